@@ -1,0 +1,129 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X render path; prints ONE JSON line.
+
+Workloads (--workload):
+  bsdf : rough-conductor BSDF operator, 2^22 independent FP32 queries
+         (PCG32 seed 42 / stream 54), complex IOR 0.27+2.78i.
+
+A step is one pass of the hot path over one batch of synthetic input that is
+already resident in HBM.  The timed region is bracketed by
+torch.cuda.synchronize() on both sides; per-launch kernel time comes from HIP
+events recorded on the stream the kernel is launched on.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+def _cpu_threads():
+    n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+# --------------------------------------------------------------------- bsdf
+def bench_bsdf(args):
+    import torch
+
+    from open_rendering_framework_amd import rt
+
+    n = 1 << 22
+    host = rt.gen_bsdf_queries(n, 42, 54)
+    dev = [t.cuda() for t in host]
+    out = [torch.empty(n, dtype=torch.float32, device="cuda") for _ in range(7)]
+    ior_re, ior_im = [0.27] * 3, [2.78] * 3
+    for _ in range(args.warmup):
+        rt.bsdf_ggx_conductor_batch(*dev, ior_re, ior_im, out=out)
+    torch.cuda.synchronize()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ev[s][0].record()
+        rt.bsdf_ggx_conductor_batch(*dev, ior_re, ior_im, out=out)
+        ev[s][1].record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = sum(a.elapsed_time(b) for a, b in ev) / args.steps
+    bytes_per_query = 52  # 24 B in (wo, alpha, u1, u2) + 28 B out (wi, pdf, RGB weight)
+    achieved = bytes_per_query * n / (kms * 1e-3) / 1e9
+    value = n * args.steps / wall / 1e6
+    line = {
+        "metric": "Mqueries/s, rough-conductor BSDF operator (2^22 FP32 queries)",
+        "value": value,
+        "unit": "Mqueries/s",
+        "ms_per_step": wall * 1e3 / args.steps,
+        "dtype": "f32",
+        "config": {"workload": "ggx_conductor_operator", "queries": n, "ior": "0.27+2.78i",
+                   "inputs": "PCG32 seed 42 seq 54"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_ggx_conductor", "kernel_ms": kms, "bytes_per_unit": bytes_per_query},
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline_bsdf(host)
+    return line
+
+
+def cpu_baseline_bsdf(host):
+    from tests.oracle_api import Oracle
+
+    orc = Oracle()
+    threads = _cpu_threads()
+    ins = [t.numpy() for t in host]
+    done, t0 = 0, time.perf_counter()
+    while True:
+        orc.bsdf_ggx_conductor(*ins, [0.27] * 3, [2.78] * 3, threads=threads)
+        done += len(ins[0])
+        el = time.perf_counter() - t0
+        if el > 10.0:
+            break
+    return {"value": done / el / 1e6, "unit": "Mqueries/s", "cores": threads, "kind": "port",
+            "sample": "%d x 2^22 operator queries (oracle FP32 restatement, OpenMP %d threads, %s)"
+                      % (done >> 22, threads, _cpu_model())}
+
+
+WORKLOADS = {"bsdf": bench_bsdf}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="bsdf", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+              "(the render path shards by image tile or sample range, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    if not torch.cuda.is_available():
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    line = WORKLOADS[args.workload](args)
+    base = {"n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "data": "synthetic"}
+    base.update(line)
+    print(json.dumps(base))
+
+
+if __name__ == "__main__":
+    main()

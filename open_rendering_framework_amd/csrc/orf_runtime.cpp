@@ -1,0 +1,64 @@
+// orf_runtime.cpp -- library-wide C-ABI pieces: errors, version, workload generators.
+#include <cmath>
+#include <cstdint>
+#include <string>
+
+#include "orf_common.h"
+
+namespace orf {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+// PCG32 (O'Neill 2014, pcg-c-basic): multiplier 6364136223846793005, XSH-RR output.
+struct Pcg32 {
+    uint64_t state = 0, inc = 1;
+    Pcg32(uint64_t initstate, uint64_t initseq)
+    {
+        inc = (initseq << 1u) | 1u;
+        next();
+        state += initstate;
+        next();
+    }
+    uint32_t next()
+    {
+        uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+        uint32_t rot = (uint32_t)(old >> 59u);
+        return (xs >> rot) | (xs << ((-rot) & 31u));
+    }
+};
+
+}  // namespace orf
+
+extern "C" const char* orf_last_error(void) { return orf::g_last_error.c_str(); }
+
+extern "C" const char* orf_version(void) { return "orf_rt 0.1 gfx950"; }
+
+extern "C" orf_status orf_gen_bsdf_queries(int64_t n, uint64_t seed, uint64_t seq, float* wo_x,
+                                           float* wo_y, float* wo_z, float* alpha, float* u1,
+                                           float* u2)
+{
+    ORF_API_BEGIN
+    ORF_REQUIRE(n >= 0, "orf_gen_bsdf_queries: n < 0");
+    ORF_REQUIRE(n == 0 || (wo_x && wo_y && wo_z && alpha && u1 && u2),
+                "orf_gen_bsdf_queries: null array");
+    orf::Pcg32 rng(seed, seq);
+    const float two_pi = 6.28318530717958647692f;
+    for (int64_t i = 0; i < n; ++i) {
+        float u[5];
+        for (float& v : u) v = (float)(rng.next() >> 8) * (1.0f / 16777216.0f);
+        float c = u[0];
+        float s = std::sqrt(std::fmax(0.0f, 1.0f - c * c));
+        float phi = two_pi * u[1];
+        wo_x[i] = s * std::cos(phi);
+        wo_y[i] = s * std::sin(phi);
+        wo_z[i] = c;
+        alpha[i] = 0.05f + 0.95f * u[2];
+        u1[i] = u[3];
+        u2[i] = u[4];
+    }
+    ORF_API_END
+}

@@ -1,0 +1,418 @@
+/*
+ * oracle_math.c -- CPU restatement of the reference's device math (TEST INFRASTRUCTURE).
+ *
+ * Follows ORF/random.h, ORF/helpers.h, ORF/sampler.h, ORF/MicrofacetGGX.h,
+ * ORF/Microfacet.h, ORF/Fresnel.h, ORF/MyComplex.h and CU/metallic_shader.cu.
+ * Only tests, the smoke check and the CPU-baseline leg of the benchmark use it.
+ *
+ * Evaluation order of every expression follows the reference so that FP32
+ * rounding matches the reference's host-compiled arithmetic; the product's
+ * kernels are compiled without FMA contraction for the same reason.
+ */
+#include <tgmath.h>
+#include <math.h>
+#include <string.h>
+#include "orf_oracle.h"
+
+#define PI_R ((R)3.14159265358979323846)
+
+/* ------------------------------------------------------------------ RNG */
+
+/* tea<N> (ORF/random.h:25-40): TEA encryption rounds with the reference's key. */
+uint32_t orc_tea_key(uint32_t v0, uint32_t v1, uint32_t rounds, const uint32_t key[4])
+{
+    uint32_t s0 = 0;
+    for (uint32_t n = 0; n < rounds; n++) {
+        s0 += 0x9e3779b9u;
+        v0 += ((v1 << 4) + key[0]) ^ (v1 + s0) ^ ((v1 >> 5) + key[1]);
+        v1 += ((v0 << 4) + key[2]) ^ (v0 + s0) ^ ((v0 >> 5) + key[3]);
+    }
+    return v0;
+}
+
+static const uint32_t REF_TEA_KEY[4] = {0xa341316cu, 0xc8013ea4u, 0xad90777du, 0x7e95761eu};
+
+uint32_t orc_tea16(uint32_t v0, uint32_t v1) { return orc_tea_key(v0, v1, 16, REF_TEA_KEY); }
+
+/* tea_hash / rnd_tea (ORF/random.h:117-126).  The value is formed in float as
+ * in the reference (it can round up to exactly 1.0f); the FP64 oracle promotes
+ * that same float so both oracles consume identical random numbers. */
+R orc_rnd_tea(uint32_t *prev)
+{
+    *prev = orc_tea16(*prev, 100u);
+    uint32_t h = *prev & 0x7FFFFFFFu;
+    float u = (float)h / (float)0x80000000u;
+    return (R)u;
+}
+
+/* lcg / rnd (ORF/random.h:43-61): Numerical Recipes LCG, 24-bit output. */
+uint32_t orc_lcg(uint32_t *prev)
+{
+    *prev = 1664525u * *prev + 1013904223u;
+    return *prev & 0x00FFFFFFu;
+}
+
+R orc_rnd(uint32_t *prev) { return (R)((float)orc_lcg(prev) / (float)0x01000000); }
+
+/* PCG32, pcg-c-basic (O'Neill 2014): published algorithm, used only to
+ * generate the synthetic operator queries. */
+uint32_t orc_pcg32_next(orc_pcg32 *r)
+{
+    uint64_t old = r->state;
+    r->state = old * 6364136223846793005ULL + r->inc;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31u));
+}
+
+void orc_pcg32_seed(orc_pcg32 *r, uint64_t initstate, uint64_t initseq)
+{
+    r->state = 0u;
+    r->inc = (initseq << 1u) | 1u;
+    orc_pcg32_next(r);
+    r->state += initstate;
+    orc_pcg32_next(r);
+}
+
+/* ------------------------------------------------------------ vectors */
+
+/* optixu normalize: multiply by the reciprocal length. */
+v3 orc_normalize(v3 v)
+{
+    R inv = (R)1 / sqrt(vdot(v, v));
+    return vscale(v, inv);
+}
+
+/* faceforward(n, i, nref) = n * sign(dot(i, nref)) (OptiX convention). */
+v3 orc_faceforward(v3 n, v3 i, v3 nref)
+{
+    return vscale(n, copysign((R)1, vdot(i, nref)));
+}
+
+/* create_onb, two-output form (ORF/helpers.h:107-119). */
+void orc_create_onb(v3 n, v3 *U, v3 *V)
+{
+    v3 u = vcross(n, V3(0, 1, 0));
+    if (vdot(u, u) < (R)1e-3f)
+        u = vcross(n, V3(1, 0, 0));
+    u = orc_normalize(u);
+    *U = u;
+    *V = vcross(n, u);
+}
+
+/* transformToLocal / transformToWorld (ORF/helpers.h:284-299). */
+v3 orc_to_local(v3 w, v3 n)
+{
+    v3 u, v;
+    orc_create_onb(n, &u, &v);
+    return orc_normalize(V3(vdot(w, u), vdot(w, v), vdot(w, n)));
+}
+
+v3 orc_to_world(v3 w, v3 n)
+{
+    v3 u, v;
+    orc_create_onb(n, &u, &v);
+    return orc_normalize(vadd(vadd(vscale(u, w.x), vscale(v, w.y)), vscale(n, w.z)));
+}
+
+/* ------------------------------------------------------------ sampler.h */
+
+/* rotate_to_normal (ORF/sampler.h:12-24, Frisvad 2012). */
+void orc_rotate_to_normal(v3 n, v3 *v)
+{
+    if (n.z < (R)-0.999999f) {
+        *v = V3(-v->y, -v->x, -v->z);
+        return;
+    }
+    R a = (R)1 / ((R)1 + n.z);
+    R b = -n.x * n.y * a;
+    v3 c0 = V3((R)1 - n.x * n.x * a, b, -n.x);
+    v3 c1 = V3(b, (R)1 - n.y * n.y * a, -n.y);
+    *v = vadd(vadd(vscale(c0, v->x), vscale(c1, v->y)), vscale(n, v->z));
+}
+
+/* sample_cosine_weighted (ORF/sampler.h:51-64) with spherical_direction (:29-34). */
+v3 orc_sample_cosine_weighted(v3 n, uint32_t *t)
+{
+    R cos_theta = sqrt(orc_rnd_tea(t));
+    R phi = (R)2 * PI_R * orc_rnd_tea(t);
+    R sin_theta = sqrt((R)1 - cos_theta * cos_theta);
+    v3 v = V3(sin_theta * cos(phi), sin_theta * sin(phi), cos_theta);
+    orc_rotate_to_normal(n, &v);
+    return v;
+}
+
+/* sample_barycentric (ORF/sampler.h:103-116). */
+v3 orc_sample_barycentric(uint32_t *t)
+{
+    R sqrt_xi1 = sqrt(orc_rnd_tea(t));
+    R xi2 = orc_rnd_tea(t);
+    return V3((R)1 - sqrt_xi1, ((R)1 - xi2) * sqrt_xi1, xi2 * sqrt_xi1);
+}
+
+/* ------------------------------------------------------------ GGX */
+
+/* ggx_alpha_i (ORF/MicrofacetGGX.h:9-17). */
+R orc_ggx_alpha_i(v3 wi, R ax, R ay)
+{
+    R inv_sin2 = (R)1 / ((R)1 - wi.z * wi.z);
+    if (wi.z == (R)1 || wi.z == (R)-1)
+        inv_sin2 = 0;
+    R cos_phi2 = wi.x * wi.x * inv_sin2;
+    R sin_phi2 = wi.y * wi.y * inv_sin2;
+    return sqrt(cos_phi2 * ax * ax + sin_phi2 * ay * ay);
+}
+
+/* ggx_G1(cos, alpha^2) (ORF/MicrofacetGGX.h:54-60): Smith G1 closed form. */
+R orc_ggx_G1_cos(R cos_theta, R alpha_sqr)
+{
+    R c2 = cos_theta * cos_theta;
+    R t2 = ((R)1 - c2) / c2;
+    return (R)2 / ((R)1 + sqrt((R)1 + alpha_sqr * t2));
+}
+
+/* ggx_G1(v, m, n, ax, ay) (ORF/MicrofacetGGX.h:63-74). */
+R orc_ggx_G1(v3 v, v3 m, v3 n, R ax, R ay)
+{
+    R alpha = orc_ggx_alpha_i(orc_normalize(orc_to_local(v, n)), ax, ay);
+    R a2 = alpha * alpha;
+    R cos_v = vdot(v, n);
+    if (cos_v * vdot(v, m) <= 0)
+        return 0;
+    return orc_ggx_G1_cos(cos_v, a2);
+}
+
+/* ggx_projected_area (ORF/MicrofacetGGX.h:33-45): cos(theta) (1 + Lambda). */
+R orc_ggx_projected_area(v3 wi, R ax, R ay)
+{
+    if (wi.z > (R)0.9999f)
+        return 1;
+    if (wi.z < (R)-0.9999f)
+        return 0;
+    R alpha = orc_ggx_alpha_i(wi, ax, ay);
+    R theta = acos(wi.z);
+    R s = sin(theta);
+    return (R)0.5f * (wi.z + sqrt(wi.z * wi.z + s * s * alpha * alpha));
+}
+
+/* ggx_slope_pdf + ggx_eval_NDF (ORF/MicrofacetGGX.h:48-53, 219-231). */
+R orc_ggx_eval_ndf(v3 wm, R ax, R ay)
+{
+    if (wm.z <= 0)
+        return 0;
+    R sx = -wm.x / wm.z;
+    R sy = -wm.y / wm.z;
+    R tmp = (R)1 + sx * sx / (ax * ax) + sy * sy / (ay * ay);
+    R p22 = (R)1 / (PI_R * ax * ay) / (tmp * tmp);
+    return p22 / (wm.z * wm.z * wm.z * wm.z);
+}
+
+/* ggx_eval_VNDF (ORF/MicrofacetGGX.h:234-251): D_wi(m) = <wi,m> D(m) / proj_area. */
+R orc_ggx_eval_vndf(v3 wi, v3 wm, R ax, R ay)
+{
+    if (wm.z <= 0)
+        return 0;
+    R area = orc_ggx_projected_area(wi, ax, ay);
+    if (area == 0)
+        return 0;
+    R c = (R)1 / area;
+    R D = orc_ggx_eval_ndf(wm, ax, ay);
+    R d = vdot(wi, wm);
+    return c * (d > 0 ? d : 0) * D;
+}
+
+/* ggx_sample_P22_11 (ORF/MicrofacetGGX.h:142-187): visible-slope sampling of
+ * the unit-roughness distribution (Heitz & d'Eon 2014, rational fit at :184). */
+v2 orc_ggx_sample_p22_11(R theta_i, R z1, R z2)
+{
+    v2 s;
+    if (theta_i < (R)0.0001f) {
+        R r = sqrt(z1 / ((R)1 - z1));
+        R phi = PI_R * (R)2 * z2;
+        s.x = r * cos(phi);
+        s.y = r * sin(phi);
+        return s;
+    }
+    R sin_i = sin(theta_i);
+    R cos_i = cos(theta_i);
+    if (!(cos_i >= (R)1e-7f)) cos_i = (R)1e-7f;  /* fmaxf(cos, 1e-7) */
+    R tan_i = sin_i / cos_i;
+    R proj_area = (R)0.5f * (cos_i + (R)1);
+    if (proj_area < (R)0.0001f || proj_area != proj_area) {
+        s.x = 0; s.y = 0;
+        return s;
+    }
+    R c = (R)1 / proj_area;
+    R A = (R)2 * z1 / cos_i / c - (R)1;
+    R B = tan_i;
+    R tmp = (R)1 / (A * A - (R)1);
+    R disc = B * B * tmp * tmp - (A * A - B * B) * tmp;
+    R D = sqrt(disc > 0 ? disc : 0);
+    R sx1 = B * tmp - D;
+    R sx2 = B * tmp + D;
+    s.x = (A < 0 || sx2 > (R)1 / tan_i) ? sx1 : sx2;
+    R S, zz2;
+    if (z2 > (R)0.5f) {
+        S = 1;
+        zz2 = (R)2 * (z2 - (R)0.5f);
+    } else {
+        S = -1;
+        zz2 = (R)2 * ((R)0.5f - z2);
+    }
+    R z = (zz2 * (zz2 * (zz2 * (R)0.27385f - (R)0.73369f) + (R)0.46341f)) /
+          (zz2 * (zz2 * (zz2 * (R)0.093073f + (R)0.309420f) - (R)1.0f) + (R)0.597999f);
+    s.y = S * z * sqrt((R)1 + s.x * s.x);
+    return s;
+}
+
+/* ggx_sample_VNDF (ORF/MicrofacetGGX.h:190-207). */
+v3 orc_ggx_sample_vndf(v3 wi, R ax, R ay, R z1, R z2)
+{
+    v3 wi11 = orc_normalize(V3(ax * wi.x, ay * wi.y, wi.z));
+    v2 s11 = orc_ggx_sample_p22_11(acos(wi11.z), z1, z2);
+    R phi = atan2(wi11.y, wi11.x);
+    R cp = cos(phi), sp = sin(phi);
+    R sx = cp * s11.x - sp * s11.y;
+    R sy = sp * s11.x + cp * s11.y;
+    sx *= ax;
+    sy *= ay;
+    if (!isfinite(sx) || sx != sx) {
+        if (wi.z > 0)
+            return V3(0, 0, 1);
+        return orc_normalize(V3(wi.x, wi.y, 0));
+    }
+    return orc_normalize(V3(-sx, -sy, (R)1));
+}
+
+/* ------------------------------------------------------------ complex Fresnel */
+
+/* MyComplex helpers (ORF/MyComplex.h).  The reference forms |c| and arg(c)
+ * in float (abs :18-23, phase :169-185); R reproduces that truncation in the
+ * FP32 oracle and removes it in the FP64 oracle. */
+static R c_abs(cplx c)
+{
+    R r = (R)c.re, i = (R)c.im;
+    return (R)sqrt((double)(r * r + i * i));
+}
+
+static R c_phase(cplx c)
+{
+    double r = c.re, i = c.im, th;
+    const double pi_ref = (double)PI_R;
+    if (r == 0 && i > 0) th = pi_ref / 2;
+    else if (r == 0 && i < 0) th = -pi_ref / 2;
+    else if (r == 0 && i == 0) th = 0;
+    else th = atan(i / r);
+    if (r < 0) th += pi_ref;
+    return (R)th;
+}
+
+static cplx C(double re, double im) { cplx c = {re, im}; return c; }
+static cplx c_add(cplx a, cplx b) { return C(a.re + b.re, a.im + b.im); }
+static cplx c_sub(cplx a, cplx b) { return C(a.re - b.re, a.im - b.im); }
+static cplx c_mul(cplx a, cplx b) { return C(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+static cplx c_scale(cplx a, double s) { return C(a.re * s, a.im * s); }
+
+/* operator/(MyComplex, MyComplex) (ORF/MyComplex.h:148-153). */
+static cplx c_div(cplx a, cplx b)
+{
+    cplx num = c_mul(a, C(b.re, -b.im));
+    R ab = c_abs(b);
+    double den = (double)(R)pow(ab, (R)2);
+    return c_scale(num, 1.0 / den);
+}
+
+/* operator/(float, MyComplex) (ORF/MyComplex.h:155-160). */
+cplx orc_cdiv_fc(R c1, cplx b)
+{
+    cplx num = C((double)c1 * b.re, (double)c1 * -b.im);
+    R ab = c_abs(b);
+    double den = (double)(R)pow(ab, (R)2);
+    return c_scale(num, 1.0 / den);
+}
+
+/* pow(MyComplex, int) (ORF/MyComplex.h:187-193). */
+static cplx c_powi(cplx c, int e)
+{
+    double ab = pow((double)c_abs(c), (double)e);
+    double th = (double)(R)(c_phase(c) * (R)e);
+    return C(ab * cos(th), ab * sin(th));
+}
+
+/* sqrt(MyComplex) (ORF/MyComplex.h:195-202): half the phase in (-pi/2, 3pi/2]. */
+static cplx c_sqrt(cplx c)
+{
+    double ab = sqrt((double)c_abs(c));
+    double th = (double)(R)(c_phase(c) / (R)2);
+    return C(ab * cos(th), ab * sin(th));
+}
+
+static R fresnel_conductor_1(R cos_i, R sin_i, cplx eta)
+{
+    cplx sin_t = c_scale(eta, (double)sin_i);
+    cplx p = c_powi(sin_t, 2);
+    cplx cos_t = c_sqrt(C((double)(R)1.0f - p.re, -p.im));
+    cplx e_ci = c_scale(eta, (double)cos_i);
+    cplx r_ort = c_div(c_sub(e_ci, cos_t), c_add(e_ci, cos_t));
+    cplx e_ct = c_mul(eta, cos_t);
+    cplx r_par = c_div(C((double)cos_i - e_ct.re, -e_ct.im), C((double)cos_i + e_ct.re, e_ct.im));
+    R a = pow(c_abs(r_ort), (R)2.0f);
+    R b = pow(c_abs(r_par), (R)2);
+    return (R)0.5f * (a + b);
+}
+
+/* fresnel_MyComplex_R (ORF/Fresnel.h:78-91): unpolarised reflectance of a
+ * conductor with relative complex IOR eta (= n1 / ior). */
+v3 orc_fresnel_conductor(v3 w_i, v3 w_m, const cplx3 *eta)
+{
+    R cos_i = vdot(w_i, w_m);
+    R sin_i = sqrt((R)1 - cos_i * cos_i);
+    return V3(fresnel_conductor_1(cos_i, sin_i, eta->x),
+              fresnel_conductor_1(cos_i, sin_i, eta->y),
+              fresnel_conductor_1(cos_i, sin_i, eta->z));
+}
+
+/* fresnel_R(cos, ior1/ior2) (ORF/Fresnel.h:14-40): dielectric, unpolarised. */
+R orc_fresnel_dielectric(R cos_theta, R eta)
+{
+    R sin_t2 = eta * eta * ((R)1 - cos_theta * cos_theta);
+    if (sin_t2 >= (R)1)
+        return 1;
+    R cos_t = sqrt((R)1 - sin_t2);
+    R rs = (eta * cos_theta - cos_t) / (eta * cos_theta + cos_t);
+    R rp = (cos_theta - eta * cos_t) / (cos_theta + eta * cos_t);
+    return (rs * rs + rp * rp) * (R)0.5f;
+}
+
+/* ------------------------------------------------------------ operator */
+
+/* Rough-conductor sampling step of CU/metallic_shader.cu:84-158 (visible
+ * normals, GGX, isotropic alpha), for a surface with normal (0,0,1).  wo is the
+ * reference's w_i (towards the viewer); *wi is its w_o (the reflected ray).
+ * weight = F(wo.m) G1(wi) (:117,152); pdf = D_wo(m) / (4 |wi.m|) through
+ * microfacet_eval_visible_normal (ORF/Microfacet.h:85-97). */
+int orc_ggx_conductor_query(v3 wo, R alpha, R u1, R u2, const cplx3 *eta,
+                            v3 *wi, R *pdf, v3 *weight)
+{
+    const v3 normal = V3(0, 0, 1);
+    v3 w_i = wo;
+    v3 ffn = orc_faceforward(normal, w_i, normal);
+    R ax = alpha, ay = alpha;
+
+    v3 m = orc_to_world(orc_ggx_sample_vndf(orc_to_local(w_i, ffn), ax, ay, u1, u2), ffn);
+    R G_i = orc_ggx_G1(w_i, m, ffn, ax, ay);
+    v3 F = orc_fresnel_conductor(w_i, m, eta);
+    v3 w_o = vadd(vneg(w_i), vscale(vscale(m, (R)2.0f), vdot(w_i, m)));
+    R G_o = orc_ggx_G1(w_o, m, ffn, ax, ay);
+
+    R D = orc_ggx_eval_vndf(orc_to_local(w_i, ffn), orc_to_local(m, ffn), ax, ay);
+    R dm = vdot(w_o, m);
+    *wi = w_o;
+    *pdf = D / ((R)4 * (dm < 0 ? -dm : dm));
+    if (G_i <= 0 || G_o <= 0) {
+        *weight = V3(0, 0, 0);
+        return 0;
+    }
+    *weight = vscale(F, G_o);
+    return 1;
+}

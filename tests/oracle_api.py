@@ -1,0 +1,136 @@
+"""numpy-facing wrappers of the CPU oracle's C entry points (tests only)."""
+import ctypes
+
+import numpy as np
+
+import oracle
+
+c_vp = ctypes.c_void_p
+
+
+def _p(a):
+    return a.ctypes.data_as(c_vp)
+
+
+def _d3(v):
+    return (ctypes.c_double * 3)(*[float(x) for x in v])
+
+
+class Oracle:
+    def __init__(self, fp64=False):
+        self.lib = oracle.lib(fp64=fp64)
+        self.fp64 = fp64
+        L = self.lib
+        L.orc_tea_key.restype = ctypes.c_uint32
+        L.orc_tea_key.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        L.orc_tea16.restype = ctypes.c_uint32
+        L.orc_tea16.argtypes = [ctypes.c_uint32, ctypes.c_uint32]
+        L.orc_pcg32_stream.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, c_vp]
+        L.orc_tea_stream.argtypes = [ctypes.c_uint32, ctypes.c_int, c_vp, c_vp]
+        L.orc_lcg_stream.argtypes = [ctypes.c_uint32, ctypes.c_int, c_vp, c_vp]
+        L.orc_gen_bsdf_queries.argtypes = [ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint64] + [c_vp] * 6
+        L.orc_bsdf_ggx_conductor_batch.argtypes = (
+            [ctypes.c_int64] + [c_vp] * 6 + [c_vp, c_vp] + [c_vp] * 7 + [ctypes.c_int]
+        )
+        L.orc_conductor_eta.argtypes = [c_vp] * 4
+        L.orc_ggx_G1_batch.argtypes = [ctypes.c_int] + [c_vp] * 6
+        L.orc_ggx_ndf_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_double, ctypes.c_double, c_vp]
+        L.orc_ggx_vndf_batch.argtypes = [ctypes.c_int, c_vp, c_vp, ctypes.c_double, ctypes.c_double, c_vp]
+        L.orc_ggx_sample_vndf_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_float, ctypes.c_float, c_vp, c_vp, c_vp]
+        L.orc_fresnel_conductor_batch.argtypes = [ctypes.c_int, c_vp, c_vp, c_vp, c_vp]
+        L.orc_fresnel_dielectric_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_float, c_vp]
+        L.orc_cosine_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_uint32, c_vp]
+
+    # ---- RNG
+    def tea(self, v0, v1, rounds=16, key=None):
+        if key is None:
+            key = (0xA341316C, 0xC8013EA4, 0xAD90777D, 0x7E95761E)
+        k = (ctypes.c_uint32 * 4)(*key)
+        return self.lib.orc_tea_key(v0, v1, rounds, k)
+
+    def pcg32(self, seed, seq, n):
+        out = np.zeros(n, np.uint32)
+        self.lib.orc_pcg32_stream(seed, seq, n, _p(out))
+        return out
+
+    def tea_stream(self, seed, n):
+        st = np.zeros(n, np.uint32)
+        u = np.zeros(n, np.float32)
+        self.lib.orc_tea_stream(seed, n, _p(st), _p(u))
+        return st, u
+
+    def lcg_stream(self, seed, n):
+        st = np.zeros(n, np.uint32)
+        u = np.zeros(n, np.float32)
+        self.lib.orc_lcg_stream(seed, n, _p(st), _p(u))
+        return st, u
+
+    # ---- operator workload
+    def gen_bsdf_queries(self, n, seed=42, seq=54):
+        arrs = [np.zeros(n, np.float32) for _ in range(6)]
+        self.lib.orc_gen_bsdf_queries(n, seed, seq, *[_p(a) for a in arrs])
+        return arrs
+
+    def conductor_eta(self, ior_re, ior_im):
+        er = np.zeros(3)
+        ei = np.zeros(3)
+        self.lib.orc_conductor_eta(_p(np.asarray(ior_re, np.float64)), _p(np.asarray(ior_im, np.float64)), _p(er), _p(ei))
+        return er + 1j * ei
+
+    def bsdf_ggx_conductor(self, wx, wy, wz, alpha, u1, u2, ior_re, ior_im, threads=8):
+        n = len(wx)
+        ins = [np.ascontiguousarray(a, np.float32) for a in (wx, wy, wz, alpha, u1, u2)]
+        outs = [np.zeros(n, np.float32) for _ in range(7)]
+        self.lib.orc_bsdf_ggx_conductor_batch(
+            n, *[_p(a) for a in ins], _d3(ior_re), _d3(ior_im), *[_p(a) for a in outs], threads
+        )
+        return outs
+
+    # ---- single functions
+    def ggx_G1(self, v, m, nrm, ax, ay):
+        n = len(v)
+        out = np.zeros(n, np.float32)
+        args = [np.ascontiguousarray(a, np.float32) for a in (v, m, nrm)]
+        axa = np.full(n, ax, np.float32) if np.isscalar(ax) else np.ascontiguousarray(ax, np.float32)
+        aya = np.full(n, ay, np.float32) if np.isscalar(ay) else np.ascontiguousarray(ay, np.float32)
+        self.lib.orc_ggx_G1_batch(n, *[_p(a) for a in args], _p(axa), _p(aya), _p(out))
+        return out
+
+    def ggx_ndf(self, wm, ax, ay):
+        wm = np.ascontiguousarray(wm, np.float64)
+        out = np.zeros(len(wm))
+        self.lib.orc_ggx_ndf_batch(len(wm), _p(wm), ax, ay, _p(out))
+        return out
+
+    def ggx_vndf(self, wi, wm, ax, ay):
+        wm = np.ascontiguousarray(wm, np.float64)
+        wi = np.ascontiguousarray(wi, np.float64)
+        out = np.zeros(len(wm))
+        self.lib.orc_ggx_vndf_batch(len(wm), _p(wi), _p(wm), ax, ay, _p(out))
+        return out
+
+    def ggx_sample_vndf(self, wi, ax, ay, z1, z2):
+        wi = np.ascontiguousarray(wi, np.float32)
+        z1 = np.ascontiguousarray(z1, np.float32)
+        z2 = np.ascontiguousarray(z2, np.float32)
+        out = np.zeros((len(z1), 3), np.float32)
+        self.lib.orc_ggx_sample_vndf_batch(len(z1), _p(wi), ax, ay, _p(z1), _p(z2), _p(out))
+        return out
+
+    def fresnel_conductor(self, cos_i, ior_re, ior_im):
+        c = np.ascontiguousarray(cos_i, np.float32)
+        out = np.zeros((len(c), 3), np.float32)
+        self.lib.orc_fresnel_conductor_batch(len(c), _p(c), _d3(ior_re), _d3(ior_im), _p(out))
+        return out
+
+    def fresnel_dielectric(self, cos_i, eta):
+        c = np.ascontiguousarray(cos_i, np.float32)
+        out = np.zeros(len(c), np.float32)
+        self.lib.orc_fresnel_dielectric_batch(len(c), _p(c), eta, _p(out))
+        return out
+
+    def sample_cosine(self, nrm, seed, n):
+        nn = np.ascontiguousarray(nrm, np.float32)
+        out = np.zeros((n, 3), np.float32)
+        self.lib.orc_cosine_batch(n, _p(nn), seed, _p(out))
+        return out
