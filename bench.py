@@ -2,6 +2,11 @@
 """Benchmark of the MI355X render path; prints ONE JSON line.
 
 Workloads (--workload):
+  path : (default) plain path tracer, 512x512, 64 spp, max depth 8: Lambert
+         ground, 327,680-triangle displaced icosphere as a GGX rough conductor
+         (alpha 0.3, ior 0.27+2.78i), 1x1 area light of radiance 10, 1024x512
+         sky + sun env map importance-sampled through an FP64-built 2D CDF.
+         One step = one launch of all 64 samples of every pixel (16.8 M paths).
   bsdf : rough-conductor BSDF operator, 2^22 independent FP32 queries
          (PCG32 seed 42 / stream 54), complex IOR 0.27+2.78i.
 
@@ -98,15 +103,93 @@ def cpu_baseline_bsdf(host):
                       % (done >> 22, threads, _cpu_model())}
 
 
-WORKLOADS = {"bsdf": bench_bsdf}
+# --------------------------------------------------------------------- path
+PATH_W = PATH_H = 512
+PATH_SPP = 64
+
+
+def bench_path(args):
+    import torch
+
+    from open_rendering_framework_amd import rt, scenes
+
+    desc = scenes.path_scene(width=PATH_W, height=PATH_H)
+    ctx = scenes.to_context(desc)
+    stats = ctx.accel_build()
+    stream = torch.cuda.current_stream()
+    for w in range(max(1, args.warmup)):
+        ctx.launch(PATH_W, PATH_H, 0, PATH_SPP, stream=stream)
+    torch.cuda.synchronize()
+    ctx.film_clear()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ev[s][0].record(stream)
+        ctx.launch(PATH_W, PATH_H, 0, PATH_SPP, stream=stream)
+        ev[s][1].record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kms = sum(a.elapsed_time(b) for a, b in ev) / args.steps
+    c = ctx.counters()
+    paths = PATH_W * PATH_H * PATH_SPP
+    assert c["paths"] == paths * args.steps, c
+    n_ext = c["ext_rays"] / c["paths"]
+    n_shd = c["shadow_rays"] / c["paths"]
+    # algorithmic bytes per path: extension segment 160 B (ray 32 B w+r, hit 16 B
+    # w+r, path state 32 B r+w), shadow segment 72 B (ray w+r + result), FP64 RGBA
+    # film 32 B per pixel shared by its spp paths
+    b_path = n_ext * 160.0 + n_shd * 72.0 + 32.0 / PATH_SPP
+    achieved = b_path * paths / (kms * 1e-3) / 1e9
+    value = paths * args.steps / wall / 1e6
+    line = {
+        "metric": "Mpaths/s at 512^2 64spp (plain megakernel path tracer)",
+        "value": value,
+        "unit": "Mpaths/s",
+        "ms_per_step": wall * 1e3 / args.steps,
+        "dtype": "f32",
+        "config": {"workload": "plain_path_tracer_512x512_64spp", "width": PATH_W, "height": PATH_H,
+                   "spp": PATH_SPP, "max_depth": 8, "triangles": int(stats["tris"]),
+                   "env": "1024x512 sky+sun, importance sampled", "form": "megakernel"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_render_mega",
+                     "kernel_ms": kms, "bytes_per_unit": b_path, "n_ext_per_path": n_ext,
+                     "n_shadow_per_path": n_shd},
+        "bvh": stats,
+        "nonfinite_samples": c["nonfinite"],
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline_path(desc)
+    return line
+
+
+def cpu_baseline_path(desc):
+    from tests.oracle_scene import OracleScene
+
+    threads = _cpu_threads()
+    orc = OracleScene(desc)
+    frames, paths, t0 = 0, 0, time.perf_counter()
+    while True:
+        orc.render(PATH_W, PATH_H, frames, 1, threads=threads)
+        frames += 1
+        paths += PATH_W * PATH_H
+        el = time.perf_counter() - t0
+        if el > 10.0:
+            break
+    return {"value": paths / el / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+            "sample": "%d frames of 512x512 at 1 spp of the same scene and seeds (oracle FP32 restatement, "
+                      "OpenMP %d threads, %s)" % (frames, threads, _cpu_model())}
+
+
+WORKLOADS = {"path": bench_path, "bsdf": bench_bsdf}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="bsdf", choices=sorted(WORKLOADS))
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="path", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -71,6 +71,106 @@ orf_status orf_gen_bsdf_queries(int64_t n, uint64_t seed, uint64_t seq,
                                 float* wo_x, float* wo_y, float* wo_z, float* alpha,
                                 float* u1, float* u2);
 
+/* ------------------------------------------------------------------------
+ * Render context: scene objects, named variables, launches, film.
+ *
+ * Mirrors the subset of the optix:: host API the reference's scene classes
+ * use.  A context owns one device, one scene and one FP64 film.
+ * ------------------------------------------------------------------------ */
+typedef struct orf_context_t* orf_context;
+
+/* optix::Context::create + setRayTypeCount/setEntryPointCount/setStackSize
+ * (ORF/OptixScene.cpp:96-117).  device = HIP ordinal. */
+orf_status orf_context_create(int device, orf_context* out);
+/* OptixScene::destroyContext (ORF/OptixScene.cpp:68-75). */
+orf_status orf_context_destroy(orf_context ctx);
+
+/* Variable scopes of context["name"] / material["name"]. */
+enum { ORF_SCOPE_CONTEXT = 0, ORF_SCOPE_MATERIAL = 1 };
+
+/* Variable setters: context["name"]->setFloat/setInt/setUint(...) and
+ * material["name"]->... with 1-4 components; setUserData(size, ptr) for
+ * structured values (e.g. the conductor "ior" as MyComplex3, 6 doubles:
+ * ORF/MetallicMaterial.cpp, ORF/MyComplex.h:7-16).  Context variables:
+ * max_depth, scene_epsilon, eye, U, V, W, frame, patch_origin, patch_dims,
+ * exception_color (ORF/Integrator.cpp:6-24, ORF/Camera.cpp:83-103).
+ * Material variables: diffuse_color, radiance, roughness, ior,
+ * microfacet_model, normal_distribution, glass_absorption. */
+orf_status orf_variable_set_float(orf_context ctx, int scope, int handle, const char* name,
+                                  const float* v, int n);
+orf_status orf_variable_set_int(orf_context ctx, int scope, int handle, const char* name,
+                                const int32_t* v, int n);
+orf_status orf_variable_set_uint(orf_context ctx, int scope, int handle, const char* name,
+                                 const uint32_t* v, int n);
+orf_status orf_variable_set_user_data(orf_context ctx, int scope, int handle, const char* name,
+                                      const void* data, size_t size);
+
+/* createMaterial + setClosestHitProgram(radiance, createProgramFromPTXFile(file,
+ * "closest_hit")) (e.g. ORF/MetallicMaterial.cpp:108-118).  `program` is the
+ * reference program file: "diffuse_shader.cu", "metallic_shader.cu",
+ * "arealight_shader.cu", "rough_transparent_shader.cu",
+ * "lambertian_interface_shader.cu".  Returns the material handle. */
+orf_status orf_material_create(orf_context ctx, const char* program, int* material);
+
+/* createGeometry + vertex/index/normal buffers + createGeometryInstance
+ * (ORF/ObjLoader.cpp:198-410).  verts: 3 floats per vertex; vindex: 3 ints per
+ * triangle; normals/nindex optional (NULL: geometric normals; nindex NULL with
+ * normals given: nindex = vindex).  One material per mesh. */
+orf_status orf_mesh_create(orf_context ctx, const float* verts, int64_t nverts, const int32_t* vindex,
+                           int64_t ntris, const float* normals, int64_t nnormals, const int32_t* nindex,
+                           int material, int* mesh);
+/* Transform node: setMatrix(transpose=false, m) with a row-major object-to-world
+ * 4x4 (ORF/Geometry.cpp:286-291); normals use its inverse transpose. */
+orf_status orf_mesh_set_transform(orf_context ctx, int mesh, const float m[16]);
+
+/* LightStruct entries of light_buffer (ORF/OptixSceneLoader.cpp:349-410). */
+orf_status orf_light_add_point(orf_context ctx, const float position[3], const float radiance[3]);
+orf_status orf_light_add_directional(orf_context ctx, const float direction[3], const float radiance[3]);
+/* TrianglesAreaLight: appends one TriangleLight per triangle to
+ * triangle_light_buffer (area = |(v1-v2) x (v0-v2)| / 2, ORF/ObjLoader.cpp:532-612)
+ * and one LightStruct {count, start, end}.  tri_verts: 9 floats per triangle in
+ * world space; tri_normals: 9 floats per triangle or NULL. */
+orf_status orf_light_add_triangles(orf_context ctx, const float* tri_verts, const float* tri_normals,
+                                   int64_t ntris, const float radiance[3]);
+orf_status orf_lights_clear(orf_context ctx);
+
+/* EnvMapBackground (ORF/Background.cpp:57-105): FP32 RGBA lat-long texture,
+ * row 0 at the nadir.  importance_sample != 0 builds the FP64 2D CDF on the GPU
+ * (spec: CU/env_cameras.cu) and adds the environment to the light list. */
+orf_status orf_envmap_set(orf_context ctx, const float* rgba, int width, int height, int importance_sample);
+/* Read back the FP32 pdf/cdf tables (host arrays: h, h, h*w, h*w). */
+orf_status orf_envmap_tables(orf_context ctx, float* marg_pdf, float* marg_cdf, float* cond_pdf,
+                             float* cond_cdf);
+
+/* Geometry changed: BVH rebuilt at the next launch (Acceleration::markDirty). */
+orf_status orf_accel_mark_dirty(orf_context ctx);
+/* Build/upload now (optional; launch does it when dirty).  stats: nodes, leaves,
+ * max_depth, max_leaf, tris, degenerate (6 int64, may be NULL). */
+orf_status orf_accel_build(orf_context ctx, int64_t* stats);
+
+/* Integrator pass over a width x height film (ORF/OptixScene.cpp:139-159 with
+ * path_tracer, CU/path_tracer.cu:32-74): samples frame0 .. frame0+spp-1 of every
+ * pixel in one launch, sample s seeded tea<16>(width*y + x, s).  The film keeps
+ * FP64 sums of finite samples and their count; it is cleared when the size
+ * changes or by orf_film_clear. */
+orf_status orf_launch(orf_context ctx, int width, int height, uint32_t frame0, uint32_t spp, void* stream);
+orf_status orf_film_clear(orf_context ctx);
+/* Film mean as RGBA (A = 1) in FP64 (rgba64) and/or FP32 (rgba32); host arrays
+ * of 4*width*height, row 0 = bottom row (launch index y = 0). */
+orf_status orf_film_read(orf_context ctx, double* rgba64, float* rgba32);
+/* Raw film (sum R, G, B, count) as FP64 host array of 4*width*height. */
+orf_status orf_film_read_sums(orf_context ctx, double* sums);
+/* Counters since the last clear: [0] non-finite samples dropped,
+ * [1] extension (closest-hit) rays, [2] shadow rays, [3] paths. */
+orf_status orf_counters(orf_context ctx, uint64_t out[4]);
+
+/* Procedural scene pieces (host).  Icosphere: call with verts = tris = normals =
+ * NULL to get the sizes.  Env: sun_dir in env coordinates, sigma in degrees. */
+orf_status orf_gen_icosphere(int level, uint32_t noise_seed, float amplitude, float frequency,
+                             float* verts, float* normals, int64_t* nverts, int32_t* tris, int64_t* ntris);
+orf_status orf_gen_sky_envmap(int width, int height, const float sun_dir[3], float sun_sigma_deg,
+                              float sun_peak, float* rgba);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,12 +11,14 @@
 #include "orf_common.h"
 #include "orf_fresnel.h"
 #include "orf_microfacet.h"
+#include "orf_render.h"
+#include "orf_shade.h"
 
 namespace orf {
 
 // eta = 1 / ior as the reference forms it (operator/(float, MyComplex),
 // ORF/MyComplex.h:155-160: conj(ior) / |ior|^2 with |ior| rounded to float).
-static void conductor_eta_host(const double ior_re[3], const double ior_im[3], ConductorEta& e)
+void conductor_eta_host(const double ior_re[3], const double ior_im[3], ConductorEta& e)
 {
     for (int c = 0; c < 3; ++c) {
         float r = (float)ior_re[c], i = (float)ior_im[c];
@@ -29,21 +31,6 @@ static void conductor_eta_host(const double ior_re[3], const double ior_im[3], C
     }
     e.gray = (ior_re[0] == ior_re[1] && ior_re[1] == ior_re[2] && ior_im[0] == ior_im[1] &&
               ior_im[1] == ior_im[2]);
-}
-
-// One visible-normal sampling step of a rough conductor with macro-normal n.
-// Returns the reflected direction, its solid-angle pdf and f cos / pdf.
-ORF_HD void ggx_conductor_sample(f3 w_i, f3 n, float ax, float ay, float z1, float z2,
-                                 const ConductorEta& eta, f3& w_o, float& pdf, f3& weight)
-{
-    f3 m = to_world(ggx_sample_visible(to_local(w_i, n), ax, ay, z1, z2), n);
-    float g_i = ggx_masking(w_i, m, n, ax, ay);
-    f3 F = fresnel_conductor(w_i, m, eta);
-    w_o = -w_i + (m * 2.0f) * dot(w_i, m);
-    float g_o = ggx_masking(w_o, m, n, ax, ay);
-    float D = ggx_vndf(to_local(w_i, n), to_local(m, n), ax, ay);
-    pdf = D / (4.0f * fabsf(dot(w_o, m)));
-    weight = (g_i <= 0.0f || g_o <= 0.0f) ? mk3(0.0f) : F * g_o;
 }
 
 __global__ __launch_bounds__(256) void k_ggx_conductor(int64_t n, const float* __restrict__ wx,

@@ -1,0 +1,115 @@
+// orf_scene.h -- device-side scene layout shared by the render kernels.
+//
+// HBM layout (all arrays 16 B aligned, structure-of-arrays where a kernel
+// streams one field):
+//   bvh      : BVH2 nodes, 64 B each, both child boxes stored in the parent so one
+//              node fetch tests two boxes (4 x float4: c0.lo,c0.hi,c1.lo,c1.hi
+//              packed, then the two child links)
+//   tri      : triangles in leaf order, 48 B each: p0, e1 = p1-p0, e2 = p2-p0 and the
+//              original triangle id (3 x float4), read by the intersection loop only
+//   shade    : per original triangle: vertex-normal indices and material id,
+//              read once per closest hit
+//   normals  : per-vertex shading normals (float3 as float4)
+//   materials, lights, triangle lights: small tables (read-only, cached)
+//   env      : RGBA FP32 texture + FP32 marginal/conditional pdf and cdf tables
+//   film     : FP64 RGBA per pixel: running sums of RGB and the finite-sample count
+#pragma once
+
+#include "orf_fresnel.h"
+#include "orf_vec.h"
+
+namespace orf {
+
+// Material program ids (the reference binds closest-hit programs by file name).
+enum MaterialKind : int {
+    MAT_LAMBERT = 0,        // diffuse_shader.cu
+    MAT_CONDUCTOR = 1,      // metallic_shader.cu
+    MAT_EMITTER = 2,        // arealight_shader.cu
+    MAT_DIELECTRIC = 3,     // rough_transparent_shader.cu
+    MAT_INTERFACE = 4,      // lambertian_interface_shader.cu
+    MAT_TRANSLUCENT = 5,    // subsurface_scattering_shader.cu / rough_subsurface_scattering_shader.cu
+    MAT_COUNT
+};
+
+// Microfacet models and distributions (ORF/structs.h:242-266).
+enum MicrofacetModel : int { WALTER_MODEL = 0, VISIBLE_NORMALS_MODEL = 1, MULTISCATTERING_MODEL = 2 };
+enum NormalDistribution : int { BECKMANN_DISTRIBUTION = 0, GGX_DISTRIBUTION = 1 };
+
+struct MaterialGPU {
+    int kind;
+    int microfacet_model;
+    int normal_distribution;
+    int pad0;
+    f3 diffuse_color;   // Lambert albedo / interface albedo
+    f3 radiance;        // emitter radiance
+    f3 glass_absorption;
+    f2 roughness;       // (ax, ay)
+    float ior;          // dielectric ior (inside / outside)
+    float pad1;
+    ConductorEta eta;   // conductor: n1 / ior per channel
+};
+
+// Light types (ORF/structs.h:146-153) + the environment as a sampled light.
+enum LightType : unsigned { POINT_LIGHT = 0, DIRECTIONAL_LIGHT = 1, TRIANGLES_AREA_LIGHT = 2,
+                            DISK_LIGHT = 3, SPHERICAL_LIGHT = 4, ENV_LIGHT = 5 };
+
+// LightStruct (ORF/structs.h:42-48): type, radiance, 3 parameters; the
+// triangle-light form stores (count, start, end) as floats (:66-74).
+struct LightGPU {
+    unsigned type;
+    f3 radiance;
+    f3 params;
+    f3 pad;
+};
+
+// TriangleLight (ORF/structs.h:95-102).
+struct TriLightGPU {
+    f3 v0, v1, v2;
+    f3 n0, n1, n2;
+    unsigned has_normals;
+    f3 emission;
+    float area;
+};
+
+struct EnvGPU {
+    const float4* tex;   // w*h texels, row j <-> v = (j + 0.5) / h, theta = (1 - v) pi
+    int w, h;
+    const float* marg_pdf;  // h
+    const float* marg_cdf;  // h
+    const float* cond_pdf;  // h rows of w
+    const float* cond_cdf;  // h rows of w
+    int sampled;            // importance sampling tables valid
+};
+
+struct CameraGPU {
+    f3 eye, U, V, W;
+    unsigned patch_origin[2];
+    unsigned patch_dims[2];
+};
+
+struct SceneGPU {
+    const float4* bvh;        // 4 float4 per node
+    const float4* tri;        // 3 float4 per leaf-ordered triangle
+    const int4* shade;        // per original triangle: nidx.xyz, material id
+    const float4* normals;    // per normal
+    const MaterialGPU* materials;
+    const LightGPU* lights;
+    int nlights;              // including the environment entry when sampled
+    int root_is_leaf;         // tiny scenes: the root is a single leaf
+    const TriLightGPU* tri_lights;
+    EnvGPU env;
+    int has_env;
+    int max_depth;
+    float scene_epsilon;
+    int pad;
+};
+
+constexpr float kRayMax = 1e27f;  // RT_DEFAULT_MAX
+constexpr int kStackDepth = 48;   // builder caps BVH depth below this
+
+// BVH child link: internal node index, or leaf = 0x80000000 | (count-1) << 27 | first
+ORF_HD bool link_is_leaf(unsigned c) { return (c & 0x80000000u) != 0; }
+ORF_HD unsigned leaf_first(unsigned c) { return c & 0x07FFFFFFu; }
+ORF_HD unsigned leaf_count(unsigned c) { return ((c >> 27) & 0xFu) + 1u; }
+
+}  // namespace orf

@@ -1,0 +1,192 @@
+// orf_shade.h -- environment, light sampling and per-material shading steps (device).
+//
+// Environment:  env_lookup follows ORF/Envmap.h:57-64 (lat-long u = (atan2(x,z)+pi)/2pi,
+//   v = 1 - acos(y)/pi) with an exact FP32 bilinear fetch, wrap = repeat on both axes
+//   (ORF/HDRLoader.cpp:231-286), texel centres at (i + 0.5) / w.
+//   env_sample follows sample_environment, ORF/Envmap.h:66-98 (two draws, marginal then
+//   conditional inverse CDF with linear remap, L = E sin(theta) 2 pi^2 / p_uv); the CDF
+//   search is a lower bound (first index with cdf >= xi).  env_pdf is the matching
+//   solid-angle density p_uv / (2 pi^2 sin(theta)).
+// Lights: ORF/LightSampler.h:20-37 (point, directional), :40-71 (triangle area light).
+// Lambert step: CU/diffuse_shader.cu:122-187; conductor step: CU/metallic_shader.cu:84-158.
+// The environment as a light (NEE + power-heuristic MIS against cosine sampling) is
+// new: the reference's env importance sampling (CU/env_cameras.cu, CU/envmap_shader.cu)
+// is never compiled and its miss program ignores emit_light.
+#pragma once
+
+#include "orf_fresnel.h"
+#include "orf_microfacet.h"
+#include "orf_rng.h"
+#include "orf_scene.h"
+
+namespace orf {
+
+ORF_HD f3 tex_bilinear(const float4* tex, int w, int h, float u, float v)
+{
+    float x = u * (float)w - 0.5f, y = v * (float)h - 0.5f;
+    float fx0 = floorf(x), fy0 = floorf(y);
+    float fx = x - fx0, fy = y - fy0;
+    int ix = (int)fx0, iy = (int)fy0;
+    ix %= w;
+    if (ix < 0) ix += w;
+    iy %= h;
+    if (iy < 0) iy += h;
+    int ix1 = ix + 1 == w ? 0 : ix + 1;
+    int iy1 = iy + 1 == h ? 0 : iy + 1;
+    float4 t00 = tex[(size_t)iy * w + ix], t10 = tex[(size_t)iy * w + ix1];
+    float4 t01 = tex[(size_t)iy1 * w + ix], t11 = tex[(size_t)iy1 * w + ix1];
+    float gx = 1.0f - fx, gy = 1.0f - fy;
+    f3 r0 = mk3(t00.x * gx + t10.x * fx, t00.y * gx + t10.y * fx, t00.z * gx + t10.z * fx);
+    f3 r1 = mk3(t01.x * gx + t11.x * fx, t01.y * gx + t11.y * fx, t01.z * gx + t11.z * fx);
+    return mk3(r0.x * gy + r1.x * fy, r0.y * gy + r1.y * fy, r0.z * gy + r1.z * fy);
+}
+
+ORF_HD f3 env_lookup(const EnvGPU& env, f3 dir)
+{
+    float theta = acosf(dir.y);
+    float phi = atan2f(dir.x, dir.z);
+    float u = (phi + kPi) * 0.5f * kInvPi;
+    float v = (float)(1.0 - (double)(theta * kInvPi));
+    return tex_bilinear(env.tex, env.w, env.h, u, v);
+}
+
+// first index i in [0, n) with cdf[i] >= xi (n - 1 if none)
+ORF_HD int cdf_lower_bound(const float* cdf, int n, float xi)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cdf[mid] >= xi) hi = mid;
+        else lo = mid + 1;
+    }
+    return lo;
+}
+
+// Solid-angle density of env_sample for direction dir.
+ORF_HD float env_pdf(const EnvGPU& env, f3 dir)
+{
+    float theta = acosf(dir.y);
+    float phi = atan2f(dir.x, dir.z);
+    float u = (phi + kPi) * 0.5f * kInvPi;
+    float v = 1.0f - theta * kInvPi;
+    int iu = min(env.w - 1, max(0, (int)(u * (float)env.w)));
+    int iv = min(env.h - 1, max(0, (int)(v * (float)env.h)));
+    float p = env.marg_pdf[iv] * env.cond_pdf[(size_t)iv * env.w + iu];
+    float st = sinf(theta);
+    return st > 0.0f ? p / (2.0f * kPi * kPi * st) : 0.0f;
+}
+
+// Sample a direction from the environment; L = radiance / pdf (solid angle).
+ORF_HD void env_sample(const EnvGPU& env, uint32_t& rng, f3& dir, f3& L, float& pdf_sa)
+{
+    const float two_pi2 = 2.0f * kPi * kPi;
+    float xi1 = rnd_tea(rng), xi2 = rnd_tea(rng);
+    int vi = cdf_lower_bound(env.marg_cdf, env.h, xi1);
+    float c0 = vi > 0 ? env.marg_cdf[vi - 1] : 0.0f;
+    float dv = vi > 0 ? (xi1 - c0) / (env.marg_cdf[vi] - c0) : xi1 / env.marg_cdf[vi];
+    float pdf_m = env.marg_pdf[vi];
+    float v = ((float)vi + dv) / (float)env.h;
+    const float* ccdf = env.cond_cdf + (size_t)vi * env.w;
+    int ui = cdf_lower_bound(ccdf, env.w, xi2);
+    float c1 = ui > 0 ? ccdf[ui - 1] : 0.0f;
+    float du = ui > 0 ? (xi2 - c1) / (ccdf[ui] - c1) : xi2 / ccdf[ui];
+    float pdf_c = env.cond_pdf[(size_t)vi * env.w + ui];
+    float u = ((float)ui + du) / (float)env.w;
+    float prob = pdf_m * pdf_c;
+    float theta = (1.0f - v) * kPi;
+    float phi = (2.0f * u - 1.0f) * kPi;
+    float st, ct, sp, cp;
+    sincosf(theta, &st, &ct);
+    sincosf(phi, &sp, &cp);
+    dir = mk3(st * sp, ct, st * cp);
+    f3 E = tex_bilinear(env.tex, env.w, env.h, u, v);
+    L = E * (st * two_pi2 / prob);
+    pdf_sa = prob / (two_pi2 * st);
+}
+
+// evaluate_direct_illumination (ORF/LightSampler.h:143-166) for one picked light.
+// Returns false for light types without a sampler here.
+ORF_HD bool sample_light(const SceneGPU& sc, const LightGPU& lt, f3 pos, uint32_t& rng, f3& dir, f3& L,
+                         float& dist, float& pdf_env)
+{
+    pdf_env = -1.0f;
+    if (lt.type == POINT_LIGHT) {
+        dir = lt.params - pos;
+        float r2 = dot(dir, dir);
+        dist = sqrtf(r2);
+        dir = dir / dist;
+        L = lt.radiance / r2;
+        return true;
+    }
+    if (lt.type == DIRECTIONAL_LIGHT) {
+        dir = normalize(-lt.params);
+        dist = kRayMax;
+        L = lt.radiance;
+        return true;
+    }
+    if (lt.type == TRIANGLES_AREA_LIGHT) {
+        unsigned ntri = (unsigned)lt.params.x;
+        unsigned pick = (unsigned)(rnd_tea(rng) * (float)ntri);
+        if (pick >= ntri) pick = ntri - 1;  // rnd_tea can round to 1.0f
+        const TriLightGPU& tl = sc.tri_lights[pick + (unsigned)lt.params.y];
+        f3 uvw = sample_barycentric(rng);
+        f3 lp = tl.v0 * uvw.x + tl.v1 * uvw.y + tl.v2 * uvw.z;
+        f3 n = tl.has_normals ? normalize(tl.n0 * uvw.x + tl.n1 * uvw.y + tl.n2 * uvw.z)
+                              : normalize(cross(tl.v1 - tl.v0, tl.v2 - tl.v0));
+        dir = lp - pos;
+        float d2 = dot(dir, dir);
+        dist = sqrtf(d2);
+        dir = normalize(dir);
+        float cp = fmaxf(dot(n, -dir), 0.0f);
+        L = lt.radiance * (tl.area * cp / d2 * (float)ntri);
+        return true;
+    }
+    if (lt.type == ENV_LIGHT) {
+        env_sample(sc.env, rng, dir, L, pdf_env);
+        dist = kRayMax;
+        return true;
+    }
+    return false;
+}
+
+// One visible-normal sampling step of a rough conductor with macro-normal n
+// (CU/metallic_shader.cu:84-158, VISIBLE_NORMALS_MODEL, GGX).  Returns the
+// reflected direction, its solid-angle pdf and f cos / pdf (= F G1(w_o));
+// the weight is 0 where the reference stops the path (G1 <= 0).
+ORF_HD void ggx_conductor_sample(f3 w_i, f3 n, float ax, float ay, float z1, float z2, const ConductorEta& eta,
+                                 f3& w_o, float& pdf, f3& weight)
+{
+    f3 m = to_world(ggx_sample_visible(to_local(w_i, n), ax, ay, z1, z2), n);
+    float g_i = ggx_masking(w_i, m, n, ax, ay);
+    f3 F = fresnel_conductor(w_i, m, eta);
+    w_o = -w_i + (m * 2.0f) * dot(w_i, m);
+    float g_o = ggx_masking(w_o, m, n, ax, ay);
+    float D = ggx_vndf(to_local(w_i, n), to_local(m, n), ax, ay);
+    pdf = D / (4.0f * fabsf(dot(w_o, m)));
+    weight = (g_i <= 0.0f || g_o <= 0.0f) ? mk3(0.0f) : F * g_o;
+}
+
+// Power-heuristic weight of strategy a against b.
+ORF_HD float mis_power(float pa, float pb)
+{
+    float a2 = pa * pa, b2 = pb * pb;
+    return a2 + b2 > 0.0f ? a2 / (a2 + b2) : 0.0f;
+}
+
+// Shading frame of a hit: interpolated vertex normal (CU/triangle_mesh.cu:334),
+// normalised again as rtTransformNormal + normalize do in the closest-hit programs.
+ORF_HD f3 shading_normal(const SceneGPU& sc, int prim, float b, float g, f3 e1, f3 e2, int4& sh)
+{
+    sh = sc.shade[prim];
+    f3 n;
+    if (sh.x < 0 || sh.y < 0 || sh.z < 0) {
+        n = normalize(cross(e1, e2));
+    } else {
+        float4 a0 = sc.normals[sh.x], a1 = sc.normals[sh.y], a2 = sc.normals[sh.z];
+        f3 n0 = mk3(a0.x, a0.y, a0.z), n1 = mk3(a1.x, a1.y, a1.z), n2 = mk3(a2.x, a2.y, a2.z);
+        n = normalize(n1 * b + n2 * g + n0 * (1.0f - b - g));
+    }
+    return normalize(n);
+}
+
+}  // namespace orf

@@ -1,0 +1,144 @@
+// orf_trace.h -- ray/triangle intersection and BVH2 traversal (device).
+//
+// Replaces OptiX traversal + intersect_triangle_earlyexit as used by
+// CU/triangle_mesh.cu:46-86 (closest hit for radiance rays, any hit for shadow
+// rays whose any-hit programs all terminate: CU/diffuse_shader.cu:33-38,
+// CU/metallic_shader.cu:38-43, CU/arealight_shader.cu:27-32).
+//
+// Triangle test: Moller-Trumbore (1997) on (p0, e1 = p1-p0, e2 = p2-p0), plain
+// FP32 (no contraction), so (t, beta, gamma) round exactly as in the CPU
+// restatement; beta weights p1, gamma weights p2, the geometric normal is
+// e1 x e2.  A hit is accepted when tmin < t < t_best (strict).
+// Box test: slabs with precomputed 1/d and o/d (explicit fma), far distance
+// inflated by 1 + 2^-21 so rounding never culls a box containing a hit.
+// Traversal stack: per-lane column of a workgroup's LDS array (stride = block
+// size), depth kStackDepth (the builder bounds the tree depth below it).
+#pragma once
+
+#include "orf_scene.h"
+
+namespace orf {
+
+struct Hit {
+    float t;
+    float b, g;   // barycentric weights of p1 and p2
+    int prim;     // original triangle id, -1 = miss
+    int leaf;     // leaf-order triangle index (geometry record)
+};
+
+ORF_HD bool intersect_tri(f3 o, f3 d, f3 p0, f3 e1, f3 e2, float tmin, float tmax, float& t, float& b, float& g)
+{
+    f3 pv = cross(d, e2);
+    float det = dot(e1, pv);
+    if (det == 0.0f) return false;
+    float inv = 1.0f / det;
+    f3 tv = o - p0;
+    float u = dot(tv, pv) * inv;
+    if (u < 0.0f || u > 1.0f) return false;
+    f3 qv = cross(tv, e1);
+    float v = dot(d, qv) * inv;
+    if (v < 0.0f || u + v > 1.0f) return false;
+    float tt = dot(e2, qv) * inv;
+    if (!(tt > tmin && tt < tmax)) return false;
+    t = tt;
+    b = u;
+    g = v;
+    return true;
+}
+
+struct RayPre {
+    f3 o, d, inv, oinv;
+};
+
+__device__ __forceinline__ RayPre ray_pre(f3 o, f3 d)
+{
+    RayPre r;
+    r.o = o;
+    r.d = d;
+    r.inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    r.oinv = mk3(o.x * r.inv.x, o.y * r.inv.y, o.z * r.inv.z);
+    return r;
+}
+
+// Entry distance of the slab test, or +inf when the box is missed.
+__device__ __forceinline__ float box_enter(const RayPre& r, float lx, float ly, float lz, float hx, float hy,
+                                           float hz, float tmin, float tmax)
+{
+    float t0x = fmaf(lx, r.inv.x, -r.oinv.x), t1x = fmaf(hx, r.inv.x, -r.oinv.x);
+    float t0y = fmaf(ly, r.inv.y, -r.oinv.y), t1y = fmaf(hy, r.inv.y, -r.oinv.y);
+    float t0z = fmaf(lz, r.inv.z, -r.oinv.z), t1z = fmaf(hz, r.inv.z, -r.oinv.z);
+    float tn = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), tmin));
+    float tf = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fminf(fmaxf(t0z, t1z), tmax));
+    tf *= 1.00000048f;
+    return tn <= tf ? tn : __builtin_huge_valf();
+}
+
+// Closest hit (any_hit == false) or first hit (any_hit == true) in (tmin, tmax).
+template <bool ANY>
+__device__ __forceinline__ bool trace(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, Hit& h,
+                                      unsigned* stk, int stride)
+{
+    RayPre r = ray_pre(o, d);
+    float tbest = tmax;
+    int prim = -1, leaf = -1;
+    float hb = 0.0f, hg = 0.0f;
+    unsigned node = 0;
+    int sp = 0;
+    bool have = true;  // `node` holds a live internal node or leaf link
+    while (have) {
+        if (!link_is_leaf(node)) {
+            const float4* nd = sc.bvh + 4 * (size_t)node;
+            float4 a = nd[0], b = nd[1], c = nd[2], l = nd[3];
+            float e0 = box_enter(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, tbest);
+            float e1 = box_enter(r, b.z, b.w, c.x, c.y, c.z, c.w, tmin, tbest);
+            unsigned l0 = __float_as_uint(l.x), l1 = __float_as_uint(l.y);
+            bool h0 = e0 != __builtin_huge_valf(), h1 = e1 != __builtin_huge_valf();
+            if (h0 && h1) {
+                unsigned nearc = e0 <= e1 ? l0 : l1;
+                unsigned farc = e0 <= e1 ? l1 : l0;
+                if (sp < kStackDepth) stk[stride * sp++] = farc;
+                node = nearc;
+            } else if (h0) {
+                node = l0;
+            } else if (h1) {
+                node = l1;
+            } else {
+                if (sp == 0) break;
+                node = stk[stride * --sp];
+            }
+            continue;
+        }
+        unsigned first = leaf_first(node), cnt = leaf_count(node);
+        for (unsigned k = 0; k < cnt; ++k) {
+            const float4* tp = sc.tri + 3 * (size_t)(first + k);
+            float4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
+            float t, bb, gg;
+            if (intersect_tri(o, d, mk3(q0.x, q0.y, q0.z), mk3(q1.x, q1.y, q1.z), mk3(q2.x, q2.y, q2.z), tmin,
+                              tbest, t, bb, gg)) {
+                tbest = t;
+                hb = bb;
+                hg = gg;
+                prim = (int)__float_as_uint(q0.w);
+                leaf = (int)(first + k);
+                if (ANY) {
+                    h.t = t;
+                    h.b = bb;
+                    h.g = gg;
+                    h.prim = prim;
+                    h.leaf = leaf;
+                    return true;
+                }
+            }
+        }
+        if (sp == 0) break;
+        node = stk[stride * --sp];
+    }
+    h.t = tbest;
+    h.b = hb;
+    h.g = hg;
+    h.prim = prim;
+    h.leaf = leaf;
+    return prim >= 0;
+}
+
+}  // namespace orf

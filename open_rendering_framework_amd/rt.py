@@ -42,6 +42,31 @@ lib.orf_version.restype = ctypes.c_char_p
 _SIGS = {
     "orf_bsdf_ggx_conductor_batch": [c_i64] + [c_vp] * 6 + [c_dblp, c_dblp] + [c_vp] * 7 + [c_vp],
     "orf_gen_bsdf_queries": [c_i64, c_u64, c_u64] + [c_vp] * 6,
+    "orf_context_create": [ctypes.c_int, ctypes.POINTER(c_vp)],
+    "orf_context_destroy": [c_vp],
+    "orf_variable_set_float": [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, c_vp, ctypes.c_int],
+    "orf_variable_set_int": [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, c_vp, ctypes.c_int],
+    "orf_variable_set_uint": [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, c_vp, ctypes.c_int],
+    "orf_variable_set_user_data": [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, c_vp, ctypes.c_size_t],
+    "orf_material_create": [c_vp, ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)],
+    "orf_mesh_create": [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_i64, c_vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int)],
+    "orf_mesh_set_transform": [c_vp, ctypes.c_int, c_vp],
+    "orf_light_add_point": [c_vp, c_vp, c_vp],
+    "orf_light_add_directional": [c_vp, c_vp, c_vp],
+    "orf_light_add_triangles": [c_vp, c_vp, c_vp, c_i64, c_vp],
+    "orf_lights_clear": [c_vp],
+    "orf_envmap_set": [c_vp, c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+    "orf_envmap_tables": [c_vp, c_vp, c_vp, c_vp, c_vp],
+    "orf_accel_mark_dirty": [c_vp],
+    "orf_accel_build": [c_vp, c_vp],
+    "orf_launch": [c_vp, ctypes.c_int, ctypes.c_int, c_u32, c_u32, c_vp],
+    "orf_film_clear": [c_vp],
+    "orf_film_read": [c_vp, c_vp, c_vp],
+    "orf_film_read_sums": [c_vp, c_vp],
+    "orf_counters": [c_vp, c_vp],
+    "orf_gen_icosphere": [ctypes.c_int, c_u32, ctypes.c_float, ctypes.c_float, c_vp, c_vp,
+                          ctypes.POINTER(c_i64), c_vp, ctypes.POINTER(c_i64)],
+    "orf_gen_sky_envmap": [ctypes.c_int, ctypes.c_int, c_vp, ctypes.c_float, ctypes.c_float, c_vp],
 }
 for _name, _args in _SIGS.items():
     _f = getattr(lib, _name)
@@ -108,3 +133,167 @@ def bsdf_ggx_conductor_batch(wo_x, wo_y, wo_z, alpha, u1, u2, ior_re, ior_im, ou
         stream_ptr(stream),
     )
     return out
+
+
+def _np_ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def gen_icosphere(level, seed, amplitude, frequency):
+    """(verts (n,3) f32, normals (n,3) f32, tris (m,3) i32) of the displaced icosphere."""
+    import numpy as np
+
+    nv, nt = c_i64(0), c_i64(0)
+    call("orf_gen_icosphere", level, seed, amplitude, frequency, None, None, ctypes.byref(nv), None, ctypes.byref(nt))
+    v = np.zeros((nv.value, 3), np.float32)
+    n = np.zeros((nv.value, 3), np.float32)
+    t = np.zeros((nt.value, 3), np.int32)
+    call("orf_gen_icosphere", level, seed, amplitude, frequency, _np_ptr(v), _np_ptr(n), ctypes.byref(nv),
+         _np_ptr(t), ctypes.byref(nt))
+    return v, n, t
+
+
+def gen_sky_envmap(width, height, sun_dir, sigma_deg, peak):
+    import numpy as np
+
+    out = np.zeros((height, width, 4), np.float32)
+    sd = (ctypes.c_float * 3)(*sun_dir)
+    call("orf_gen_sky_envmap", width, height, sd, sigma_deg, peak, _np_ptr(out))
+    return out
+
+
+SCOPE_CONTEXT, SCOPE_MATERIAL = 0, 1
+
+
+class Context:
+    """A render context (C-ABI orf_context) on one device."""
+
+    def __init__(self, device=0):
+        h = c_vp()
+        call("orf_context_create", device, ctypes.byref(h))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            call("orf_context_destroy", self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- variables
+    def _scope(self, material):
+        return (SCOPE_CONTEXT, 0) if material is None else (SCOPE_MATERIAL, material)
+
+    def set_float(self, name, *v, material=None):
+        sc, h = self._scope(material)
+        arr = (ctypes.c_float * len(v))(*v)
+        call("orf_variable_set_float", self.handle, sc, h, name.encode(), arr, len(v))
+
+    def set_int(self, name, *v, material=None):
+        sc, h = self._scope(material)
+        arr = (ctypes.c_int32 * len(v))(*v)
+        call("orf_variable_set_int", self.handle, sc, h, name.encode(), arr, len(v))
+
+    def set_uint(self, name, *v, material=None):
+        sc, h = self._scope(material)
+        arr = (ctypes.c_uint32 * len(v))(*v)
+        call("orf_variable_set_uint", self.handle, sc, h, name.encode(), arr, len(v))
+
+    def set_conductor_ior(self, material, ior):
+        """ior: three (re, im) pairs, passed as MyComplex3 user data."""
+        flat = [float(x) for pair in ior for x in pair]
+        arr = (ctypes.c_double * 6)(*flat)
+        call("orf_variable_set_user_data", self.handle, SCOPE_MATERIAL, material, b"ior", arr, 48)
+
+    # ---- scene objects
+    def material(self, program):
+        h = ctypes.c_int()
+        call("orf_material_create", self.handle, program.encode(), ctypes.byref(h))
+        return h.value
+
+    def mesh(self, verts, tris, material, normals=None, nindex=None):
+        import numpy as np
+
+        v = np.ascontiguousarray(verts, np.float32)
+        t = np.ascontiguousarray(tris, np.int32)
+        n = None if normals is None else np.ascontiguousarray(normals, np.float32)
+        ni = None if nindex is None else np.ascontiguousarray(nindex, np.int32)
+        h = ctypes.c_int()
+        call("orf_mesh_create", self.handle, _np_ptr(v), len(v), _np_ptr(t), len(t), _np_ptr(n),
+             0 if n is None else len(n), _np_ptr(ni), material, ctypes.byref(h))
+        return h.value
+
+    def set_transform(self, mesh, m4):
+        import numpy as np
+
+        m = np.ascontiguousarray(m4, np.float32).reshape(16)
+        call("orf_mesh_set_transform", self.handle, mesh, _np_ptr(m))
+
+    def point_light(self, position, radiance):
+        call("orf_light_add_point", self.handle, (ctypes.c_float * 3)(*position), (ctypes.c_float * 3)(*radiance))
+
+    def directional_light(self, direction, radiance):
+        call("orf_light_add_directional", self.handle, (ctypes.c_float * 3)(*direction),
+             (ctypes.c_float * 3)(*radiance))
+
+    def triangle_light(self, tri_verts, tri_normals, radiance):
+        import numpy as np
+
+        tv = np.ascontiguousarray(tri_verts, np.float32).reshape(-1, 9)
+        tn = None if tri_normals is None else np.ascontiguousarray(tri_normals, np.float32).reshape(-1, 9)
+        call("orf_light_add_triangles", self.handle, _np_ptr(tv), _np_ptr(tn), len(tv),
+             (ctypes.c_float * 3)(*radiance))
+
+    def envmap(self, rgba, importance_sample=True):
+        import numpy as np
+
+        e = np.ascontiguousarray(rgba, np.float32)
+        h, w = e.shape[:2]
+        call("orf_envmap_set", self.handle, _np_ptr(e), w, h, 1 if importance_sample else 0)
+
+    def env_tables(self, w, h):
+        import numpy as np
+
+        out = [np.zeros(h, np.float32), np.zeros(h, np.float32), np.zeros(h * w, np.float32),
+               np.zeros(h * w, np.float32)]
+        call("orf_envmap_tables", self.handle, *[_np_ptr(a) for a in out])
+        return out
+
+    def accel_build(self):
+        import numpy as np
+
+        st = np.zeros(6, np.int64)
+        call("orf_accel_build", self.handle, _np_ptr(st))
+        return dict(zip(("nodes", "leaves", "max_depth", "max_leaf", "tris", "degenerate"), st.tolist()))
+
+    # ---- launches and film
+    def launch(self, width, height, frame0=0, spp=1, stream=None):
+        call("orf_launch", self.handle, width, height, frame0, spp, stream_ptr(stream))
+
+    def film_clear(self):
+        call("orf_film_clear", self.handle)
+
+    def film(self, width, height):
+        import numpy as np
+
+        out = np.zeros((height, width, 4), np.float64)
+        call("orf_film_read", self.handle, _np_ptr(out), None)
+        return out
+
+    def film_sums(self, width, height):
+        import numpy as np
+
+        out = np.zeros((height, width, 4), np.float64)
+        call("orf_film_read_sums", self.handle, _np_ptr(out))
+        return out
+
+    def counters(self):
+        import numpy as np
+
+        out = np.zeros(4, np.uint64)
+        call("orf_counters", self.handle, _np_ptr(out))
+        return dict(zip(("nonfinite", "ext_rays", "shadow_rays", "paths"), [int(x) for x in out]))

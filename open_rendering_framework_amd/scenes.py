@@ -1,0 +1,188 @@
+"""Scene descriptions for the render workloads (host side, numpy).
+
+A SceneDesc is the flattened form of what the reference's JSON loader builds
+(ORF/OptixSceneLoader.cpp:105-177): meshes with one material each, the light
+list, the environment map, the pinhole camera and the integrator settings.
+`to_context` pushes it through the C-ABI into a render context.
+
+The benchmark scene ("plain path tracer", 512x512, 64 spp, max depth 8):
+  ground   : 2-triangle quad, x,z in [-10, 10] at y = -1, Lambert albedo 0.5
+  sphere   : icosphere level 7 (327,680 triangles), radius 1 at the origin,
+             displaced radially by 3D value noise (lattice seed 7, amplitude
+             0.05, frequency 4); GGX rough conductor, alpha 0.3, ior 0.27+2.78i
+  light    : 1x1 quad at y = 3 facing -y, radiance 10 (emitter + triangle light)
+  env      : procedural 1024x512 FP32 sky gradient + Gaussian sun lobe
+             (sigma 0.5 deg, peak 5e4, direction SUN_DIR), importance sampled
+  camera   : eye (0, 1.5, 4) looking at the origin, up +y, vfov 40
+"""
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# sun at 35 deg elevation, azimuth 150 deg in the env convention
+# dir = (sin th sin ph, cos th, sin th cos ph), th = 55 deg, ph = 150 deg
+_TH, _PH = math.radians(55.0), math.radians(150.0)
+SUN_DIR = (math.sin(_TH) * math.sin(_PH), math.cos(_TH), math.sin(_TH) * math.cos(_PH))
+
+
+@dataclass
+class Material:
+    program: str                      # reference closest-hit program file, e.g. "metallic_shader.cu"
+    floats: dict = field(default_factory=dict)   # name -> tuple of floats
+    uints: dict = field(default_factory=dict)    # name -> tuple of ints
+    ior_complex: tuple = None         # conductor ior: ((re, im), (re, im), (re, im))
+
+
+@dataclass
+class Mesh:
+    verts: np.ndarray                 # (n, 3) float32
+    tris: np.ndarray                  # (m, 3) int32
+    material: int
+    normals: np.ndarray = None        # (k, 3) float32
+    nindex: np.ndarray = None         # (m, 3) int32 (None: = tris when normals are per vertex)
+
+
+@dataclass
+class Light:
+    kind: str                         # "point" | "directional" | "triangles"
+    radiance: tuple
+    position: tuple = None
+    direction: tuple = None
+    tri_verts: np.ndarray = None      # (t, 9) float32
+    tri_normals: np.ndarray = None    # (t, 9) float32
+
+
+@dataclass
+class SceneDesc:
+    meshes: list
+    materials: list
+    lights: list
+    env: np.ndarray = None            # (h, w, 4) float32
+    env_importance: bool = True
+    eye: tuple = (0.0, 0.0, 0.0)
+    U: tuple = (1.0, 0.0, 0.0)
+    V: tuple = (0.0, 1.0, 0.0)
+    W: tuple = (0.0, 0.0, -1.0)
+    width: int = 512
+    height: int = 512
+    max_depth: int = 8
+    scene_epsilon: float = 1e-4
+
+    def flat(self):
+        """World-space triangle soup: verts, tris, normals, nindex, tri_mat."""
+        V, T, N, NI, M = [], [], [], [], []
+        vb = nb = 0
+        for m in self.meshes:
+            V.append(m.verts.astype(np.float32))
+            T.append(m.tris.astype(np.int32) + vb)
+            M.append(np.full(len(m.tris), m.material, np.int32))
+            if m.normals is not None:
+                N.append(m.normals.astype(np.float32))
+                ni = m.nindex if m.nindex is not None else m.tris
+                NI.append(ni.astype(np.int32) + nb)
+                nb += len(m.normals)
+            else:
+                NI.append(np.full((len(m.tris), 3), -1, np.int32))
+            vb += len(m.verts)
+        N = np.concatenate(N) if N else np.zeros((0, 3), np.float32)
+        return (np.concatenate(V), np.concatenate(T), N, np.concatenate(NI), np.concatenate(M))
+
+
+def pinhole(eye, lookat, up, vfov, width, height):
+    """U, V, W of PinholeCamera::calculateCameraVariables (ORF/Camera.cpp:83-103)."""
+    eye = np.asarray(eye, np.float64)
+    W = np.asarray(lookat, np.float64) - eye
+    wlen = np.linalg.norm(W)
+    U = np.cross(W, up)
+    U /= np.linalg.norm(U)
+    V = np.cross(U, W)
+    V /= np.linalg.norm(V)
+    vlen = wlen * math.tan(0.5 * vfov * math.pi / 180.0)
+    V *= vlen
+    U *= vlen * (width / height)
+    f = lambda x: tuple(float(np.float32(c)) for c in x)  # noqa: E731
+    return f(eye), f(U), f(V), f(W)
+
+
+def quad(y, half, facing_down):
+    """Axis-aligned square at height y; returns verts, tris with the given facing."""
+    v = np.array([[-half, y, -half], [half, y, -half], [half, y, half], [-half, y, half]], np.float32)
+    if facing_down:
+        t = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    else:
+        t = np.array([[0, 2, 1], [0, 3, 2]], np.int32)
+    return v, t
+
+
+def icosphere(level=7, seed=7, amplitude=0.05, frequency=4.0):
+    from . import rt
+
+    return rt.gen_icosphere(level, seed, amplitude, frequency)
+
+
+def sky_envmap(width=1024, height=512, sun_dir=SUN_DIR, sigma_deg=0.5, peak=5e4):
+    from . import rt
+
+    return rt.gen_sky_envmap(width, height, sun_dir, sigma_deg, peak)
+
+
+def path_scene(width=512, height=512, level=7, ground_mat="lambert", sphere_mat="conductor", env=True,
+               env_size=(1024, 512), max_depth=8, alpha=0.3, env_importance=True):
+    """The plain-path-tracer benchmark scene (module docstring); smaller `level`,
+    image and env sizes give the same scene family at test scale."""
+    mats = [
+        Material("diffuse_shader.cu", floats={"diffuse_color": (0.5, 0.5, 0.5)}),
+        Material("metallic_shader.cu", floats={"roughness": (alpha, alpha)},
+                 uints={"microfacet_model": (1,), "normal_distribution": (1,)},
+                 ior_complex=((0.27, 2.78),) * 3),
+        Material("arealight_shader.cu", floats={"radiance": (10.0, 10.0, 10.0)}),
+    ]
+    prog = {"lambert": 0, "conductor": 1}
+    gv, gt = quad(-1.0, 10.0, facing_down=False)
+    ground = Mesh(gv, gt, prog[ground_mat], normals=np.tile([[0, 1, 0]], (4, 1)).astype(np.float32))
+    sv, sn, st = icosphere(level)
+    sphere = Mesh(sv, st, prog[sphere_mat], normals=sn)
+    lv, lt = quad(3.0, 0.5, facing_down=True)
+    ln = np.tile([[0, -1, 0]], (4, 1)).astype(np.float32)
+    light_mesh = Mesh(lv, lt, 2, normals=ln)
+    tri_v = lv[lt].reshape(-1, 9)
+    tri_n = ln[lt].reshape(-1, 9)
+    lights = [Light("triangles", (10.0, 10.0, 10.0), tri_verts=tri_v, tri_normals=tri_n)]
+    eye, U, V, W = pinhole((0.0, 1.5, 4.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 40.0, width, height)
+    envmap = sky_envmap(*env_size) if env else None
+    return SceneDesc([ground, sphere, light_mesh], mats, lights, envmap, env_importance, eye, U, V, W,
+                     width, height, max_depth, 1e-4)
+
+
+def to_context(desc, ctx=None):
+    """Create (or fill) a render context from a SceneDesc through the C-ABI."""
+    from . import rt
+
+    ctx = ctx or rt.Context()
+    ctx.set_int("max_depth", desc.max_depth)
+    ctx.set_float("scene_epsilon", desc.scene_epsilon)
+    for name in ("eye", "U", "V", "W"):
+        ctx.set_float(name, *getattr(desc, name))
+    handles = []
+    for m in desc.materials:
+        h = ctx.material(m.program)
+        for k, v in m.floats.items():
+            ctx.set_float(k, *v, material=h)
+        for k, v in m.uints.items():
+            ctx.set_uint(k, *v, material=h)
+        if m.ior_complex is not None:
+            ctx.set_conductor_ior(h, m.ior_complex)
+        handles.append(h)
+    for m in desc.meshes:
+        ctx.mesh(m.verts, m.tris, handles[m.material], m.normals, m.nindex)
+    for l in desc.lights:
+        if l.kind == "point":
+            ctx.point_light(l.position, l.radiance)
+        elif l.kind == "directional":
+            ctx.directional_light(l.direction, l.radiance)
+        else:
+            ctx.triangle_light(l.tri_verts, l.tri_normals, l.radiance)
+    if desc.env is not None:
+        ctx.envmap(desc.env, desc.env_importance)
+    return ctx

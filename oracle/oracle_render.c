@@ -1,2 +1,809 @@
-/* oracle_render.c -- CPU restatement of the per-pixel render path (TEST INFRASTRUCTURE). */
+/*
+ * oracle_render.c -- CPU restatement of the per-pixel render path (TEST INFRASTRUCTURE).
+ *
+ * Recursive like the reference programs (each closest hit traces its child and
+ * adds child * weight), so the summation order is the reference's; the product
+ * kernels iterate with a throughput instead.  The acceleration structure is the
+ * oracle's own object-median BVH with a double-precision, slightly inflated box
+ * test; the triangle test is Moller-Trumbore in R, the same arithmetic as the
+ * product's, so hits agree to the bit in the FP32 oracle.
+ *
+ * New parts with no reference counterpart (marked where they occur): the
+ * environment as a sampled light with power-heuristic MIS (the reference's env
+ * importance sampling, CU/env_cameras.cu + CU/envmap_shader.cu, is never
+ * compiled), clamping of light picks where rnd_tea rounds to 1.0f, and the
+ * film as sums of finite samples.
+ */
 #include "oracle_render.h"
+
+#include <stdlib.h>
+#include <string.h>
+#include <tgmath.h>
+
+#define PI_R ((R)3.14159265358979323846)
+#define INV_PI_R ((R)0.318309886183790671538)
+#define RAY_MAX ((R)1e27f)
+
+typedef struct {
+    double lo[3], hi[3];
+    int32_t left, right;  /* left < 0: leaf */
+    int32_t first, count;
+} bnode;
+
+typedef struct {
+    /* geometry (world space) */
+    int64_t ntris;
+    v3* p0;
+    v3* e1;
+    v3* e2;
+    int32_t* nidx;  /* 3 per tri or NULL */
+    v3* normals;
+    int32_t* tri_mat;
+    /* bvh */
+    bnode* nodes;
+    int32_t nnodes;
+    int32_t* order;
+    /* materials and lights */
+    int32_t nmats;
+    orc_material* mats;
+    cplx3* eta;
+    int32_t nlights;
+    orc_light* lights;
+    orc_trilight* trilights;
+    /* environment */
+    float* env;
+    int32_t env_w, env_h, env_is, has_env;
+    R* mpdf;
+    R* mcdf;
+    R* cpdf;
+    R* ccdf;
+    /* camera and integrator */
+    v3 eye, U, V, W;
+    int32_t max_depth;
+    R eps;
+} oscene;
+
+/* ---------------------------------------------------------------- BVH */
+
+typedef struct {
+    double c[3];
+    int32_t id;
+} cref;
+
+
+static void cref_swap(cref* a, cref* b)
+{
+    cref t = *a;
+    *a = *b;
+    *b = t;
+}
+
+/* quickselect: place the k-th smallest (by c[axis]) at position k */
+static void select_k(cref* a, int64_t n, int64_t k, int axis)
+{
+    int64_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+        double pivot = a[(lo + hi) / 2].c[axis];
+        int64_t i = lo, j = hi;
+        while (i <= j) {
+            while (a[i].c[axis] < pivot) i++;
+            while (a[j].c[axis] > pivot) j--;
+            if (i <= j) {
+                cref_swap(&a[i], &a[j]);
+                i++;
+                j--;
+            }
+        }
+        if (k <= j) hi = j;
+        else if (k >= i) lo = i;
+        else break;
+    }
+}
+
+static void tri_box(const oscene* s, int32_t t, double lo[3], double hi[3])
+{
+    v3 p[3] = {s->p0[t], vadd(s->p0[t], s->e1[t]), vadd(s->p0[t], s->e2[t])};
+    for (int a = 0; a < 3; a++) {
+        lo[a] = 1e300;
+        hi[a] = -1e300;
+    }
+    for (int k = 0; k < 3; k++) {
+        double q[3] = {p[k].x, p[k].y, p[k].z};
+        for (int a = 0; a < 3; a++) {
+            if (q[a] < lo[a]) lo[a] = q[a];
+            if (q[a] > hi[a]) hi[a] = q[a];
+        }
+    }
+}
+
+static int32_t build_rec(oscene* s, cref* refs, int64_t first, int64_t n, int32_t* cap)
+{
+    if (s->nnodes >= *cap) {
+        *cap *= 2;
+        s->nodes = (bnode*)realloc(s->nodes, sizeof(bnode) * (size_t)*cap);
+    }
+    int32_t me = s->nnodes++;
+    bnode nd;
+    for (int a = 0; a < 3; a++) {
+        nd.lo[a] = 1e300;
+        nd.hi[a] = -1e300;
+    }
+    double clo[3] = {1e300, 1e300, 1e300}, chi[3] = {-1e300, -1e300, -1e300};
+    for (int64_t i = first; i < first + n; i++) {
+        double lo[3], hi[3];
+        tri_box(s, refs[i].id, lo, hi);
+        for (int a = 0; a < 3; a++) {
+            if (lo[a] < nd.lo[a]) nd.lo[a] = lo[a];
+            if (hi[a] > nd.hi[a]) nd.hi[a] = hi[a];
+            if (refs[i].c[a] < clo[a]) clo[a] = refs[i].c[a];
+            if (refs[i].c[a] > chi[a]) chi[a] = refs[i].c[a];
+        }
+    }
+    /* inflate so a float hit on a box face is never culled */
+    for (int a = 0; a < 3; a++) {
+        double pad = 1e-6 * (nd.hi[a] - nd.lo[a]) + 1e-6 * (fabs(nd.lo[a]) + fabs(nd.hi[a])) + 1e-30;
+        nd.lo[a] -= pad;
+        nd.hi[a] += pad;
+    }
+    if (n <= 4) {
+        nd.left = nd.right = -1;
+        nd.first = (int32_t)first;
+        nd.count = (int32_t)n;
+        s->nodes[me] = nd;
+        return me;
+    }
+    int axis = 0;
+    for (int a = 1; a < 3; a++)
+        if (chi[a] - clo[a] > chi[axis] - clo[axis]) axis = a;
+    int64_t half = n / 2;
+    select_k(refs + first, n, half, axis);
+    nd.first = nd.count = 0;
+    s->nodes[me] = nd;
+    int32_t l = build_rec(s, refs, first, half, cap);
+    int32_t r = build_rec(s, refs, first + half, n - half, cap);
+    s->nodes[me].left = l;
+    s->nodes[me].right = r;
+    return me;
+}
+
+static void build_bvh(oscene* s)
+{
+    int64_t n = 0;
+    cref* refs = (cref*)malloc(sizeof(cref) * (size_t)(s->ntris + 1));
+    for (int64_t t = 0; t < s->ntris; t++) {
+        /* mesh_bounds: triangles of zero or non-finite area get no box (CU/triangle_mesh.cu:88-105) */
+        R area = sqrt(vdot(vcross(s->e1[t], s->e2[t]), vcross(s->e1[t], s->e2[t])));
+        float af = (float)area;
+        if (!(af > 0.0f) || isinf(af)) continue;
+        double lo[3], hi[3];
+        tri_box(s, (int32_t)t, lo, hi);
+        for (int a = 0; a < 3; a++) refs[n].c[a] = 0.5 * (lo[a] + hi[a]);
+        refs[n].id = (int32_t)t;
+        n++;
+    }
+    int32_t cap = 1024;
+    s->nodes = (bnode*)malloc(sizeof(bnode) * (size_t)cap);
+    s->nnodes = 0;
+    if (n > 0) build_rec(s, refs, 0, n, &cap);
+    s->order = (int32_t*)malloc(sizeof(int32_t) * (size_t)(n + 1));
+    for (int64_t i = 0; i < n; i++) s->order[i] = refs[i].id;
+    free(refs);
+}
+
+static int box_hit(const bnode* b, const double o[3], const double inv[3], double tmin, double tmax, double* tn)
+{
+    double t0 = tmin, t1 = tmax;
+    for (int a = 0; a < 3; a++) {
+        double ta = (b->lo[a] - o[a]) * inv[a], tb = (b->hi[a] - o[a]) * inv[a];
+        if (ta != ta) ta = -1e300; /* 0 * inf on a box face: no constraint */
+        if (tb != tb) tb = 1e300;
+        if (ta > tb) {
+            double x = ta;
+            ta = tb;
+            tb = x;
+        }
+        if (ta > t0) t0 = ta;
+        if (tb < t1) t1 = tb;
+    }
+    *tn = t0;
+    return t0 <= t1;
+}
+
+/* Moller-Trumbore (1997), same operation order as the product's kernels. */
+static int isect(v3 o, v3 d, v3 p0, v3 e1, v3 e2, R tmin, R tmax, R* t, R* b, R* g)
+{
+    v3 pv = vcross(d, e2);
+    R det = vdot(e1, pv);
+    if (det == 0) return 0;
+    R inv = (R)1 / det;
+    v3 tv = vsub(o, p0);
+    R u = vdot(tv, pv) * inv;
+    if (u < 0 || u > 1) return 0;
+    v3 qv = vcross(tv, e1);
+    R v = vdot(d, qv) * inv;
+    if (v < 0 || u + v > 1) return 0;
+    R tt = vdot(e2, qv) * inv;
+    if (!(tt > tmin && tt < tmax)) return 0;
+    *t = tt;
+    *b = u;
+    *g = v;
+    return 1;
+}
+
+typedef struct {
+    R t, b, g;
+    int32_t prim;
+} ohit;
+
+static int trace(const oscene* s, v3 o, v3 d, R tmin, R tmax, int any, ohit* h)
+{
+    h->prim = -1;
+    h->t = tmax;
+    if (s->nnodes == 0) return 0;
+    double od[3] = {o.x, o.y, o.z};
+    double inv[3] = {1.0 / (double)d.x, 1.0 / (double)d.y, 1.0 / (double)d.z};
+    int32_t stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const bnode* b = &s->nodes[stack[--sp]];
+        double tn;
+        if (!box_hit(b, od, inv, (double)tmin, (double)h->t * (1.0 + 1e-6), &tn)) continue;
+        if (b->left < 0) {
+            for (int32_t k = 0; k < b->count; k++) {
+                int32_t id = s->order[b->first + k];
+                R t, bb, gg;
+                if (isect(o, d, s->p0[id], s->e1[id], s->e2[id], tmin, h->t, &t, &bb, &gg)) {
+                    h->t = t;
+                    h->b = bb;
+                    h->g = gg;
+                    h->prim = id;
+                    if (any) return 1;
+                }
+            }
+            continue;
+        }
+        stack[sp++] = b->right;
+        stack[sp++] = b->left;
+    }
+    return h->prim >= 0;
+}
+
+static int trace_brute(const oscene* s, v3 o, v3 d, R tmin, R tmax, ohit* h)
+{
+    h->prim = -1;
+    h->t = tmax;
+    for (int64_t id = 0; id < s->ntris; id++) {
+        R t, bb, gg;
+        R area = sqrt(vdot(vcross(s->e1[id], s->e2[id]), vcross(s->e1[id], s->e2[id])));
+        if (!((float)area > 0.0f) || isinf((float)area)) continue;
+        if (isect(o, d, s->p0[id], s->e1[id], s->e2[id], tmin, h->t, &t, &bb, &gg)) {
+            h->t = t;
+            h->b = bb;
+            h->g = gg;
+            h->prim = (int32_t)id;
+        }
+    }
+    return h->prim >= 0;
+}
+
+/* ---------------------------------------------------------------- environment */
+
+/* tex2D, normalized coordinates, repeat wrap, linear filter (ORF/HDRLoader.cpp:231-286). */
+static v3 tex_bilinear(const oscene* s, R u, R v)
+{
+    int w = s->env_w, h = s->env_h;
+    R x = u * (R)w - (R)0.5f, y = v * (R)h - (R)0.5f;
+    R fx0 = floor(x), fy0 = floor(y);
+    R fx = x - fx0, fy = y - fy0;
+    int ix = (int)fx0 % w, iy = (int)fy0 % h;
+    if (ix < 0) ix += w;
+    if (iy < 0) iy += h;
+    int ix1 = ix + 1 == w ? 0 : ix + 1, iy1 = iy + 1 == h ? 0 : iy + 1;
+    const float* t00 = s->env + 4 * ((size_t)iy * w + ix);
+    const float* t10 = s->env + 4 * ((size_t)iy * w + ix1);
+    const float* t01 = s->env + 4 * ((size_t)iy1 * w + ix);
+    const float* t11 = s->env + 4 * ((size_t)iy1 * w + ix1);
+    R gx = (R)1 - fx, gy = (R)1 - fy;
+    R r[3];
+    for (int k = 0; k < 3; k++) {
+        R a = (R)t00[k] * gx + (R)t10[k] * fx;
+        R b = (R)t01[k] * gx + (R)t11[k] * fx;
+        r[k] = a * gy + b * fy;
+    }
+    return V3(r[0], r[1], r[2]);
+}
+
+/* env_lookup (ORF/Envmap.h:57-64). */
+static v3 env_lookup(const oscene* s, v3 dir)
+{
+    R theta = acos(dir.y);
+    R phi = atan2(dir.x, dir.z);
+    R u = (phi + PI_R) * (R)0.5f * INV_PI_R;
+    R v = (R)(1.0 - (double)(theta * INV_PI_R));
+    return tex_bilinear(s, u, v);
+}
+
+/* cdf_bsearch_marginal / cdf_bsearch_conditional (ORF/Envmap.h:23-55), with the
+ * out-of-range reads at index -1 and n guarded. */
+static int env_bsearch(const R* cdf, int n, R xi)
+{
+    unsigned ts = (unsigned)n;
+    unsigned middle = ts = ts >> 1;
+    while (ts > 0) {
+        unsigned odd = ts & 1u;
+        ts >>= 1;
+        unsigned tmp = ts + odd;
+        R left = middle > 0 ? cdf[middle - 1] : (R)-1;
+        if (xi > cdf[middle]) middle = middle + tmp;
+        else if (xi < left) middle = middle - tmp;
+        if (middle >= (unsigned)n) middle = (unsigned)n - 1;
+    }
+    return (int)middle;
+}
+
+int orc_env_bsearch(const double* cdf, int n, double xi)
+{
+    R* c = (R*)malloc(sizeof(R) * (size_t)n);
+    for (int i = 0; i < n; i++) c[i] = (R)cdf[i];
+    int r = env_bsearch(c, n, (R)xi);
+    free(c);
+    return r;
+}
+
+/* env_luminance_camera / env_marginal_camera / env_pdf_camera (CU/env_cameras.cu:18-75)
+ * evaluated in double; every row's last conditional cdf entry and the last
+ * marginal entry are set to 1, zero rows get a uniform conditional. */
+static void build_env_tables(oscene* s)
+{
+    int w = s->env_w, h = s->env_h;
+    double* rowmean = (double*)malloc(sizeof(double) * (size_t)h);
+    s->mpdf = (R*)malloc(sizeof(R) * (size_t)h);
+    s->mcdf = (R*)malloc(sizeof(R) * (size_t)h);
+    s->cpdf = (R*)malloc(sizeof(R) * (size_t)w * h);
+    s->ccdf = (R*)malloc(sizeof(R) * (size_t)w * h);
+    double* lum = (double*)malloc(sizeof(double) * (size_t)w);
+    for (int j = 0; j < h; j++) {
+        double theta = (1.0 - ((double)j + 0.5) / (double)h) * 3.14159265358979323846;
+        double st = sin(theta), sum = 0.0;
+        for (int i = 0; i < w; i++) {
+            const float* t = s->env + 4 * ((size_t)j * w + i);
+            lum[i] = ((double)t[0] * 0.2989 + (double)t[1] * 0.5866 + (double)t[2] * 0.1145) * st;
+            sum += lum[i];
+        }
+        double mean = sum / (double)w, run = 0.0;
+        rowmean[j] = mean;
+        for (int i = 0; i < w; i++) {
+            run += lum[i];
+            size_t k = (size_t)j * w + i;
+            if (mean > 0.0) {
+                s->cpdf[k] = (R)(lum[i] / mean);
+                s->ccdf[k] = i == w - 1 ? (R)1 : (R)(run / (double)w / mean);
+            } else {
+                s->cpdf[k] = 1;
+                s->ccdf[k] = i == w - 1 ? (R)1 : (R)((double)(i + 1) / (double)w);
+            }
+        }
+    }
+    double msum = 0.0;
+    for (int j = 0; j < h; j++) msum += rowmean[j];
+    double mmean = msum / (double)h, run = 0.0;
+    for (int j = 0; j < h; j++) {
+        run += rowmean[j];
+        s->mpdf[j] = mmean > 0.0 ? (R)(rowmean[j] / mmean) : (R)1;
+        s->mcdf[j] = j == h - 1 ? (R)1 : (mmean > 0.0 ? (R)(run / (double)h / mmean) : (R)((j + 1.0) / h));
+    }
+    free(lum);
+    free(rowmean);
+}
+
+/* Solid-angle density of env_sample in direction dir (new: MIS). */
+static R env_pdf(const oscene* s, v3 dir)
+{
+    R theta = acos(dir.y);
+    R phi = atan2(dir.x, dir.z);
+    R u = (phi + PI_R) * (R)0.5f * INV_PI_R;
+    R v = (R)1 - theta * INV_PI_R;
+    int iu = (int)(u * (R)s->env_w), iv = (int)(v * (R)s->env_h);
+    if (iu < 0) iu = 0;
+    if (iu > s->env_w - 1) iu = s->env_w - 1;
+    if (iv < 0) iv = 0;
+    if (iv > s->env_h - 1) iv = s->env_h - 1;
+    R p = s->mpdf[iv] * s->cpdf[(size_t)iv * s->env_w + iu];
+    R st = sin(theta);
+    return st > 0 ? p / ((R)2 * PI_R * PI_R * st) : 0;
+}
+
+/* sample_environment (ORF/Envmap.h:66-98): returns L = E / pdf and the pdf. */
+static void env_sample(const oscene* s, uint32_t* t, v3* dir, v3* L, R* pdf_sa)
+{
+    const R two_pi2 = (R)2 * PI_R * PI_R;
+    R xi1 = orc_rnd_tea(t), xi2 = orc_rnd_tea(t);
+    int vi = env_bsearch(s->mcdf, s->env_h, xi1);
+    R dv = vi > 0 ? (xi1 - s->mcdf[vi - 1]) / (s->mcdf[vi] - s->mcdf[vi - 1]) : xi1 / s->mcdf[vi];
+    R pdf_m = s->mpdf[vi];
+    R v = ((R)vi + dv) / (R)s->env_h;
+    const R* cc = s->ccdf + (size_t)vi * s->env_w;
+    int ui = env_bsearch(cc, s->env_w, xi2);
+    R du = ui > 0 ? (xi2 - cc[ui - 1]) / (cc[ui] - cc[ui - 1]) : xi2 / cc[ui];
+    R pdf_c = s->cpdf[(size_t)vi * s->env_w + ui];
+    R u = ((R)ui + du) / (R)s->env_w;
+    R prob = pdf_m * pdf_c;
+    R theta = ((R)1 - v) * PI_R;
+    R phi = ((R)2 * u - (R)1) * PI_R;
+    R st = sin(theta), ct = cos(theta), sp = sin(phi), cp = cos(phi);
+    *dir = V3(st * sp, ct, st * cp);
+    v3 E = tex_bilinear(s, u, v);
+    *L = vscale(E, st * two_pi2 / prob);
+    *pdf_sa = prob / (two_pi2 * st);
+}
+
+/* ---------------------------------------------------------------- lights */
+
+/* evaluate_direct_illumination (ORF/LightSampler.h:143-166): point (:20-28),
+ * directional (:31-37), triangle area light (:40-71), environment (new). */
+static void direct_light(const oscene* s, const orc_light* lt, v3 pos, uint32_t* seed, v3* dir, v3* L, R* dist,
+                         R* pdf_env)
+{
+    *pdf_env = -1;
+    v3 rad = V3(lt->radiance[0], lt->radiance[1], lt->radiance[2]);
+    v3 prm = V3(lt->params[0], lt->params[1], lt->params[2]);
+    if (lt->type == ORC_POINT) {
+        *dir = vsub(prm, pos);
+        R r2 = vdot(*dir, *dir);
+        *dist = sqrt(r2);
+        *dir = V3(dir->x / *dist, dir->y / *dist, dir->z / *dist);
+        *L = V3(rad.x / r2, rad.y / r2, rad.z / r2);
+    } else if (lt->type == ORC_DIRECTIONAL) {
+        *dir = orc_normalize(vneg(prm));
+        *dist = RAY_MAX;
+        *L = rad;
+    } else if (lt->type == ORC_TRIANGLES) {
+        uint32_t ntri = (uint32_t)lt->params[0];
+        uint32_t pick = (uint32_t)(orc_rnd_tea(seed) * (R)ntri);
+        if (pick >= ntri) pick = ntri - 1; /* new: rnd_tea may round to 1.0f */
+        const orc_trilight* tl = &s->trilights[pick + (uint32_t)lt->params[1]];
+        v3 uvw = orc_sample_barycentric(seed);
+        v3 a = V3(tl->v[0], tl->v[1], tl->v[2]), b = V3(tl->v[3], tl->v[4], tl->v[5]),
+           c = V3(tl->v[6], tl->v[7], tl->v[8]);
+        v3 lp = vadd(vadd(vscale(a, uvw.x), vscale(b, uvw.y)), vscale(c, uvw.z));
+        v3 n;
+        if (tl->has_normals) {
+            v3 n0 = V3(tl->n[0], tl->n[1], tl->n[2]), n1 = V3(tl->n[3], tl->n[4], tl->n[5]),
+               n2 = V3(tl->n[6], tl->n[7], tl->n[8]);
+            n = orc_normalize(vadd(vadd(vscale(n0, uvw.x), vscale(n1, uvw.y)), vscale(n2, uvw.z)));
+        } else {
+            n = orc_normalize(vcross(vsub(b, a), vsub(c, a)));
+        }
+        *dir = vsub(lp, pos);
+        R d2 = vdot(*dir, *dir);
+        *dist = sqrt(d2);
+        *dir = orc_normalize(*dir);
+        R cp = vdot(n, vneg(*dir));
+        if (!(cp >= 0)) cp = 0;
+        *L = vscale(rad, (R)tl->area * cp / d2 * (R)ntri);
+    } else if (lt->type == ORC_ENV) {
+        env_sample(s, seed, dir, L, pdf_env);
+        *dist = RAY_MAX;
+    } else {
+        *dir = V3(0, 0, 1);
+        *L = V3(0, 0, 0);
+        *dist = 0;
+    }
+}
+
+static R mis_power(R a, R b)
+{
+    R a2 = a * a, b2 = b * b;
+    return a2 + b2 > 0 ? a2 / (a2 + b2) : 0;
+}
+
+/* ---------------------------------------------------------------- shading */
+
+typedef struct {
+    uint64_t ext, shd;
+} ostats;
+
+static v3 radiance(const oscene* s, v3 o, v3 d, int depth, int emit_light, uint32_t* seed, R prev_pdf, ostats* st);
+
+static v3 hit_normal(const oscene* s, const ohit* h)
+{
+    int32_t t = h->prim;
+    v3 n;
+    if (!s->nidx || s->nidx[3 * t] < 0 || s->nidx[3 * t + 1] < 0 || s->nidx[3 * t + 2] < 0) {
+        n = orc_normalize(vcross(s->e1[t], s->e2[t]));
+    } else {
+        v3 n0 = s->normals[s->nidx[3 * t]], n1 = s->normals[s->nidx[3 * t + 1]], n2 = s->normals[s->nidx[3 * t + 2]];
+        /* normalize(n1*beta + n2*gamma + n0*(1-beta-gamma)) (CU/triangle_mesh.cu:334) */
+        n = orc_normalize(vadd(vadd(vscale(n1, h->b), vscale(n2, h->g)), vscale(n0, (R)1 - h->b - h->g)));
+    }
+    /* normalize(rtTransformNormal(RT_OBJECT_TO_WORLD, shading_normal)): world-space geometry */
+    return orc_normalize(n);
+}
+
+/* CU/diffuse_shader.cu:122-187 (+ environment MIS, new). */
+static v3 shade_lambert(const oscene* s, const orc_material* m, v3 o, v3 d, const ohit* h, int depth,
+                        int* emit_light, uint32_t* t, ostats* st)
+{
+    v3 result = V3(0, 0, 0);
+    if (depth > s->max_depth) return result;
+    v3 hit_pos = vadd(o, vscale(d, h->t));
+    v3 normal = hit_normal(s, h);
+    v3 ffn = orc_faceforward(normal, vneg(d), normal);
+    v3 rho = V3(m->diffuse[0], m->diffuse[1], m->diffuse[2]);
+    uint32_t light_idx = (uint32_t)((R)s->nlights * orc_rnd_tea(t));
+    if (s->nlights > 0) {
+        if (light_idx >= (uint32_t)s->nlights) light_idx = (uint32_t)s->nlights - 1; /* new: see header */
+        R light_pdf = (R)1 / (R)s->nlights;
+        v3 w_l, rad;
+        R dist, penv;
+        direct_light(s, &s->lights[light_idx], hit_pos, t, &w_l, &rad, &dist, &penv);
+        rad = V3(rad.x / light_pdf, rad.y / light_pdf, rad.z / light_pdf);
+        R cos_theta = vdot(ffn, w_l);
+        if (cos_theta > 0) {
+            ohit sh;
+            st->shd++;
+            R V = trace(s, hit_pos, w_l, s->eps, dist - s->eps, 1, &sh) ? (R)0 : (R)1;
+            v3 Li = vscale(rad, V);
+            v3 c = vmul(vscale(vscale(Li, INV_PI_R), cos_theta), rho);
+            if (penv >= 0) c = vscale(c, mis_power(penv * light_pdf, cos_theta * INV_PI_R));
+            result = vadd(result, c);
+            *emit_light = 0;
+        }
+    }
+    R prob = (rho.x + rho.y + rho.z) / (R)3.0f;
+    R xi = orc_rnd_tea(t);
+    if (xi < prob) {
+        v3 nd = orc_sample_cosine_weighted(ffn, t);
+        R pdf = s->has_env && s->env_is ? vdot(ffn, nd) * INV_PI_R : (R)-1;
+        v3 child = radiance(s, hit_pos, nd, depth + 1, *emit_light, t, pdf, st);
+        v3 cr = vmul(child, rho);
+        result = vadd(result, V3(cr.x / prob, cr.y / prob, cr.z / prob));
+    }
+    return result;
+}
+
+/* CU/metallic_shader.cu:45-162, visible-normal branch. */
+static v3 shade_conductor(const oscene* s, const orc_material* m, const cplx3* eta, v3 o, v3 d, const ohit* h,
+                          int depth, uint32_t* seed, ostats* st)
+{
+    if (depth >= s->max_depth) return V3(0, 0, 0);
+    v3 hit_point = vadd(o, vscale(d, h->t));
+    v3 normal = hit_normal(s, h);
+    v3 ffn = orc_faceforward(normal, vneg(d), normal);
+    v3 w_i = vneg(d);
+    R ax = m->roughness[0], ay = m->roughness[1];
+    R z1 = orc_rnd_tea(seed);
+    R z2 = orc_rnd_tea(seed);
+    v3 mn = orc_to_world(orc_ggx_sample_vndf(orc_to_local(w_i, ffn), ax, ay, z1, z2), ffn);
+    R G_i = orc_ggx_G1(w_i, mn, ffn, ax, ay);
+    if (G_i <= 0) return V3(0, 0, 0);
+    v3 F = orc_fresnel_conductor(w_i, mn, eta);
+    v3 w_o = vadd(vneg(w_i), vscale(vscale(mn, (R)2.0f), vdot(w_i, mn)));
+    R G_o = orc_ggx_G1(w_o, mn, ffn, ax, ay);
+    if (G_o <= 0) return V3(0, 0, 0);
+    v3 child = radiance(s, hit_point, w_o, depth + 1, 1, seed, (R)-1, st);
+    v3 weight = vscale(F, G_o);
+    return vmul(child, weight);
+}
+
+static v3 radiance(const oscene* s, v3 o, v3 d, int depth, int emit_light, uint32_t* seed, R prev_pdf, ostats* st)
+{
+    ohit h;
+    st->ext++;
+    if (!trace(s, o, d, s->eps, RAY_MAX, 0, &h)) {
+        if (!s->has_env) return V3(0, 0, 0);
+        v3 E = env_lookup(s, d);
+        if (prev_pdf > 0 && s->env_is) E = vscale(E, mis_power(prev_pdf, env_pdf(s, d) / (R)s->nlights));
+        return E;
+    }
+    const orc_material* m = &s->mats[s->tri_mat[h.prim]];
+    if (m->kind == ORC_EMITTER) {
+        /* CU/arealight_shader.cu:36-50 */
+        v3 normal = hit_normal(s, &h);
+        if (vdot(normal, vneg(d)) > 0 && emit_light) return V3(m->radiance[0], m->radiance[1], m->radiance[2]);
+        return V3(0, 0, 0);
+    }
+    if (m->kind == ORC_LAMBERT) return shade_lambert(s, m, o, d, &h, depth, &emit_light, seed, st);
+    if (m->kind == ORC_CONDUCTOR) return shade_conductor(s, m, &s->eta[s->tri_mat[h.prim]], o, d, &h, depth, seed, st);
+    return V3(0, 0, 0);
+}
+
+/* path_tracer (CU/path_tracer.cu:32-74) for one (pixel, frame). */
+static v3 pixel_sample(const oscene* s, int W, int H, int x, int y, uint32_t frame, ostats* st)
+{
+    uint32_t seed = orc_tea16((uint32_t)W * (uint32_t)y + (uint32_t)x, frame);
+    R jx = orc_rnd_tea(&seed);
+    R jy = orc_rnd_tea(&seed);
+    R ipx = ((R)x + jx) / (R)W * (R)2.0f - (R)1.0f;
+    R ipy = ((R)y + jy) / (R)H * (R)2.0f - (R)1.0f;
+    v3 dir = orc_normalize(vadd(vadd(vscale(s->U, ipx), vscale(s->V, ipy)), s->W));
+    return radiance(s, s->eye, dir, 0, 1, &seed, (R)-1, st);
+}
+
+/* ---------------------------------------------------------------- API */
+
+void* orc_scene_create(const orc_scene_desc* d)
+{
+    oscene* s = (oscene*)calloc(1, sizeof(oscene));
+    s->ntris = d->ntris;
+    s->p0 = (v3*)malloc(sizeof(v3) * (size_t)(d->ntris + 1));
+    s->e1 = (v3*)malloc(sizeof(v3) * (size_t)(d->ntris + 1));
+    s->e2 = (v3*)malloc(sizeof(v3) * (size_t)(d->ntris + 1));
+    for (int64_t t = 0; t < d->ntris; t++) {
+        const int32_t* ti = d->idx + 3 * t;
+        /* e1 = p1 - p0, e2 = p2 - p0 rounded in float, as the product stores them */
+        float p[3][3];
+        for (int k = 0; k < 3; k++)
+            for (int a = 0; a < 3; a++) p[k][a] = d->verts[3 * ti[k] + a];
+        s->p0[t] = V3(p[0][0], p[0][1], p[0][2]);
+        s->e1[t] = V3(p[1][0] - p[0][0], p[1][1] - p[0][1], p[1][2] - p[0][2]);
+        s->e2[t] = V3(p[2][0] - p[0][0], p[2][1] - p[0][1], p[2][2] - p[0][2]);
+    }
+    if (d->nidx && d->normals) {
+        s->nidx = (int32_t*)malloc(sizeof(int32_t) * 3 * (size_t)(d->ntris + 1));
+        memcpy(s->nidx, d->nidx, sizeof(int32_t) * 3 * (size_t)d->ntris);
+        s->normals = (v3*)malloc(sizeof(v3) * (size_t)(d->nnormals + 1));
+        for (int64_t k = 0; k < d->nnormals; k++)
+            s->normals[k] = V3(d->normals[3 * k], d->normals[3 * k + 1], d->normals[3 * k + 2]);
+    }
+    s->tri_mat = (int32_t*)malloc(sizeof(int32_t) * (size_t)(d->ntris + 1));
+    memcpy(s->tri_mat, d->tri_mat, sizeof(int32_t) * (size_t)d->ntris);
+    s->nmats = d->nmats;
+    s->mats = (orc_material*)malloc(sizeof(orc_material) * (size_t)(d->nmats + 1));
+    memcpy(s->mats, d->mats, sizeof(orc_material) * (size_t)d->nmats);
+    s->eta = (cplx3*)malloc(sizeof(cplx3) * (size_t)(d->nmats + 1));
+    for (int i = 0; i < d->nmats; i++) {
+        /* eta = n1 / ior (CU/metallic_shader.cu:59-61) */
+        cplx c0 = {d->mats[i].ior_re[0], d->mats[i].ior_im[0]}, c1 = {d->mats[i].ior_re[1], d->mats[i].ior_im[1]},
+             c2 = {d->mats[i].ior_re[2], d->mats[i].ior_im[2]};
+        s->eta[i].x = orc_cdiv_fc((R)1.0f, c0);
+        s->eta[i].y = orc_cdiv_fc((R)1.0f, c1);
+        s->eta[i].z = orc_cdiv_fc((R)1.0f, c2);
+    }
+    s->has_env = d->env != NULL && d->env_w > 0 && d->env_h > 0;
+    s->env_is = s->has_env && d->env_is;
+    int extra = s->env_is ? 1 : 0;
+    s->nlights = d->nlights + extra;
+    s->lights = (orc_light*)calloc((size_t)(s->nlights + 1), sizeof(orc_light));
+    memcpy(s->lights, d->lights, sizeof(orc_light) * (size_t)d->nlights);
+    if (extra) s->lights[d->nlights].type = ORC_ENV;
+    s->trilights = (orc_trilight*)malloc(sizeof(orc_trilight) * (size_t)(d->ntrilights + 1));
+    memcpy(s->trilights, d->trilights, sizeof(orc_trilight) * (size_t)d->ntrilights);
+    if (s->has_env) {
+        s->env_w = d->env_w;
+        s->env_h = d->env_h;
+        s->env = (float*)malloc(sizeof(float) * 4 * (size_t)d->env_w * d->env_h);
+        memcpy(s->env, d->env, sizeof(float) * 4 * (size_t)d->env_w * d->env_h);
+        if (s->env_is) build_env_tables(s);
+    }
+    s->eye = V3(d->eye[0], d->eye[1], d->eye[2]);
+    s->U = V3(d->U[0], d->U[1], d->U[2]);
+    s->V = V3(d->V[0], d->V[1], d->V[2]);
+    s->W = V3(d->W[0], d->W[1], d->W[2]);
+    s->max_depth = d->max_depth;
+    s->eps = d->scene_epsilon;
+    build_bvh(s);
+    return s;
+}
+
+void orc_scene_destroy(void* p)
+{
+    oscene* s = (oscene*)p;
+    if (!s) return;
+    free(s->p0);
+    free(s->e1);
+    free(s->e2);
+    free(s->nidx);
+    free(s->normals);
+    free(s->tri_mat);
+    free(s->nodes);
+    free(s->order);
+    free(s->mats);
+    free(s->eta);
+    free(s->lights);
+    free(s->trilights);
+    free(s->env);
+    free(s->mpdf);
+    free(s->mcdf);
+    free(s->cpdf);
+    free(s->ccdf);
+    free(s);
+}
+
+void orc_render(void* p, int W, int H, int x0, int y0, int x1, int y1, uint32_t frame0, uint32_t spp, double* out,
+                uint64_t* counters, int threads)
+{
+    const oscene* s = (const oscene*)p;
+    int rw = x1 - x0, rh = y1 - y0;
+    uint64_t bad = 0, ext = 0, shd = 0;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads) reduction(+ : bad, ext, shd)
+    for (int64_t i = 0; i < (int64_t)rw * rh; i++) {
+        int x = x0 + (int)(i % rw), y = y0 + (int)(i / rw);
+        double acc[4] = {0, 0, 0, 0};
+        ostats st = {0, 0};
+        for (uint32_t f = frame0; f < frame0 + spp; f++) {
+            v3 L = pixel_sample(s, W, H, x, y, f, &st);
+            if (isfinite(L.x) && isfinite(L.y) && isfinite(L.z)) {
+                acc[0] += (double)L.x;
+                acc[1] += (double)L.y;
+                acc[2] += (double)L.z;
+                acc[3] += 1.0;
+            } else {
+                bad++;
+            }
+        }
+        ext += st.ext;
+        shd += st.shd;
+        memcpy(out + 4 * i, acc, sizeof(acc));
+    }
+    if (counters) {
+        counters[0] = bad;
+        counters[1] = ext;
+        counters[2] = shd;
+        counters[3] = (uint64_t)rw * rh * spp;
+    }
+}
+
+void orc_render_samples(void* p, int W, int H, const int32_t* px, const int32_t* py, int n, uint32_t frame,
+                        double* out, int threads)
+{
+    const oscene* s = (const oscene*)p;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads)
+    for (int i = 0; i < n; i++) {
+        ostats st = {0, 0};
+        v3 L = pixel_sample(s, W, H, px[i], py[i], frame, &st);
+        out[3 * i] = (double)L.x;
+        out[3 * i + 1] = (double)L.y;
+        out[3 * i + 2] = (double)L.z;
+    }
+}
+
+void orc_env_tables(void* p, double* mpdf, double* mcdf, double* cpdf, double* ccdf)
+{
+    const oscene* s = (const oscene*)p;
+    if (!s->env_is) return;
+    size_t w = s->env_w, h = s->env_h;
+    for (size_t j = 0; j < h; j++) {
+        mpdf[j] = (double)s->mpdf[j];
+        mcdf[j] = (double)s->mcdf[j];
+    }
+    for (size_t k = 0; k < w * h; k++) {
+        cpdf[k] = (double)s->cpdf[k];
+        ccdf[k] = (double)s->ccdf[k];
+    }
+}
+
+void orc_closest_hits(void* p, int n, const float* o, const float* d, float tmin, float* t, int32_t* prim,
+                      int brute_force)
+{
+    const oscene* s = (const oscene*)p;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; i++) {
+        ohit h;
+        v3 oo = V3(o[3 * i], o[3 * i + 1], o[3 * i + 2]), dd = V3(d[3 * i], d[3 * i + 1], d[3 * i + 2]);
+        if (brute_force) trace_brute(s, oo, dd, (R)tmin, RAY_MAX, &h);
+        else trace(s, oo, dd, (R)tmin, RAY_MAX, 0, &h);
+        t[i] = (float)h.t;
+        prim[i] = h.prim;
+    }
+}
+
+/* Draws of env_sample (one TEA stream per draw, seeded tea16(i, seed)) with
+ * their returned pdf and the pdf env_pdf evaluates at the drawn direction. */
+void orc_env_sample_batch(void* p, uint32_t seed, int n, double* dir, double* pdf_sample, double* pdf_eval)
+{
+    const oscene* s = (const oscene*)p;
+    if (!s->env_is) return;
+    for (int i = 0; i < n; i++) {
+        uint32_t t = orc_tea16((uint32_t)i, seed);
+        v3 d, L;
+        R pdf;
+        env_sample(s, &t, &d, &L, &pdf);
+        dir[3 * i] = (double)d.x;
+        dir[3 * i + 1] = (double)d.y;
+        dir[3 * i + 2] = (double)d.z;
+        pdf_sample[i] = (double)pdf;
+        pdf_eval[i] = (double)env_pdf(s, d);
+    }
+}

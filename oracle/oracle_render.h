@@ -1,5 +1,83 @@
-/* oracle_render.h -- CPU restatement of the per-pixel render path (TEST INFRASTRUCTURE). */
+/* oracle_render.h -- CPU restatement of the per-pixel render path (TEST INFRASTRUCTURE).
+ *
+ * A scene is given as flattened world-space arrays; the oracle builds its own
+ * acceleration structure (object-median BVH, independent of the product's
+ * binned-SAH builder) and renders by following the reference's programs
+ * recursively: CU/path_tracer.cu (ray generation), CU/diffuse_shader.cu,
+ * CU/metallic_shader.cu, CU/arealight_shader.cu (closest hits),
+ * CU/envmap_background.cu (miss), ORF/LightSampler.h, ORF/Envmap.h.
+ */
 #ifndef ORF_ORACLE_RENDER_H
 #define ORF_ORACLE_RENDER_H
+#include <stdint.h>
+
 #include "orf_oracle.h"
+
+enum { ORC_LAMBERT = 0, ORC_CONDUCTOR = 1, ORC_EMITTER = 2, ORC_DIELECTRIC = 3 };
+enum { ORC_POINT = 0, ORC_DIRECTIONAL = 1, ORC_TRIANGLES = 2, ORC_ENV = 5 };
+
+typedef struct {
+    int32_t kind;
+    int32_t microfacet_model;
+    int32_t normal_distribution;
+    int32_t pad;
+    float diffuse[3];
+    float radiance[3];
+    float absorption[3];
+    float roughness[2];
+    float ior;
+    double ior_re[3], ior_im[3];
+} orc_material;
+
+typedef struct {
+    uint32_t type;
+    float radiance[3];
+    float params[3];  /* point: position; directional: direction; triangles: count, start, end */
+} orc_light;
+
+typedef struct {
+    float v[9];
+    float n[9];
+    uint32_t has_normals;
+    float area;
+} orc_trilight;
+
+typedef struct {
+    int64_t nverts, ntris, nnormals;
+    const float* verts;
+    const int32_t* idx;
+    const float* normals;
+    const int32_t* nidx;     /* NULL: geometric normals */
+    const int32_t* tri_mat;
+    int32_t nmats;
+    const orc_material* mats;
+    int32_t nlights;         /* the environment entry, if any, is appended by the oracle */
+    const orc_light* lights;
+    int32_t ntrilights;
+    const orc_trilight* trilights;
+    const float* env;        /* RGBA FP32, NULL = no environment */
+    int32_t env_w, env_h, env_is;
+    float eye[3], U[3], V[3], W[3];
+    int32_t max_depth;
+    float scene_epsilon;
+} orc_scene_desc;
+
+void* orc_scene_create(const orc_scene_desc* d);
+void orc_scene_destroy(void* s);
+/* Sum of samples frame0 .. frame0+spp-1 for pixels [x0,x1) x [y0,y1) of a
+ * W x H film: out has 4 doubles per pixel (R, G, B sums of finite samples,
+ * count); counters: [0] non-finite, [1] extension rays, [2] shadow rays, [3] paths. */
+void orc_render(void* s, int W, int H, int x0, int y0, int x1, int y1, uint32_t frame0, uint32_t spp,
+                double* out, uint64_t* counters, int threads);
+/* Per-sample radiance of one frame for a pixel list: out has 3 doubles per pixel. */
+void orc_render_samples(void* s, int W, int H, const int32_t* px, const int32_t* py, int n, uint32_t frame,
+                        double* out, int threads);
+/* The environment tables the oracle built (FP64 -> out arrays as double). */
+void orc_env_tables(void* s, double* marg_pdf, double* marg_cdf, double* cond_pdf, double* cond_cdf);
+void orc_env_sample_batch(void* s, uint32_t seed, int n, double* dir, double* pdf_sample, double* pdf_eval);
+int orc_env_bsearch(const double* cdf, int n, double xi);
+/* Closest hit of a batch of rays against the oracle BVH (t, prim) and by brute force. */
+void orc_closest_hits(void* s, int n, const float* o, const float* d, float tmin, float* t, int32_t* prim,
+                      int brute_force);
+
 #endif

@@ -1,0 +1,181 @@
+"""Hands a SceneDesc to the CPU oracle renderer (tests and the CPU baseline only)."""
+import ctypes
+
+import numpy as np
+
+import oracle
+
+c_vp = ctypes.c_void_p
+
+
+class OrcMaterial(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("microfacet_model", ctypes.c_int32),
+                ("normal_distribution", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("diffuse", ctypes.c_float * 3), ("radiance", ctypes.c_float * 3),
+                ("absorption", ctypes.c_float * 3), ("roughness", ctypes.c_float * 2), ("ior", ctypes.c_float),
+                ("ior_re", ctypes.c_double * 3), ("ior_im", ctypes.c_double * 3)]
+
+
+class OrcLight(ctypes.Structure):
+    _fields_ = [("type", ctypes.c_uint32), ("radiance", ctypes.c_float * 3), ("params", ctypes.c_float * 3)]
+
+
+class OrcTriLight(ctypes.Structure):
+    _fields_ = [("v", ctypes.c_float * 9), ("n", ctypes.c_float * 9), ("has_normals", ctypes.c_uint32),
+                ("area", ctypes.c_float)]
+
+
+class OrcSceneDesc(ctypes.Structure):
+    _fields_ = [("nverts", ctypes.c_int64), ("ntris", ctypes.c_int64), ("nnormals", ctypes.c_int64),
+                ("verts", c_vp), ("idx", c_vp), ("normals", c_vp), ("nidx", c_vp), ("tri_mat", c_vp),
+                ("nmats", ctypes.c_int32), ("mats", c_vp), ("nlights", ctypes.c_int32), ("lights", c_vp),
+                ("ntrilights", ctypes.c_int32), ("trilights", c_vp), ("env", c_vp), ("env_w", ctypes.c_int32),
+                ("env_h", ctypes.c_int32), ("env_is", ctypes.c_int32), ("eye", ctypes.c_float * 3),
+                ("U", ctypes.c_float * 3), ("V", ctypes.c_float * 3), ("W", ctypes.c_float * 3),
+                ("max_depth", ctypes.c_int32), ("scene_epsilon", ctypes.c_float)]
+
+
+_KIND = {"diffuse_shader.cu": 0, "metallic_shader.cu": 1, "arealight_shader.cu": 2, "rough_transparent_shader.cu": 3}
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class OracleScene:
+    """An oracle scene built from a SceneDesc; keeps the arrays alive."""
+
+    def __init__(self, desc, fp64=False):
+        self.lib = L = oracle.lib(fp64=fp64)
+        L.orc_scene_create.restype = c_vp
+        L.orc_scene_create.argtypes = [ctypes.POINTER(OrcSceneDesc)]
+        L.orc_scene_destroy.argtypes = [c_vp]
+        L.orc_render.argtypes = [c_vp] + [ctypes.c_int] * 6 + [ctypes.c_uint32, ctypes.c_uint32, c_vp, c_vp,
+                                                               ctypes.c_int]
+        L.orc_render_samples.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, c_vp, c_vp, ctypes.c_int,
+                                         ctypes.c_uint32, c_vp, ctypes.c_int]
+        L.orc_env_tables.argtypes = [c_vp] * 5
+        L.orc_env_sample_batch.argtypes = [c_vp, ctypes.c_uint32, ctypes.c_int, c_vp, c_vp, c_vp]
+        L.orc_env_bsearch.restype = ctypes.c_int
+        L.orc_env_bsearch.argtypes = [c_vp, ctypes.c_int, ctypes.c_double]
+        L.orc_closest_hits.argtypes = [c_vp, ctypes.c_int, c_vp, c_vp, ctypes.c_float, c_vp, c_vp, ctypes.c_int]
+        self.desc = desc
+        V, T, N, NI, M = desc.flat()
+        self.arrays = [np.ascontiguousarray(V), np.ascontiguousarray(T), np.ascontiguousarray(N),
+                       np.ascontiguousarray(NI), np.ascontiguousarray(M)]
+        mats = (OrcMaterial * max(1, len(desc.materials)))()
+        for i, m in enumerate(desc.materials):
+            mm = mats[i]
+            mm.kind = _KIND[m.program]
+            mm.microfacet_model = m.uints.get("microfacet_model", (1,))[0]
+            mm.normal_distribution = m.uints.get("normal_distribution", (1,))[0]
+            for k, v in enumerate(m.floats.get("diffuse_color", (0.8, 0.8, 0.8))):
+                mm.diffuse[k] = v
+            for k, v in enumerate(m.floats.get("radiance", (1.0, 1.0, 1.0))):
+                mm.radiance[k] = v
+            for k, v in enumerate(m.floats.get("glass_absorption", (0.0, 0.0, 0.0))):
+                mm.absorption[k] = v
+            r = m.floats.get("roughness", (0.1, 0.1))
+            mm.roughness[0] = r[0]
+            mm.roughness[1] = r[1] if len(r) > 1 else r[0]
+            mm.ior = m.floats.get("ior", (1.5,))[0]
+            ior = m.ior_complex or ((1.5, 0.0),) * 3
+            for k in range(3):
+                mm.ior_re[k] = ior[k][0]
+                mm.ior_im[k] = ior[k][1]
+        lights, tls = [], []
+        for l in desc.lights:
+            ol = OrcLight()
+            ol.radiance[:] = list(l.radiance)
+            if l.kind == "point":
+                ol.type = 0
+                ol.params[:] = list(l.position)
+            elif l.kind == "directional":
+                ol.type = 1
+                d = np.asarray(l.direction, np.float64)
+                d = (d / np.linalg.norm(d)).astype(np.float32)
+                ol.params[:] = [float(x) for x in d]
+            else:
+                ol.type = 2
+                n = len(l.tri_verts)
+                ol.params[:] = [float(n), float(len(tls)), float(len(tls) + n - 1)]
+                for t in range(n):
+                    tl = OrcTriLight()
+                    v = np.asarray(l.tri_verts[t], np.float32)
+                    tl.v[:] = [float(x) for x in v]
+                    if l.tri_normals is not None:
+                        tl.n[:] = [float(x) for x in np.asarray(l.tri_normals[t], np.float32)]
+                        tl.has_normals = 1
+                    a, b, c = v[0:3], v[3:6], v[6:9]
+                    tl.area = np.float32(0.5 * float(np.linalg.norm(np.cross(b - c, a - c).astype(np.float32))))
+                    tls.append(tl)
+            lights.append(ol)
+        self.mats = mats
+        self.lights = (OrcLight * max(1, len(lights)))(*lights)
+        self.tls = (OrcTriLight * max(1, len(tls)))(*tls)
+        self.env = None if desc.env is None else np.ascontiguousarray(desc.env, np.float32)
+        d = OrcSceneDesc()
+        d.nverts, d.ntris, d.nnormals = len(V), len(T), len(N)
+        d.verts, d.idx, d.normals, d.nidx, d.tri_mat = [_p(a) for a in self.arrays]
+        if len(N) == 0:
+            d.normals = None
+            d.nidx = None
+        d.nmats = len(desc.materials)
+        d.mats = ctypes.cast(self.mats, c_vp)
+        d.nlights = len(lights)
+        d.lights = ctypes.cast(self.lights, c_vp)
+        d.ntrilights = len(tls)
+        d.trilights = ctypes.cast(self.tls, c_vp)
+        if self.env is not None:
+            d.env = _p(self.env)
+            d.env_h, d.env_w = self.env.shape[:2]
+            d.env_is = 1 if desc.env_importance else 0
+        for name in ("eye", "U", "V", "W"):
+            getattr(d, name)[:] = list(getattr(desc, name))
+        d.max_depth = desc.max_depth
+        d.scene_epsilon = desc.scene_epsilon
+        self.cdesc = d
+        self.handle = L.orc_scene_create(ctypes.byref(d))
+
+    def close(self):
+        if self.handle:
+            self.lib.orc_scene_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def render(self, width, height, frame0=0, spp=1, rect=None, threads=8):
+        x0, y0, x1, y1 = rect if rect is not None else (0, 0, width, height)
+        out = np.zeros((y1 - y0, x1 - x0, 4), np.float64)
+        cnt = np.zeros(4, np.uint64)
+        self.lib.orc_render(self.handle, width, height, x0, y0, x1, y1, frame0, spp, _p(out), _p(cnt), threads)
+        return out, dict(zip(("nonfinite", "ext_rays", "shadow_rays", "paths"), [int(x) for x in cnt]))
+
+    def samples(self, width, height, px, py, frame, threads=8):
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        out = np.zeros((len(px), 3), np.float64)
+        self.lib.orc_render_samples(self.handle, width, height, _p(px), _p(py), len(px), frame, _p(out), threads)
+        return out
+
+    def env_tables(self):
+        h, w = self.env.shape[:2]
+        out = [np.zeros(h), np.zeros(h), np.zeros(h * w), np.zeros(h * w)]
+        self.lib.orc_env_tables(self.handle, *[_p(a) for a in out])
+        return out
+
+    def env_samples(self, n, seed=1):
+        d = np.zeros((n, 3))
+        ps = np.zeros(n)
+        pe = np.zeros(n)
+        self.lib.orc_env_sample_batch(self.handle, seed, n, _p(d), _p(ps), _p(pe))
+        return d, ps, pe
+
+    def closest_hits(self, o, d, tmin=1e-4, brute_force=False):
+        o = np.ascontiguousarray(o, np.float32)
+        d = np.ascontiguousarray(d, np.float32)
+        t = np.zeros(len(o), np.float32)
+        prim = np.zeros(len(o), np.int32)
+        self.lib.orc_closest_hits(self.handle, len(o), _p(o), _p(d), tmin, _p(t), _p(prim), 1 if brute_force else 0)
+        return t, prim

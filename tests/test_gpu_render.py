@@ -1,0 +1,122 @@
+"""GPU parity of the megakernel path tracer (through the C-ABI) against the CPU oracle.
+
+Per-sample tolerance (per pixel and channel, one sample of one frame):
+|gpu - oracle| <= 1e-5 |oracle| + 1e-6.  Both sides round +,-,*,/ identically
+(no contraction), so differences start only in acosf/atan2f/sincosf (~1-2 ulp);
+a path whose branch flips on such a difference (a silhouette, a Russian-roulette
+threshold, a CDF cell edge) diverges completely, so the bar is a fraction:
+>= 99% of samples within tolerance.  Converged images are compared with the
+FP64 oracle through the relative RMSE stated in each test.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from open_rendering_framework_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+RTOL, ATOL = 1e-5, 1e-6
+
+
+def _say(*a):
+    print(*a)
+    sys.stdout.flush()
+
+
+def _small(level=4, w=64, h=64, env_size=(256, 128), **kw):
+    return scenes.path_scene(width=w, height=h, level=level, env_size=env_size, **kw)
+
+
+def _gpu_samples(ctx, w, h, frame):
+    ctx.film_clear()
+    ctx.launch(w, h, frame, 1)
+    s = ctx.film_sums(w, h)
+    assert np.all(s[..., 3] <= 1)
+    return s[..., :3]
+
+
+def _frac_ok(got, want):
+    err = np.abs(got - want)
+    ok = (err <= RTOL * np.abs(want) + ATOL).all(axis=-1)
+    return ok.mean(), ok
+
+
+def test_env_tables_match_oracle():
+    from tests.oracle_scene import OracleScene
+
+    d = _small(level=1)
+    ctx = scenes.to_context(d)
+    h, w = d.env.shape[:2]
+    g = ctx.env_tables(w, h)
+    o = OracleScene(d).env_tables()
+    for a, b in zip(g, o):
+        np.testing.assert_allclose(a, b, rtol=2e-7, atol=1e-12)
+
+
+def test_per_sample_parity_small_scene():
+    from tests.oracle_scene import OracleScene
+
+    d = _small()
+    ctx = scenes.to_context(d)
+    stats = ctx.accel_build()
+    _say("bvh", stats)
+    assert stats["max_depth"] < 48 and stats["degenerate"] == 0
+    orc = OracleScene(d)
+    ys, xs = np.mgrid[0:64, 0:64]
+    for frame in (0, 1, 17):
+        got = _gpu_samples(ctx, 64, 64, frame)
+        want = orc.samples(64, 64, xs.ravel(), ys.ravel(), frame).reshape(64, 64, 3)
+        frac, ok = _frac_ok(got, want)
+        _say("frame %d: within tol %.5f (%d/%d), max |d| %.3g" % (frame, frac, ok.sum(), ok.size,
+                                                                 np.abs(got - want).max()))
+        assert frac >= 0.99
+
+
+def test_image_and_ray_counts_vs_fp64_oracle():
+    from tests.oracle_scene import OracleScene
+
+    d = _small()
+    ctx = scenes.to_context(d)
+    spp = 32
+    ctx.film_clear()
+    ctx.launch(64, 64, 0, spp)
+    img = ctx.film(64, 64)[..., :3]
+    c = ctx.counters()
+    ref, rc = OracleScene(d, fp64=True).render(64, 64, 0, spp)
+    ref = ref[..., :3] / ref[..., 3:4]
+    rel_rmse = np.sqrt(((img - ref) ** 2).mean()) / ref.mean()
+    _say("rel rmse vs fp64 oracle %.3g; counters gpu %s oracle %s" % (rel_rmse, c, rc))
+    assert c["paths"] == 64 * 64 * spp and c["nonfinite"] == 0
+    assert abs(c["ext_rays"] - rc["ext_rays"]) <= 0.005 * rc["ext_rays"]
+    assert abs(c["shadow_rays"] - rc["shadow_rays"]) <= 0.005 * rc["shadow_rays"]
+    assert rel_rmse < 2e-2
+
+
+def test_full_scene_samples_and_determinism():
+    """The benchmark scene (512x512, 327,680-triangle sphere, 1024x512 env)."""
+    from tests.oracle_scene import OracleScene
+
+    d = scenes.path_scene()
+    ctx = scenes.to_context(d)
+    st = ctx.accel_build()
+    _say("bvh", st)
+    assert st["tris"] == 327680 + 4 and st["max_depth"] < 48
+    ctx.film_clear()
+    ctx.launch(512, 512, 0, 4)
+    a = ctx.film_sums(512, 512)
+    ctx.film_clear()
+    ctx.launch(512, 512, 0, 4)
+    b = ctx.film_sums(512, 512)
+    assert np.array_equal(a, b)
+    assert np.isfinite(a).all() and np.all(a[..., 3] == 4)
+    c = ctx.counters()
+    assert c["paths"] == 512 * 512 * 4
+    rng = np.random.default_rng(5)
+    px, py = rng.integers(0, 512, 3000), rng.integers(0, 512, 3000)
+    got = _gpu_samples(ctx, 512, 512, 3)[py, px]
+    want = OracleScene(d).samples(512, 512, px, py, 3)
+    frac, _ = _frac_ok(got, want)
+    _say("full scene frame 3: within tol %.5f" % frac)
+    assert frac >= 0.99
