@@ -179,11 +179,138 @@ ORF_HD f3 ggx_sample_visible(f3 wi, float ax, float ay, float z1, float z2)
     float sy = sp * s11.x + cp * s11.y;
     sx *= ax;
     sy *= ay;
-    if (!isfinite(sx)) {
+    if (!__builtin_isfinite(sx)) {
         if (wi.z > 0.0f) return mk3(0.0f, 0.0f, 1.0f);
         return normalize(mk3(wi.x, wi.y, 0.0f));
     }
     return normalize(mk3(-sx, -sy, 1.0f));
+}
+
+// ---------------------------------------------------------------- Beckmann
+//
+// ORF/MicrofacetBeckmann.h:13-21 (alpha_i), :68-98 (G1 rational fit), :161-210
+// (visible-slope sampling by Newton-bisection in erf space), :213-231 (VNDF).
+// The reference evaluates erfinv in FP64 (Giles 2010 double-precision
+// polynomial, ORF/helpers.h:301-377) and a few sub-expressions in double; on the
+// GPU everything is FP32 with Giles' single-precision erfinv polynomial from the
+// same paper (max relative error 2.4e-7 against a library erfinv).
+
+constexpr float k2SqrtPi = 1.12837916709551257390f;
+
+ORF_HD float erfinv_f32(float x)
+{
+    float w = -logf((1.0f - x) * (1.0f + x)), p;
+    if (w < 5.0f) {
+        w = w - 2.5f;
+        p = 2.81022636e-08f;
+        p = 3.43273939e-07f + p * w;
+        p = -3.5233877e-06f + p * w;
+        p = -4.39150654e-06f + p * w;
+        p = 0.00021858087f + p * w;
+        p = -0.00125372503f + p * w;
+        p = -0.00417768164f + p * w;
+        p = 0.246640727f + p * w;
+        p = 1.50140941f + p * w;
+    } else {
+        w = sqrtf(w) - 3.0f;
+        p = -0.000200214257f;
+        p = 0.000100950558f + p * w;
+        p = 0.00134934322f + p * w;
+        p = -0.00367342844f + p * w;
+        p = 0.00573950773f + p * w;
+        p = -0.0076224613f + p * w;
+        p = 0.00943887047f + p * w;
+        p = 1.00167406f + p * w;
+        p = 2.83297682f + p * w;
+    }
+    return p * x;
+}
+
+ORF_HD float beckmann_smith_g1(float cos_theta, float a2)
+{
+    float c2 = cos_theta * cos_theta;
+    float t2 = (1.0f - c2) / c2;
+    float a_sqr = 1.0f / (a2 * t2);
+    float a = sqrtf(a_sqr);
+    return a < 1.6f ? (3.535f * a + 2.181f * a_sqr) / (1.0f + 2.276f * a + 2.577f * a_sqr) : 1.0f;
+}
+
+ORF_HD float beckmann_masking(f3 v, f3 m, f3 n, float ax, float ay)
+{
+    float width = ggx_alpha_proj(to_local(v, n), ax, ay);
+    float cv = dot(v, n);
+    if (dot(v, m) * cv <= 0.0f) return 0.0f;
+    return beckmann_smith_g1(cv, width * width);
+}
+
+ORF_HD f2 beckmann_visible_slope(float theta_i, float z1, float z2)
+{
+    if (theta_i < 0.0001f) {
+        float r = sqrtf(-logf(z1));
+        float sp, cp;
+        sincosf(kPi * 2.0f * z2, &sp, &cp);
+        return mk2(r * cp, r * sp);
+    }
+    float sin_i, cos_i;
+    sincosf(theta_i, &sin_i, &cos_i);
+    float slope_i = cos_i / sin_i;
+    float a = cos_i / sin_i;
+    float proj = 0.5f * (erff(a) + 1.0f) * cos_i + 0.25f * k2SqrtPi * sin_i * expf(-a * a);
+    if (proj < 0.0001f || proj != proj) return mk2(0.0f, 0.0f);
+    float c = 1.0f / proj;
+    float emin = -0.9999f;
+    float emax = fmaxf(emin, erff(slope_i));
+    float ecur = 0.5f * (emin + emax);
+    for (int it = 0; it < 256 && emax - emin > 0.00001f; ++it) {
+        if (!(ecur >= emin && ecur <= emax)) ecur = 0.5f * (emin + emax);
+        float s = erfinv_f32(ecur);
+        float cdf = (s >= slope_i) ? 1.0f
+                                   : c * (0.25f * k2SqrtPi * sin_i * expf(-s * s) +
+                                          cos_i * (0.5f + 0.5f * erff(s)));
+        float diff = cdf - z1;
+        if (fabsf(diff) < 0.00001f) break;
+        if (diff > 0.0f) {
+            if (emax == ecur) break;
+            emax = ecur;
+        } else {
+            if (emin == ecur) break;
+            emin = ecur;
+        }
+        float deriv = 0.5f * c * cos_i - 0.5f * c * sin_i * s;
+        ecur -= diff / deriv;
+    }
+    return mk2(erfinv_f32(fminf(emax, fmaxf(emin, ecur))), erfinv_f32(2.0f * z2 - 1.0f));
+}
+
+ORF_HD f3 beckmann_sample_visible(f3 wi, float ax, float ay, float z1, float z2)
+{
+    f3 w11 = normalize(mk3(ax * wi.x, ay * wi.y, wi.z));
+    f2 s11 = beckmann_visible_slope(acosf(w11.z), z1, z2);
+    float phi = atan2f(w11.y, w11.x);
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    float sx = cp * s11.x - sp * s11.y;
+    float sy = sp * s11.x + cp * s11.y;
+    sx *= ax;
+    sy *= ay;
+    if (!__builtin_isfinite(sx)) {
+        if (wi.z > 0.0f) return mk3(0.0f, 0.0f, 1.0f);
+        return normalize(mk3(wi.x, wi.y, 0.0f));
+    }
+    return normalize(mk3(-sx, -sy, 1.0f));
+}
+
+// microfacet_sample_visible_normal / masking_G1 dispatch (ORF/Microfacet.h:24-32, 72-83).
+ORF_HD f3 sample_visible_normal(f3 wi, f3 n, float ax, float ay, float z1, float z2, int dist)
+{
+    f3 l = to_local(wi, n);
+    f3 m = dist == 1 ? ggx_sample_visible(l, ax, ay, z1, z2) : beckmann_sample_visible(l, ax, ay, z1, z2);
+    return to_world(m, n);
+}
+
+ORF_HD float masking_g1(f3 v, f3 m, f3 n, float ax, float ay, int dist)
+{
+    return dist == 1 ? ggx_masking(v, m, n, ax, ay) : beckmann_masking(v, m, n, ax, ay);
 }
 
 }  // namespace orf

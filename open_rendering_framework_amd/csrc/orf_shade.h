@@ -166,6 +166,69 @@ ORF_HD void ggx_conductor_sample(f3 w_i, f3 n, float ax, float ay, float z1, flo
     weight = (g_i <= 0.0f || g_o <= 0.0f) ? mk3(0.0f) : F * g_o;
 }
 
+// Conductor step for either distribution (CU/metallic_shader.cu:84-158, visible
+// normals): weight F G1(w_o), 0 where the reference stops.
+ORF_HD void conductor_step(f3 w_i, f3 n, const MaterialGPU& m, float z1, float z2, f3& w_o, f3& weight)
+{
+    if (m.normal_distribution == GGX_DISTRIBUTION) {
+        float pdf;
+        ggx_conductor_sample(w_i, n, m.roughness.x, m.roughness.y, z1, z2, m.eta, w_o, pdf, weight);
+        return;
+    }
+    f3 mn = sample_visible_normal(w_i, n, m.roughness.x, m.roughness.y, z1, z2, BECKMANN_DISTRIBUTION);
+    float g_i = beckmann_masking(w_i, mn, n, m.roughness.x, m.roughness.y);
+    f3 F = fresnel_conductor(w_i, mn, m.eta);
+    w_o = -w_i + (mn * 2.0f) * dot(w_i, mn);
+    float g_o = beckmann_masking(w_o, mn, n, m.roughness.x, m.roughness.y);
+    weight = (g_i <= 0.0f || g_o <= 0.0f) ? mk3(0.0f) : F * g_o;
+}
+
+// fresnel_R(cos1, cos2, ior1/ior2) (ORF/Fresnel.h:26-32).
+ORF_HD float fresnel_R3(float c1, float c2, float eta)
+{
+    float rs = (eta * c1 - c2) / (eta * c1 + c2);
+    float rp = (c1 - eta * c2) / (c1 + eta * c2);
+    return (rs * rs + rp * rp) * 0.5f;
+}
+
+// One step of the rough dielectric (CU/rough_transparent_shader.cu:66-225,
+// visible-normal model): Beer absorption with Russian roulette when leaving the
+// medium, a visible microfacet normal, Fresnel-weighted choice between
+// reflection and refraction (get_rough_glass_rays, :39-55), weight G1(w_o).
+// Returns false where the reference returns 0; `factor` multiplies the throughput.
+ORF_HD bool dielectric_step(f3 d, float t_hit, f3 normal, f3 ffn, const MaterialGPU& m, uint32_t& rng,
+                            f3& new_dir, f3& factor)
+{
+    f3 w_i = -d;
+    f3 beam = mk3(1.0f);
+    if (dot(-d, normal) < 0.0f) {
+        f3 e = (-t_hit) * m.glass_absorption;
+        beam = mk3(expf(e.x), expf(e.y), expf(e.z));
+        float prob = (beam.x + beam.y + beam.z) / 3.0f;
+        if (rnd_tea(rng) >= prob) return false;
+        beam = beam / prob;
+    }
+    float z1 = rnd_tea(rng);
+    float z2 = rnd_tea(rng);
+    float a = m.roughness.x;
+    int dist = m.normal_distribution;
+    f3 mn = sample_visible_normal(w_i, ffn, a, a, z1, z2, dist);
+    float eta = dot(normal, w_i) < 0.0f ? m.ior : 1.0f / m.ior;
+    float g_i = masking_g1(w_i, mn, ffn, a, a, dist);
+    if (g_i <= 0.0f) return false;
+    float ct = fabsf(dot(mn, -d));
+    float st2 = eta * eta * (1.0f - ct * ct);
+    float ctt = sqrtf(1.0f - st2);
+    float R = st2 < 1.0f ? fresnel_R3(ct, ctt, eta) : 1.0f;
+    float rr = rnd_tea(rng);
+    f3 dir = rr > R ? eta * d + mn * (eta * ct - ctt) : d - (mn * 2.0f) * dot(mn, d);
+    float g_o = masking_g1(dir, mn, ffn, a, a, dist);
+    if (g_o <= 0.0f) return false;
+    new_dir = dir;
+    factor = beam * g_o;
+    return true;
+}
+
 // Power-heuristic weight of strategy a against b.
 ORF_HD float mis_power(float pa, float pb)
 {

@@ -50,7 +50,7 @@ struct RayPre {
     f3 o, d, inv, oinv;
 };
 
-__device__ __forceinline__ RayPre ray_pre(f3 o, f3 d)
+ORF_HD RayPre ray_pre(f3 o, f3 d)
 {
     RayPre r;
     r.o = o;
@@ -61,7 +61,7 @@ __device__ __forceinline__ RayPre ray_pre(f3 o, f3 d)
 }
 
 // Entry distance of the slab test, or +inf when the box is missed.
-__device__ __forceinline__ float box_enter(const RayPre& r, float lx, float ly, float lz, float hx, float hy,
+ORF_HD float box_enter(const RayPre& r, float lx, float ly, float lz, float hx, float hy,
                                            float hz, float tmin, float tmax)
 {
     float t0x = fmaf(lx, r.inv.x, -r.oinv.x), t1x = fmaf(hx, r.inv.x, -r.oinv.x);
@@ -75,7 +75,7 @@ __device__ __forceinline__ float box_enter(const RayPre& r, float lx, float ly, 
 
 // Closest hit (any_hit == false) or first hit (any_hit == true) in (tmin, tmax).
 template <bool ANY>
-__device__ __forceinline__ bool trace(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, Hit& h,
+ORF_HD bool trace(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, Hit& h,
                                       unsigned* stk, int stride)
 {
     RayPre r = ray_pre(o, d);
@@ -91,7 +91,7 @@ __device__ __forceinline__ bool trace(const SceneGPU& sc, f3 o, f3 d, float tmin
             float4 a = nd[0], b = nd[1], c = nd[2], l = nd[3];
             float e0 = box_enter(r, a.x, a.y, a.z, a.w, b.x, b.y, tmin, tbest);
             float e1 = box_enter(r, b.z, b.w, c.x, c.y, c.z, c.w, tmin, tbest);
-            unsigned l0 = __float_as_uint(l.x), l1 = __float_as_uint(l.y);
+            unsigned l0 = f2u(l.x), l1 = f2u(l.y);
             bool h0 = e0 != __builtin_huge_valf(), h1 = e1 != __builtin_huge_valf();
             if (h0 && h1) {
                 unsigned nearc = e0 <= e1 ? l0 : l1;
@@ -118,7 +118,7 @@ __device__ __forceinline__ bool trace(const SceneGPU& sc, f3 o, f3 d, float tmin
                 tbest = t;
                 hb = bb;
                 hg = gg;
-                prim = (int)__float_as_uint(q0.w);
+                prim = (int)f2u(q0.w);
                 leaf = (int)(first + k);
                 if (ANY) {
                     h.t = t;

@@ -416,3 +416,239 @@ int orc_ggx_conductor_query(v3 wo, R alpha, R u1, R u2, const cplx3 *eta,
     *weight = vscale(F, G_o);
     return 1;
 }
+
+/* ------------------------------------------------------------ erfinv (FP64) */
+
+/* erfinv (ORF/helpers.h:301-377): Giles (2010), "Approximating the erfinv
+ * function", double-precision branches w < 6.25, w < 16, else. */
+double orc_erfinv(double x)
+{
+    double w = -log((1.0 - x) * (1.0 + x)), p;
+    if (w < 6.25) {
+        w = w - 3.125;
+        p = -3.6444120640178196996e-21;
+        p = -1.685059138182016589e-19 + p * w;
+        p = 1.2858480715256400167e-18 + p * w;
+        p = 1.115787767802518096e-17 + p * w;
+        p = -1.333171662854620906e-16 + p * w;
+        p = 2.0972767875968561637e-17 + p * w;
+        p = 6.6376381343583238325e-15 + p * w;
+        p = -4.0545662729752068639e-14 + p * w;
+        p = -8.1519341976054721522e-14 + p * w;
+        p = 2.6335093153082322977e-12 + p * w;
+        p = -1.2975133253453532498e-11 + p * w;
+        p = -5.4154120542946279317e-11 + p * w;
+        p = 1.051212273321532285e-09 + p * w;
+        p = -4.1126339803469836976e-09 + p * w;
+        p = -2.9070369957882005086e-08 + p * w;
+        p = 4.2347877827932403518e-07 + p * w;
+        p = -1.3654692000834678645e-06 + p * w;
+        p = -1.3882523362786468719e-05 + p * w;
+        p = 0.0001867342080340571352 + p * w;
+        p = -0.00074070253416626697512 + p * w;
+        p = -0.0060336708714301490533 + p * w;
+        p = 0.24015818242558961693 + p * w;
+        p = 1.6536545626831027356 + p * w;
+    } else if (w < 16.0) {
+        w = sqrt(w) - 3.25;
+        p = 2.2137376921775787049e-09;
+        p = 9.0756561938885390979e-08 + p * w;
+        p = -2.7517406297064545428e-07 + p * w;
+        p = 1.8239629214389227755e-08 + p * w;
+        p = 1.5027403968909827627e-06 + p * w;
+        p = -4.013867526981545969e-06 + p * w;
+        p = 2.9234449089955446044e-06 + p * w;
+        p = 1.2475304481671778723e-05 + p * w;
+        p = -4.7318229009055733981e-05 + p * w;
+        p = 6.8284851459573175448e-05 + p * w;
+        p = 2.4031110387097893999e-05 + p * w;
+        p = -0.0003550375203628474796 + p * w;
+        p = 0.00095328937973738049703 + p * w;
+        p = -0.0016882755560235047313 + p * w;
+        p = 0.0024914420961078508066 + p * w;
+        p = -0.0037512085075692412107 + p * w;
+        p = 0.005370914553590063617 + p * w;
+        p = 1.0052589676941592334 + p * w;
+        p = 3.0838856104922207635 + p * w;
+    } else {
+        w = sqrt(w) - 5.0;
+        p = -2.7109920616438573243e-11;
+        p = -2.5556418169965252055e-10 + p * w;
+        p = 1.5076572693500548083e-09 + p * w;
+        p = -3.7894654401267369937e-09 + p * w;
+        p = 7.6157012080783393804e-09 + p * w;
+        p = -1.4960026627149240478e-08 + p * w;
+        p = 2.9147953450901080826e-08 + p * w;
+        p = -6.7711997758452339498e-08 + p * w;
+        p = 2.2900482228026654717e-07 + p * w;
+        p = -9.9298272942317002539e-07 + p * w;
+        p = 4.5260625972231537039e-06 + p * w;
+        p = -1.9681778105531670567e-05 + p * w;
+        p = 7.5995277030017761139e-05 + p * w;
+        p = -0.00021503011930044477347 + p * w;
+        p = -0.00013871931833623122026 + p * w;
+        p = 1.0103004648645343977 + p * w;
+        p = 4.8499064014085844221 + p * w;
+    }
+    return p * x;
+}
+
+/* ------------------------------------------------------------ Beckmann */
+
+#define M_2_SQRTPI_R ((R)1.12837916709551257390f)
+
+/* beckmann_alpha_i (ORF/MicrofacetBeckmann.h:13-21) equals ggx_alpha_i. */
+R orc_beckmann_alpha_i(v3 wi, R ax, R ay) { return orc_ggx_alpha_i(wi, ax, ay); }
+
+/* beckmann_G1(cos, alpha^2) (ORF/MicrofacetBeckmann.h:68-75): Walter et al. 2007
+ * rational fit of the Smith masking; the denominator's 2.577 is a double literal. */
+R orc_beckmann_G1_cos(R cos_theta, R alpha_sqr)
+{
+    R c2 = cos_theta * cos_theta;
+    R t2 = ((R)1 - c2) / c2;
+    R a_sqr = (R)1 / (alpha_sqr * t2);
+    R a = sqrt(a_sqr);
+    if (!(a < (R)1.6f))
+        return 1;
+    R num = (R)3.535f * a + (R)2.181f * a_sqr;
+    double den = (double)((R)1.0f + (R)2.276f * a) + 2.577 * (double)a_sqr;
+    return (R)((double)num / den);
+}
+
+/* beckmann_G1(v, m, n, ax, ay) (ORF/MicrofacetBeckmann.h:89-98). */
+R orc_beckmann_G1(v3 v, v3 m, v3 n, R ax, R ay)
+{
+    R width = orc_beckmann_alpha_i(orc_to_local(v, n), ax, ay);
+    R cos_v = vdot(v, n);
+    if (vdot(v, m) * cos_v <= 0)
+        return 0;
+    return orc_beckmann_G1_cos(cos_v, width * width);
+}
+
+/* beckmann_projected_area (ORF/MicrofacetBeckmann.h:45-58). */
+R orc_beckmann_projected_area(v3 wi, R ax, R ay)
+{
+    if (wi.z > (R)0.9999f)
+        return 1;
+    if (wi.z < (R)-0.9999f)
+        return 0;
+    R alpha = orc_beckmann_alpha_i(wi, ax, ay);
+    R theta = acos(wi.z);
+    R a = (R)1 / tan(theta) / alpha;
+    double t1 = (double)((R)0.5f * ((R)erf(a) + (R)1.0f) * wi.z);
+    double t2 = 0.25 * (double)M_2_SQRTPI_R * (double)alpha * (double)sin(theta) * (double)exp(-a * a);
+    return (R)(t1 + t2);
+}
+
+/* beckmann_slope_pdf + beckmann_eval_NDF + beckmann_eval_VNDF (ORF/MicrofacetBeckmann.h:61-65, 249-281). */
+R orc_beckmann_eval_ndf(v3 wm, R ax, R ay)
+{
+    if (wm.z <= 0)
+        return 0;
+    R sx = -wm.x / wm.z;
+    R sy = -wm.y / wm.z;
+    R p22 = (R)1 / (PI_R * ax * ay) * exp(-sx * sx / (ax * ax) - sy * sy / (ay * ay));
+    return p22 / (wm.z * wm.z * wm.z * wm.z);
+}
+
+R orc_beckmann_eval_vndf(v3 wi, v3 wm, R ax, R ay)
+{
+    if (wm.z <= 0)
+        return 0;
+    R area = orc_beckmann_projected_area(wi, ax, ay);
+    if (area == 0)
+        return 0;
+    R c = (R)1 / area;
+    R D = orc_beckmann_eval_ndf(wm, ax, ay);
+    R d = vdot(wi, wm);
+    return c * (d > 0 ? d : 0) * D;
+}
+
+/* beckmann_sample_P22_11 (ORF/MicrofacetBeckmann.h:161-210): Newton-bisection
+ * inversion of the visible-slope CDF in erf space (Heitz & d'Eon 2014).  The
+ * reference's loop has no iteration bound; `max_iter` (256) only guards
+ * against a non-terminating input and is never reached on finite inputs. */
+v2 orc_beckmann_sample_p22_11(R theta_i, R z1, R z2)
+{
+    v2 slope;
+    if (theta_i < (R)0.0001f) {
+        R r = sqrt(-log(z1));
+        R phi = PI_R * 2 * z2;
+        slope.x = r * cos(phi);
+        slope.y = r * sin(phi);
+        return slope;
+    }
+    R sin_i = sin(theta_i);
+    R cos_i = cos(theta_i);
+    R slope_i = cos_i / sin_i;
+    R a = cos_i / sin_i;
+    R proj_area = (R)0.5f * ((R)erf(a) + (R)1.0f) * cos_i + (R)0.25f * M_2_SQRTPI_R * sin_i * exp(-a * a);
+    if (proj_area < (R)0.0001f || proj_area != proj_area) {
+        slope.x = slope.y = 0;
+        return slope;
+    }
+    R c = (R)1 / proj_area;
+    R erf_min = (R)-0.9999f;
+    R e = (R)erf(slope_i);
+    R erf_max = erf_min < e ? e : erf_min; /* max(erf_min, erf(slope_i)) */
+    R erf_cur = (R)0.5f * (erf_min + erf_max);
+    for (int it = 0; it < 256 && erf_max - erf_min > (R)0.00001f; it++) {
+        if (!(erf_cur >= erf_min && erf_cur <= erf_max))
+            erf_cur = (R)0.5f * (erf_min + erf_max);
+        R s = (R)orc_erfinv((double)erf_cur);
+        R cdf;
+        if (s >= slope_i) {
+            cdf = 1;
+        } else {
+            double A = 0.25 * (double)M_2_SQRTPI_R * (double)sin_i * (double)exp(-s * s);
+            R B = cos_i * ((R)0.5f + (R)0.5f * (R)erf(s));
+            cdf = (R)((double)c * (A + (double)B));
+        }
+        R diff = cdf - z1;
+        if (fabs(diff) < (R)0.00001f)
+            break;
+        if (diff > 0) {
+            if (erf_max == erf_cur)
+                break;
+            erf_max = erf_cur;
+        } else {
+            if (erf_min == erf_cur)
+                break;
+            erf_min = erf_cur;
+        }
+        R deriv = (R)0.5f * c * cos_i - (R)0.5f * c * sin_i * s;
+        erf_cur -= diff / deriv;
+    }
+    R lo = erf_min < erf_cur ? erf_cur : erf_min;     /* max(erf_min, erf_cur) */
+    R cl = lo < erf_max ? lo : erf_max;               /* min(erf_max, .) */
+    slope.x = (R)orc_erfinv((double)cl);
+    slope.y = (R)orc_erfinv((double)((R)2.0f * z2 - (R)1.0f));
+    return slope;
+}
+
+/* beckmann_sample_VNDF (ORF/MicrofacetBeckmann.h:213-231). */
+v3 orc_beckmann_sample_vndf(v3 wi, R ax, R ay, R z1, R z2)
+{
+    v3 wi11 = orc_normalize(V3(ax * wi.x, ay * wi.y, wi.z));
+    v2 s11 = orc_beckmann_sample_p22_11(acos(wi11.z), z1, z2);
+    R phi = atan2(wi11.y, wi11.x);
+    R cp = cos(phi), sp = sin(phi);
+    R sx = cp * s11.x - sp * s11.y;
+    R sy = sp * s11.x + cp * s11.y;
+    sx *= ax;
+    sy *= ay;
+    if (!isfinite(sx) || sx != sx) {
+        if (wi.z > 0)
+            return V3(0, 0, 1);
+        return orc_normalize(V3(wi.x, wi.y, 0));
+    }
+    return orc_normalize(V3(-sx, -sy, (R)1));
+}
+
+/* fresnel_R(cos1, cos2, ior1/ior2) (ORF/Fresnel.h:26-32). */
+R orc_fresnel_R3(R cos1, R cos2, R eta)
+{
+    R rs = (eta * cos1 - cos2) / (eta * cos1 + cos2);
+    R rp = (cos1 - eta * cos2) / (cos1 + eta * cos2);
+    return (rs * rs + rp * rp) * (R)0.5f;
+}

@@ -587,6 +587,61 @@ static v3 shade_conductor(const oscene* s, const orc_material* m, const cplx3* e
     return vmul(child, weight);
 }
 
+/* masking_G1 / microfacet_sample_visible_normal dispatch (ORF/Microfacet.h:24-32, 72-83). */
+static R masking(v3 v, v3 m, v3 n, R a, int dist)
+{
+    return dist == 1 ? orc_ggx_G1(v, m, n, a, a) : orc_beckmann_G1(v, m, n, a, a);
+}
+
+static v3 sample_visible(v3 w_i, v3 n, R a, R z1, R z2, int dist)
+{
+    v3 l = orc_to_local(w_i, n);
+    v3 m = dist == 1 ? orc_ggx_sample_vndf(l, a, a, z1, z2) : orc_beckmann_sample_vndf(l, a, a, z1, z2);
+    return orc_to_world(m, n);
+}
+
+/* CU/rough_transparent_shader.cu:66-225, visible-normal model, with
+ * get_rough_glass_rays (:39-55). */
+static v3 shade_dielectric(const oscene* s, const orc_material* m, v3 o, v3 d, const ohit* h, int depth,
+                           uint32_t* t, ostats* st)
+{
+    v3 zero = V3(0, 0, 0);
+    if (depth >= s->max_depth) return zero;
+    v3 hit_point = vadd(o, vscale(d, h->t));
+    v3 normal = hit_normal(s, h);
+    v3 ffn = orc_faceforward(normal, vneg(d), normal);
+    v3 w_i = vneg(d);
+    v3 beam = V3(1, 1, 1);
+    R cos_theta = vdot(vneg(d), normal);
+    if (cos_theta < 0) {
+        beam = V3(exp(-h->t * (R)m->absorption[0]), exp(-h->t * (R)m->absorption[1]), exp(-h->t * (R)m->absorption[2]));
+        R prob = (beam.x + beam.y + beam.z) / (R)3.0f;
+        if (orc_rnd_tea(t) >= prob) return zero;
+        beam = V3(beam.x / prob, beam.y / prob, beam.z / prob);
+    }
+    R z1 = orc_rnd_tea(t);
+    R z2 = orc_rnd_tea(t);
+    R a = m->roughness[0];
+    int dist = m->normal_distribution;
+    v3 mn = sample_visible(w_i, ffn, a, z1, z2, dist);
+    R eta = vdot(normal, w_i) < 0 ? (R)m->ior : (R)1.0f / (R)m->ior;
+    R G_i = masking(w_i, mn, ffn, a, dist);
+    if (G_i <= 0) return zero;
+    R ct = fabs(vdot(mn, vneg(d)));
+    R st2 = eta * eta * ((R)1.0f - ct * ct);
+    R ctt = sqrt((R)1.0f - st2);
+    R Rf = st2 < (R)1.0f ? orc_fresnel_R3(ct, ctt, eta) : (R)1.0f;
+    v3 refr = vadd(vscale(d, eta), vscale(mn, eta * ct - ctt));
+    v3 refl = vsub(d, vscale(vscale(mn, (R)2.0f), vdot(mn, d)));
+    R rr = orc_rnd_tea(t);
+    v3 dir = rr > Rf ? refr : refl;
+    R G_o = masking(dir, mn, ffn, a, dist);
+    if (G_o <= 0) return zero;
+    v3 child = radiance(s, hit_point, dir, depth + 1, 1, t, (R)-1, st);
+    v3 result = vscale(child, G_o);
+    return vmul(result, beam);
+}
+
 static v3 radiance(const oscene* s, v3 o, v3 d, int depth, int emit_light, uint32_t* seed, R prev_pdf, ostats* st)
 {
     ohit h;
@@ -606,6 +661,7 @@ static v3 radiance(const oscene* s, v3 o, v3 d, int depth, int emit_light, uint3
     }
     if (m->kind == ORC_LAMBERT) return shade_lambert(s, m, o, d, &h, depth, &emit_light, seed, st);
     if (m->kind == ORC_CONDUCTOR) return shade_conductor(s, m, &s->eta[s->tri_mat[h.prim]], o, d, &h, depth, seed, st);
+    if (m->kind == ORC_DIELECTRIC) return shade_dielectric(s, m, o, d, &h, depth, seed, st);
     return V3(0, 0, 0);
 }
 

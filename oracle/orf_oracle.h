@@ -86,6 +86,17 @@ cplx orc_cdiv_fc(R c1, cplx c2);
 v3 orc_fresnel_conductor(v3 w_i, v3 w_m, const cplx3 *eta);
 R orc_fresnel_dielectric(R cos_theta, R ior1_over_ior2);
 
+double orc_erfinv(double x);
+R orc_beckmann_alpha_i(v3 wi, R ax, R ay);
+R orc_beckmann_G1_cos(R cos_theta, R alpha_sqr);
+R orc_beckmann_G1(v3 v, v3 m, v3 n, R ax, R ay);
+R orc_beckmann_projected_area(v3 wi, R ax, R ay);
+R orc_beckmann_eval_ndf(v3 wm, R ax, R ay);
+R orc_beckmann_eval_vndf(v3 wi, v3 wm, R ax, R ay);
+v2 orc_beckmann_sample_p22_11(R theta_i, R z1, R z2);
+v3 orc_beckmann_sample_vndf(v3 wi, R ax, R ay, R z1, R z2);
+R orc_fresnel_R3(R cos1, R cos2, R eta);
+
 /* One query of the rough-conductor operator (CU/metallic_shader.cu:84-158,
  * visible-normal branch, GGX, surface normal (0,0,1)).  Returns 1 when the
  * path continues, 0 when the reference would stop (G1 <= 0). */
