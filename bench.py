@@ -158,6 +158,8 @@ def bench_path(args):
         "bvh": stats,
         "nonfinite_samples": c["nonfinite"],
     }
+    if not args.no_batch:
+        line["batched"] = batch_measure(1, 1)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_path(desc)
     return line
@@ -181,7 +183,55 @@ def cpu_baseline_path(desc):
                       "OpenMP %d threads, %s)" % (frames, threads, _cpu_model())}
 
 
-WORKLOADS = {"path": bench_path, "bsdf": bench_bsdf}
+# --------------------------------------------------------------------- batch
+def batch_measure(steps, warmup, spp_scale=1.0, size_scale=1.0):
+    """64 uneven renders of the scene family: autotuned wavefront vs one
+    megakernel launch per render (same scene, seeds and material tables)."""
+    import torch
+
+    from open_rendering_framework_amd import rt, scenes
+
+    desc, items, remap = scenes.batch_workload(spp_scale=spp_scale, size_scale=size_scale)
+    ctx = scenes.to_context(desc)
+    ctx.accel_build()
+    paths = sum(it["width"] * it["height"] * it["spp"] for it in items)
+    best, report = ctx.autotune(items, remap, spp_divisor=16)
+    for _ in range(max(1, warmup)):
+        ctx.render_batch(items, remap, mode=rt.BATCH_WAVEFRONT, config=best)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.render_batch(items, remap, mode=rt.BATCH_WAVEFRONT, config=best)
+    torch.cuda.synchronize()
+    wf = (time.perf_counter() - t0) / steps
+    c = ctx.batch_counters()
+    mk = ctx.render_batch(items, remap, mode=rt.BATCH_MEGAKERNEL)
+    torch.cuda.synchronize()
+    return {
+        "renders": len(items), "paths": paths,
+        "wavefront_mpaths_s": paths / wf / 1e6, "wavefront_s": wf,
+        "per_render_megakernel_mpaths_s": paths / mk / 1e6, "per_render_megakernel_s": mk,
+        "speedup_vs_per_render_megakernel": mk / wf,
+        "config": best.as_dict(), "autotune": report,
+        "ext_rays_per_path": c["ext_rays"] / paths, "shadow_rays_per_path": c["shadow_rays"] / paths,
+    }
+
+
+def bench_batch(args):
+    b = batch_measure(args.steps, args.warmup)
+    line = {
+        "metric": "Mpaths/s, 64 uneven renders in one batched wavefront pass",
+        "value": b["wavefront_mpaths_s"],
+        "unit": "Mpaths/s",
+        "ms_per_step": b["wavefront_s"] * 1e3,
+        "dtype": "f32",
+        "config": {"workload": "batched_wavefront_64_renders", "renders": b["renders"], "paths": b["paths"]},
+        "batched": b,
+    }
+    return line
+
+
+WORKLOADS = {"path": bench_path, "bsdf": bench_bsdf, "batch": bench_batch}
 
 
 def main():
@@ -191,6 +241,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="path", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-wavefront measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "

@@ -164,6 +164,53 @@ orf_status orf_film_read_sums(orf_context ctx, double* sums);
  * [1] extension (closest-hit) rays, [2] shadow rays, [3] paths. */
 orf_status orf_counters(orf_context ctx, uint64_t out[4]);
 
+/* ------------------------------------------------------------------------
+ * Batched renders: many uneven renders of the context's scene in one call.
+ *
+ * No counterpart in the reference, which launches one integrator pass per
+ * frame (ORF/OptixScene.cpp:139-159); each item here is that pass for one
+ * camera/film, sample s of item i seeded tea<16>(width*y + x, frame0 + s).
+ * remap (n x remap_stride ints, NULL = identity) gives each item its own
+ * material table: geometry material id m shades as material remap[i][m].
+ * ------------------------------------------------------------------------ */
+typedef struct orf_render_item {
+    int32_t width, height;
+    uint32_t spp, frame0;
+    float eye[3], U[3], V[3], W[3];
+} orf_render_item;
+
+/* Wavefront parameters (the autotuner's search space). */
+typedef struct orf_wavefront_config {
+    int32_t block_size;     /* traversal workgroup size: 64, 128 or 256 */
+    int32_t pool_size;      /* path slots in flight (multiple of 256) */
+    int32_t sort_policy;    /* 1: per-material shade queues, 0: one shade kernel */
+    float regen_threshold;  /* refill free slots when continuing paths < threshold * pool */
+    int32_t use_graph;      /* 1: bounce loop replayed as a hipGraph */
+    int32_t graph_pairs;    /* bounce-iteration pairs per graph replay */
+} orf_wavefront_config;
+
+enum { ORF_BATCH_WAVEFRONT = 0, ORF_BATCH_MEGAKERNEL = 1 };
+
+orf_status orf_wavefront_default_config(orf_wavefront_config* cfg);
+/* mode ORF_BATCH_WAVEFRONT: all items in one wavefront pass; ORF_BATCH_MEGAKERNEL:
+ * one megakernel launch per item.  cfg NULL = defaults.  *seconds (may be NULL)
+ * receives the wall time from the first launch to completion. */
+orf_status orf_render_batch(orf_context ctx, const orf_render_item* items, int32_t n, const int32_t* remap,
+                            int32_t remap_stride, int32_t mode, const orf_wavefront_config* cfg, void* stream,
+                            double* seconds);
+/* Mean RGBA (A = 1) of item `item` of the last batch, FP64, 4*width*height, row 0 = bottom. */
+orf_status orf_batch_film_read(orf_context ctx, int32_t item, double* rgba);
+/* Raw FP64 sums (R, G, B, count) of item `item`. */
+orf_status orf_batch_film_sums(orf_context ctx, int32_t item, double* sums);
+/* [0] non-finite samples, [1] extension rays, [2] shadow rays, [3] paths of the last batch. */
+orf_status orf_batch_counters(orf_context ctx, uint64_t out[4]);
+/* Coordinate search over block size, pool size, sort policy and refill threshold,
+ * timing the wavefront on the batch with every spp divided by spp_divisor.
+ * Writes the fastest configuration and a JSON report of all trials. */
+orf_status orf_wavefront_autotune(orf_context ctx, const orf_render_item* items, int32_t n, const int32_t* remap,
+                                  int32_t remap_stride, uint32_t spp_divisor, orf_wavefront_config* best,
+                                  char* report, size_t report_size);
+
 /* Procedural scene pieces (host).  Icosphere: call with verts = tris = normals =
  * NULL to get the sizes.  Env: sun_dir in env coordinates, sigma in degrees. */
 orf_status orf_gen_icosphere(int level, uint32_t noise_seed, float amplitude, float frequency,

@@ -16,23 +16,6 @@
 
 namespace orf {
 
-// eta = 1 / ior as the reference forms it (operator/(float, MyComplex),
-// ORF/MyComplex.h:155-160: conj(ior) / |ior|^2 with |ior| rounded to float).
-void conductor_eta_host(const double ior_re[3], const double ior_im[3], ConductorEta& e)
-{
-    for (int c = 0; c < 3; ++c) {
-        float r = (float)ior_re[c], i = (float)ior_im[c];
-        float ab = sqrtf(r * r + i * i);
-        double den = (double)powf(ab, 2.0f);
-        double er = ior_re[c] * (1.0 / den);
-        double ei = -ior_im[c] * (1.0 / den);
-        e.eta[c] = cf{(float)er, (float)ei};
-        e.eta2[c] = cf{(float)(er * er - ei * ei), (float)(2.0 * er * ei)};
-    }
-    e.gray = (ior_re[0] == ior_re[1] && ior_re[1] == ior_re[2] && ior_im[0] == ior_im[1] &&
-              ior_im[1] == ior_im[2]);
-}
-
 __global__ __launch_bounds__(256) void k_ggx_conductor(int64_t n, const float* __restrict__ wx,
                                                        const float* __restrict__ wy,
                                                        const float* __restrict__ wz,

@@ -13,86 +13,16 @@
 #include <string>
 #include <vector>
 
-#include "orf_bvh.h"
 #include "orf_common.h"
-#include "orf_render.h"
+#include "orf_context.h"
 
 namespace orf {
-
-struct DevBuf {
-    void* p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    void ensure(size_t n)
-    {
-        if (n <= bytes && p) return;
-        release();
-        if (n == 0) n = 16;
-        ORF_HIP_CHECK(hipMalloc(&p, n));
-        bytes = n;
-    }
-    template <class T>
-    void upload(const std::vector<T>& v)
-    {
-        ensure(v.size() * sizeof(T));
-        if (!v.empty()) ORF_HIP_CHECK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    }
-    template <class T>
-    T* as() const
-    {
-        return (T*)p;
-    }
-};
-
-struct MeshRec {
-    std::vector<float> v, n;
-    std::vector<int32_t> vi, ni;
-    int material = 0;
-    float xf[16];
-    bool has_xf = false;
-};
 
 }  // namespace orf
 
-struct orf_context_t {
-    int device = 0;
-    // context variables (defaults of ORF/Integrator.cpp:6-13)
-    int max_depth = 50;
-    float scene_epsilon = 1e-4f;
-    orf::CameraGPU cam{};
-    bool patch_set = false;
-    unsigned frame = 0;
-    float exception_color[3] = {1.0f, 0.0f, 0.0f};
-    // scene records
-    std::vector<orf::MaterialGPU> mats;
-    std::vector<orf::MeshRec> meshes;
-    std::vector<orf::LightGPU> lights;
-    std::vector<orf::TriLightGPU> tri_lights;
-    std::vector<float4> env_tex;
-    int env_w = 0, env_h = 0;
-    bool env_is = false;
-    bool dirty_geo = true, dirty_mat = true, dirty_light = true, dirty_env = true;
-    orf::BvhStats bvh_stats;
-    // device copies
-    orf::DevBuf d_bvh, d_tri, d_shade, d_normals, d_mats, d_lights, d_trilights;
-    orf::DevBuf d_env, d_mpdf, d_mcdf, d_cpdf, d_ccdf, d_envscratch;
-    orf::DevBuf d_film, d_counters;
-    int film_w = 0, film_h = 0;
-    int nlights_dev = 0;
-};
-
 namespace orf {
 
-static orf_context_t* ctx_check(orf_context ctx)
+orf_context_t* ctx_check(orf_context ctx)
 {
     ORF_REQUIRE(ctx != nullptr, "null context");
     ORF_HIP_CHECK(hipSetDevice(ctx->device));
@@ -309,7 +239,7 @@ static void upload_lights(orf_context_t* c)
     c->dirty_light = false;
 }
 
-static void prepare(orf_context_t* c, hipStream_t stream)
+void prepare(orf_context_t* c, hipStream_t stream)
 {
     if (c->dirty_geo) build_geometry(c, stream);
     if (c->dirty_mat) {
@@ -322,7 +252,7 @@ static void prepare(orf_context_t* c, hipStream_t stream)
     if (c->dirty_light) upload_lights(c);
 }
 
-static SceneGPU scene_view(orf_context_t* c)
+SceneGPU scene_view(orf_context_t* c)
 {
     SceneGPU s;
     std::memset(&s, 0, sizeof(s));
@@ -650,7 +580,7 @@ extern "C" orf_status orf_launch(orf_context ctx, int width, int height, uint32_
         cam.patch_dims[1] = (unsigned)height;
     }
     orf::FilmGPU film{c->d_film.as<double>(), c->d_counters.as<unsigned long long>(), width, height};
-    orf::launch_megakernel(orf::scene_view(c), cam, film, frame0, spp, st);
+    orf::launch_megakernel(orf::scene_view(c), cam, film, nullptr, frame0, spp, st);
     ORF_API_END
 }
 

@@ -1,0 +1,63 @@
+// orf_context.h -- the render context behind the C-ABI handle (internal).
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "orf_bvh.h"
+#include "orf_devbuf.h"
+#include "orf_render.h"
+#include "orf_wavefront_host.h"
+
+namespace orf {
+
+struct MeshRec {
+    std::vector<float> v, n;
+    std::vector<int32_t> vi, ni;
+    int material = 0;
+    float xf[16];
+    bool has_xf = false;
+};
+
+}  // namespace orf
+
+struct orf_context_t {
+    int device = 0;
+    // context variables (defaults of ORF/Integrator.cpp:6-13)
+    int max_depth = 50;
+    float scene_epsilon = 1e-4f;
+    orf::CameraGPU cam{};
+    bool patch_set = false;
+    unsigned frame = 0;
+    float exception_color[3] = {1.0f, 0.0f, 0.0f};
+    // scene records
+    std::vector<orf::MaterialGPU> mats;
+    std::vector<orf::MeshRec> meshes;
+    std::vector<orf::LightGPU> lights;
+    std::vector<orf::TriLightGPU> tri_lights;
+    std::vector<float4> env_tex;
+    int env_w = 0, env_h = 0;
+    bool env_is = false;
+    bool dirty_geo = true, dirty_mat = true, dirty_light = true, dirty_env = true;
+    orf::BvhStats bvh_stats;
+    // device copies
+    orf::DevBuf d_bvh, d_tri, d_shade, d_normals, d_mats, d_lights, d_trilights;
+    orf::DevBuf d_env, d_mpdf, d_mcdf, d_cpdf, d_ccdf, d_envscratch;
+    orf::DevBuf d_film, d_counters;
+    int film_w = 0, film_h = 0;
+    int nlights_dev = 0;
+    // batched renders
+    orf::DevBuf d_batch_renders, d_batch_remap, d_batch_film, d_batch_stats;
+    std::vector<orf::WfRender> batch_renders;
+    long long batch_pixels = 0;
+    orf::WavefrontPool wf_pool;
+    double last_batch_seconds = 0.0;
+    long long last_batch_iterations = 0;
+};
+
+namespace orf {
+orf_context_t* ctx_check(orf_context ctx);
+void prepare(orf_context_t* c, hipStream_t stream);
+SceneGPU scene_view(orf_context_t* c);
+}  // namespace orf
+

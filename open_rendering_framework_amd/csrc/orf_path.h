@@ -1,4 +1,5 @@
-// orf_path.h -- one camera path of the integrator pass (device; also host-callable).
+// orf_path.h -- the integrator's per-vertex shading step and the camera path
+// loop built on it (device; also host-callable).
 //
 // The recursion of the reference's closest-hit programs (rtTrace from inside
 // CU/diffuse_shader.cu, CU/metallic_shader.cu, CU/rough_transparent_shader.cu)
@@ -9,6 +10,10 @@
 // (CU/metallic_shader.cu:47, CU/rough_transparent_shader.cu:68); emitters and the
 // environment answer at any depth.  emit_light follows CU/path_tracer.cu:46,
 // CU/diffuse_shader.cu:160,176 and CU/metallic_shader.cu:127.
+//
+// shade_vertex() is the one place the shading arithmetic lives: the megakernel
+// calls it inside its loop, the wavefront kernels call it on queued hits, so
+// both forms produce bit-identical per-sample radiance.
 #pragma once
 
 #include "orf_shade.h"
@@ -18,6 +23,23 @@ namespace orf {
 
 struct PathStats {
     unsigned ext = 0, shd = 0;
+};
+
+struct PathState {
+    f3 o, d;             // current ray
+    f3 thr;              // throughput
+    f3 L;                // radiance gathered so far
+    uint32_t rng;        // TEA state (prd.seed)
+    int depth;           // prd.depth
+    int emit;            // prd.emit_light
+    float prev_bsdf_pdf; // > 0: the ray left a Lambert vertex that sampled lights (env MIS)
+};
+
+struct ShadowRay {
+    f3 o, d;
+    float tmax;
+    f3 contrib;  // added to L when the segment is unoccluded
+    bool valid;
 };
 
 // Camera ray of pixel (x, y) of a w x h film for sample `frame`
@@ -33,100 +55,140 @@ ORF_HD uint32_t camera_ray(const CameraGPU& cam, int w, int h, int x, int y, uns
     return rng;
 }
 
-ORF_HD f3 path_radiance(const SceneGPU& sc, f3 o, f3 d, uint32_t& rng, unsigned* stk, int stride, PathStats& st)
+ORF_HD void path_begin(PathState& ps, f3 eye, f3 dir, uint32_t rng)
 {
-    f3 L = mk3(0.0f), thr = mk3(1.0f);
-    int depth = 0, emit = 1;
-    float prev_bsdf_pdf = -1.0f;  // > 0: the ray left a Lambert vertex that sampled lights (env MIS)
+    ps.o = eye;
+    ps.d = dir;
+    ps.thr = mk3(1.0f);
+    ps.L = mk3(0.0f);
+    ps.rng = rng;
+    ps.depth = 0;
+    ps.emit = 1;
+    ps.prev_bsdf_pdf = -1.0f;
+}
+
+// Material of a hit through the per-render material table (identity when null).
+ORF_HD const MaterialGPU& hit_material(const SceneGPU& sc, const int* remap, int id)
+{
+    return sc.materials[remap ? remap[id] : id];
+}
+
+// Environment answer of a ray that left the scene (CU/envmap_background.cu:20-23 + MIS).
+ORF_HD void shade_miss(const SceneGPU& sc, PathState& ps)
+{
+    if (!sc.has_env) return;
+    f3 E = env_lookup(sc.env, ps.d);
+    if (ps.prev_bsdf_pdf > 0.0f && sc.env.sampled)
+        E = E * mis_power(ps.prev_bsdf_pdf, env_pdf(sc.env, ps.d) / (float)sc.nlights);
+    ps.L += ps.thr * E;
+}
+
+// Shade one surface vertex.  Returns true when the path continues with ps.o/ps.d
+// as its next ray; `sr` receives the NEE shadow segment, if any.
+ORF_HD bool shade_vertex(const SceneGPU& sc, const int* remap, PathState& ps, const Hit& h, ShadowRay& sr)
+{
+    sr.valid = false;
     const float eps = sc.scene_epsilon;
+    f3 d = ps.d;
+    f3 p = ps.o + h.t * d;
+    const float4* tg = sc.tri + 3 * (size_t)h.leaf;
+    float4 q1 = tg[1], q2 = tg[2];
+    int4 sh;
+    f3 normal = shading_normal(sc, h.prim, h.b, h.g, mk3(q1.x, q1.y, q1.z), mk3(q2.x, q2.y, q2.z), sh);
+    f3 ffn = faceforward(normal, -d, normal);
+    const MaterialGPU& mat = hit_material(sc, remap, sh.w);
+    if (mat.kind == MAT_EMITTER) {
+        if (dot(normal, -d) > 0.0f && ps.emit) ps.L += ps.thr * mat.radiance;
+        return false;
+    }
+    if (mat.kind == MAT_LAMBERT) {
+        if (ps.depth > sc.max_depth) return false;
+        f3 rho = mat.diffuse_color;
+        unsigned li = (unsigned)((float)sc.nlights * rnd_tea(ps.rng));
+        if (sc.nlights > 0) {
+            if (li >= (unsigned)sc.nlights) li = sc.nlights - 1;
+            float light_pdf = 1.0f / (float)sc.nlights;
+            LightGPU lt = sc.lights[li];
+            f3 wl = mk3(0.0f), rad = mk3(0.0f);
+            float dist = 0.0f, penv;
+            sample_light(sc, lt, p, ps.rng, wl, rad, dist, penv);
+            rad = rad / light_pdf;
+            float cs = dot(ffn, wl);
+            if (cs > 0.0f) {
+                f3 c = rad * kInvPi * cs * rho;
+                if (penv >= 0.0f) c = c * mis_power(penv * light_pdf, cs * kInvPi);
+                sr.o = p;
+                sr.d = wl;
+                sr.tmax = dist - eps;
+                sr.contrib = ps.thr * c;
+                sr.valid = true;
+                ps.emit = 0;
+            }
+        }
+        float prob = (rho.x + rho.y + rho.z) / 3.0f;
+        float xi = rnd_tea(ps.rng);
+        if (!(xi < prob)) return false;
+        f3 nd = sample_cosine(ffn, ps.rng);
+        ps.prev_bsdf_pdf = sc.env.sampled ? dot(ffn, nd) * kInvPi : -1.0f;
+        ps.thr = ps.thr * (rho / prob);
+        ps.o = p;
+        ps.d = nd;
+        ps.depth++;
+        return true;
+    }
+    if (mat.kind == MAT_CONDUCTOR) {
+        if (ps.depth >= sc.max_depth) return false;
+        float z1 = rnd_tea(ps.rng);
+        float z2 = rnd_tea(ps.rng);
+        f3 wo, w;
+        conductor_step(-d, ffn, mat, z1, z2, wo, w);
+        if (w.x == 0.0f && w.y == 0.0f && w.z == 0.0f) return false;
+        ps.thr = ps.thr * w;
+        ps.o = p;
+        ps.d = wo;
+        ps.depth++;
+        ps.emit = 1;
+        ps.prev_bsdf_pdf = -1.0f;
+        return true;
+    }
+    if (mat.kind == MAT_DIELECTRIC) {
+        if (ps.depth >= sc.max_depth) return false;
+        f3 nd, f;
+        if (!dielectric_step(d, h.t, normal, ffn, mat, ps.rng, nd, f)) return false;
+        ps.thr = ps.thr * f;
+        ps.o = p;
+        ps.d = nd;
+        ps.depth++;
+        ps.emit = 1;
+        ps.prev_bsdf_pdf = -1.0f;
+        return true;
+    }
+    return false;  // materials without a radiance program here
+}
+
+// The whole path of one camera sample (megakernel form).
+ORF_HD f3 path_radiance(const SceneGPU& sc, const int* remap, f3 o, f3 d, uint32_t rng, unsigned* stk, int stride,
+                        PathStats& st)
+{
+    PathState ps;
+    path_begin(ps, o, d, rng);
     for (;;) {
         Hit h;
         st.ext++;
-        if (!trace<false>(sc, o, d, eps, kRayMax, h, stk, stride)) {
-            if (sc.has_env) {
-                f3 E = env_lookup(sc.env, d);
-                if (prev_bsdf_pdf > 0.0f && sc.env.sampled)
-                    E = E * mis_power(prev_bsdf_pdf, env_pdf(sc.env, d) / (float)sc.nlights);
-                L += thr * E;
-            }
+        if (!trace<false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk, stride)) {
+            shade_miss(sc, ps);
             break;
         }
-        f3 p = o + h.t * d;
-        const float4* tg = sc.tri + 3 * (size_t)h.leaf;
-        float4 q1 = tg[1], q2 = tg[2];
-        int4 sh;
-        f3 normal = shading_normal(sc, h.prim, h.b, h.g, mk3(q1.x, q1.y, q1.z), mk3(q2.x, q2.y, q2.z), sh);
-        f3 ffn = faceforward(normal, -d, normal);
-        const MaterialGPU& mat = sc.materials[sh.w];
-        if (mat.kind == MAT_EMITTER) {
-            if (dot(normal, -d) > 0.0f && emit) L += thr * mat.radiance;
-            break;
+        ShadowRay sr;
+        bool more = shade_vertex(sc, remap, ps, h, sr);
+        if (sr.valid) {
+            Hit hs;
+            st.shd++;
+            if (!trace<true>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) ps.L += sr.contrib;
         }
-        if (mat.kind == MAT_LAMBERT) {
-            if (depth > sc.max_depth) break;
-            f3 rho = mat.diffuse_color;
-            unsigned li = (unsigned)((float)sc.nlights * rnd_tea(rng));
-            if (sc.nlights > 0) {
-                if (li >= (unsigned)sc.nlights) li = sc.nlights - 1;
-                float light_pdf = 1.0f / (float)sc.nlights;
-                LightGPU lt = sc.lights[li];
-                f3 wl = mk3(0.0f), rad = mk3(0.0f);
-                float dist = 0.0f, penv;
-                sample_light(sc, lt, p, rng, wl, rad, dist, penv);
-                rad = rad / light_pdf;
-                float cs = dot(ffn, wl);
-                if (cs > 0.0f) {
-                    Hit hs;
-                    st.shd++;
-                    bool occ = trace<true>(sc, p, wl, eps, dist - eps, hs, stk, stride);
-                    f3 Li = rad * (occ ? 0.0f : 1.0f);
-                    f3 c = Li * kInvPi * cs * rho;
-                    if (penv >= 0.0f) c = c * mis_power(penv * light_pdf, cs * kInvPi);
-                    L += thr * c;
-                    emit = 0;
-                }
-            }
-            float prob = (rho.x + rho.y + rho.z) / 3.0f;
-            float xi = rnd_tea(rng);
-            if (!(xi < prob)) break;
-            f3 nd = sample_cosine(ffn, rng);
-            prev_bsdf_pdf = sc.env.sampled ? dot(ffn, nd) * kInvPi : -1.0f;
-            thr = thr * (rho / prob);
-            o = p;
-            d = nd;
-            depth++;
-            continue;
-        }
-        if (mat.kind == MAT_CONDUCTOR) {
-            if (depth >= sc.max_depth) break;
-            float z1 = rnd_tea(rng);
-            float z2 = rnd_tea(rng);
-            f3 wo, w;
-            conductor_step(-d, ffn, mat, z1, z2, wo, w);
-            if (w.x == 0.0f && w.y == 0.0f && w.z == 0.0f) break;
-            thr = thr * w;
-            o = p;
-            d = wo;
-            depth++;
-            emit = 1;
-            prev_bsdf_pdf = -1.0f;
-            continue;
-        }
-        if (mat.kind == MAT_DIELECTRIC) {
-            if (depth >= sc.max_depth) break;
-            f3 nd, f;
-            if (!dielectric_step(d, h.t, normal, ffn, mat, rng, nd, f)) break;
-            thr = thr * f;
-            o = p;
-            d = nd;
-            depth++;
-            emit = 1;
-            prev_bsdf_pdf = -1.0f;
-            continue;
-        }
-        break;  // materials without a radiance program in this kernel
+        if (!more) break;
     }
-    return L;
+    return ps.L;
 }
 
 }  // namespace orf

@@ -14,8 +14,9 @@ struct FilmGPU {
 };
 
 // Megakernel: every (pixel, sample) path of frames frame0 .. frame0+spp-1.
-void launch_megakernel(const SceneGPU& sc, const CameraGPU& cam, const FilmGPU& film, unsigned frame0,
-                       unsigned spp, hipStream_t stream);
+// remap: per-render material table (device, nullptr = identity).
+void launch_megakernel(const SceneGPU& sc, const CameraGPU& cam, const FilmGPU& film, const int* remap,
+                       unsigned frame0, unsigned spp, hipStream_t stream);
 
 // FP64 construction of the environment's FP32 pdf/cdf tables (spec CU/env_cameras.cu).
 // scratch: (h + 1) doubles.

@@ -15,8 +15,8 @@ namespace orf {
 constexpr int kMegaBlock = 64;
 
 __global__ __launch_bounds__(kMegaBlock) void k_render_mega(SceneGPU sc, CameraGPU cam, FilmGPU film,
-                                                            unsigned frame0, unsigned spp, int chunks,
-                                                            unsigned per, int tiles_x)
+                                                            const int* remap, unsigned frame0, unsigned spp,
+                                                            int chunks, unsigned per, int tiles_x)
 {
     __shared__ unsigned stack_lds[kStackDepth * kMegaBlock];
     unsigned* stk = stack_lds + threadIdx.x;
@@ -37,7 +37,7 @@ __global__ __launch_bounds__(kMegaBlock) void k_render_mega(SceneGPU sc, CameraG
         if (inside) {
             f3 dir;
             uint32_t rng = camera_ray(cam, film.width, film.height, x, y, s, dir);
-            L = path_radiance(sc, cam.eye, dir, rng, stk, kMegaBlock, st);
+            L = path_radiance(sc, remap, cam.eye, dir, rng, stk, kMegaBlock, st);
         }
         if (finite3(L)) {
             sr += (double)L.x;
@@ -66,8 +66,8 @@ __global__ __launch_bounds__(kMegaBlock) void k_render_mega(SceneGPU sc, CameraG
     atomicAdd(film.counters + 3, (unsigned long long)(s1 - s0));
 }
 
-void launch_megakernel(const SceneGPU& sc, const CameraGPU& cam, const FilmGPU& film, unsigned frame0,
-                       unsigned spp, hipStream_t stream)
+void launch_megakernel(const SceneGPU& sc, const CameraGPU& cam, const FilmGPU& film, const int* remap,
+                       unsigned frame0, unsigned spp, hipStream_t stream)
 {
     if (spp == 0 || film.width <= 0 || film.height <= 0) return;
     int tiles_x = (film.width + 7) / 8, tiles_y = (film.height + 7) / 8;
@@ -79,8 +79,8 @@ void launch_megakernel(const SceneGPU& sc, const CameraGPU& cam, const FilmGPU& 
     chunks = (int)((spp + per - 1) / per);
     int64_t blocks = tiles * chunks;
     ORF_REQUIRE(blocks < 0x7fffffffLL, "launch: image too large");
-    hipLaunchKernelGGL(k_render_mega, dim3((unsigned)blocks), dim3(kMegaBlock), 0, stream, sc, cam, film, frame0,
-                       spp, chunks, per, tiles_x);
+    hipLaunchKernelGGL(k_render_mega, dim3((unsigned)blocks), dim3(kMegaBlock), 0, stream, sc, cam, film, remap,
+                       frame0, spp, chunks, per, tiles_x);
     ORF_HIP_CHECK(hipGetLastError());
 }
 

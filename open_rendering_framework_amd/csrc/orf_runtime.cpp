@@ -4,8 +4,27 @@
 #include <string>
 
 #include "orf_common.h"
+#include "orf_render.h"
 
 namespace orf {
+
+// eta = 1 / ior as the reference forms it (operator/(float, MyComplex),
+// ORF/MyComplex.h:155-160: conj(ior) / |ior|^2 with |ior| rounded to float).
+void conductor_eta_host(const double ior_re[3], const double ior_im[3], ConductorEta& e)
+{
+    for (int c = 0; c < 3; ++c) {
+        float r = (float)ior_re[c], i = (float)ior_im[c];
+        float ab = sqrtf(r * r + i * i);
+        double den = (double)powf(ab, 2.0f);
+        double er = ior_re[c] * (1.0 / den);
+        double ei = -ior_im[c] * (1.0 / den);
+        e.eta[c] = cf{(float)er, (float)ei};
+        e.eta2[c] = cf{(float)(er * er - ei * ei), (float)(2.0 * er * ei)};
+    }
+    e.gray = (ior_re[0] == ior_re[1] && ior_re[1] == ior_re[2] && ior_im[0] == ior_im[1] &&
+              ior_im[1] == ior_im[2]);
+}
+
 
 static thread_local std::string g_last_error;
 

@@ -67,6 +67,13 @@ _SIGS = {
     "orf_gen_icosphere": [ctypes.c_int, c_u32, ctypes.c_float, ctypes.c_float, c_vp, c_vp,
                           ctypes.POINTER(c_i64), c_vp, ctypes.POINTER(c_i64)],
     "orf_gen_sky_envmap": [ctypes.c_int, ctypes.c_int, c_vp, ctypes.c_float, ctypes.c_float, c_vp],
+    "orf_wavefront_default_config": [c_vp],
+    "orf_render_batch": [c_vp, c_vp, ctypes.c_int32, c_vp, ctypes.c_int32, ctypes.c_int32, c_vp, c_vp, c_vp],
+    "orf_batch_film_read": [c_vp, ctypes.c_int32, c_vp],
+    "orf_batch_film_sums": [c_vp, ctypes.c_int32, c_vp],
+    "orf_batch_counters": [c_vp, c_vp],
+    "orf_wavefront_autotune": [c_vp, c_vp, ctypes.c_int32, c_vp, ctypes.c_int32, c_u32, c_vp, ctypes.c_char_p,
+                               ctypes.c_size_t],
 }
 for _name, _args in _SIGS.items():
     _f = getattr(lib, _name)
@@ -163,6 +170,31 @@ def gen_sky_envmap(width, height, sun_dir, sigma_deg, peak):
 
 
 SCOPE_CONTEXT, SCOPE_MATERIAL = 0, 1
+BATCH_WAVEFRONT, BATCH_MEGAKERNEL = 0, 1
+
+
+class RenderItem(ctypes.Structure):
+    """orf_render_item: one render of a batch."""
+
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("spp", ctypes.c_uint32),
+                ("frame0", ctypes.c_uint32), ("eye", ctypes.c_float * 3), ("U", ctypes.c_float * 3),
+                ("V", ctypes.c_float * 3), ("W", ctypes.c_float * 3)]
+
+
+class WavefrontConfig(ctypes.Structure):
+    """orf_wavefront_config: the wavefront's tunable parameters."""
+
+    _fields_ = [("block_size", ctypes.c_int32), ("pool_size", ctypes.c_int32), ("sort_policy", ctypes.c_int32),
+                ("regen_threshold", ctypes.c_float), ("use_graph", ctypes.c_int32), ("graph_pairs", ctypes.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+    @classmethod
+    def default(cls):
+        c = cls()
+        call("orf_wavefront_default_config", ctypes.byref(c))
+        return c
 
 
 class Context:
@@ -297,3 +329,55 @@ class Context:
         out = np.zeros(4, np.uint64)
         call("orf_counters", self.handle, _np_ptr(out))
         return dict(zip(("nonfinite", "ext_rays", "shadow_rays", "paths"), [int(x) for x in out]))
+
+    # ---- batched renders
+    @staticmethod
+    def _items(items):
+        arr = (RenderItem * len(items))()
+        for i, it in enumerate(items):
+            arr[i].width, arr[i].height = it["width"], it["height"]
+            arr[i].spp, arr[i].frame0 = it["spp"], it["frame0"]
+            for k in ("eye", "U", "V", "W"):
+                getattr(arr[i], k)[:] = list(it[k])
+        return arr
+
+    def render_batch(self, items, remap=None, mode=BATCH_WAVEFRONT, config=None, stream=None):
+        """Render every item; returns the wall time in seconds."""
+        import numpy as np
+
+        arr = self._items(items)
+        rm = None if remap is None else np.ascontiguousarray(remap, np.int32)
+        stride = 0 if rm is None else rm.shape[1]
+        secs = ctypes.c_double()
+        call("orf_render_batch", self.handle, arr, len(items), _np_ptr(rm), stride, mode,
+             None if config is None else ctypes.byref(config), stream_ptr(stream), ctypes.byref(secs))
+        return secs.value
+
+    def batch_film(self, item, width, height, sums=False):
+        import numpy as np
+
+        out = np.zeros((height, width, 4), np.float64)
+        call("orf_batch_film_sums" if sums else "orf_batch_film_read", self.handle, item, _np_ptr(out))
+        return out
+
+    def batch_counters(self):
+        import numpy as np
+
+        out = np.zeros(4, np.uint64)
+        call("orf_batch_counters", self.handle, _np_ptr(out))
+        return dict(zip(("nonfinite", "ext_rays", "shadow_rays", "paths"), [int(x) for x in out]))
+
+    def autotune(self, items, remap=None, spp_divisor=16):
+        """(best WavefrontConfig, report dict) of the wavefront coordinate search."""
+        import json
+
+        import numpy as np
+
+        arr = self._items(items)
+        rm = None if remap is None else np.ascontiguousarray(remap, np.int32)
+        stride = 0 if rm is None else rm.shape[1]
+        best = WavefrontConfig()
+        buf = ctypes.create_string_buffer(1 << 16)
+        call("orf_wavefront_autotune", self.handle, arr, len(items), _np_ptr(rm), stride, spp_divisor,
+             ctypes.byref(best), buf, len(buf))
+        return best, json.loads(buf.value.decode())

@@ -155,6 +155,69 @@ def path_scene(width=512, height=512, level=7, ground_mat="lambert", sphere_mat=
                      width, height, max_depth, 1e-4)
 
 
+class Pcg32:
+    """PCG32 (O'Neill 2014, pcg-c-basic): initstate/initseq seeding, XSH-RR output."""
+
+    MASK = (1 << 64) - 1
+
+    def __init__(self, initstate, initseq):
+        self.state = 0
+        self.inc = ((initseq << 1) | 1) & self.MASK
+        self.next()
+        self.state = (self.state + initstate) & self.MASK
+        self.next()
+
+    def next(self):
+        old = self.state
+        self.state = (old * 6364136223846793005 + self.inc) & self.MASK
+        xs = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return ((xs >> rot) | (xs << ((-rot) & 31))) & 0xFFFFFFFF
+
+    def uniform(self):
+        return (self.next() >> 8) * (1.0 / 16777216.0)
+
+
+BATCH_SPP = (4, 16, 64, 256)
+BATCH_MATERIALS = ("lambert", "conductor", "dielectric")
+
+
+def _round8(x):
+    return int(min(1024, max(64, 8 * round(x / 8.0))))
+
+
+def batch_workload(n=64, seed=1234, level=7, env_size=(1024, 512), max_depth=8, spp_scale=1.0, size_scale=1.0):
+    """The batched workload: `n` renders of the path-tracer scene family.
+
+    Render r draws from PCG32(seed, r): frame0 (its seed), width and height
+    2^U[6,10] rounded to a multiple of 8, spp uniform in {4, 16, 64, 256}, and
+    the sphere's material uniform among Lambert (albedo 0.5), GGX conductor
+    (alpha 0.3, ior 0.27+2.78i) and Beckmann rough dielectric (alpha 0.3,
+    ior 1.5, no absorption).  spp_scale / size_scale shrink the batch for tests.
+    Returns (SceneDesc, items, remap) for Context.render_batch.
+    """
+    desc = path_scene(width=512, height=512, level=level, env_size=env_size, max_depth=max_depth)
+    desc.materials.append(Material("rough_transparent_shader.cu",
+                                   floats={"roughness": (0.3,), "ior": (1.5,), "glass_absorption": (0.0, 0.0, 0.0)},
+                                   uints={"microfacet_model": (1,), "normal_distribution": (0,)}))
+    desc.materials.append(Material("diffuse_shader.cu", floats={"diffuse_color": (0.5, 0.5, 0.5)}))
+    sphere_handle = {"lambert": 4, "conductor": 1, "dielectric": 3}
+    items, remap = [], []
+    for r in range(n):
+        g = Pcg32(seed, r)
+        frame0 = g.next()
+        w = _round8(2.0 ** (6.0 + 4.0 * g.uniform()) * size_scale)
+        h = _round8(2.0 ** (6.0 + 4.0 * g.uniform()) * size_scale)
+        spp = BATCH_SPP[min(3, int(4.0 * g.uniform()))]
+        mat = BATCH_MATERIALS[min(2, int(3.0 * g.uniform()))]
+        spp = max(1, int(round(spp * spp_scale)))
+        eye, U, V, W = pinhole((0.0, 1.5, 4.0), (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), 40.0, w, h)
+        items.append({"width": w, "height": h, "spp": spp, "frame0": frame0, "eye": eye, "U": U, "V": V, "W": W,
+                      "material": mat})
+        remap.append([0, sphere_handle[mat], 2, 3, 4])
+    return desc, items, np.array(remap, np.int32)
+
+
 def to_context(desc, ctx=None):
     """Create (or fill) a render context from a SceneDesc through the C-ABI."""
     from . import rt
