@@ -211,6 +211,35 @@ orf_status orf_wavefront_autotune(orf_context ctx, const orf_render_item* items,
                                   int32_t remap_stride, uint32_t spp_divisor, orf_wavefront_config* best,
                                   char* report, size_t report_size);
 
+/* ------------------------------------------------------------------------
+ * Differentiable renders (no counterpart in the reference).
+ *
+ * A Lambert material's albedo and a GGX conductor's roughness can come from
+ * textures (nearest texel of a planar x-z or a spherical mapping of the hit
+ * point).  orf_diff_render writes per-view mean images; orf_diff_adjoint runs
+ * the path-replay adjoint: given dLoss/dImage it adds dLoss/dTexel into the
+ * gradient buffers (device, caller-zeroed) using samples decorrelated from the
+ * primal's (frame index + frame_shift).  Sampling and Russian roulette are
+ * detached (the BSDF weight is differentiated at the sampled direction).
+ * Textures, images and gradients are device FP32 arrays:
+ * albedo tex_h*tex_w*3, roughness tex_h*tex_w, images nviews*h*w*3.
+ * ------------------------------------------------------------------------ */
+typedef struct orf_diff_desc {
+    int32_t albedo_material;      /* material handle (diffuse_shader) or -1 */
+    int32_t albedo_uv;            /* 0: planar x-z, 1: spherical */
+    float albedo_uv_params[4];    /* planar: x0, z0, x1, z1; spherical: centre x, y, z, unused */
+    int32_t roughness_material;   /* material handle (metallic_shader, GGX) or -1 */
+    int32_t roughness_uv;
+    float roughness_uv_params[4];
+    int32_t tex_w, tex_h;
+} orf_diff_desc;
+
+orf_status orf_diff_render(orf_context ctx, const orf_diff_desc* desc, const orf_render_item* views, int32_t nviews,
+                           const float* albedo, const float* roughness, float* images, void* stream);
+orf_status orf_diff_adjoint(orf_context ctx, const orf_diff_desc* desc, const orf_render_item* views,
+                            int32_t nviews, const float* albedo, const float* roughness, const float* d_images,
+                            float* g_albedo, float* g_roughness, uint32_t frame_shift, void* stream);
+
 /* Procedural scene pieces (host).  Icosphere: call with verts = tris = normals =
  * NULL to get the sizes.  Env: sun_dir in env coordinates, sigma in degrees. */
 orf_status orf_gen_icosphere(int level, uint32_t noise_seed, float amplitude, float frequency,

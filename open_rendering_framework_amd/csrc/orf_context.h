@@ -52,6 +52,8 @@ struct orf_context_t {
     long long batch_pixels = 0;
     orf::WavefrontPool wf_pool;
     double last_batch_seconds = 0.0;
+    // differentiable renders
+    orf::DevBuf d_diff_views, d_diff_stats;
     long long last_batch_iterations = 0;
 };
 

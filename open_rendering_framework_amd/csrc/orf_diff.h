@@ -1,0 +1,113 @@
+// orf_diff.h -- differentiable form: textured albedo/roughness and path-replay adjoint.
+//
+// No counterpart in the reference (it has neither textures on these materials --
+// CU/diffuse_shader.cu:130 reads a texture but shades with diffuse_color -- nor
+// derivatives).  Two materials take their parameter from a texture, looked up at
+// the nearest texel of a planar (x,z) or spherical mapping of the hit position:
+//   Lambert  : albedo rho(uv) (RGB), Russian roulette on mean(rho) as in
+//              CU/diffuse_shader.cu:165-168;
+//   conductor: GGX roughness alpha(uv) (isotropic), visible-normal sampling.
+// The estimator is the plain integrator's (shade_with() with the texel's values),
+// so a constant texture reproduces the plain render.  Sampling is "detached": the
+// BSDF weight f cos / pdf is differentiated with the sampled direction held fixed,
+// which gives per vertex
+//   Lambert  : d w_c / d rho_c = w_c / rho_c, and the NEE term is linear in rho_c;
+//   conductor: d log w / d alpha = d log D(m) + d log G1(wi) + d log G1(wo),
+//              D = 1 / (pi a^2 cos^4 (1 + tan^2/a^2)^2), G1 = 2 / (1 + sqrt(1 + a^2 tan^2)).
+// Path replay (Vicini et al. 2021): pass 1 computes the sample's radiance L; pass 2
+// replays the same random numbers, keeping L_rem = radiance still to come, and at
+// each textured vertex adds  delta * L_rem / rho  (Lambert, per channel, before the
+// vertex's own NEE term is removed) or  <delta, L_rem> * d log w / d alpha.
+#pragma once
+
+#include "orf_path.h"
+
+namespace orf {
+
+enum DiffUV : int { UV_PLANAR_XZ = 0, UV_SPHERICAL = 1 };
+
+struct DiffParams {
+    const float* albedo;  // tex_h * tex_w * 3
+    const float* rough;   // tex_h * tex_w
+    int tw, th;
+    int albedo_mat, rough_mat;  // material handles carrying the textures (-1: none)
+    int albedo_uv, rough_uv;
+    float albedo_uvp[4], rough_uvp[4];  // planar: x0, z0, x1, z1; spherical: cx, cy, cz, -
+};
+
+ORF_HD int diff_texel(int mode, const float* uvp, f3 p, int tw, int th)
+{
+    float u, v;
+    if (mode == UV_PLANAR_XZ) {
+        u = (p.x - uvp[0]) / (uvp[2] - uvp[0]);
+        v = (p.z - uvp[1]) / (uvp[3] - uvp[1]);
+    } else {
+        f3 q = normalize(p - mk3(uvp[0], uvp[1], uvp[2]));
+        u = (atan2f(q.x, q.z) + kPi) * 0.5f * kInvPi;
+        v = acosf(fminf(1.0f, fmaxf(-1.0f, q.y))) * kInvPi;
+    }
+    int iu = (int)floorf(u * (float)tw), iv = (int)floorf(v * (float)th);
+    iu = min(tw - 1, max(0, iu));
+    iv = min(th - 1, max(0, iv));
+    return iv * tw + iu;
+}
+
+// d log (f cos / pdf) / d alpha of the GGX conductor at fixed sampled direction.
+ORF_HD float ggx_dlogw_dalpha(f3 wi, f3 wo, f3 n, float a)
+{
+    f3 m = normalize(wi + wo);
+    float cm = dot(m, n);
+    float t2m = (1.0f - cm * cm) / (cm * cm);
+    float a2 = a * a;
+    float dlogD = (2.0f * t2m - 2.0f * a2) / (a * (a2 + t2m));
+    float r = 0.0f;
+    for (int k = 0; k < 2; ++k) {
+        float c = dot(k ? wo : wi, n);
+        float T = (1.0f - c * c) / (c * c);
+        float s = sqrtf(1.0f + a2 * T);
+        r += -(a * T / s) / (1.0f + s);
+    }
+    return dlogD + r;
+}
+
+struct DiffVertex {
+    int kind;   // 0: no textured parameter, 1: albedo texel, 2: roughness texel
+    int texel;
+    f3 rho;
+    float dlogw;
+};
+
+// One vertex of the differentiable estimator: the plain shading step with the
+// texel's parameter substituted, plus what the adjoint needs.
+ORF_HD bool diff_vertex(const SceneGPU& sc, const DiffParams& dp, PathState& ps, const Hit& h, ShadowRay& sr,
+                        DiffVertex& dv)
+{
+    HitGeom g = hit_geometry(sc, ps, h);
+    int mid = g.sh.w;
+    MaterialGPU mat = sc.materials[mid];
+    dv.kind = 0;
+    if (mid == dp.albedo_mat && dp.albedo) {
+        int t = diff_texel(dp.albedo_uv, dp.albedo_uvp, g.p, dp.tw, dp.th);
+        mat.diffuse_color = mk3(dp.albedo[3 * t], dp.albedo[3 * t + 1], dp.albedo[3 * t + 2]);
+        dv.kind = 1;
+        dv.texel = t;
+        dv.rho = mat.diffuse_color;
+    } else if (mid == dp.rough_mat && dp.rough) {
+        int t = diff_texel(dp.rough_uv, dp.rough_uvp, g.p, dp.tw, dp.th);
+        mat.roughness = mk2(dp.rough[t], dp.rough[t]);
+        dv.kind = 2;
+        dv.texel = t;
+    }
+    f3 wi = -ps.d;
+    bool more = shade_with(sc, mat, ps, h, g, sr);
+    if (dv.kind == 1 && mat.kind != MAT_LAMBERT) dv.kind = 0;
+    if (dv.kind == 2) {
+        if (mat.kind == MAT_CONDUCTOR && more && mat.normal_distribution == GGX_DISTRIBUTION)
+            dv.dlogw = ggx_dlogw_dalpha(wi, ps.d, g.ffn, mat.roughness.x);
+        else
+            dv.kind = 0;
+    }
+    return more;
+}
+
+}  // namespace orf

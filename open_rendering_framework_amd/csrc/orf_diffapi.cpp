@@ -1,0 +1,106 @@
+// orf_diffapi.cpp -- C-ABI of the differentiable form (primal and adjoint launches).
+#include <cstring>
+#include <vector>
+
+#include "orf_common.h"
+#include "orf_context.h"
+#include "orf_diff_host.h"
+
+namespace orf {
+
+static DiffParams diff_params(orf_context_t* c, const orf_diff_desc* d, const float* albedo, const float* rough)
+{
+    ORF_REQUIRE(d, "diff: null description");
+    ORF_REQUIRE(d->tex_w > 0 && d->tex_h > 0 && d->tex_w <= 8192 && d->tex_h <= 8192, "diff: bad texture size");
+    int nm = (int)c->mats.size();
+    ORF_REQUIRE(d->albedo_material < nm && d->roughness_material < nm, "diff: material handle out of range");
+    ORF_REQUIRE(d->albedo_material < 0 || albedo, "diff: albedo texture required");
+    ORF_REQUIRE(d->roughness_material < 0 || rough, "diff: roughness texture required");
+    ORF_REQUIRE(d->albedo_material < 0 || c->mats[d->albedo_material].kind == MAT_LAMBERT,
+                "diff: the albedo texture applies to a Lambert (diffuse_shader) material");
+    ORF_REQUIRE(d->roughness_material < 0 || (c->mats[d->roughness_material].kind == MAT_CONDUCTOR &&
+                                              c->mats[d->roughness_material].normal_distribution == GGX_DISTRIBUTION),
+                "diff: the roughness texture applies to a GGX conductor (metallic_shader) material");
+    ORF_REQUIRE(d->albedo_uv == 0 || d->albedo_uv == 1, "diff: albedo_uv must be 0 or 1");
+    ORF_REQUIRE(d->roughness_uv == 0 || d->roughness_uv == 1, "diff: roughness_uv must be 0 or 1");
+    DiffParams p;
+    std::memset(&p, 0, sizeof(p));
+    p.albedo = d->albedo_material >= 0 ? albedo : nullptr;
+    p.rough = d->roughness_material >= 0 ? rough : nullptr;
+    p.tw = d->tex_w;
+    p.th = d->tex_h;
+    p.albedo_mat = d->albedo_material;
+    p.rough_mat = d->roughness_material;
+    p.albedo_uv = d->albedo_uv;
+    p.rough_uv = d->roughness_uv;
+    for (int k = 0; k < 4; ++k) {
+        p.albedo_uvp[k] = d->albedo_uv_params[k];
+        p.rough_uvp[k] = d->roughness_uv_params[k];
+    }
+    return p;
+}
+
+static void stage_views(orf_context_t* c, const orf_render_item* v, int n, hipStream_t st, int& w, int& h)
+{
+    ORF_REQUIRE(v && n > 0 && n <= 4096, "diff: 1..4096 views required");
+    w = v[0].width;
+    h = v[0].height;
+    std::vector<WfRender> R(n);
+    for (int i = 0; i < n; ++i) {
+        ORF_REQUIRE(v[i].width == w && v[i].height == h, "diff: all views must share one size");
+        ORF_REQUIRE(w > 0 && h > 0 && w <= 16384 && h <= 16384 && v[i].spp > 0, "diff: bad view");
+        std::memset(&R[i], 0, sizeof(WfRender));
+        R[i].cam.eye = mk3(v[i].eye[0], v[i].eye[1], v[i].eye[2]);
+        R[i].cam.U = mk3(v[i].U[0], v[i].U[1], v[i].U[2]);
+        R[i].cam.V = mk3(v[i].V[0], v[i].V[1], v[i].V[2]);
+        R[i].cam.W = mk3(v[i].W[0], v[i].W[1], v[i].W[2]);
+        R[i].width = w;
+        R[i].height = h;
+        R[i].spp = v[i].spp;
+        R[i].frame0 = v[i].frame0;
+    }
+    // the previous launch may still read the view table
+    ORF_HIP_CHECK(hipStreamSynchronize(st));
+    c->d_diff_views.upload(R);
+    c->d_diff_stats.ensure(8 * sizeof(unsigned long long));
+}
+
+}  // namespace orf
+
+using namespace orf;
+
+extern "C" orf_status orf_diff_render(orf_context ctx, const orf_diff_desc* desc, const orf_render_item* views,
+                                      int32_t nviews, const float* albedo, const float* roughness, float* images,
+                                      void* stream)
+{
+    ORF_API_BEGIN
+    orf_context_t* c = ctx_check(ctx);
+    ORF_REQUIRE(images, "orf_diff_render: null image buffer");
+    hipStream_t st = (hipStream_t)stream;
+    prepare(c, st);
+    DiffParams dp = diff_params(c, desc, albedo, roughness);
+    int w, h;
+    stage_views(c, views, nviews, st, w, h);
+    launch_diff(scene_view(c), dp, c->d_diff_views.as<WfRender>(), nviews, w, h, false, 0, images, nullptr,
+                nullptr, nullptr, c->d_diff_stats.as<unsigned long long>(), st);
+    ORF_API_END
+}
+
+extern "C" orf_status orf_diff_adjoint(orf_context ctx, const orf_diff_desc* desc, const orf_render_item* views,
+                                       int32_t nviews, const float* albedo, const float* roughness,
+                                       const float* d_images, float* g_albedo, float* g_roughness,
+                                       uint32_t frame_shift, void* stream)
+{
+    ORF_API_BEGIN
+    orf_context_t* c = ctx_check(ctx);
+    ORF_REQUIRE(d_images, "orf_diff_adjoint: null image gradient");
+    hipStream_t st = (hipStream_t)stream;
+    prepare(c, st);
+    DiffParams dp = diff_params(c, desc, albedo, roughness);
+    int w, h;
+    stage_views(c, views, nviews, st, w, h);
+    launch_diff(scene_view(c), dp, c->d_diff_views.as<WfRender>(), nviews, w, h, true, frame_shift, nullptr,
+                d_images, desc->albedo_material >= 0 ? g_albedo : nullptr,
+                desc->roughness_material >= 0 ? g_roughness : nullptr, c->d_diff_stats.as<unsigned long long>(), st);
+    ORF_API_END
+}

@@ -83,20 +83,43 @@ ORF_HD void shade_miss(const SceneGPU& sc, PathState& ps)
     ps.L += ps.thr * E;
 }
 
+// Geometry of a hit: position, shading normal, the normal facing the ray, and
+// the shade record (vertex-normal indices, geometry material id).
+struct HitGeom {
+    f3 p, normal, ffn;
+    int4 sh;
+};
+
+ORF_HD HitGeom hit_geometry(const SceneGPU& sc, const PathState& ps, const Hit& h)
+{
+    HitGeom g;
+    g.p = ps.o + h.t * ps.d;
+    const float4* tg = sc.tri + 3 * (size_t)h.leaf;
+    float4 q1 = tg[1], q2 = tg[2];
+    g.normal = shading_normal(sc, h.prim, h.b, h.g, mk3(q1.x, q1.y, q1.z), mk3(q2.x, q2.y, q2.z), g.sh);
+    g.ffn = faceforward(g.normal, -ps.d, g.normal);
+    return g;
+}
+
+ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps, const Hit& h, const HitGeom& g,
+                       ShadowRay& sr);
+
 // Shade one surface vertex.  Returns true when the path continues with ps.o/ps.d
 // as its next ray; `sr` receives the NEE shadow segment, if any.
 ORF_HD bool shade_vertex(const SceneGPU& sc, const int* remap, PathState& ps, const Hit& h, ShadowRay& sr)
 {
+    HitGeom g = hit_geometry(sc, ps, h);
+    return shade_with(sc, hit_material(sc, remap, g.sh.w), ps, h, g, sr);
+}
+
+// Shade a vertex with material `mat` (the material programs' closest hits).
+ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps, const Hit& h, const HitGeom& g,
+                       ShadowRay& sr)
+{
     sr.valid = false;
     const float eps = sc.scene_epsilon;
     f3 d = ps.d;
-    f3 p = ps.o + h.t * d;
-    const float4* tg = sc.tri + 3 * (size_t)h.leaf;
-    float4 q1 = tg[1], q2 = tg[2];
-    int4 sh;
-    f3 normal = shading_normal(sc, h.prim, h.b, h.g, mk3(q1.x, q1.y, q1.z), mk3(q2.x, q2.y, q2.z), sh);
-    f3 ffn = faceforward(normal, -d, normal);
-    const MaterialGPU& mat = hit_material(sc, remap, sh.w);
+    f3 p = g.p, normal = g.normal, ffn = g.ffn;
     if (mat.kind == MAT_EMITTER) {
         if (dot(normal, -d) > 0.0f && ps.emit) ps.L += ps.thr * mat.radiance;
         return false;

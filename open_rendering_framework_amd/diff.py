@@ -1,0 +1,83 @@
+"""Differentiable renders: a torch autograd Function (libtorch C++ extension)
+over the C-ABI's path-replay adjoint.
+
+build_extension() compiles the extension in-tree (torch.utils.cpp_extension
+with an explicit build directory); load_extension() imports the built module
+and fails loudly if it is missing.
+"""
+import glob
+import importlib.util
+import os
+
+from . import INCLUDE_DIR, PACKAGE_DIR
+
+EXT_NAME = "orf_torch_autograd"
+BUILD_DIR = os.path.join(PACKAGE_DIR, "_torch_build")
+SOURCE = os.path.join(PACKAGE_DIR, "torch_ext", "orf_autograd.cpp")
+
+_ext = None
+
+
+def build_extension(verbose=False):
+    import torch.utils.cpp_extension as ce
+
+    os.makedirs(BUILD_DIR, exist_ok=True)
+    os.environ.setdefault("PYTORCH_ROCM_ARCH", "gfx950")
+    ce.load(name=EXT_NAME, sources=[SOURCE], build_directory=BUILD_DIR, verbose=verbose, with_cuda=True,
+            extra_include_paths=[INCLUDE_DIR],
+            extra_cflags=["-O2", "-D__HIP_PLATFORM_AMD__"],
+            extra_ldflags=["-L" + PACKAGE_DIR, "-lorf_rt", "-Wl,-rpath," + PACKAGE_DIR],
+            is_python_module=True)
+
+
+def load_extension():
+    global _ext
+    if _ext is not None:
+        return _ext
+    import torch  # noqa: F401  (libtorch must be loaded first)
+
+    from . import rt  # noqa: F401  (liborf_rt.so bound to torch's HIP runtime)
+
+    hits = glob.glob(os.path.join(BUILD_DIR, EXT_NAME + "*.so"))
+    if not hits:
+        raise ImportError("%s not built in %s; run open_rendering_framework_amd.diff.build_extension()"
+                          % (EXT_NAME, BUILD_DIR))
+    spec = importlib.util.spec_from_file_location(EXT_NAME, hits[0])
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    _ext = mod
+    return mod
+
+
+class DiffRenderer:
+    """Renders views of a context's scene as a differentiable function of an
+    albedo texture (on a Lambert material) and a roughness texture (on a GGX
+    conductor): images = renderer(albedo[h,w,3], roughness[h,w])."""
+
+    def __init__(self, ctx, views, albedo_material, roughness_material, tex_size=(64, 64),
+                 albedo_uv=(0, (-10.0, -10.0, 10.0, 10.0)), roughness_uv=(1, (0.0, 0.0, 0.0, 0.0)),
+                 frame_shift=0x40000000):
+        ext = load_extension()
+        self.ctx = ctx
+        desc = [albedo_material, albedo_uv[0], *albedo_uv[1], roughness_material, roughness_uv[0], *roughness_uv[1],
+                tex_size[0], tex_size[1]]
+        vv = [[v["width"], v["height"], v["spp"], v["frame0"], *v["eye"], *v["U"], *v["V"], *v["W"]] for v in views]
+        self.views = views
+        self.tex_size = tex_size
+        self.setup = ext.make_setup(ctx.handle.value, [float(x) for x in desc], [[float(x) for x in v] for v in vv],
+                                    int(frame_shift))
+        self._ext = ext
+
+    def __call__(self, albedo, roughness):
+        return self._ext.render(albedo, roughness, self.setup)
+
+    def close(self):
+        if self.setup:
+            self._ext.free_setup(self.setup)
+            self.setup = 0
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
