@@ -43,8 +43,8 @@ ORF_HD int diff_texel(int mode, const float* uvp, f3 p, int tw, int th)
         v = (p.z - uvp[1]) / (uvp[3] - uvp[1]);
     } else {
         f3 q = normalize(p - mk3(uvp[0], uvp[1], uvp[2]));
-        u = (atan2f(q.x, q.z) + kPi) * 0.5f * kInvPi;
-        v = acosf(fminf(1.0f, fmaxf(-1.0f, q.y))) * kInvPi;
+        u = (orf_atan2f(q.x, q.z) + kPi) * 0.5f * kInvPi;
+        v = orf_acosf(fminf(1.0f, fmaxf(-1.0f, q.y))) * kInvPi;
     }
     int iu = (int)floorf(u * (float)tw), iv = (int)floorf(v * (float)th);
     iu = min(tw - 1, max(0, iu));
@@ -108,6 +108,61 @@ ORF_HD bool diff_vertex(const SceneGPU& sc, const DiffParams& dp, PathState& ps,
             dv.kind = 0;
     }
     return more;
+}
+
+// Pass 1: the sample's radiance (the plain estimator with textured parameters).
+ORF_HD f3 diff_path(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32_t rng, unsigned* stk, int stride)
+{
+    PathState ps;
+    path_begin(ps, o, d, rng);
+    for (;;) {
+        Hit h;
+        if (!trace<false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk, stride)) {
+            shade_miss(sc, ps);
+            break;
+        }
+        ShadowRay sr;
+        DiffVertex dv;
+        bool more = diff_vertex(sc, dp, ps, h, sr, dv);
+        if (sr.valid) {
+            Hit hs;
+            if (!trace<true>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) ps.L += sr.contrib;
+        }
+        if (!more) break;
+    }
+    return ps.L;
+}
+
+// Pass 2: replay the same random numbers and hand each textured vertex's
+// gradient terms to `sink` (albedo(texel, channel, value), rough(texel, value));
+// delta = dLoss/dL of this sample, L_total its pass-1 radiance.
+template <class Sink>
+ORF_HD void diff_replay(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32_t rng, f3 L_total, f3 delta,
+                        Sink& sink, unsigned* stk, int stride)
+{
+    PathState ps;
+    path_begin(ps, o, d, rng);
+    for (;;) {
+        Hit h;
+        if (!trace<false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk, stride)) break;
+        f3 rem_before = L_total - ps.L;  // radiance from this vertex on
+        ShadowRay sr;
+        DiffVertex dv;
+        bool more = diff_vertex(sc, dp, ps, h, sr, dv);
+        if (sr.valid) {
+            Hit hs;
+            if (!trace<true>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) ps.L += sr.contrib;
+        }
+        if (dv.kind == 1) {
+            if (dv.rho.x != 0.0f) sink.albedo(dv.texel, 0, delta.x * rem_before.x / dv.rho.x);
+            if (dv.rho.y != 0.0f) sink.albedo(dv.texel, 1, delta.y * rem_before.y / dv.rho.y);
+            if (dv.rho.z != 0.0f) sink.albedo(dv.texel, 2, delta.z * rem_before.z / dv.rho.z);
+        } else if (dv.kind == 2) {
+            f3 rem_after = L_total - ps.L;  // radiance after the conductor's scattering
+            sink.rough(dv.texel, dot(delta, rem_after) * dv.dlogw);
+        }
+        if (!more) break;
+    }
 }
 
 }  // namespace orf

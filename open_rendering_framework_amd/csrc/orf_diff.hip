@@ -17,57 +17,19 @@ namespace orf {
 
 constexpr int kDiffBlock = 64;
 
-__device__ f3 diff_path(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32_t rng, unsigned* stk)
-{
-    PathState ps;
-    path_begin(ps, o, d, rng);
-    for (;;) {
-        Hit h;
-        if (!trace<false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk, kDiffBlock)) {
-            shade_miss(sc, ps);
-            break;
-        }
-        ShadowRay sr;
-        DiffVertex dv;
-        bool more = diff_vertex(sc, dp, ps, h, sr, dv);
-        if (sr.valid) {
-            Hit hs;
-            if (!trace<true>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, kDiffBlock)) ps.L += sr.contrib;
-        }
-        if (!more) break;
+// Texel gradients land in global FP32 arrays through atomics.
+struct AtomicSink {
+    float* g_albedo;
+    float* g_rough;
+    __device__ void albedo(int texel, int c, float v)
+    {
+        if (g_albedo) atomicAdd(g_albedo + 3 * (size_t)texel + c, v);
     }
-    return ps.L;
-}
-
-// Pass 2: replay and scatter gradients; delta = dLoss/dL of this sample.
-__device__ void diff_replay(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32_t rng, f3 L_total,
-                            f3 delta, float* g_albedo, float* g_rough, unsigned* stk)
-{
-    PathState ps;
-    path_begin(ps, o, d, rng);
-    for (;;) {
-        Hit h;
-        if (!trace<false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk, kDiffBlock)) break;
-        f3 rem_before = L_total - ps.L;  // radiance from this vertex on
-        ShadowRay sr;
-        DiffVertex dv;
-        bool more = diff_vertex(sc, dp, ps, h, sr, dv);
-        if (sr.valid) {
-            Hit hs;
-            if (!trace<true>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, kDiffBlock)) ps.L += sr.contrib;
-        }
-        if (dv.kind == 1 && g_albedo) {
-            float* g = g_albedo + 3 * (size_t)dv.texel;
-            if (dv.rho.x != 0.0f) atomicAdd(g + 0, delta.x * rem_before.x / dv.rho.x);
-            if (dv.rho.y != 0.0f) atomicAdd(g + 1, delta.y * rem_before.y / dv.rho.y);
-            if (dv.rho.z != 0.0f) atomicAdd(g + 2, delta.z * rem_before.z / dv.rho.z);
-        } else if (dv.kind == 2 && g_rough) {
-            f3 rem_after = L_total - ps.L;
-            atomicAdd(g_rough + dv.texel, dot(delta, rem_after) * dv.dlogw);
-        }
-        if (!more) break;
+    __device__ void rough(int texel, float v)
+    {
+        if (g_rough) atomicAdd(g_rough + texel, v);
     }
-}
+};
 
 __global__ __launch_bounds__(kDiffBlock) void k_diff(SceneGPU sc, DiffParams dp, const WfRender* __restrict__ views,
                                                      int tiles_x, int tiles_per_view, int adjoint,
@@ -91,7 +53,7 @@ __global__ __launch_bounds__(kDiffBlock) void k_diff(SceneGPU sc, DiffParams dp,
         for (unsigned s = 0; s < R.spp; ++s) {
             f3 dir;
             uint32_t rng = camera_ray(R.cam, R.width, R.height, x, y, R.frame0 + s, dir);
-            f3 L = diff_path(sc, dp, R.cam.eye, dir, rng, stk);
+            f3 L = diff_path(sc, dp, R.cam.eye, dir, rng, stk, kDiffBlock);
             if (finite3(L)) {
                 acc += L;
                 cnt++;
@@ -104,18 +66,19 @@ __global__ __launch_bounds__(kDiffBlock) void k_diff(SceneGPU sc, DiffParams dp,
         images[3 * px + 1] = acc.y * inv;
         images[3 * px + 2] = acc.z * inv;
     } else {
+        AtomicSink sink{g_albedo, g_rough};
         float sc_inv = 1.0f / (float)max(1u, R.spp);
         f3 delta = mk3(dimages[3 * px], dimages[3 * px + 1], dimages[3 * px + 2]) * sc_inv;
         if (delta.x == 0.0f && delta.y == 0.0f && delta.z == 0.0f) return;
         for (unsigned s = 0; s < R.spp; ++s) {
             f3 dir;
             uint32_t rng = camera_ray(R.cam, R.width, R.height, x, y, R.frame0 + frame_shift + s, dir);
-            f3 L = diff_path(sc, dp, R.cam.eye, dir, rng, stk);
+            f3 L = diff_path(sc, dp, R.cam.eye, dir, rng, stk, kDiffBlock);
             if (!finite3(L)) {
                 bad++;
                 continue;
             }
-            diff_replay(sc, dp, R.cam.eye, dir, rng, L, delta, g_albedo, g_rough, stk);
+            diff_replay(sc, dp, R.cam.eye, dir, rng, L, delta, sink, stk, kDiffBlock);
         }
     }
     if (bad) atomicAdd(counters, (unsigned long long)bad);

@@ -7,9 +7,9 @@
 //   GGX              ORF/MicrofacetGGX.h:9-17 (alpha_i), :33-45 (projected area),
 //                    :48-60 (slope pdf, Smith G1), :63-74 (G1 with frame),
 //                    :142-187 (P22_11), :190-207 (VNDF sampling), :219-251 (NDF, VNDF eval)
-// Transcendentals are the device libm (acosf, atan2f, sincosf); the identity
-// cos(acos x) is deliberately NOT simplified: the reference rounds through
-// theta, and matching that rounding keeps grazing-angle queries within 1e-5.
+// Transcendentals are the shared FP32 functions of orf_math.h (the same bits on
+// the GPU and in the CPU oracle); the identity cos(acos x) is deliberately NOT
+// simplified: the reference rounds through theta, and so do these kernels.
 #pragma once
 
 #include "orf_vec.h"
@@ -55,7 +55,7 @@ ORF_HD f3 sample_cosine(f3 n, uint32_t& rng)
     float phi = 2.0f * kPi * rnd_tea(rng);
     float st = sqrtf(1.0f - ct * ct);
     float sp, cp;
-    sincosf(phi, &sp, &cp);
+    orf_sincosf(phi, &sp, &cp);
     return rotate_to_normal(n, mk3(st * cp, st * sp, ct));
 }
 
@@ -102,7 +102,7 @@ ORF_HD float ggx_proj_area(f3 wi, float ax, float ay)
     if (wi.z > 0.9999f) return 1.0f;
     if (wi.z < -0.9999f) return 0.0f;
     float a = ggx_alpha_proj(wi, ax, ay);
-    float s = sinf(acosf(wi.z));
+    float s = orf_sinf(orf_acosf(wi.z));
     return 0.5f * (wi.z + sqrtf(wi.z * wi.z + s * s * a * a));
 }
 
@@ -134,11 +134,11 @@ ORF_HD f2 ggx_visible_slope(float theta_i, float z1, float z2)
     if (theta_i < 0.0001f) {
         float r = sqrtf(z1 / (1.0f - z1));
         float sp, cp;
-        sincosf(kPi * 2.0f * z2, &sp, &cp);
+        orf_sincosf(kPi * 2.0f * z2, &sp, &cp);
         return mk2(r * cp, r * sp);
     }
     float sin_i, cos_i;
-    sincosf(theta_i, &sin_i, &cos_i);
+    orf_sincosf(theta_i, &sin_i, &cos_i);
     cos_i = fmaxf(cos_i, 1e-7f);
     float tan_i = sin_i / cos_i;
     float proj = 0.5f * (cos_i + 1.0f);
@@ -171,10 +171,10 @@ ORF_HD f2 ggx_visible_slope(float theta_i, float z1, float z2)
 ORF_HD f3 ggx_sample_visible(f3 wi, float ax, float ay, float z1, float z2)
 {
     f3 w11 = normalize(mk3(ax * wi.x, ay * wi.y, wi.z));
-    f2 s11 = ggx_visible_slope(acosf(w11.z), z1, z2);
-    float phi = atan2f(w11.y, w11.x);
+    f2 s11 = ggx_visible_slope(orf_acosf(w11.z), z1, z2);
+    float phi = orf_atan2f(w11.y, w11.x);
     float sp, cp;
-    sincosf(phi, &sp, &cp);
+    orf_sincosf(phi, &sp, &cp);
     float sx = cp * s11.x - sp * s11.y;
     float sy = sp * s11.x + cp * s11.y;
     sx *= ax;
@@ -199,7 +199,7 @@ constexpr float k2SqrtPi = 1.12837916709551257390f;
 
 ORF_HD float erfinv_f32(float x)
 {
-    float w = -logf((1.0f - x) * (1.0f + x)), p;
+    float w = -orf_logf((1.0f - x) * (1.0f + x)), p;
     if (w < 5.0f) {
         w = w - 2.5f;
         p = 2.81022636e-08f;
@@ -246,27 +246,27 @@ ORF_HD float beckmann_masking(f3 v, f3 m, f3 n, float ax, float ay)
 ORF_HD f2 beckmann_visible_slope(float theta_i, float z1, float z2)
 {
     if (theta_i < 0.0001f) {
-        float r = sqrtf(-logf(z1));
+        float r = sqrtf(-orf_logf(z1));
         float sp, cp;
-        sincosf(kPi * 2.0f * z2, &sp, &cp);
+        orf_sincosf(kPi * 2.0f * z2, &sp, &cp);
         return mk2(r * cp, r * sp);
     }
     float sin_i, cos_i;
-    sincosf(theta_i, &sin_i, &cos_i);
+    orf_sincosf(theta_i, &sin_i, &cos_i);
     float slope_i = cos_i / sin_i;
     float a = cos_i / sin_i;
-    float proj = 0.5f * (erff(a) + 1.0f) * cos_i + 0.25f * k2SqrtPi * sin_i * expf(-a * a);
+    float proj = 0.5f * (orf_erff(a) + 1.0f) * cos_i + 0.25f * k2SqrtPi * sin_i * orf_expf(-a * a);
     if (proj < 0.0001f || proj != proj) return mk2(0.0f, 0.0f);
     float c = 1.0f / proj;
     float emin = -0.9999f;
-    float emax = fmaxf(emin, erff(slope_i));
+    float emax = fmaxf(emin, orf_erff(slope_i));
     float ecur = 0.5f * (emin + emax);
     for (int it = 0; it < 256 && emax - emin > 0.00001f; ++it) {
         if (!(ecur >= emin && ecur <= emax)) ecur = 0.5f * (emin + emax);
         float s = erfinv_f32(ecur);
         float cdf = (s >= slope_i) ? 1.0f
-                                   : c * (0.25f * k2SqrtPi * sin_i * expf(-s * s) +
-                                          cos_i * (0.5f + 0.5f * erff(s)));
+                                   : c * (0.25f * k2SqrtPi * sin_i * orf_expf(-s * s) +
+                                          cos_i * (0.5f + 0.5f * orf_erff(s)));
         float diff = cdf - z1;
         if (fabsf(diff) < 0.00001f) break;
         if (diff > 0.0f) {
@@ -285,10 +285,10 @@ ORF_HD f2 beckmann_visible_slope(float theta_i, float z1, float z2)
 ORF_HD f3 beckmann_sample_visible(f3 wi, float ax, float ay, float z1, float z2)
 {
     f3 w11 = normalize(mk3(ax * wi.x, ay * wi.y, wi.z));
-    f2 s11 = beckmann_visible_slope(acosf(w11.z), z1, z2);
-    float phi = atan2f(w11.y, w11.x);
+    f2 s11 = beckmann_visible_slope(orf_acosf(w11.z), z1, z2);
+    float phi = orf_atan2f(w11.y, w11.x);
     float sp, cp;
-    sincosf(phi, &sp, &cp);
+    orf_sincosf(phi, &sp, &cp);
     float sx = cp * s11.x - sp * s11.y;
     float sy = sp * s11.x + cp * s11.y;
     sx *= ax;

@@ -43,8 +43,8 @@ ORF_HD f3 tex_bilinear(const float4* tex, int w, int h, float u, float v)
 
 ORF_HD f3 env_lookup(const EnvGPU& env, f3 dir)
 {
-    float theta = acosf(dir.y);
-    float phi = atan2f(dir.x, dir.z);
+    float theta = orf_acosf(dir.y);
+    float phi = orf_atan2f(dir.x, dir.z);
     float u = (phi + kPi) * 0.5f * kInvPi;
     float v = (float)(1.0 - (double)(theta * kInvPi));
     return tex_bilinear(env.tex, env.w, env.h, u, v);
@@ -65,14 +65,14 @@ ORF_HD int cdf_lower_bound(const float* cdf, int n, float xi)
 // Solid-angle density of env_sample for direction dir.
 ORF_HD float env_pdf(const EnvGPU& env, f3 dir)
 {
-    float theta = acosf(dir.y);
-    float phi = atan2f(dir.x, dir.z);
+    float theta = orf_acosf(dir.y);
+    float phi = orf_atan2f(dir.x, dir.z);
     float u = (phi + kPi) * 0.5f * kInvPi;
     float v = 1.0f - theta * kInvPi;
     int iu = min(env.w - 1, max(0, (int)(u * (float)env.w)));
     int iv = min(env.h - 1, max(0, (int)(v * (float)env.h)));
     float p = env.marg_pdf[iv] * env.cond_pdf[(size_t)iv * env.w + iu];
-    float st = sinf(theta);
+    float st = orf_sinf(theta);
     return st > 0.0f ? p / (2.0f * kPi * kPi * st) : 0.0f;
 }
 
@@ -96,8 +96,8 @@ ORF_HD void env_sample(const EnvGPU& env, uint32_t& rng, f3& dir, f3& L, float& 
     float theta = (1.0f - v) * kPi;
     float phi = (2.0f * u - 1.0f) * kPi;
     float st, ct, sp, cp;
-    sincosf(theta, &st, &ct);
-    sincosf(phi, &sp, &cp);
+    orf_sincosf(theta, &st, &ct);
+    orf_sincosf(phi, &sp, &cp);
     dir = mk3(st * sp, ct, st * cp);
     f3 E = tex_bilinear(env.tex, env.w, env.h, u, v);
     L = E * (st * two_pi2 / prob);
@@ -203,7 +203,7 @@ ORF_HD bool dielectric_step(f3 d, float t_hit, f3 normal, f3 ffn, const Material
     f3 beam = mk3(1.0f);
     if (dot(-d, normal) < 0.0f) {
         f3 e = (-t_hit) * m.glass_absorption;
-        beam = mk3(expf(e.x), expf(e.y), expf(e.z));
+        beam = mk3(orf_expf(e.x), orf_expf(e.y), orf_expf(e.z));
         float prob = (beam.x + beam.y + beam.z) / 3.0f;
         if (rnd_tea(rng) >= prob) return false;
         beam = beam / prob;

@@ -10,6 +10,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "orf_math.h"
+
 #define ORF_HD __host__ __device__ __forceinline__
 
 namespace orf {

@@ -1,9 +1,8 @@
 """Differentiable renders: a torch autograd Function (libtorch C++ extension)
 over the C-ABI's path-replay adjoint.
 
-build_extension() compiles the extension in-tree (torch.utils.cpp_extension
-with an explicit build directory); load_extension() imports the built module
-and fails loudly if it is missing.
+build_extension() compiles the extension in-tree with one g++ command;
+load_extension() imports the built module and fails loudly if it is missing.
 """
 import glob
 import importlib.util
@@ -19,15 +18,30 @@ _ext = None
 
 
 def build_extension(verbose=False):
+    """Compile torch_ext/orf_autograd.cpp into BUILD_DIR with one explicit g++
+    command (host C++ against libtorch and the HIP runtime; no source rewriting)."""
+    import subprocess
+    import sysconfig
+
+    import torch
     import torch.utils.cpp_extension as ce
 
     os.makedirs(BUILD_DIR, exist_ok=True)
-    os.environ.setdefault("PYTORCH_ROCM_ARCH", "gfx950")
-    ce.load(name=EXT_NAME, sources=[SOURCE], build_directory=BUILD_DIR, verbose=verbose, with_cuda=True,
-            extra_include_paths=[INCLUDE_DIR],
-            extra_cflags=["-O2", "-D__HIP_PLATFORM_AMD__"],
-            extra_ldflags=["-L" + PACKAGE_DIR, "-lorf_rt", "-Wl,-rpath," + PACKAGE_DIR],
-            is_python_module=True)
+    suffix = sysconfig.get_config_var("EXT_SUFFIX") or ".so"
+    out = os.path.join(BUILD_DIR, EXT_NAME + suffix)
+    abi = int(bool(torch._C._GLIBCXX_USE_CXX11_ABI))
+    cmd = ["g++", "-std=c++17", "-O2", "-fPIC", "-shared", "-D__HIP_PLATFORM_AMD__",
+           "-DTORCH_API_INCLUDE_EXTENSION_H", "-DTORCH_EXTENSION_NAME=" + EXT_NAME,
+           "-D_GLIBCXX_USE_CXX11_ABI=%d" % abi, "-I" + INCLUDE_DIR, "-I" + sysconfig.get_paths()["include"]]
+    cmd += ["-I" + d for d in ce.include_paths("cuda")]
+    cmd += [SOURCE, "-o", out + ".tmp"]
+    cmd += ["-L" + d for d in ce.library_paths("cuda")]
+    cmd += ["-L" + PACKAGE_DIR, "-lorf_rt", "-Wl,-rpath," + PACKAGE_DIR,
+            "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-ltorch_hip", "-ltorch_python", "-lamdhip64"]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.run(cmd, check=True)
+    os.replace(out + ".tmp", out)
 
 
 def load_extension():

@@ -74,6 +74,8 @@ _SIGS = {
     "orf_batch_counters": [c_vp, c_vp],
     "orf_wavefront_autotune": [c_vp, c_vp, ctypes.c_int32, c_vp, ctypes.c_int32, c_u32, c_vp, ctypes.c_char_p,
                                ctypes.c_size_t],
+    "orf_diff_render": [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp, c_vp, c_vp, c_vp],
+    "orf_diff_adjoint": [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp],
 }
 for _name, _args in _SIGS.items():
     _f = getattr(lib, _name)
@@ -195,6 +197,26 @@ class WavefrontConfig(ctypes.Structure):
         c = cls()
         call("orf_wavefront_default_config", ctypes.byref(c))
         return c
+
+
+class DiffDesc(ctypes.Structure):
+    """orf_diff_desc: which materials carry the albedo / roughness textures."""
+
+    _fields_ = [("albedo_material", ctypes.c_int32), ("albedo_uv", ctypes.c_int32),
+                ("albedo_uv_params", ctypes.c_float * 4), ("roughness_material", ctypes.c_int32),
+                ("roughness_uv", ctypes.c_int32), ("roughness_uv_params", ctypes.c_float * 4),
+                ("tex_w", ctypes.c_int32), ("tex_h", ctypes.c_int32)]
+
+    @classmethod
+    def make(cls, albedo_material, roughness_material, tex_size, albedo_uv=(0, (-10.0, -10.0, 10.0, 10.0)),
+             roughness_uv=(1, (0.0, 0.0, 0.0, 0.0))):
+        d = cls()
+        d.albedo_material, d.roughness_material = albedo_material, roughness_material
+        d.albedo_uv, d.roughness_uv = albedo_uv[0], roughness_uv[0]
+        d.albedo_uv_params[:] = [float(x) for x in albedo_uv[1]]
+        d.roughness_uv_params[:] = [float(x) for x in roughness_uv[1]]
+        d.tex_w, d.tex_h = tex_size
+        return d
 
 
 class Context:
@@ -366,6 +388,28 @@ class Context:
         out = np.zeros(4, np.uint64)
         call("orf_batch_counters", self.handle, _np_ptr(out))
         return dict(zip(("nonfinite", "ext_rays", "shadow_rays", "paths"), [int(x) for x in out]))
+
+    def diff_render(self, desc, views, albedo, roughness, images=None, stream=None):
+        """Per-view mean images (nviews, h, w, 3) of the textured scene (device tensors)."""
+        arr = self._items(views)
+        if images is None:
+            images = torch.empty((len(views), views[0]["height"], views[0]["width"], 3), dtype=torch.float32,
+                                 device=(albedo if albedo is not None else roughness).device)
+        call("orf_diff_render", self.handle, ctypes.byref(desc), arr, len(views), ptr(albedo), ptr(roughness),
+             ptr(images), stream_ptr(stream))
+        return images
+
+    def diff_adjoint(self, desc, views, albedo, roughness, d_images, g_albedo=None, g_roughness=None,
+                     frame_shift=0x40000000, stream=None):
+        """Adds dLoss/dtexture into (g_albedo, g_roughness) (zeroed here when not given)."""
+        arr = self._items(views)
+        if g_albedo is None and albedo is not None:
+            g_albedo = torch.zeros_like(albedo)
+        if g_roughness is None and roughness is not None:
+            g_roughness = torch.zeros_like(roughness)
+        call("orf_diff_adjoint", self.handle, ctypes.byref(desc), arr, len(views), ptr(albedo), ptr(roughness),
+             ptr(d_images), ptr(g_albedo), ptr(g_roughness), frame_shift, stream_ptr(stream))
+        return g_albedo, g_roughness
 
     def autotune(self, items, remap=None, spp_divisor=16):
         """(best WavefrontConfig, report dict) of the wavefront coordinate search."""

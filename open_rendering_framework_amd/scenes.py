@@ -218,6 +218,38 @@ def batch_workload(n=64, seed=1234, level=7, env_size=(1024, 512), max_depth=8, 
     return desc, items, np.array(remap, np.int32)
 
 
+def ring_views(n=8, size=256, spp=16, radius=4.0, height=1.5, frame0=0, vfov=40.0):
+    """`n` pinhole views on a ring of `radius` at `height`, looking at the origin;
+    view i starts at frame frame0 + i * spp."""
+    views = []
+    for i in range(n):
+        a = 2.0 * math.pi * i / n
+        eye = (radius * math.sin(a), height, radius * math.cos(a))
+        e, U, V, W = pinhole(eye, (0.0, 0.0, 0.0), (0.0, 1.0, 0.0), vfov, size, size)
+        views.append({"width": size, "height": size, "spp": spp, "frame0": frame0 + i * spp,
+                      "eye": e, "U": U, "V": V, "W": W})
+    return views
+
+
+def diff_textures(tex=64, seed=99, albedo_range=(0.2, 0.8), rough_range=(0.1, 0.6)):
+    """Random albedo (tex, tex, 3) and roughness (tex, tex) textures, PCG32(seed, 1/2)."""
+    ga, gr = Pcg32(seed, 1), Pcg32(seed, 2)
+    a = np.array([ga.uniform() for _ in range(tex * tex * 3)], np.float32).reshape(tex, tex, 3)
+    r = np.array([gr.uniform() for _ in range(tex * tex)], np.float32).reshape(tex, tex)
+    a = albedo_range[0] + (albedo_range[1] - albedo_range[0]) * a
+    r = rough_range[0] + (rough_range[1] - rough_range[0]) * r
+    return a.astype(np.float32), r.astype(np.float32)
+
+
+# Differentiable form of path_scene: albedo texture on the Lambert ground
+# (planar x,z map over the 20 x 20 quad), roughness texture on the conductor
+# sphere (spherical map about the origin).
+DIFF_ALBEDO_MATERIAL = 0
+DIFF_ROUGHNESS_MATERIAL = 1
+DIFF_ALBEDO_UV = (0, (-10.0, -10.0, 10.0, 10.0))
+DIFF_ROUGHNESS_UV = (1, (0.0, 0.0, 0.0, 0.0))
+
+
 def to_context(desc, ctx=None):
     """Create (or fill) a render context from a SceneDesc through the C-ABI."""
     from . import rt

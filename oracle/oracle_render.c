@@ -20,6 +20,8 @@
 #include <string.h>
 #include <tgmath.h>
 
+#include "oracle_fmath.h"
+
 #define PI_R ((R)3.14159265358979323846)
 #define INV_PI_R ((R)0.318309886183790671538)
 #define RAY_MAX ((R)1e27f)
@@ -61,7 +63,19 @@ typedef struct {
     v3 eye, U, V, W;
     int32_t max_depth;
     R eps;
+    /* differentiable form (new, no reference counterpart): textured Lambert albedo
+     * and GGX-conductor roughness; *_eval shade, *_frozen drive sampling and RR */
+    int32_t diff_on, tex_w, tex_h, alb_mat, rough_mat, alb_uv, rough_uv;
+    R alb_uvp[4], rough_uvp[4];
+    R* alb_eval;
+    R* alb_frozen;
+    R* rough_eval;
+    R* rough_frozen;
 } oscene;
+
+typedef struct {
+    v3 eye, U, V, W;
+} ocam;
 
 /* ---------------------------------------------------------------- BVH */
 
@@ -317,8 +331,8 @@ static v3 tex_bilinear(const oscene* s, R u, R v)
 /* env_lookup (ORF/Envmap.h:57-64). */
 static v3 env_lookup(const oscene* s, v3 dir)
 {
-    R theta = acos(dir.y);
-    R phi = atan2(dir.x, dir.z);
+    R theta = F_ACOS(dir.y);
+    R phi = F_ATAN2(dir.x, dir.z);
     R u = (phi + PI_R) * (R)0.5f * INV_PI_R;
     R v = (R)(1.0 - (double)(theta * INV_PI_R));
     return tex_bilinear(s, u, v);
@@ -400,8 +414,8 @@ static void build_env_tables(oscene* s)
 /* Solid-angle density of env_sample in direction dir (new: MIS). */
 static R env_pdf(const oscene* s, v3 dir)
 {
-    R theta = acos(dir.y);
-    R phi = atan2(dir.x, dir.z);
+    R theta = F_ACOS(dir.y);
+    R phi = F_ATAN2(dir.x, dir.z);
     R u = (phi + PI_R) * (R)0.5f * INV_PI_R;
     R v = (R)1 - theta * INV_PI_R;
     int iu = (int)(u * (R)s->env_w), iv = (int)(v * (R)s->env_h);
@@ -410,7 +424,7 @@ static R env_pdf(const oscene* s, v3 dir)
     if (iv < 0) iv = 0;
     if (iv > s->env_h - 1) iv = s->env_h - 1;
     R p = s->mpdf[iv] * s->cpdf[(size_t)iv * s->env_w + iu];
-    R st = sin(theta);
+    R st = F_SIN(theta);
     return st > 0 ? p / ((R)2 * PI_R * PI_R * st) : 0;
 }
 
@@ -431,7 +445,7 @@ static void env_sample(const oscene* s, uint32_t* t, v3* dir, v3* L, R* pdf_sa)
     R prob = pdf_m * pdf_c;
     R theta = ((R)1 - v) * PI_R;
     R phi = ((R)2 * u - (R)1) * PI_R;
-    R st = sin(theta), ct = cos(theta), sp = sin(phi), cp = cos(phi);
+    R st = F_SIN(theta), ct = F_COS(theta), sp = F_SIN(phi), cp = F_COS(phi);
     *dir = V3(st * sp, ct, st * cp);
     v3 E = tex_bilinear(s, u, v);
     *L = vscale(E, st * two_pi2 / prob);
@@ -521,6 +535,26 @@ static v3 hit_normal(const oscene* s, const ohit* h)
     return orc_normalize(n);
 }
 
+/* Differentiable form (new): nearest texel of a planar (x,z) map over
+ * [uvp0,uvp2] x [uvp1,uvp3] (mode 0) or a spherical map about uvp[0..2] (mode 1).
+ * Parity unpinned beyond the GPU comparison: no reference counterpart. */
+static int diff_texel(int mode, const R* uvp, v3 p, int tw, int th)
+{
+    R u, v;
+    if (mode == 0) {
+        u = (p.x - uvp[0]) / (uvp[2] - uvp[0]);
+        v = (p.z - uvp[1]) / (uvp[3] - uvp[1]);
+    } else {
+        v3 q = orc_normalize(V3(p.x - uvp[0], p.y - uvp[1], p.z - uvp[2]));
+        u = (F_ATAN2(q.x, q.z) + PI_R) * (R)0.5f * INV_PI_R;
+        v = F_ACOS(fmin((R)1, fmax((R)-1, q.y))) * INV_PI_R;
+    }
+    int iu = (int)floor(u * (R)tw), iv = (int)floor(v * (R)th);
+    iu = iu < 0 ? 0 : (iu > tw - 1 ? tw - 1 : iu);
+    iv = iv < 0 ? 0 : (iv > th - 1 ? th - 1 : iv);
+    return iv * tw + iu;
+}
+
 /* CU/diffuse_shader.cu:122-187 (+ environment MIS, new). */
 static v3 shade_lambert(const oscene* s, const orc_material* m, v3 o, v3 d, const ohit* h, int depth,
                         int* emit_light, uint32_t* t, ostats* st)
@@ -531,6 +565,14 @@ static v3 shade_lambert(const oscene* s, const orc_material* m, v3 o, v3 d, cons
     v3 normal = hit_normal(s, h);
     v3 ffn = orc_faceforward(normal, vneg(d), normal);
     v3 rho = V3(m->diffuse[0], m->diffuse[1], m->diffuse[2]);
+    R prob = (rho.x + rho.y + rho.z) / (R)3.0f;
+    if (s->diff_on && s->alb_eval && (int32_t)(m - s->mats) == s->alb_mat) {
+        /* textured albedo (new): evaluate at rho_eval, roulette on the frozen albedo */
+        int k = diff_texel(s->alb_uv, s->alb_uvp, hit_pos, s->tex_w, s->tex_h);
+        rho = V3(s->alb_eval[3 * k], s->alb_eval[3 * k + 1], s->alb_eval[3 * k + 2]);
+        const R* f = s->alb_frozen + 3 * k;
+        prob = (f[0] + f[1] + f[2]) / (R)3.0f;
+    }
     uint32_t light_idx = (uint32_t)((R)s->nlights * orc_rnd_tea(t));
     if (s->nlights > 0) {
         if (light_idx >= (uint32_t)s->nlights) light_idx = (uint32_t)s->nlights - 1; /* new: see header */
@@ -551,7 +593,6 @@ static v3 shade_lambert(const oscene* s, const orc_material* m, v3 o, v3 d, cons
             *emit_light = 0;
         }
     }
-    R prob = (rho.x + rho.y + rho.z) / (R)3.0f;
     R xi = orc_rnd_tea(t);
     if (xi < prob) {
         v3 nd = orc_sample_cosine_weighted(ffn, t);
@@ -573,6 +614,13 @@ static v3 shade_conductor(const oscene* s, const orc_material* m, const cplx3* e
     v3 ffn = orc_faceforward(normal, vneg(d), normal);
     v3 w_i = vneg(d);
     R ax = m->roughness[0], ay = m->roughness[1];
+    R ae = ax;  /* evaluation roughness of the differentiable form */
+    if (s->diff_on && s->rough_eval && (int32_t)(m - s->mats) == s->rough_mat) {
+        /* textured roughness (new): sample with the frozen alpha, evaluate at alpha_eval */
+        int k = diff_texel(s->rough_uv, s->rough_uvp, hit_point, s->tex_w, s->tex_h);
+        ax = ay = s->rough_frozen[k];
+        ae = s->rough_eval[k];
+    }
     R z1 = orc_rnd_tea(seed);
     R z2 = orc_rnd_tea(seed);
     v3 mn = orc_to_world(orc_ggx_sample_vndf(orc_to_local(w_i, ffn), ax, ay, z1, z2), ffn);
@@ -584,6 +632,15 @@ static v3 shade_conductor(const oscene* s, const orc_material* m, const cplx3* e
     if (G_o <= 0) return V3(0, 0, 0);
     v3 child = radiance(s, hit_point, w_o, depth + 1, 1, seed, (R)-1, st);
     v3 weight = vscale(F, G_o);
+    if (ae != ax) {
+        /* f cos / pdf at alpha_eval for a direction drawn with the frozen alpha:
+         * F G1(wi) G1(wo) D(m) / (G1_0(wi) D_0(m)) with the separable G2 the
+         * reference uses (CU/metallic_shader.cu:117-121) */
+        v3 ml = orc_to_local(mn, ffn);
+        R num = orc_ggx_G1(w_i, mn, ffn, ae, ae) * orc_ggx_G1(w_o, mn, ffn, ae, ae) * orc_ggx_eval_ndf(ml, ae, ae);
+        R den = G_i * orc_ggx_eval_ndf(ml, ax, ay);
+        weight = vscale(F, num / den);
+    }
     return vmul(child, weight);
 }
 
@@ -614,7 +671,7 @@ static v3 shade_dielectric(const oscene* s, const orc_material* m, v3 o, v3 d, c
     v3 beam = V3(1, 1, 1);
     R cos_theta = vdot(vneg(d), normal);
     if (cos_theta < 0) {
-        beam = V3(exp(-h->t * (R)m->absorption[0]), exp(-h->t * (R)m->absorption[1]), exp(-h->t * (R)m->absorption[2]));
+        beam = V3(F_EXP(-h->t * (R)m->absorption[0]), F_EXP(-h->t * (R)m->absorption[1]), F_EXP(-h->t * (R)m->absorption[2]));
         R prob = (beam.x + beam.y + beam.z) / (R)3.0f;
         if (orc_rnd_tea(t) >= prob) return zero;
         beam = V3(beam.x / prob, beam.y / prob, beam.z / prob);
@@ -666,15 +723,21 @@ static v3 radiance(const oscene* s, v3 o, v3 d, int depth, int emit_light, uint3
 }
 
 /* path_tracer (CU/path_tracer.cu:32-74) for one (pixel, frame). */
-static v3 pixel_sample(const oscene* s, int W, int H, int x, int y, uint32_t frame, ostats* st)
+static v3 pixel_sample_cam(const oscene* s, const ocam* c, int W, int H, int x, int y, uint32_t frame, ostats* st)
 {
     uint32_t seed = orc_tea16((uint32_t)W * (uint32_t)y + (uint32_t)x, frame);
     R jx = orc_rnd_tea(&seed);
     R jy = orc_rnd_tea(&seed);
     R ipx = ((R)x + jx) / (R)W * (R)2.0f - (R)1.0f;
     R ipy = ((R)y + jy) / (R)H * (R)2.0f - (R)1.0f;
-    v3 dir = orc_normalize(vadd(vadd(vscale(s->U, ipx), vscale(s->V, ipy)), s->W));
-    return radiance(s, s->eye, dir, 0, 1, &seed, (R)-1, st);
+    v3 dir = orc_normalize(vadd(vadd(vscale(c->U, ipx), vscale(c->V, ipy)), c->W));
+    return radiance(s, c->eye, dir, 0, 1, &seed, (R)-1, st);
+}
+
+static v3 pixel_sample(const oscene* s, int W, int H, int x, int y, uint32_t frame, ostats* st)
+{
+    ocam c = {s->eye, s->U, s->V, s->W};
+    return pixel_sample_cam(s, &c, W, H, x, y, frame, st);
 }
 
 /* ---------------------------------------------------------------- API */
@@ -764,6 +827,10 @@ void orc_scene_destroy(void* p)
     free(s->mcdf);
     free(s->cpdf);
     free(s->ccdf);
+    free(s->alb_eval);
+    free(s->alb_frozen);
+    free(s->rough_eval);
+    free(s->rough_frozen);
     free(s);
 }
 
@@ -862,4 +929,85 @@ void orc_env_sample_batch(void* p, uint32_t seed, int n, double* dir, double* pd
         pdf_sample[i] = (double)pdf;
         pdf_eval[i] = (double)env_pdf(s, d);
     }
+}
+
+/* ---------------------------------------------------------------- differentiable form (new) */
+
+static R* dup_tex(const double* src, size_t n)
+{
+    if (!src) return NULL;
+    R* d = (R*)malloc(sizeof(R) * (n + 1));
+    for (size_t i = 0; i < n; i++) d[i] = (R)src[i];
+    return d;
+}
+
+/* Textures of the differentiable form.  albedo_* are tex_h x tex_w x 3, rough_*
+ * tex_h x tex_w; *_frozen (NULL: same as *_eval) drive sampling and Russian
+ * roulette, *_eval the BSDF weights and NEE, so the estimator is a smooth
+ * function of *_eval at fixed random numbers (what a finite difference of the
+ * detached-sampling adjoint differentiates).  mat < 0 disables a texture. */
+void orc_diff_set(void* p, int tex_w, int tex_h, int alb_mat, int alb_uv, const double* alb_uvp, int rough_mat,
+                  int rough_uv, const double* rough_uvp, const double* albedo_eval, const double* albedo_frozen,
+                  const double* rough_eval, const double* rough_frozen)
+{
+    oscene* s = (oscene*)p;
+    free(s->alb_eval);
+    free(s->alb_frozen);
+    free(s->rough_eval);
+    free(s->rough_frozen);
+    size_t n = (size_t)tex_w * (size_t)tex_h;
+    s->tex_w = tex_w;
+    s->tex_h = tex_h;
+    s->alb_mat = alb_mat;
+    s->rough_mat = rough_mat;
+    s->alb_uv = alb_uv;
+    s->rough_uv = rough_uv;
+    for (int k = 0; k < 4; k++) {
+        s->alb_uvp[k] = (R)alb_uvp[k];
+        s->rough_uvp[k] = (R)rough_uvp[k];
+    }
+    s->alb_eval = alb_mat >= 0 ? dup_tex(albedo_eval, 3 * n) : NULL;
+    s->alb_frozen = alb_mat >= 0 ? dup_tex(albedo_frozen ? albedo_frozen : albedo_eval, 3 * n) : NULL;
+    s->rough_eval = rough_mat >= 0 ? dup_tex(rough_eval, n) : NULL;
+    s->rough_frozen = rough_mat >= 0 ? dup_tex(rough_frozen ? rough_frozen : rough_eval, n) : NULL;
+    s->diff_on = (s->alb_eval || s->rough_eval) && n > 0;
+}
+
+/* Views: cams holds 12 floats per view (eye, U, V, W).  images (may be NULL)
+ * receives per-view mean images over the finite samples of frames
+ * frame0[v] + shift + s, s < spp; the return value is
+ * sum over views, pixels, channels of dimg * (sum of finite samples) / spp,
+ * the linear loss whose gradient the adjoint accumulates. */
+double orc_diff_eval(void* p, int W, int H, int nviews, const float* cams, const uint32_t* frame0, uint32_t spp,
+                     uint32_t shift, const double* dimg, double* images, int threads)
+{
+    const oscene* s = (const oscene*)p;
+    double loss = 0;
+    int64_t npx = (int64_t)W * H;
+    for (int v = 0; v < nviews; v++) {
+        const float* c = cams + 12 * v;
+        ocam cam = {V3(c[0], c[1], c[2]), V3(c[3], c[4], c[5]), V3(c[6], c[7], c[8]), V3(c[9], c[10], c[11])};
+        double part = 0;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(threads) reduction(+ : part)
+        for (int64_t i = 0; i < npx; i++) {
+            int x = (int)(i % W), y = (int)(i / W);
+            double acc[3] = {0, 0, 0}, cnt = 0;
+            ostats st = {0, 0};
+            for (uint32_t f = 0; f < spp; f++) {
+                v3 L = pixel_sample_cam(s, &cam, W, H, x, y, frame0[v] + shift + f, &st);
+                if (isfinite(L.x) && isfinite(L.y) && isfinite(L.z)) {
+                    acc[0] += (double)L.x;
+                    acc[1] += (double)L.y;
+                    acc[2] += (double)L.z;
+                    cnt += 1;
+                }
+            }
+            size_t o = 3 * ((size_t)v * npx + i);
+            if (dimg) part += (dimg[o] * acc[0] + dimg[o + 1] * acc[1] + dimg[o + 2] * acc[2]) / (double)spp;
+            if (images)
+                for (int k = 0; k < 3; k++) images[o + k] = cnt > 0 ? acc[k] / cnt : 0.0;
+        }
+        loss += part;
+    }
+    return loss;
 }

@@ -79,5 +79,12 @@ int orc_env_bsearch(const double* cdf, int n, double xi);
 /* Closest hit of a batch of rays against the oracle BVH (t, prim) and by brute force. */
 void orc_closest_hits(void* s, int n, const float* o, const float* d, float tmin, float* t, int32_t* prim,
                       int brute_force);
+/* Differentiable form (new): textured albedo / roughness with frozen sampling,
+ * and per-view evaluation of images and of the linear loss sum(dimg * mean L). */
+void orc_diff_set(void* s, int tex_w, int tex_h, int alb_mat, int alb_uv, const double* alb_uvp, int rough_mat,
+                  int rough_uv, const double* rough_uvp, const double* albedo_eval, const double* albedo_frozen,
+                  const double* rough_eval, const double* rough_frozen);
+double orc_diff_eval(void* s, int W, int H, int nviews, const float* cams, const uint32_t* frame0, uint32_t spp,
+                     uint32_t shift, const double* dimg, double* images, int threads);
 
 #endif

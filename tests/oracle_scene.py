@@ -59,6 +59,11 @@ class OracleScene:
         L.orc_env_bsearch.restype = ctypes.c_int
         L.orc_env_bsearch.argtypes = [c_vp, ctypes.c_int, ctypes.c_double]
         L.orc_closest_hits.argtypes = [c_vp, ctypes.c_int, c_vp, c_vp, ctypes.c_float, c_vp, c_vp, ctypes.c_int]
+        L.orc_diff_set.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_vp, ctypes.c_int,
+                                   ctypes.c_int, c_vp, c_vp, c_vp, c_vp, c_vp]
+        L.orc_diff_eval.restype = ctypes.c_double
+        L.orc_diff_eval.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_vp, c_vp, ctypes.c_uint32,
+                                    ctypes.c_uint32, c_vp, c_vp, ctypes.c_int]
         self.desc = desc
         V, T, N, NI, M = desc.flat()
         self.arrays = [np.ascontiguousarray(V), np.ascontiguousarray(T), np.ascontiguousarray(N),
@@ -179,3 +184,30 @@ class OracleScene:
         prim = np.zeros(len(o), np.int32)
         self.lib.orc_closest_hits(self.handle, len(o), _p(o), _p(d), tmin, _p(t), _p(prim), 1 if brute_force else 0)
         return t, prim
+
+    # ------------------------------------------------ differentiable form
+    def diff_set(self, tex_size, albedo_material, roughness_material, albedo=None, roughness=None,
+                 albedo_frozen=None, roughness_frozen=None, albedo_uv=(0, (-10.0, -10.0, 10.0, 10.0)),
+                 roughness_uv=(1, (0.0, 0.0, 0.0, 0.0))):
+        """Textured albedo (tex_h, tex_w, 3) on a Lambert material and roughness
+        (tex_h, tex_w) on a GGX conductor; *_frozen (default: the same values)
+        drive sampling and Russian roulette."""
+        tw, th = tex_size
+        f64 = lambda a: None if a is None else np.ascontiguousarray(a, np.float64)
+        self._diff_keep = [f64(albedo), f64(albedo_frozen), f64(roughness), f64(roughness_frozen),
+                           np.asarray(albedo_uv[1], np.float64), np.asarray(roughness_uv[1], np.float64)]
+        a, af, r, rf, au, ru = self._diff_keep
+        self.lib.orc_diff_set(self.handle, tw, th, albedo_material, albedo_uv[0], _p(au), roughness_material,
+                              roughness_uv[0], _p(ru), _p(a), _p(af), _p(r), _p(rf))
+
+    def diff_eval(self, views, shift=0, dimg=None, images=True, threads=8):
+        """(loss, images): per-view mean images over frames frame0 + shift + s and
+        the linear loss sum(dimg * sum_s L_s / spp)."""
+        W, H, spp = views[0]["width"], views[0]["height"], views[0]["spp"]
+        cams = np.array([list(v["eye"]) + list(v["U"]) + list(v["V"]) + list(v["W"]) for v in views], np.float32)
+        f0 = np.array([v["frame0"] for v in views], np.uint32)
+        img = np.zeros((len(views), H, W, 3)) if images else None
+        dd = None if dimg is None else np.ascontiguousarray(dimg, np.float64)
+        loss = self.lib.orc_diff_eval(self.handle, W, H, len(views), _p(cams), _p(f0), spp, shift, _p(dd), _p(img),
+                                      threads)
+        return loss, img
