@@ -9,6 +9,10 @@ Workloads (--workload):
          One step = one launch of all 64 samples of every pixel (16.8 M paths).
   bsdf : rough-conductor BSDF operator, 2^22 independent FP32 queries
          (PCG32 seed 42 / stream 54), complex IOR 0.27+2.78i.
+  batch: 64 uneven renders of the scene family, autotuned wavefront vs one
+         megakernel launch per render.
+  diff : inverse rendering step (primal + L2 loss + path-replay adjoint +
+         texture update) over 8 ring views at 256x256, 16 + 16 spp.
 
 A step is one pass of the hot path over one batch of synthetic input that is
 already resident in HBM.  The timed region is bracketed by
@@ -160,6 +164,8 @@ def bench_path(args):
     }
     if not args.no_batch:
         line["batched"] = batch_measure(1, 1)
+    if not args.no_diff:
+        line["differentiable"] = diff_measure(5, 1)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_path(desc)
     return line
@@ -231,7 +237,87 @@ def bench_batch(args):
     return line
 
 
-WORKLOADS = {"path": bench_path, "bsdf": bench_bsdf, "batch": bench_batch}
+# --------------------------------------------------------------------- diff
+DIFF_SIZE, DIFF_VIEWS, DIFF_SPP, DIFF_TEX = 256, 8, 16, 64
+
+
+def diff_setup(size=DIFF_SIZE, views_n=DIFF_VIEWS, spp=DIFF_SPP, tex=DIFF_TEX, level=7):
+    """Config-5 inverse rendering: the path-tracer scene with a roughness texture on
+    the conductor sphere and an albedo texture on the ground, 8 ring views, targets
+    rendered from the seed-99 ground-truth textures; the optimisation starts from
+    constant textures (albedo 0.5, alpha 0.3)."""
+    import torch
+
+    from open_rendering_framework_amd import diff, scenes
+
+    desc = scenes.path_scene(width=size, height=size, level=level)
+    ctx = scenes.to_context(desc)
+    ctx.accel_build()
+    views = scenes.ring_views(n=views_n, size=size, spp=spp)
+    R = diff.DiffRenderer(ctx, views, scenes.DIFF_ALBEDO_MATERIAL, scenes.DIFF_ROUGHNESS_MATERIAL, (tex, tex),
+                          scenes.DIFF_ALBEDO_UV, scenes.DIFF_ROUGHNESS_UV)
+    a_gt, r_gt = scenes.diff_textures(tex, seed=99, rough_range=(0.05, 0.8))
+    with torch.no_grad():
+        target = R(torch.from_numpy(a_gt).cuda(), torch.from_numpy(r_gt).cuda())
+    alb = torch.full((tex, tex, 3), 0.5, device="cuda", requires_grad=True)
+    rough = torch.full((tex, tex), 0.3, device="cuda", requires_grad=True)
+    return ctx, R, target, alb, rough
+
+
+def diff_step(R, target, alb, rough, lr_a=0.5, lr_r=0.5):
+    """One gradient step: primal render, L2 image loss, path-replay adjoint, update."""
+    import torch
+
+    img = R(alb, rough)
+    loss = ((img - target) ** 2).mean()
+    loss.backward()
+    with torch.no_grad():
+        alb -= lr_a * alb.grad
+        rough -= lr_r * rough.grad
+        alb.clamp_(0.02, 0.98)
+        rough.clamp_(0.05, 0.8)
+        alb.grad = None
+        rough.grad = None
+    return loss
+
+
+def diff_measure(steps, warmup):
+    import torch
+
+    ctx, R, target, alb, rough = diff_setup()
+    losses = []
+    for _ in range(max(1, warmup)):
+        losses.append(float(diff_step(R, target, alb, rough)))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        loss = diff_step(R, target, alb, rough)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    losses.append(float(loss))
+    R.close()
+    paths = DIFF_SIZE * DIFF_SIZE * DIFF_SPP * DIFF_VIEWS
+    return {"grad_steps_per_s": steps / wall, "ms_per_step": wall * 1e3 / steps, "views": DIFF_VIEWS,
+            "size": DIFF_SIZE, "spp_primal": DIFF_SPP, "spp_adjoint": DIFF_SPP, "texture": DIFF_TEX,
+            "primal_paths_per_step": paths, "replayed_paths_per_step": paths,
+            "loss_first": losses[0], "loss_last": losses[-1], "steps": steps}
+
+
+def bench_diff(args):
+    d = diff_measure(args.steps, args.warmup)
+    return {
+        "metric": "adjoint grad-steps/s (8 views 256^2, 16+16 spp, 64x64 albedo + roughness textures)",
+        "value": d["grad_steps_per_s"],
+        "unit": "grad-steps/s",
+        "ms_per_step": d["ms_per_step"],
+        "dtype": "f32",
+        "config": {"workload": "differentiable_inverse_rendering", "views": DIFF_VIEWS, "size": DIFF_SIZE,
+                   "spp_primal": DIFF_SPP, "spp_adjoint": DIFF_SPP, "texture": DIFF_TEX},
+        "differentiable": d,
+    }
+
+
+WORKLOADS = {"path": bench_path, "bsdf": bench_bsdf, "batch": bench_batch, "diff": bench_diff}
 
 
 def main():
@@ -242,6 +328,7 @@ def main():
     ap.add_argument("--workload", default="path", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-wavefront measurement")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-render measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
