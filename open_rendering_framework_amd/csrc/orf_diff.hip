@@ -37,7 +37,7 @@ __global__ __launch_bounds__(kDiffBlock) void k_diff(SceneGPU sc, DiffParams dp,
                                                      float* g_albedo, float* g_rough,
                                                      unsigned long long* counters)
 {
-    __shared__ unsigned stack_lds[kStackDepth * kDiffBlock];
+    __shared__ unsigned stack_lds[kLdsStack * kDiffBlock];
     unsigned* stk = stack_lds + threadIdx.x;
     int view = blockIdx.x / tiles_per_view;
     int tile = blockIdx.x % tiles_per_view;

@@ -13,12 +13,16 @@
 namespace orf {
 
 constexpr int kMegaBlock = 64;
+// Minimum waves per SIMD the register allocator must allow (occupancy vs spills).
+#ifndef ORF_MEGA_WAVES
+#define ORF_MEGA_WAVES 4
+#endif
 
-__global__ __launch_bounds__(kMegaBlock) void k_render_mega(SceneGPU sc, CameraGPU cam, FilmGPU film,
+__global__ __launch_bounds__(kMegaBlock, ORF_MEGA_WAVES) void k_render_mega(SceneGPU sc, CameraGPU cam, FilmGPU film,
                                                             const int* remap, unsigned frame0, unsigned spp,
                                                             int chunks, unsigned per, int tiles_x)
 {
-    __shared__ unsigned stack_lds[kStackDepth * kMegaBlock];
+    __shared__ unsigned stack_lds[kLdsStack * kMegaBlock];
     unsigned* stk = stack_lds + threadIdx.x;
     int tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
     int x = (tile % tiles_x) * 8 + (threadIdx.x & 7);

@@ -106,6 +106,14 @@ struct SceneGPU {
 
 constexpr float kRayMax = 1e27f;  // RT_DEFAULT_MAX
 constexpr int kStackDepth = 48;   // builder caps BVH depth below this
+// Traversal stack entries per lane kept in LDS; deeper entries spill to a
+// private (scratch) array.  Small LDS columns keep traversal kernels at high
+// occupancy (LDS per wave64 = 256 B * kLdsStack).
+#ifndef ORF_LDS_STACK
+#define ORF_LDS_STACK 16
+#endif
+constexpr int kLdsStack = ORF_LDS_STACK;
+static_assert(kLdsStack >= 1 && kLdsStack <= kStackDepth, "bad LDS stack size");
 
 // BVH child link: internal node index, or leaf = 0x80000000 | (count-1) << 27 | first
 ORF_HD bool link_is_leaf(unsigned c) { return (c & 0x80000000u) != 0; }

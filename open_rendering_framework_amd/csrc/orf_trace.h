@@ -11,8 +11,9 @@
 // e1 x e2.  A hit is accepted when tmin < t < t_best (strict).
 // Box test: slabs with precomputed 1/d and o/d (explicit fma), far distance
 // inflated by 1 + 2^-21 so rounding never culls a box containing a hit.
-// Traversal stack: per-lane column of a workgroup's LDS array (stride = block
-// size), depth kStackDepth (the builder bounds the tree depth below it).
+// Traversal stack: the first kLdsStack entries of a lane live in a column of
+// the workgroup's LDS array (stride = block size), entries up to kStackDepth in
+// a private array (the builder bounds the tree depth below kStackDepth).
 #pragma once
 
 #include "orf_scene.h"
@@ -84,6 +85,16 @@ ORF_HD bool trace(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, Hit& h
     float hb = 0.0f, hg = 0.0f;
     unsigned node = 0;
     int sp = 0;
+    unsigned ovf[kStackDepth - kLdsStack + 1];
+    auto push = [&](unsigned v) {
+        if (sp < kLdsStack) stk[stride * sp] = v;
+        else ovf[sp - kLdsStack] = v;
+        ++sp;
+    };
+    auto pop = [&]() -> unsigned {
+        --sp;
+        return sp < kLdsStack ? stk[stride * sp] : ovf[sp - kLdsStack];
+    };
     bool have = true;  // `node` holds a live internal node or leaf link
     while (have) {
         if (!link_is_leaf(node)) {
@@ -96,7 +107,7 @@ ORF_HD bool trace(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, Hit& h
             if (h0 && h1) {
                 unsigned nearc = e0 <= e1 ? l0 : l1;
                 unsigned farc = e0 <= e1 ? l1 : l0;
-                if (sp < kStackDepth) stk[stride * sp++] = farc;
+                if (sp < kStackDepth) push(farc);
                 node = nearc;
             } else if (h0) {
                 node = l0;
@@ -104,7 +115,7 @@ ORF_HD bool trace(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, Hit& h
                 node = l1;
             } else {
                 if (sp == 0) break;
-                node = stk[stride * --sp];
+                node = pop();
             }
             continue;
         }
@@ -131,7 +142,7 @@ ORF_HD bool trace(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, Hit& h
             }
         }
         if (sp == 0) break;
-        node = stk[stride * --sp];
+        node = pop();
     }
     h.t = tbest;
     h.b = hb;

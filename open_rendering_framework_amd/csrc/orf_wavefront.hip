@@ -71,7 +71,7 @@ __global__ __launch_bounds__(256) void k_wf_regen(WfView v, int parity)
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_wf_extend(WfView v, int parity)
 {
-    __shared__ unsigned stack_lds[kStackDepth * BLOCK];
+    __shared__ unsigned stack_lds[kLdsStack * BLOCK];
     unsigned* stk = stack_lds + threadIdx.x;
     unsigned n = v.cnt[C_EXT0 + parity];
     const int* q = v.ext_q[parity];
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfView v, int parity, int kind
 template <int BLOCK>
 __global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfView v)
 {
-    __shared__ unsigned stack_lds[kStackDepth * BLOCK];
+    __shared__ unsigned stack_lds[kLdsStack * BLOCK];
     unsigned* stk = stack_lds + threadIdx.x;
     unsigned n = v.cnt[C_SHADOW];
     unsigned rays = 0;

@@ -169,3 +169,44 @@ void orc_cosine_batch(int n, const float *nrm, uint32_t seed, float *out)
         out[3 * i] = (float)d.x; out[3 * i + 1] = (float)d.y; out[3 * i + 2] = (float)d.z;
     }
 }
+
+/* Beckmann restatements, vectorised for the closed-form pinning tests. */
+void orc_beckmann_ndf_batch(int n, const double *wm, double ax, double ay, double *out)
+{
+    for (int i = 0; i < n; i++)
+        out[i] = (double)orc_beckmann_eval_ndf(V3((R)wm[3 * i], (R)wm[3 * i + 1], (R)wm[3 * i + 2]), (R)ax, (R)ay);
+}
+
+void orc_beckmann_vndf_batch(int n, const double *wi, const double *wm, double ax, double ay, double *out)
+{
+    for (int i = 0; i < n; i++)
+        out[i] = (double)orc_beckmann_eval_vndf(V3((R)wi[0], (R)wi[1], (R)wi[2]),
+                                                V3((R)wm[3 * i], (R)wm[3 * i + 1], (R)wm[3 * i + 2]), (R)ax, (R)ay);
+}
+
+void orc_beckmann_proj_batch(int n, const double *wi, double ax, double ay, double *out)
+{
+    for (int i = 0; i < n; i++)
+        out[i] = (double)orc_beckmann_projected_area(V3((R)wi[3 * i], (R)wi[3 * i + 1], (R)wi[3 * i + 2]), (R)ax,
+                                                     (R)ay);
+}
+
+void orc_beckmann_G1_cos_batch(int n, const double *cos_t, double alpha, double *out)
+{
+    for (int i = 0; i < n; i++) out[i] = (double)orc_beckmann_G1_cos((R)cos_t[i], (R)(alpha * alpha));
+}
+
+void orc_beckmann_sample_vndf_batch(int n, const float *wi, float ax, float ay, const float *z1, const float *z2,
+                                    float *out)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        v3 m = orc_beckmann_sample_vndf(V3(wi[0], wi[1], wi[2]), ax, ay, z1[i], z2[i]);
+        out[3 * i] = (float)m.x; out[3 * i + 1] = (float)m.y; out[3 * i + 2] = (float)m.z;
+    }
+}
+
+void orc_erfinv_batch(int n, const double *x, double *out)
+{
+    for (int i = 0; i < n; i++) out[i] = orc_erfinv(x[i]);
+}

@@ -40,6 +40,13 @@ class Oracle:
         L.orc_fresnel_conductor_batch.argtypes = [ctypes.c_int, c_vp, c_vp, c_vp, c_vp]
         L.orc_fresnel_dielectric_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_float, c_vp]
         L.orc_cosine_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_uint32, c_vp]
+        L.orc_beckmann_ndf_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_double, ctypes.c_double, c_vp]
+        L.orc_beckmann_vndf_batch.argtypes = [ctypes.c_int, c_vp, c_vp, ctypes.c_double, ctypes.c_double, c_vp]
+        L.orc_beckmann_proj_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_double, ctypes.c_double, c_vp]
+        L.orc_beckmann_G1_cos_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_double, c_vp]
+        L.orc_beckmann_sample_vndf_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_float, ctypes.c_float, c_vp, c_vp,
+                                                     c_vp]
+        L.orc_erfinv_batch.argtypes = [ctypes.c_int, c_vp, c_vp]
 
     # ---- RNG
     def tea(self, v0, v1, rounds=16, key=None):
@@ -133,4 +140,44 @@ class Oracle:
         nn = np.ascontiguousarray(nrm, np.float32)
         out = np.zeros((n, 3), np.float32)
         self.lib.orc_cosine_batch(n, _p(nn), seed, _p(out))
+        return out
+
+    # ---- Beckmann
+    def beckmann_ndf(self, wm, ax, ay):
+        wm = np.ascontiguousarray(wm, np.float64)
+        out = np.zeros(len(wm))
+        self.lib.orc_beckmann_ndf_batch(len(wm), _p(wm), ax, ay, _p(out))
+        return out
+
+    def beckmann_vndf(self, wi, wm, ax, ay):
+        wi = np.ascontiguousarray(wi, np.float64)
+        wm = np.ascontiguousarray(wm, np.float64)
+        out = np.zeros(len(wm))
+        self.lib.orc_beckmann_vndf_batch(len(wm), _p(wi), _p(wm), ax, ay, _p(out))
+        return out
+
+    def beckmann_proj(self, wi, ax, ay):
+        wi = np.ascontiguousarray(np.atleast_2d(wi), np.float64)
+        out = np.zeros(len(wi))
+        self.lib.orc_beckmann_proj_batch(len(wi), _p(wi), ax, ay, _p(out))
+        return out
+
+    def beckmann_G1_cos(self, cos_t, alpha):
+        c = np.ascontiguousarray(cos_t, np.float64)
+        out = np.zeros(len(c))
+        self.lib.orc_beckmann_G1_cos_batch(len(c), _p(c), alpha, _p(out))
+        return out
+
+    def beckmann_sample_vndf(self, wi, ax, ay, z1, z2):
+        wi = np.ascontiguousarray(wi, np.float32)
+        z1 = np.ascontiguousarray(z1, np.float32)
+        z2 = np.ascontiguousarray(z2, np.float32)
+        out = np.zeros((len(z1), 3), np.float32)
+        self.lib.orc_beckmann_sample_vndf_batch(len(z1), _p(wi), ax, ay, _p(z1), _p(z2), _p(out))
+        return out
+
+    def erfinv(self, x):
+        x = np.ascontiguousarray(x, np.float64)
+        out = np.zeros(len(x))
+        self.lib.orc_erfinv_batch(len(x), _p(x), _p(out))
         return out
