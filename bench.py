@@ -29,6 +29,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+# HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE passes of these exact
+# workloads, scaled by a 1 GiB copy kernel (profiles/hbm_traffic_pmc.md)
+TRAFFIC_PATH_512_64SPP = 8.76e9
+TRAFFIC_BSDF_2E22 = 218.4e6
 
 
 def _cpu_threads():
@@ -81,8 +85,9 @@ def bench_bsdf(args):
         "config": {"workload": "ggx_conductor_operator", "queries": n, "ior": "0.27+2.78i",
                    "inputs": "PCG32 seed 42 seq 54"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_ggx_conductor", "kernel_ms": kms, "bytes_per_unit": bytes_per_query},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_BSDF_2E22,
+                     "kernel": "k_ggx_conductor", "kernel_ms": kms, "bytes_per_unit": bytes_per_query,
+                     "algorithmic_bytes_per_launch": bytes_per_query * n},
     }
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_bsdf(host)
@@ -156,7 +161,8 @@ def bench_path(args):
                    "spp": PATH_SPP, "max_depth": 8, "triangles": int(stats["tris"]),
                    "env": "1024x512 sky+sun, importance sampled", "form": "megakernel"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_render_mega",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": TRAFFIC_PATH_512_64SPP,
+                     "algorithmic_bytes_per_launch": b_path * paths, "kernel": "k_render_mega",
                      "kernel_ms": kms, "bytes_per_unit": b_path, "n_ext_per_path": n_ext,
                      "n_shadow_per_path": n_shd},
         "bvh": stats,

@@ -186,6 +186,60 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         ps.prev_bsdf_pdf = -1.0f;
         return true;
     }
+    if (mat.kind == MAT_INTERFACE) {
+        // Lambertian interface (CU/lambertian_interface_shader.cu:51-173): a Lambert
+        // base under a smooth dielectric of index ior.  NEE through both Fresnel
+        // transmissions; the continuation is the specular reflection with the
+        // probability R_i of the incident Fresnel, else a cosine diffuse bounce.
+        if (ps.depth > sc.max_depth) return false;
+        f3 rho = mat.diffuse_color;
+        float n12 = 1.0f / mat.ior;
+        float r10 = two_C1(mat.ior);
+        f3 denom = mk3(1.0f - rho.x * r10, 1.0f - rho.y * r10, 1.0f - rho.z * r10);
+        float ci = dot(-d, ffn);
+        float Ri = fresnel_R_cos(ci, n12);
+        float Ti = 1.0f - Ri;
+        unsigned li = (unsigned)((float)sc.nlights * rnd_tea(ps.rng));
+        if (sc.nlights > 0) {
+            if (li >= (unsigned)sc.nlights) li = sc.nlights - 1;
+            float light_pdf = 1.0f / (float)sc.nlights;
+            LightGPU lt = sc.lights[li];
+            f3 wl = mk3(0.0f), rad = mk3(0.0f);
+            float dist = 0.0f, penv;
+            sample_light(sc, lt, p, ps.rng, wl, rad, dist, penv);
+            rad = rad / light_pdf;
+            float cl = dot(ffn, wl);
+            if (cl > 0.0f) {
+                float Tl = 1.0f - fresnel_R_cos(cl, n12);
+                f3 c = rad * kInvPi * cl * rho * n12 * n12 * Tl * Ti / denom;
+                if (penv >= 0.0f) c = c * mis_power(penv * light_pdf, cl * kInvPi);
+                sr.o = p;
+                sr.d = wl;
+                sr.tmax = dist;  // CU/lambertian_interface_shader.cu:100 (no epsilon)
+                sr.contrib = ps.thr * c;
+                sr.valid = true;
+                ps.emit = 0;
+            }
+        }
+        float xi = rnd_tea(ps.rng);
+        if (xi > Ri) {
+            f3 nd = sample_cosine(ffn, ps.rng);
+            float cr = dot(nd, ffn);
+            float Tr = 1.0f - fresnel_R_cos(cr, n12);
+            ps.prev_bsdf_pdf = sc.env.sampled ? cr * kInvPi : -1.0f;
+            ps.thr = ps.thr * (rho * n12 * n12 * Tr / denom);
+            ps.o = p;
+            ps.d = nd;
+            ps.depth++;
+            return true;
+        }
+        ps.o = p;
+        ps.d = d - (normal * 2.0f) * dot(normal, d);  // reflect about the shading normal
+        ps.depth++;
+        ps.emit = 1;
+        ps.prev_bsdf_pdf = -1.0f;
+        return true;
+    }
     return false;  // materials without a radiance program here
 }
 

@@ -13,9 +13,28 @@
 namespace orf {
 
 constexpr int kMegaBlock = 64;
+
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8),
+// and each XCD has its own L2.  Renumbering so that XCD x receives one contiguous
+// run of logical blocks (consecutive image tiles) keeps each XCD's primary and
+// shadow rays in one image region, so its L2 holds that region's BVH nodes and
+// triangles.  A bijection on [0, nb) for any nb.
+#ifndef ORF_XCD_REMAP
+#define ORF_XCD_REMAP 1
+#endif
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
+{
+#if ORF_XCD_REMAP
+    unsigned per = nb >> 3, rem = nb & 7u, x = b & 7u, k = b >> 3;
+    return x * per + (x < rem ? x : rem) + k;
+#else
+    (void)nb;
+    return b;
+#endif
+}
 // Minimum waves per SIMD the register allocator must allow (occupancy vs spills).
 #ifndef ORF_MEGA_WAVES
-#define ORF_MEGA_WAVES 4
+#define ORF_MEGA_WAVES 5
 #endif
 
 __global__ __launch_bounds__(kMegaBlock, ORF_MEGA_WAVES) void k_render_mega(SceneGPU sc, CameraGPU cam, FilmGPU film,
@@ -24,7 +43,8 @@ __global__ __launch_bounds__(kMegaBlock, ORF_MEGA_WAVES) void k_render_mega(Scen
 {
     __shared__ unsigned stack_lds[kLdsStack * kMegaBlock];
     unsigned* stk = stack_lds + threadIdx.x;
-    int tile = blockIdx.x / chunks, chunk = blockIdx.x % chunks;
+    unsigned blk = xcd_block(blockIdx.x, gridDim.x);
+    int tile = blk / chunks, chunk = blk % chunks;
     int x = (tile % tiles_x) * 8 + (threadIdx.x & 7);
     int y = (tile / tiles_x) * 8 + (threadIdx.x >> 3);
     if (x >= film.width || y >= film.height) return;

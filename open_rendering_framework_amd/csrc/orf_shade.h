@@ -191,6 +191,24 @@ ORF_HD float fresnel_R3(float c1, float c2, float eta)
     return (rs * rs + rp * rp) * 0.5f;
 }
 
+// Fresnel reflectance through an interface of relative index eta = n1/n2 at
+// incidence cosine c; 1 under total internal reflection (the branch written out
+// in CU/lambertian_interface_shader.cu:68-75).
+ORF_HD float fresnel_R_cos(float c, float eta)
+{
+    float st2 = eta * eta * (1.0f - c * c);
+    if (!(st2 < 1.0f)) return 1.0f;
+    return fresnel_R3(c, sqrtf(1.0f - st2), eta);
+}
+
+// two_C1(n) (ORF/Fresnel.h:93-101): twice the first Fresnel moment, the
+// published polynomial fits of d'Eon & Irving 2011 (one per side of n = 1).
+ORF_HD float two_C1(float n)
+{
+    if (n >= 1.0f) return -9.23372f + n * (22.2272f + n * (-20.9292f + n * (10.2291f + n * (-2.54396f + n * 0.254913f))));
+    return 0.919317f + n * (-3.4793f + n * (6.75335f + n * (-7.80989f + n * (4.98554f - n * 1.36881f))));
+}
+
 // One step of the rough dielectric (CU/rough_transparent_shader.cu:66-225,
 // visible-normal model): Beer absorption with Russian roulette when leaving the
 // medium, a visible microfacet normal, Fresnel-weighted choice between

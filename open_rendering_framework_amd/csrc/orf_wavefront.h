@@ -183,7 +183,7 @@ ORF_HD int hit_kind(const WfView& v, int slot, const Hit& h)
     const int* remap = v.remap + v.renders[v.st[slot].z].remap_offset;
     int k = hit_material(v.sc, remap, sh.w).kind;
     if (k == MAT_EMITTER) return Q_EMIT;
-    if (k == MAT_LAMBERT) return Q_LAMBERT;
+    if (k == MAT_LAMBERT || k == MAT_INTERFACE) return Q_LAMBERT;
     if (k == MAT_CONDUCTOR) return Q_COND;
     if (k == MAT_DIELECTRIC) return Q_DIEL;
     return Q_EMIT;  // kinds without a program end their path like an emitter that does not emit

@@ -699,6 +699,81 @@ static v3 shade_dielectric(const oscene* s, const orc_material* m, v3 o, v3 d, c
     return vmul(result, beam);
 }
 
+/* Fresnel reflectance at incidence cosine c through relative index eta; 1 under
+ * total internal reflection (CU/lambertian_interface_shader.cu:68-75). */
+static R fresnel_R_cos(R c, R eta)
+{
+    R st2 = eta * eta * ((R)1.0f - c * c);
+    if (!(st2 < (R)1.0f)) return 1;
+    return orc_fresnel_R3(c, sqrt((R)1.0f - st2), eta);
+}
+
+/* two_C1 (ORF/Fresnel.h:93-101), the published polynomial fits. */
+static R two_C1(R n)
+{
+    if (n >= (R)1.0f)
+        return (R)-9.23372f + n * ((R)22.2272f + n * ((R)-20.9292f + n * ((R)10.2291f + n * ((R)-2.54396f + n * (R)0.254913f))));
+    return (R)0.919317f + n * ((R)-3.4793f + n * ((R)6.75335f + n * ((R)-7.80989f + n * ((R)4.98554f - n * (R)1.36881f))));
+}
+
+/* CU/lambertian_interface_shader.cu:51-173 (+ environment MIS, new). */
+static v3 shade_interface(const oscene* s, const orc_material* m, v3 o, v3 d, const ohit* h, int depth,
+                          int emit_light, uint32_t* t, ostats* st)
+{
+    v3 result = V3(0, 0, 0);
+    if (depth > s->max_depth) return result;
+    v3 hit_pos = vadd(o, vscale(d, h->t));
+    v3 normal = hit_normal(s, h);
+    v3 ffn = orc_faceforward(normal, vneg(d), normal);
+    v3 rho = V3(m->diffuse[0], m->diffuse[1], m->diffuse[2]);
+    R ior = (R)m->ior;
+    R n12 = (R)1.0f / ior;
+    R r10 = two_C1(ior);
+    v3 den = V3((R)1.0f - rho.x * r10, (R)1.0f - rho.y * r10, (R)1.0f - rho.z * r10);
+    R cos_i = vdot(vneg(d), ffn);
+    R R_i = fresnel_R_cos(cos_i, n12);
+    R T_i = (R)1.0f - R_i;
+    uint32_t light_idx = (uint32_t)((R)s->nlights * orc_rnd_tea(t));
+    if (s->nlights > 0) {
+        if (light_idx >= (uint32_t)s->nlights) light_idx = (uint32_t)s->nlights - 1; /* new: see header */
+        R light_pdf = (R)1 / (R)s->nlights;
+        v3 w_l, rad;
+        R dist, penv;
+        direct_light(s, &s->lights[light_idx], hit_pos, t, &w_l, &rad, &dist, &penv);
+        rad = V3(rad.x / light_pdf, rad.y / light_pdf, rad.z / light_pdf);
+        R cos_l = vdot(ffn, w_l);
+        if (cos_l > 0) {
+            ohit sh;
+            st->shd++;
+            R V = trace(s, hit_pos, w_l, s->eps, dist, 1, &sh) ? (R)0 : (R)1;  /* tmax = dist (:100) */
+            v3 Li = vscale(rad, V);
+            R T_l = (R)1.0f - fresnel_R_cos(cos_l, n12);
+            v3 c = vmul(vscale(vscale(Li, INV_PI_R), cos_l), rho);
+            c = vscale(vscale(vscale(vscale(c, n12), n12), T_l), T_i);
+            c = V3(c.x / den.x, c.y / den.y, c.z / den.z);
+            if (penv >= 0) c = vscale(c, mis_power(penv * light_pdf, cos_l * INV_PI_R));
+            result = vadd(result, c);
+            emit_light = 0;
+        }
+    }
+    R xi = orc_rnd_tea(t);
+    if (xi > R_i) {
+        v3 nd = orc_sample_cosine_weighted(ffn, t);
+        R cos_r = vdot(nd, ffn);
+        R T_r = (R)1.0f - fresnel_R_cos(cos_r, n12);
+        R pdf = s->has_env && s->env_is ? cos_r * INV_PI_R : (R)-1;
+        v3 child = radiance(s, hit_pos, nd, depth + 1, emit_light, t, pdf, st);
+        v3 cr = vscale(vscale(vscale(vmul(child, rho), n12), n12), T_r);
+        result = vadd(result, V3(cr.x / den.x, cr.y / den.y, cr.z / den.z));
+    } else {
+        /* reflect(ray.direction, normal): i - 2 n dot(n, i) */
+        v3 rd = vsub(d, vscale(vscale(normal, (R)2.0f), vdot(normal, d)));
+        v3 child = radiance(s, hit_pos, rd, depth + 1, 1, t, (R)-1, st);
+        result = vadd(result, child);
+    }
+    return result;
+}
+
 static v3 radiance(const oscene* s, v3 o, v3 d, int depth, int emit_light, uint32_t* seed, R prev_pdf, ostats* st)
 {
     ohit h;
@@ -719,6 +794,7 @@ static v3 radiance(const oscene* s, v3 o, v3 d, int depth, int emit_light, uint3
     if (m->kind == ORC_LAMBERT) return shade_lambert(s, m, o, d, &h, depth, &emit_light, seed, st);
     if (m->kind == ORC_CONDUCTOR) return shade_conductor(s, m, &s->eta[s->tri_mat[h.prim]], o, d, &h, depth, seed, st);
     if (m->kind == ORC_DIELECTRIC) return shade_dielectric(s, m, o, d, &h, depth, seed, st);
+    if (m->kind == ORC_INTERFACE) return shade_interface(s, m, o, d, &h, depth, emit_light, seed, st);
     return V3(0, 0, 0);
 }
 

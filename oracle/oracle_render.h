@@ -13,7 +13,7 @@
 
 #include "orf_oracle.h"
 
-enum { ORC_LAMBERT = 0, ORC_CONDUCTOR = 1, ORC_EMITTER = 2, ORC_DIELECTRIC = 3 };
+enum { ORC_LAMBERT = 0, ORC_CONDUCTOR = 1, ORC_EMITTER = 2, ORC_DIELECTRIC = 3, ORC_INTERFACE = 4 };
 enum { ORC_POINT = 0, ORC_DIRECTIONAL = 1, ORC_TRIANGLES = 2, ORC_ENV = 5 };
 
 typedef struct {

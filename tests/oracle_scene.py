@@ -35,7 +35,8 @@ class OrcSceneDesc(ctypes.Structure):
                 ("max_depth", ctypes.c_int32), ("scene_epsilon", ctypes.c_float)]
 
 
-_KIND = {"diffuse_shader.cu": 0, "metallic_shader.cu": 1, "arealight_shader.cu": 2, "rough_transparent_shader.cu": 3}
+_KIND = {"diffuse_shader.cu": 0, "metallic_shader.cu": 1, "arealight_shader.cu": 2, "rough_transparent_shader.cu": 3,
+         "lambertian_interface_shader.cu": 4}
 
 
 def _p(a):

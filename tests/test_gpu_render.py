@@ -71,7 +71,48 @@ def test_per_sample_parity_small_scene():
         frac, ok = _frac_ok(got, want)
         _say("frame %d: within tol %.5f (%d/%d), max |d| %.3g" % (frame, frac, ok.sum(), ok.size,
                                                                  np.abs(got - want).max()))
-        assert frac >= 0.99
+        assert frac >= 0.999
+
+
+def _material_variant(kind):
+    d = _small(w=48, h=48)
+    if kind == "interface_sphere":
+        d.materials.append(scenes.Material("lambertian_interface_shader.cu",
+                                           floats={"diffuse_color": (0.6, 0.3, 0.2), "ior": (1.5,)}))
+        d.meshes[1].material = len(d.materials) - 1
+    elif kind == "interface_ground":
+        d.materials.append(scenes.Material("lambertian_interface_shader.cu",
+                                           floats={"diffuse_color": (0.6, 0.3, 0.2), "ior": (1.3,)}))
+        d.meshes[0].material = len(d.materials) - 1
+    elif kind == "beckmann_glass":
+        d.materials.append(scenes.Material("rough_transparent_shader.cu",
+                                           floats={"roughness": (0.3,), "ior": (1.5,),
+                                                   "glass_absorption": (0.2, 0.1, 0.05)},
+                                           uints={"microfacet_model": (1,), "normal_distribution": (0,)}))
+        d.meshes[1].material = len(d.materials) - 1
+    elif kind == "lambert_sphere":
+        d.meshes[1].material = 0
+    return d
+
+
+# per-sample bar per material: the Beckmann dielectric runs erfinv in FP32 on
+# the GPU where the reference (and the oracle) use FP64, and its Newton-bisection
+# loop amplifies that difference on a fraction of samples
+@pytest.mark.parametrize("kind,bar", [("interface_sphere", 0.999), ("interface_ground", 0.999),
+                                      ("lambert_sphere", 0.999), ("beckmann_glass", 0.99)])
+def test_per_sample_parity_materials(kind, bar):
+    from tests.oracle_scene import OracleScene
+
+    d = _material_variant(kind)
+    ctx = scenes.to_context(d)
+    orc = OracleScene(d)
+    ys, xs = np.mgrid[0:48, 0:48]
+    for frame in (0, 5):
+        got = _gpu_samples(ctx, 48, 48, frame)
+        want = orc.samples(48, 48, xs.ravel(), ys.ravel(), frame).reshape(48, 48, 3)
+        frac, ok = _frac_ok(got, want)
+        _say("%s frame %d: within tol %.5f, max |d| %.3g" % (kind, frame, frac, np.abs(got - want).max()))
+        assert frac >= bar
 
 
 def test_image_and_ray_counts_vs_fp64_oracle():
@@ -119,4 +160,4 @@ def test_full_scene_samples_and_determinism():
     want = OracleScene(d).samples(512, 512, px, py, 3)
     frac, _ = _frac_ok(got, want)
     _say("full scene frame 3: within tol %.5f" % frac)
-    assert frac >= 0.99
+    assert frac >= 0.999
