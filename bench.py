@@ -219,11 +219,16 @@ def batch_measure(steps, warmup, spp_scale=1.0, size_scale=1.0):
     c = ctx.batch_counters()
     mk = ctx.render_batch(items, remap, mode=rt.BATCH_MEGAKERNEL)
     torch.cuda.synchronize()
+    ctx.render_batch(items, remap, mode=rt.BATCH_PERSISTENT)  # warm
+    pe = min(ctx.render_batch(items, remap, mode=rt.BATCH_PERSISTENT) for _ in range(max(1, steps)))
+    torch.cuda.synchronize()
     return {
         "renders": len(items), "paths": paths,
         "wavefront_mpaths_s": paths / wf / 1e6, "wavefront_s": wf,
         "per_render_megakernel_mpaths_s": paths / mk / 1e6, "per_render_megakernel_s": mk,
         "speedup_vs_per_render_megakernel": mk / wf,
+        "persistent_mpaths_s": paths / pe / 1e6, "persistent_s": pe,
+        "persistent_speedup_vs_per_render_megakernel": mk / pe,
         "config": best.as_dict(), "autotune": report,
         "ext_rays_per_path": c["ext_rays"] / paths, "shadow_rays_per_path": c["shadow_rays"] / paths,
     }

@@ -189,12 +189,14 @@ typedef struct orf_wavefront_config {
     int32_t graph_pairs;    /* bounce-iteration pairs per graph replay */
 } orf_wavefront_config;
 
-enum { ORF_BATCH_WAVEFRONT = 0, ORF_BATCH_MEGAKERNEL = 1 };
+enum { ORF_BATCH_WAVEFRONT = 0, ORF_BATCH_MEGAKERNEL = 1, ORF_BATCH_PERSISTENT = 2 };
 
 orf_status orf_wavefront_default_config(orf_wavefront_config* cfg);
 /* mode ORF_BATCH_WAVEFRONT: all items in one wavefront pass; ORF_BATCH_MEGAKERNEL:
- * one megakernel launch per item.  cfg NULL = defaults.  *seconds (may be NULL)
- * receives the wall time from the first launch to completion. */
+ * one megakernel launch per item; ORF_BATCH_PERSISTENT: one persistent megakernel
+ * launch over every item (work items of 8x8 pixels x up to 8 samples, per-XCD
+ * queues with stealing).  cfg (wavefront only) NULL = defaults.  *seconds (may be
+ * NULL) receives the wall time from the first launch to completion. */
 orf_status orf_render_batch(orf_context ctx, const orf_render_item* items, int32_t n, const int32_t* remap,
                             int32_t remap_stride, int32_t mode, const orf_wavefront_config* cfg, void* stream,
                             double* seconds);

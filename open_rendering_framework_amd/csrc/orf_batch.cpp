@@ -98,7 +98,31 @@ static double run_batch(orf_context_t* c, long long paths, int mode, const Wavef
     auto* stats = c->d_batch_stats.as<unsigned long long>();
     ORF_HIP_CHECK(hipDeviceSynchronize());
     auto t0 = std::chrono::steady_clock::now();
-    if (mode == ORF_BATCH_MEGAKERNEL) {
+    if (mode == ORF_BATCH_PERSISTENT) {
+        // work items: (render, 8x8 tile, chunk of up to 8 samples), render by render
+        std::vector<PersistRender> pr(c->batch_renders.size());
+        long long items = 0;
+        for (size_t i = 0; i < pr.size(); ++i) {
+            const WfRender& r = c->batch_renders[i];
+            PersistRender& p = pr[i];
+            p.item_begin = items;
+            p.tiles_x = (r.width + 7) / 8;
+            p.per = std::max(1u, std::min(r.spp, 8u));
+            p.chunks = (int)((r.spp + p.per - 1) / p.per);
+            p.pad = 0;
+            items += (long long)p.tiles_x * ((r.height + 7) / 8) * p.chunks;
+        }
+        c->d_batch_items.upload(pr);
+        c->d_batch_qctr.ensure(8 * sizeof(unsigned long long));
+        ORF_HIP_CHECK(hipMemsetAsync(c->d_batch_qctr.p, 0, 8 * sizeof(unsigned long long), st));
+        int cus = 0;
+        ORF_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+        int grid = std::max(8, cus * 4 * ORF_MEGA_WAVES);
+        launch_batch_persistent(sc, c->d_batch_renders.as<WfRender>(), c->d_batch_items.as<PersistRender>(),
+                                (int)pr.size(), c->d_batch_remap.as<int>(), film, stats,
+                                c->d_batch_qctr.as<unsigned long long>(), items, grid, st);
+        ORF_HIP_CHECK(hipStreamSynchronize(st));
+    } else if (mode == ORF_BATCH_MEGAKERNEL) {
         const int* remap = c->d_batch_remap.as<int>();
         for (const WfRender& r : c->batch_renders) {
             FilmGPU f{film + 4 * (size_t)r.film_offset, stats, r.width, r.height};
@@ -135,7 +159,8 @@ extern "C" orf_status orf_render_batch(orf_context ctx, const orf_render_item* i
 {
     ORF_API_BEGIN
     orf_context_t* c = ctx_check(ctx);
-    ORF_REQUIRE(mode == ORF_BATCH_WAVEFRONT || mode == ORF_BATCH_MEGAKERNEL, "orf_render_batch: bad mode");
+    ORF_REQUIRE(mode == ORF_BATCH_WAVEFRONT || mode == ORF_BATCH_MEGAKERNEL || mode == ORF_BATCH_PERSISTENT,
+                "orf_render_batch: bad mode");
     WavefrontConfig w = to_cfg(cfg);
     if (mode == ORF_BATCH_WAVEFRONT) validate_config(w);
     hipStream_t st = (hipStream_t)stream;

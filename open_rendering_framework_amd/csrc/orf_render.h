@@ -7,6 +7,32 @@
 
 namespace orf {
 
+// Minimum waves per SIMD the register allocator must allow (occupancy vs spills).
+#ifndef ORF_MEGA_WAVES
+#define ORF_MEGA_WAVES 5
+#endif
+
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8),
+// and each XCD has its own L2.  Renumbering so that XCD x receives one contiguous
+// run of logical blocks (consecutive image tiles) keeps each XCD's rays in one
+// image region.  Off by default: with static dispatch the contiguous bands give
+// the XCDs unequal work (the 512x512 path-tracer image ran 1.7x slower), so the
+// batched persistent kernel gets locality from per-XCD queues with stealing
+// instead.  A bijection on [0, nb) for any nb.
+#ifndef ORF_XCD_REMAP
+#define ORF_XCD_REMAP 0
+#endif
+__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
+{
+#if ORF_XCD_REMAP
+    unsigned per = nb >> 3, rem = nb & 7u, x = b & 7u, k = b >> 3;
+    return x * per + (x < rem ? x : rem) + k;
+#else
+    (void)nb;
+    return b;
+#endif
+}
+
 struct FilmGPU {
     double* sums;                  // 4 per pixel: R, G, B sums and finite count
     unsigned long long* counters;  // [0] non-finite, [1] extension rays, [2] shadow rays, [3] paths
@@ -17,6 +43,22 @@ struct FilmGPU {
 // remap: per-render material table (device, nullptr = identity).
 void launch_megakernel(const SceneGPU& sc, const CameraGPU& cam, const FilmGPU& film, const int* remap,
                        unsigned frame0, unsigned spp, hipStream_t stream);
+
+// Persistent batched megakernel: one launch for every item of a batch.  Work
+// items are (render, 8x8 tile, sample chunk) in render order, split into 8
+// contiguous partitions, one per XCD; a wave takes items from its own XCD's
+// partition, then steals from the others.
+struct PersistRender {
+    long long item_begin;  // first work item of the render
+    int tiles_x, chunks;
+    unsigned per;          // samples per chunk
+    int pad;
+};
+struct WfRender;
+// qctr: 8 zeroed u64 queue counters (device).  grid: workgroups of 64 lanes.
+void launch_batch_persistent(const SceneGPU& sc, const WfRender* renders, const PersistRender* pr, int nrenders,
+                             const int* remap, double* film, unsigned long long* stats, unsigned long long* qctr,
+                             long long total_items, int grid, hipStream_t stream);
 
 // FP64 construction of the environment's FP32 pdf/cdf tables (spec CU/env_cameras.cu).
 // scratch: (h + 1) doubles.

@@ -14,29 +14,6 @@ namespace orf {
 
 constexpr int kMegaBlock = 64;
 
-// Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8),
-// and each XCD has its own L2.  Renumbering so that XCD x receives one contiguous
-// run of logical blocks (consecutive image tiles) keeps each XCD's primary and
-// shadow rays in one image region, so its L2 holds that region's BVH nodes and
-// triangles.  A bijection on [0, nb) for any nb.
-#ifndef ORF_XCD_REMAP
-#define ORF_XCD_REMAP 1
-#endif
-__device__ __forceinline__ unsigned xcd_block(unsigned b, unsigned nb)
-{
-#if ORF_XCD_REMAP
-    unsigned per = nb >> 3, rem = nb & 7u, x = b & 7u, k = b >> 3;
-    return x * per + (x < rem ? x : rem) + k;
-#else
-    (void)nb;
-    return b;
-#endif
-}
-// Minimum waves per SIMD the register allocator must allow (occupancy vs spills).
-#ifndef ORF_MEGA_WAVES
-#define ORF_MEGA_WAVES 5
-#endif
-
 __global__ __launch_bounds__(kMegaBlock, ORF_MEGA_WAVES) void k_render_mega(SceneGPU sc, CameraGPU cam, FilmGPU film,
                                                             const int* remap, unsigned frame0, unsigned spp,
                                                             int chunks, unsigned per, int tiles_x)
@@ -48,19 +25,20 @@ __global__ __launch_bounds__(kMegaBlock, ORF_MEGA_WAVES) void k_render_mega(Scen
     int x = (tile % tiles_x) * 8 + (threadIdx.x & 7);
     int y = (tile / tiles_x) * 8 + (threadIdx.x >> 3);
     if (x >= film.width || y >= film.height) return;
-    unsigned s0 = frame0 + (unsigned)chunk * per;
-    unsigned s1 = min(s0 + per, frame0 + spp);
-    if (s0 >= s1) return;
+    // local sample range of this chunk (frame = frame0 + k wraps like the seeds do)
+    unsigned k0 = (unsigned)chunk * per;
+    unsigned k1 = min(k0 + per, spp);
+    if (k0 >= k1) return;
     bool inside = !((unsigned)x < cam.patch_origin[0] || (unsigned)x > cam.patch_origin[0] + cam.patch_dims[0] ||
                     (unsigned)y < cam.patch_origin[1] || (unsigned)y > cam.patch_origin[1] + cam.patch_dims[1]);
     double sr = 0.0, sg = 0.0, sb = 0.0;
     unsigned cnt = 0, bad = 0;
     PathStats st;
-    for (unsigned s = s0; s < s1; ++s) {
+    for (unsigned k = k0; k < k1; ++k) {
         f3 L = mk3(0.0f);
         if (inside) {
             f3 dir;
-            uint32_t rng = camera_ray(cam, film.width, film.height, x, y, s, dir);
+            uint32_t rng = camera_ray(cam, film.width, film.height, x, y, frame0 + k, dir);
             L = path_radiance(sc, remap, cam.eye, dir, rng, stk, kMegaBlock, st);
         }
         if (finite3(L)) {
@@ -87,7 +65,7 @@ __global__ __launch_bounds__(kMegaBlock, ORF_MEGA_WAVES) void k_render_mega(Scen
     if (bad) atomicAdd(film.counters + 0, (unsigned long long)bad);
     atomicAdd(film.counters + 1, (unsigned long long)st.ext);
     atomicAdd(film.counters + 2, (unsigned long long)st.shd);
-    atomicAdd(film.counters + 3, (unsigned long long)(s1 - s0));
+    atomicAdd(film.counters + 3, (unsigned long long)(k1 - k0));
 }
 
 void launch_megakernel(const SceneGPU& sc, const CameraGPU& cam, const FilmGPU& film, const int* remap,

@@ -172,7 +172,7 @@ def gen_sky_envmap(width, height, sun_dir, sigma_deg, peak):
 
 
 SCOPE_CONTEXT, SCOPE_MATERIAL = 0, 1
-BATCH_WAVEFRONT, BATCH_MEGAKERNEL = 0, 1
+BATCH_WAVEFRONT, BATCH_MEGAKERNEL, BATCH_PERSISTENT = 0, 1, 2
 
 
 class RenderItem(ctypes.Structure):

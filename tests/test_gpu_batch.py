@@ -3,7 +3,8 @@
 The wavefront form runs shade_vertex() -- the megakernel's own shading code --
 on queued hits, so for every item its film must equal the per-item megakernel
 film up to the order of FP64 atomic additions (relative 1e-9), for every
-wavefront configuration; ray counts must match exactly.  Item images are also
+wavefront configuration; ray counts must match exactly.  The persistent batched
+megakernel runs the same path loop over different sample chunks: same bar.  Item images are also
 compared with the FP64 oracle (relative RMSE stated below).
 """
 import sys
@@ -63,6 +64,20 @@ def test_wavefront_equals_per_item_megakernel(batch, cfg):
     assert counts["ext_rays"] == ref_counts["ext_rays"]
     assert counts["shadow_rays"] == ref_counts["shadow_rays"]
     assert counts["paths"] == ref_counts["paths"] == sum(it["width"] * it["height"] * it["spp"] for it in items)
+
+
+def test_persistent_equals_per_item_megakernel(batch):
+    d, items, remap, ctx, ref, ref_counts = batch
+    secs = ctx.render_batch(items, remap, mode=rt.BATCH_PERSISTENT)
+    got = _films(ctx, items)
+    counts = ctx.batch_counters()
+    _say("persistent: %.4f s, counters %s (megakernel %s)" % (secs, counts, ref_counts))
+    for i, (a, b) in enumerate(zip(got, ref)):
+        assert np.array_equal(a[..., 3], b[..., 3]), i
+        np.testing.assert_allclose(a[..., :3], b[..., :3], rtol=1e-9, atol=1e-12)
+    assert counts["ext_rays"] == ref_counts["ext_rays"]
+    assert counts["shadow_rays"] == ref_counts["shadow_rays"]
+    assert counts["paths"] == ref_counts["paths"]
 
 
 def test_batch_items_vs_fp64_oracle(batch):
