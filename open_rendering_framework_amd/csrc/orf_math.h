@@ -15,6 +15,7 @@
  *                 quadrant rules of C99 F.9.1.4 for finite arguments;
  *                 offsets by pi, pi/2, pi/4 are added as float head + tail;
  *   log/exp     : Cephes logf/expf (frexp / ldexp done on the bits);
+ *   pow         : exp(y log x) with pow's special cases;
  *   erf         : x P(x^2) on [0,1] and 1 - exp(-x^2) Q(u) on [1,3.92]
  *                 (coefficients fitted here, relative-error weighted least
  *                 squares of erf(x)/x and erfc(x) exp(x^2)), 1 beyond.
@@ -298,6 +299,18 @@ ORF_MATH_FN float orf_erff(float xx)
         r = 1.0f;
     }
     return xx < 0.0f ? -r : r;
+}
+
+/* x^y = exp(y log x) for x > 0 (not correctly rounded; the same bits on every
+ * side), with pow's special cases for x = 0, x = 1, y = 0 and x < 0 (NaN unless
+ * y is 0). */
+ORF_MATH_FN float orf_powf(float x, float y)
+{
+    if (y == 0.0f || x == 1.0f) return 1.0f;
+    if (x != x || y != y) return x + y;
+    if (x == 0.0f) return y > 0.0f ? 0.0f : orf_u2f(0x7f800000u);
+    if (x < 0.0f) return orf_nanf();
+    return orf_expf(y * orf_logf(x));
 }
 
 #endif /* ORF_MATH_H */

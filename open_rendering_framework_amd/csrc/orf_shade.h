@@ -209,6 +209,91 @@ ORF_HD float two_C1(float n)
     return 0.919317f + n * (-3.4793f + n * (6.75335f + n * (-7.80989f + n * (4.98554f - n * 1.36881f))));
 }
 
+// ---------------------------------------------------------------- multiple scattering
+// Microsurface random walk of Heitz et al. 2016 on a Gaussian-height GGX surface
+// (ORF/MicrofacetGGX.h:19-29, 93-107, 254-305, 412-438; heights ORF/helpers.h:380-391).
+
+// ggx_smith_lambda (ORF/MicrofacetGGX.h:20-29); wi local, may point downwards.
+ORF_HD float ggx_smith_lambda(f3 wi, float ax, float ay)
+{
+    if (wi.z > 0.99999f) return 0.0f;
+    if (wi.z < -0.99999f) return -1.0f;
+    float theta = orf_acosf(wi.z);
+    float a = 1.0f / orf_tanf(theta) / ggx_alpha_proj(wi, ax, ay);
+    return 0.5f * (-1.0f + sqrtf(1.0f + 1.0f / (a * a)) * (a > 0.0f ? 1.0f : -1.0f));
+}
+
+// GaussianHeightCDF / GaussianHeightInvCDF (ORF/helpers.h:380-391).
+ORF_HD float height_cdf(float h) { return 0.5f + 0.5f * orf_erff(0.707106781186547524401f * h); }
+ORF_HD float height_inv_cdf(float z) { return 1.41421356237309504880f * erfinv_f32(2.0f * z - 1.0f); }
+
+// ggx_G(wi, h): masking of wi at height h, C1(h)^Lambda (ORF/MicrofacetGGX.h:93-107).
+ORF_HD float ggx_masking_height(f3 wi, float h, float ax, float ay)
+{
+    if (wi.z > 0.9999f) return 1.0f;
+    if (wi.z <= 0.0f) return 0.0f;
+    return orf_powf(height_cdf(h), ggx_smith_lambda(wi, ax, ay));
+}
+
+// ggx_sample_height (ORF/MicrofacetGGX.h:254-275): next height along wr from hr,
+// or kRayMax when the ray leaves the microsurface.
+ORF_HD float ggx_sample_height(f3 wr, float hr, float z, float ax, float ay)
+{
+    if (wr.z > 0.9999f) return kRayMax;
+    if (wr.z < -0.9999f) return height_inv_cdf(z * height_cdf(hr));
+    if (fabsf(wr.z) < 0.00001f) return hr;
+    float g1 = ggx_masking_height(wr, hr, ax, ay);
+    if (z > 1.0f - g1) return kRayMax;
+    return height_inv_cdf(height_cdf(hr) / orf_powf(1.0f - z, 1.0f / ggx_smith_lambda(wr, ax, ay)));
+}
+
+// ggx_sample_dielectric_phase (ORF/MicrofacetGGX.h:278-305): visible normal from
+// the side `outside` names, Fresnel choice between reflection and refraction
+// (which flips `outside`).  Draws z1, z2, then the selector.
+ORF_HD f3 ggx_sample_dielectric_phase(f3 w_i, float m_eta, uint32_t& rng, bool& outside, float ax, float ay)
+{
+    float z1 = rnd_tea(rng);
+    float z2 = rnd_tea(rng);
+    bool wi_out = outside;
+    float eta = wi_out ? m_eta : 1.0f / m_eta;
+    f3 w_m = wi_out ? ggx_sample_visible(w_i, ax, ay, z1, z2) : -ggx_sample_visible(-w_i, ax, ay, z1, z2);
+    float c = fabsf(dot(w_m, w_i));
+    float st2 = eta * eta * (1.0f - c * c);
+    float ct = sqrtf(1.0f - st2);
+    float F = st2 < 1.0f ? fresnel_R3(c, ct, eta) : 1.0f;
+    if (rnd_tea(rng) <= F) return normalize(-w_i + (w_m * 2.0f) * dot(w_i, w_m));
+    outside = !wi_out;
+    return normalize((-eta) * w_i + w_m * (eta * c - ct));
+}
+
+// Scattering events allowed per walk before the path is ended (the reference's
+// loop has no bound).
+constexpr int kMsWalkCap = 10;
+
+// ggx_multiscattering_dielectric_BSDF_sample (ORF/MicrofacetGGX.h:412-438): exit
+// direction of the walk started by local_wi (weight 1).  Returns false when the
+// walk exceeds kMsWalkCap scattering events.
+ORF_HD bool ggx_ms_dielectric_sample(f3 local_wi, float eta, float ax, float ay, uint32_t& rng, f3& w_out)
+{
+    float hr = 1.0f + height_inv_cdf(0.9999f);
+    f3 w_o = -local_wi;
+    bool outside = (-w_o.z) > 0.0f;
+    int order = 0;
+    for (;;) {
+        float z = rnd_tea(rng);
+        hr = outside ? ggx_sample_height(w_o, hr, z, ax, ay) : -ggx_sample_height(-w_o, -hr, z, ax, ay);
+        if (hr == kRayMax || hr == -kRayMax) break;
+        if (++order > kMsWalkCap) return false;
+        w_o = ggx_sample_dielectric_phase(-w_o, eta, rng, outside, ax, ay);
+        if (hr != hr || w_o.z != w_o.z) {
+            w_o = mk3(0.0f, 0.0f, 1.0f);
+            break;
+        }
+    }
+    w_out = w_o;
+    return true;
+}
+
 // One step of the rough dielectric (CU/rough_transparent_shader.cu:66-225,
 // visible-normal model): Beer absorption with Russian roulette when leaving the
 // medium, a visible microfacet normal, Fresnel-weighted choice between
@@ -230,6 +315,15 @@ ORF_HD bool dielectric_step(f3 d, float t_hit, f3 normal, f3 ffn, const Material
     float z2 = rnd_tea(rng);
     float a = m.roughness.x;
     int dist = m.normal_distribution;
+    if (m.microfacet_model == MULTISCATTERING_MODEL && dist == GGX_DISTRIBUTION) {
+        // CU/rough_transparent_shader.cu:98-116: the walk starts from the
+        // un-flipped normal's frame (z1, z2 above are drawn and unused, as there)
+        f3 lo;
+        if (!ggx_ms_dielectric_sample(to_local(w_i, normal), 1.0f / m.ior, a, a, rng, lo)) return false;
+        new_dir = to_world(lo, normal);
+        factor = beam;
+        return true;
+    }
     f3 mn = sample_visible_normal(w_i, ffn, a, a, z1, z2, dist);
     float eta = dot(normal, w_i) < 0.0f ? m.ior : 1.0f / m.ior;
     float g_i = masking_g1(w_i, mn, ffn, a, a, dist);

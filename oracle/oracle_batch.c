@@ -210,3 +210,21 @@ void orc_erfinv_batch(int n, const double *x, double *out)
 {
     for (int i = 0; i < n; i++) out[i] = orc_erfinv(x[i]);
 }
+
+/* Microsurface walks from one local direction: exit direction, event count and
+ * completion flag (0: exceeded `cap`), one TEA stream per walk (tea16(i, seed)). */
+void orc_ggx_ms_sample_batch(int n, const float* wi, float eta, float alpha, uint32_t seed, int cap, float* wo,
+                             int32_t* order, int32_t* ok)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        uint32_t t = orc_tea16((uint32_t)i, seed);
+        v3 out = V3(0, 0, 0);
+        int o = 0;
+        ok[i] = orc_ggx_ms_dielectric_sample(V3(wi[0], wi[1], wi[2]), eta, alpha, alpha, &t, cap, &out, &o);
+        order[i] = o;
+        wo[3 * i] = (float)out.x;
+        wo[3 * i + 1] = (float)out.y;
+        wo[3 * i + 2] = (float)out.z;
+    }
+}

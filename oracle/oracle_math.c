@@ -655,6 +655,88 @@ R orc_fresnel_R3(R cos1, R cos2, R eta)
     return (rs * rs + rp * rp) * (R)0.5f;
 }
 
+/* ---- microsurface multiple scattering: Heitz et al. 2016 random walk on a
+ * Gaussian-height GGX surface (ORF/MicrofacetGGX.h:19-29, 93-107, 254-305,
+ * 412-438; heights ORF/helpers.h:380-391). */
+
+#define RAY_MAX_R ((R)1e27f)
+
+/* ggx_smith_lambda (ORF/MicrofacetGGX.h:20-29). */
+R orc_ggx_smith_lambda(v3 wi, R ax, R ay)
+{
+    if (wi.z > (R)0.99999f) return 0;
+    if (wi.z < (R)-0.99999f) return -1;
+    R theta = F_ACOS(wi.z);
+    R a = (R)1.0f / F_TAN(theta) / orc_ggx_alpha_i(wi, ax, ay);
+    return (R)0.5f * ((R)-1.0f + sqrt((R)1 + (R)1 / (a * a)) * (a > 0 ? (R)1 : (R)-1));
+}
+
+/* GaussianHeightCDF / GaussianHeightInvCDF (ORF/helpers.h:380-391): erf in R,
+ * erfinv in double rounded to R, as the reference mixes them. */
+R orc_height_cdf(R h) { return (R)0.5f + (R)0.5f * (R)F_ERF((R)0.707106781186547524401f * h); }
+R orc_height_inv_cdf(R z) { return (R)1.41421356237309504880f * (R)orc_erfinv((double)((R)2.0f * z - (R)1.0f)); }
+
+/* ggx_G(wi, h) (ORF/MicrofacetGGX.h:93-107). */
+R orc_ggx_masking_height(v3 wi, R h, R ax, R ay)
+{
+    if (wi.z > (R)0.9999f) return 1;
+    if (wi.z <= 0) return 0;
+    return F_POW(orc_height_cdf(h), orc_ggx_smith_lambda(wi, ax, ay));
+}
+
+/* ggx_sample_height (ORF/MicrofacetGGX.h:254-275). */
+R orc_ggx_sample_height(v3 wr, R hr, R z, R ax, R ay)
+{
+    if (wr.z > (R)0.9999f) return RAY_MAX_R;
+    if (wr.z < (R)-0.9999f) return orc_height_inv_cdf(z * orc_height_cdf(hr));
+    if (fabs(wr.z) < (R)0.00001f) return hr;
+    R g1 = orc_ggx_masking_height(wr, hr, ax, ay);
+    if (z > (R)1.0f - g1) return RAY_MAX_R;
+    return orc_height_inv_cdf(orc_height_cdf(hr) / F_POW((R)1.0f - z, (R)1.0f / orc_ggx_smith_lambda(wr, ax, ay)));
+}
+
+/* ggx_sample_dielectric_phase (ORF/MicrofacetGGX.h:278-305); *outside flips on
+ * refraction (the reference passes one flag as both wi_outside and wo_outside). */
+static v3 ggx_sample_dielectric_phase(v3 w_i, R m_eta, uint32_t* seed, int* outside, R ax, R ay)
+{
+    R z1 = orc_rnd_tea(seed);
+    R z2 = orc_rnd_tea(seed);
+    int wi_out = *outside;
+    R eta = wi_out ? m_eta : (R)1.0f / m_eta;
+    v3 w_m = wi_out ? orc_ggx_sample_vndf(w_i, ax, ay, z1, z2) : vneg(orc_ggx_sample_vndf(vneg(w_i), ax, ay, z1, z2));
+    R c = fabs(vdot(w_m, w_i));
+    R st2 = eta * eta * ((R)1.0f - c * c);
+    R ct = sqrt((R)1.0f - st2);
+    R F = st2 < (R)1.0f ? orc_fresnel_R3(c, ct, eta) : (R)1.0f;
+    if (orc_rnd_tea(seed) <= F) return orc_normalize(vadd(vneg(w_i), vscale(vscale(w_m, (R)2.0f), vdot(w_i, w_m))));
+    *outside = !wi_out;
+    return orc_normalize(vadd(vscale(w_i, -eta), vscale(w_m, eta * c - ct)));
+}
+
+/* ggx_multiscattering_dielectric_BSDF_sample (ORF/MicrofacetGGX.h:412-438) with
+ * at most `cap` scattering events (the reference's loop is unbounded); returns
+ * 0 when the walk exceeds the cap.  *order receives the event count. */
+int orc_ggx_ms_dielectric_sample(v3 local_wi, R eta, R ax, R ay, uint32_t* seed, int cap, v3* w_out, int* order)
+{
+    R hr = (R)1.0f + orc_height_inv_cdf((R)0.9999f);
+    v3 w_o = vneg(local_wi);
+    int outside = (-w_o.z) > 0;
+    *order = 0;
+    for (;;) {
+        R z = orc_rnd_tea(seed);
+        hr = outside ? orc_ggx_sample_height(w_o, hr, z, ax, ay) : -orc_ggx_sample_height(vneg(w_o), -hr, z, ax, ay);
+        if (hr == RAY_MAX_R || hr == -RAY_MAX_R) break;
+        if (++*order > cap) return 0;
+        w_o = ggx_sample_dielectric_phase(vneg(w_o), eta, seed, &outside, ax, ay);
+        if (hr != hr || w_o.z != w_o.z) {
+            w_o = V3(0, 0, 1);
+            break;
+        }
+    }
+    *w_out = w_o;
+    return 1;
+}
+
 /* The shared FP32 transcendentals, exported for their accuracy tests:
  * fn 0 sin, 1 cos, 2 tan, 3 asin, 4 acos, 5 atan, 6 atan2(x, y), 7 log, 8 exp, 9 erf. */
 void orc_fmath_eval(int fn, int64_t n, const float* x, const float* y, float* out)

@@ -25,6 +25,7 @@
 #define PI_R ((R)3.14159265358979323846)
 #define INV_PI_R ((R)0.318309886183790671538)
 #define RAY_MAX ((R)1e27f)
+#define MS_WALK_CAP 10 /* scattering events per microsurface walk (new: the reference's loop is unbounded) */
 
 typedef struct {
     double lo[3], hi[3];
@@ -680,6 +681,18 @@ static v3 shade_dielectric(const oscene* s, const orc_material* m, v3 o, v3 d, c
     R z2 = orc_rnd_tea(t);
     R a = m->roughness[0];
     int dist = m->normal_distribution;
+    if (m->microfacet_model == 2 && dist == 1) {
+        /* multiple-scattering model (CU/rough_transparent_shader.cu:98-116): z1, z2
+         * above are drawn and unused; the walk runs in the frame of `normal` */
+        v3 lo;
+        int order;
+        if (!orc_ggx_ms_dielectric_sample(orc_to_local(w_i, normal), (R)1.0f / (R)m->ior, a, a, t, MS_WALK_CAP, &lo,
+                                          &order))
+            return zero;
+        v3 dir = orc_to_world(lo, normal);
+        v3 child = radiance(s, hit_point, dir, depth + 1, 1, t, (R)-1, st);
+        return vmul(child, beam);
+    }
     v3 mn = sample_visible(w_i, ffn, a, z1, z2, dist);
     R eta = vdot(normal, w_i) < 0 ? (R)m->ior : (R)1.0f / (R)m->ior;
     R G_i = masking(w_i, mn, ffn, a, dist);

@@ -96,6 +96,13 @@ R orc_beckmann_eval_vndf(v3 wi, v3 wm, R ax, R ay);
 v2 orc_beckmann_sample_p22_11(R theta_i, R z1, R z2);
 v3 orc_beckmann_sample_vndf(v3 wi, R ax, R ay, R z1, R z2);
 R orc_fresnel_R3(R cos1, R cos2, R eta);
+/* microsurface multiple scattering (Heitz et al. 2016) */
+R orc_ggx_smith_lambda(v3 wi, R ax, R ay);
+R orc_height_cdf(R h);
+R orc_height_inv_cdf(R z);
+R orc_ggx_masking_height(v3 wi, R h, R ax, R ay);
+R orc_ggx_sample_height(v3 wr, R hr, R z, R ax, R ay);
+int orc_ggx_ms_dielectric_sample(v3 local_wi, R eta, R ax, R ay, uint32_t* seed, int cap, v3* w_out, int* order);
 
 /* One query of the rough-conductor operator (CU/metallic_shader.cu:84-158,
  * visible-normal branch, GGX, surface normal (0,0,1)).  Returns 1 when the

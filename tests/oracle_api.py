@@ -47,6 +47,8 @@ class Oracle:
         L.orc_beckmann_sample_vndf_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_float, ctypes.c_float, c_vp, c_vp,
                                                      c_vp]
         L.orc_erfinv_batch.argtypes = [ctypes.c_int, c_vp, c_vp]
+        L.orc_ggx_ms_sample_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
+                                              ctypes.c_int, c_vp, c_vp, c_vp]
 
     # ---- RNG
     def tea(self, v0, v1, rounds=16, key=None):
@@ -181,3 +183,12 @@ class Oracle:
         out = np.zeros(len(x))
         self.lib.orc_erfinv_batch(len(x), _p(x), _p(out))
         return out
+
+    def ggx_ms_sample(self, wi, eta, alpha, n, seed=1, cap=10):
+        """(exit directions (n, 3), scattering orders, completed flags) of n walks."""
+        wi = np.ascontiguousarray(wi, np.float32)
+        wo = np.zeros((n, 3), np.float32)
+        order = np.zeros(n, np.int32)
+        ok = np.zeros(n, np.int32)
+        self.lib.orc_ggx_ms_sample_batch(n, _p(wi), eta, alpha, seed, cap, _p(wo), _p(order), _p(ok))
+        return wo, order, ok
