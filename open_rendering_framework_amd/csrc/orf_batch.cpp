@@ -8,6 +8,7 @@
 
 #include "orf_common.h"
 #include "orf_context.h"
+#include "orf_mega.h"
 
 namespace orf {
 
@@ -85,7 +86,7 @@ static long long stage_batch(orf_context_t* c, const orf_render_item* items, int
     c->d_batch_remap.upload(rm);
     c->d_batch_film.ensure((size_t)pixels * 4 * sizeof(double));
     ORF_HIP_CHECK(hipMemset(c->d_batch_film.p, 0, (size_t)pixels * 4 * sizeof(double)));
-    c->d_batch_stats.ensure(8 * sizeof(unsigned long long));
+    c->d_batch_stats.ensure(16 * sizeof(unsigned long long));  // [0..3] counters, [8..15] queue counters
     ORF_HIP_CHECK(hipMemset(c->d_batch_stats.p, 0, c->d_batch_stats.bytes));
     c->batch_pixels = pixels;
     return paths;
@@ -107,25 +108,20 @@ static double run_batch(orf_context_t* c, long long paths, int mode, const Wavef
             PersistRender& p = pr[i];
             p.item_begin = items;
             p.tiles_x = (r.width + 7) / 8;
-            p.per = std::max(1u, std::min(r.spp, 8u));
+            p.per = std::max(1u, std::min(r.spp, kMegaChunk));
             p.chunks = (int)((r.spp + p.per - 1) / p.per);
             p.pad = 0;
             items += (long long)p.tiles_x * ((r.height + 7) / 8) * p.chunks;
         }
         c->d_batch_items.upload(pr);
-        c->d_batch_qctr.ensure(8 * sizeof(unsigned long long));
-        ORF_HIP_CHECK(hipMemsetAsync(c->d_batch_qctr.p, 0, 8 * sizeof(unsigned long long), st));
-        int cus = 0;
-        ORF_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-        int grid = std::max(8, cus * 4 * ORF_MEGA_WAVES);
         launch_batch_persistent(sc, c->d_batch_renders.as<WfRender>(), c->d_batch_items.as<PersistRender>(),
-                                (int)pr.size(), c->d_batch_remap.as<int>(), film, stats,
-                                c->d_batch_qctr.as<unsigned long long>(), items, grid, st);
+                                (int)pr.size(), c->d_batch_remap.as<int>(), film, stats, stats + 8, items,
+                                persistent_grid(items), st);
         ORF_HIP_CHECK(hipStreamSynchronize(st));
     } else if (mode == ORF_BATCH_MEGAKERNEL) {
         const int* remap = c->d_batch_remap.as<int>();
         for (const WfRender& r : c->batch_renders) {
-            FilmGPU f{film + 4 * (size_t)r.film_offset, stats, r.width, r.height};
+            FilmGPU f{film + 4 * (size_t)r.film_offset, stats, r.width, r.height, stats + 8};
             launch_megakernel(sc, r.cam, f, remap + r.remap_offset, r.frame0, r.spp, st);
         }
         ORF_HIP_CHECK(hipStreamSynchronize(st));

@@ -289,7 +289,7 @@ static void film_ensure(orf_context_t* c, int w, int h)
     if (w == c->film_w && h == c->film_h && c->d_film.p) return;
     c->d_film.release();
     c->d_film.ensure((size_t)w * h * 4 * sizeof(double));
-    c->d_counters.ensure(8 * sizeof(unsigned long long));
+    c->d_counters.ensure(16 * sizeof(unsigned long long));  // [0..3] counters, [8..15] queue counters
     c->film_w = w;
     c->film_h = h;
     film_clear(c);
@@ -579,7 +579,8 @@ extern "C" orf_status orf_launch(orf_context ctx, int width, int height, uint32_
         cam.patch_dims[0] = (unsigned)width;
         cam.patch_dims[1] = (unsigned)height;
     }
-    orf::FilmGPU film{c->d_film.as<double>(), c->d_counters.as<unsigned long long>(), width, height};
+    orf::FilmGPU film{c->d_film.as<double>(), c->d_counters.as<unsigned long long>(), width, height,
+                      c->d_counters.as<unsigned long long>() + 8};
     orf::launch_megakernel(orf::scene_view(c), cam, film, nullptr, frame0, spp, st);
     ORF_API_END
 }

@@ -47,7 +47,7 @@ struct orf_context_t {
     int film_w = 0, film_h = 0;
     int nlights_dev = 0;
     // batched renders
-    orf::DevBuf d_batch_renders, d_batch_remap, d_batch_film, d_batch_stats, d_batch_items, d_batch_qctr;
+    orf::DevBuf d_batch_renders, d_batch_remap, d_batch_film, d_batch_stats, d_batch_items;
     std::vector<orf::WfRender> batch_renders;
     long long batch_pixels = 0;
     orf::WavefrontPool wf_pool;

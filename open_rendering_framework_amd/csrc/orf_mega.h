@@ -1,0 +1,74 @@
+// orf_mega.h -- per-lane body and work fetch shared by the persistent megakernels (device).
+//
+// Work item = (8x8 pixel tile, chunk of up to kMegaChunk samples) of one render.
+// A launch's items are cut into kXcdParts contiguous partitions; workgroup b
+// (one wave64) starts on partition b % kXcdParts -- the XCD it runs on, as
+// workgroups are dealt to the 8 XCDs round-robin -- and steals from the other
+// partitions once its own is drained, so neighbouring tiles share an L2 while
+// no XCD idles on a cheap image region.
+#pragma once
+
+#include "orf_path.h"
+#include "orf_render.h"
+
+namespace orf {
+
+constexpr int kMegaBlock = 64;
+constexpr int kXcdParts = 8;
+#ifndef ORF_MEGA_CHUNK
+#define ORF_MEGA_CHUNK 4
+#endif
+constexpr unsigned kMegaChunk = ORF_MEGA_CHUNK;  // samples per work item
+
+// Next item of partition `part` ([lo, hi)) for the whole wave, or -1 when drained.
+__device__ __forceinline__ long long mega_next(unsigned long long* qctr, int part, long long lo, long long hi)
+{
+    unsigned got = 0;
+    if (threadIdx.x == 0) got = (unsigned)atomicAdd(qctr + part, 1ull);
+    got = __shfl(got, 0);
+    long long item = lo + (long long)got;
+    return item < hi ? item : -1;
+}
+
+// Samples frame0 + k, k in [k0, k1), of pixel (x, y): FP64 sums of the finite
+// radiances into px (plain read-modify-write when this item is the pixel's only
+// writer in the launch, FP64 atomics otherwise).  Pixels outside the camera
+// patch (CU/path_tracer.cu:36-43) record zero radiance.
+__device__ __forceinline__ void mega_pixel(const SceneGPU& sc, const CameraGPU& cam, int w, int h, int x, int y,
+                                           const int* remap, unsigned frame0, unsigned k0, unsigned k1, bool owner,
+                                           double* px, unsigned* stk, PathStats& st, unsigned long long& bad)
+{
+    bool inside = !((unsigned)x < cam.patch_origin[0] || (unsigned)x > cam.patch_origin[0] + cam.patch_dims[0] ||
+                    (unsigned)y < cam.patch_origin[1] || (unsigned)y > cam.patch_origin[1] + cam.patch_dims[1]);
+    double sr = 0.0, sg = 0.0, sb = 0.0;
+    unsigned cnt = 0;
+    for (unsigned k = k0; k < k1; ++k) {
+        f3 L = mk3(0.0f);
+        if (inside) {
+            f3 dir;
+            uint32_t rng = camera_ray(cam, w, h, x, y, frame0 + k, dir);
+            L = path_radiance(sc, remap, cam.eye, dir, rng, stk, kMegaBlock, st);
+        }
+        if (finite3(L)) {
+            sr += (double)L.x;
+            sg += (double)L.y;
+            sb += (double)L.z;
+            cnt++;
+        } else {
+            bad++;
+        }
+    }
+    if (owner) {
+        px[0] += sr;
+        px[1] += sg;
+        px[2] += sb;
+        px[3] += (double)cnt;
+    } else {
+        atomicAdd(px + 0, sr);
+        atomicAdd(px + 1, sg);
+        atomicAdd(px + 2, sb);
+        atomicAdd(px + 3, (double)cnt);
+    }
+}
+
+}  // namespace orf

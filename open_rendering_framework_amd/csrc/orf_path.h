@@ -17,6 +17,7 @@
 #pragma once
 
 #include "orf_shade.h"
+#include "orf_sss.h"
 #include "orf_trace.h"
 
 namespace orf {
@@ -235,6 +236,50 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         }
         ps.o = p;
         ps.d = d - (normal * 2.0f) * dot(normal, d);  // reflect about the shading normal
+        ps.depth++;
+        ps.emit = 1;
+        ps.prev_bsdf_pdf = -1.0f;
+        return true;
+    }
+    if (mat.kind == MAT_TRANSLUCENT) {
+        // CU/subsurface_scattering_shader.cu:55-198: Fresnel choice between the
+        // specular reflection and the refraction; entering from outside also adds
+        // the diffusion term (sss_gather, before the path continues); inside, Beer
+        // extinction with Russian roulette first.
+        if (ps.depth > sc.max_depth) return false;
+        const ScatteringGPU& P = sc.sss[mat.sss];
+        f3 no = normal, wo = -d;
+        float recip = 1.0f / P.relative_ior;
+        float reflect_xi = rnd_tea(ps.rng);
+        float co = dot(wo, no);
+        bool inside = co < 0.0f;
+        f3 beam = mk3(1.0f);
+        if (inside) {
+            f3 ext = P.dipole_model == 1 ? P.deltaEddExtinction : P.extinction;
+            beam = exp3((-h.t) * ext);
+            float prob = (beam.x + beam.y + beam.z) / 3.0f;
+            if (rnd_tea(ps.rng) >= prob) return false;
+            beam = beam / prob;
+            recip = P.relative_ior;
+            no = -no;
+            co = -co;
+        }
+        float st2 = recip * recip * (1.0f - co * co);
+        float ct = 1.0f, R = 1.0f;
+        if (st2 < 1.0f) {
+            ct = sqrtf(1.0f - st2);
+            R = fresnel_R3(co, ct, recip);
+        }
+        ps.thr = ps.thr * beam;
+        f3 nd;
+        if (reflect_xi >= R) {
+            if (!inside && mat.sss == sc.probes.sss) ps.L += ps.thr * sss_gather(sc, P, mat.sss, p, no, ps.rng);
+            nd = recip * (co * no - wo) - no * ct;
+        } else {
+            nd = 2.0f * co * no - wo;
+        }
+        ps.o = p;
+        ps.d = nd;
         ps.depth++;
         ps.emit = 1;
         ps.prev_bsdf_pdf = -1.0f;

@@ -8,18 +8,17 @@
 // other partitions in turn; every wave exits when all 8 are drained.  Each lane
 // traces the chunk's samples of its pixel with the same path loop as the
 // per-render megakernel and adds FP64 sums to the batch film (plain RMW when one
-// chunk covers the pixel's samples, FP64 atomics otherwise).
+// chunk covers the pixel's samples, FP64 atomics otherwise).  Item fetch and the
+// per-lane body are the plain megakernel's (orf_mega.h).
 #include <algorithm>
 
 #include "orf_common.h"
-#include "orf_path.h"
-#include "orf_render.h"
+#include "orf_mega.h"
 #include "orf_wavefront.h"
 
 namespace orf {
 
-constexpr int kPersistBlock = 64;
-constexpr int kXcdParts = 8;
+constexpr int kPersistBlock = kMegaBlock;
 
 __global__ __launch_bounds__(kPersistBlock, ORF_MEGA_WAVES) void k_batch_persistent(
     SceneGPU sc, const WfRender* __restrict__ renders, const PersistRender* __restrict__ pr, int nrenders,
@@ -36,11 +35,8 @@ __global__ __launch_bounds__(kPersistBlock, ORF_MEGA_WAVES) void k_batch_persist
         const int part = (home + q) % kXcdParts;
         const long long lo = total * part / kXcdParts, hi = total * (part + 1) / kXcdParts;
         for (;;) {
-            unsigned got = 0;
-            if (lane == 0) got = (unsigned)atomicAdd(qctr + part, 1ull);
-            got = __shfl(got, 0);
-            const long long item = lo + (long long)got;
-            if (item >= hi) break;  // uniform across the wave
+            const long long item = mega_next(qctr, part, lo, hi);
+            if (item < 0) break;  // uniform across the wave
             int a = 0, b = nrenders - 1;
             while (a < b) {
                 int m = (a + b + 1) >> 1;
@@ -54,35 +50,9 @@ __global__ __launch_bounds__(kPersistBlock, ORF_MEGA_WAVES) void k_batch_persist
             const int x = (tile % P.tiles_x) * 8 + (int)(lane & 7), y = (tile / P.tiles_x) * 8 + (int)(lane >> 3);
             if (x < R.width && y < R.height) {  // every lane reaches the next item fetch
                 const unsigned k0 = (unsigned)chunk * P.per, k1 = min(k0 + P.per, R.spp);
-                const int* rm = remap + R.remap_offset;
-                double sr = 0.0, sg = 0.0, sb = 0.0;
-                unsigned cnt = 0;
-                for (unsigned k = k0; k < k1; ++k) {
-                    f3 dir;
-                    uint32_t rng = camera_ray(R.cam, R.width, R.height, x, y, R.frame0 + k, dir);
-                    f3 L = path_radiance(sc, rm, R.cam.eye, dir, rng, stk, kPersistBlock, st);
-                    if (finite3(L)) {
-                        sr += (double)L.x;
-                        sg += (double)L.y;
-                        sb += (double)L.z;
-                        cnt++;
-                    } else {
-                        bad++;
-                    }
-                }
+                mega_pixel(sc, R.cam, R.width, R.height, x, y, remap + R.remap_offset, R.frame0, k0, k1,
+                           P.chunks == 1, film + 4 * ((size_t)R.film_offset + (size_t)y * R.width + x), stk, st, bad);
                 npaths += k1 - k0;
-                double* px = film + 4 * ((size_t)R.film_offset + (size_t)y * R.width + x);
-                if (P.chunks == 1) {
-                    px[0] += sr;
-                    px[1] += sg;
-                    px[2] += sb;
-                    px[3] += (double)cnt;
-                } else {
-                    atomicAdd(px + 0, sr);
-                    atomicAdd(px + 1, sg);
-                    atomicAdd(px + 2, sb);
-                    atomicAdd(px + 3, (double)cnt);
-                }
             }
         }
     }
@@ -100,6 +70,7 @@ void launch_batch_persistent(const SceneGPU& sc, const WfRender* renders, const 
     // per-partition counters are 32-bit once read back
     ORF_REQUIRE(total_items / kXcdParts < 0xF0000000LL, "persistent batch: too many work items");
     if (total_items == 0) return;
+    zero_queue_counters(qctr, stream);
     hipLaunchKernelGGL(k_batch_persistent, dim3((unsigned)grid), dim3(kPersistBlock), 0, stream, sc, renders, pr,
                        nrenders, remap, film, stats, qctr, total_items);
     ORF_HIP_CHECK(hipGetLastError());

@@ -9,7 +9,7 @@ namespace orf {
 
 // Minimum waves per SIMD the register allocator must allow (occupancy vs spills).
 #ifndef ORF_MEGA_WAVES
-#define ORF_MEGA_WAVES 5
+#define ORF_MEGA_WAVES 6
 #endif
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8),
@@ -37,7 +37,13 @@ struct FilmGPU {
     double* sums;                  // 4 per pixel: R, G, B sums and finite count
     unsigned long long* counters;  // [0] non-finite, [1] extension rays, [2] shadow rays, [3] paths
     int width, height;
+    unsigned long long* qctr;      // 8 work-queue counters (device, zeroed by each launch)
 };
+
+// Workgroups that fill the current device at the megakernels' occupancy (at most `items`).
+int persistent_grid(long long items);
+// Zero 8 queue counters with a kernel (safe inside captured streams).
+void zero_queue_counters(unsigned long long* qctr, hipStream_t stream);
 
 // Megakernel: every (pixel, sample) path of frames frame0 .. frame0+spp-1.
 // remap: per-render material table (device, nullptr = identity).

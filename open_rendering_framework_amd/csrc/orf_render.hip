@@ -1,88 +1,84 @@
 // orf_render.hip -- plain megakernel path tracer and environment CDF build (gfx950).
 //
-// Megakernel: one wave64 workgroup per (8x8 pixel tile, sample chunk); each lane
-// walks all paths of its pixel's sample range (path loop in orf_path.h) and adds
-// FP64 sums of its finite samples to the film (plain RMW when one chunk covers
-// all samples, else FP64 atomics).
+// Megakernel: persistent wave64 workgroups take (8x8 pixel tile, chunk of
+// kMegaChunk samples) work items from per-XCD queues (orf_mega.h); each lane
+// walks its pixel's samples of the chunk (path loop in orf_path.h) and adds FP64
+// sums of its finite samples to the film (plain RMW when one chunk covers all
+// samples, else FP64 atomics).
 #include <algorithm>
 
 #include "orf_common.h"
-#include "orf_path.h"
-#include "orf_render.h"
+#include "orf_mega.h"
 
 namespace orf {
 
-constexpr int kMegaBlock = 64;
-
 __global__ __launch_bounds__(kMegaBlock, ORF_MEGA_WAVES) void k_render_mega(SceneGPU sc, CameraGPU cam, FilmGPU film,
-                                                            const int* remap, unsigned frame0, unsigned spp,
-                                                            int chunks, unsigned per, int tiles_x)
+                                                                             const int* remap, unsigned frame0,
+                                                                             unsigned spp, int tiles_x, int chunks,
+                                                                             long long items)
 {
     __shared__ unsigned stack_lds[kLdsStack * kMegaBlock];
     unsigned* stk = stack_lds + threadIdx.x;
-    unsigned blk = xcd_block(blockIdx.x, gridDim.x);
-    int tile = blk / chunks, chunk = blk % chunks;
-    int x = (tile % tiles_x) * 8 + (threadIdx.x & 7);
-    int y = (tile / tiles_x) * 8 + (threadIdx.x >> 3);
-    if (x >= film.width || y >= film.height) return;
-    // local sample range of this chunk (frame = frame0 + k wraps like the seeds do)
-    unsigned k0 = (unsigned)chunk * per;
-    unsigned k1 = min(k0 + per, spp);
-    if (k0 >= k1) return;
-    bool inside = !((unsigned)x < cam.patch_origin[0] || (unsigned)x > cam.patch_origin[0] + cam.patch_dims[0] ||
-                    (unsigned)y < cam.patch_origin[1] || (unsigned)y > cam.patch_origin[1] + cam.patch_dims[1]);
-    double sr = 0.0, sg = 0.0, sb = 0.0;
-    unsigned cnt = 0, bad = 0;
+    const int home = (int)(blockIdx.x % kXcdParts);
+    unsigned long long bad = 0, npaths = 0;
     PathStats st;
-    for (unsigned k = k0; k < k1; ++k) {
-        f3 L = mk3(0.0f);
-        if (inside) {
-            f3 dir;
-            uint32_t rng = camera_ray(cam, film.width, film.height, x, y, frame0 + k, dir);
-            L = path_radiance(sc, remap, cam.eye, dir, rng, stk, kMegaBlock, st);
-        }
-        if (finite3(L)) {
-            sr += (double)L.x;
-            sg += (double)L.y;
-            sb += (double)L.z;
-            cnt++;
-        } else {
-            bad++;
+    for (int q = 0; q < kXcdParts; ++q) {
+        const int part = (home + q) % kXcdParts;
+        const long long lo = items * part / kXcdParts, hi = items * (part + 1) / kXcdParts;
+        for (;;) {
+            const long long item = mega_next(film.qctr, part, lo, hi);
+            if (item < 0) break;  // uniform across the wave
+            const int tile = (int)(item / chunks), chunk = (int)(item - (long long)tile * chunks);
+            const int x = (tile % tiles_x) * 8 + (int)(threadIdx.x & 7), y = (tile / tiles_x) * 8 + (int)(threadIdx.x >> 3);
+            if (x < film.width && y < film.height) {
+                const unsigned k0 = (unsigned)chunk * kMegaChunk, k1 = min(k0 + kMegaChunk, spp);
+                mega_pixel(sc, cam, film.width, film.height, x, y, remap, frame0, k0, k1, chunks == 1,
+                           film.sums + 4 * ((size_t)y * film.width + x), stk, st, bad);
+                npaths += k1 - k0;
+            }
         }
     }
-    double* px = film.sums + 4 * ((size_t)y * film.width + x);
-    if (chunks == 1) {
-        px[0] += sr;
-        px[1] += sg;
-        px[2] += sb;
-        px[3] += (double)cnt;
-    } else {
-        atomicAdd(px + 0, sr);
-        atomicAdd(px + 1, sg);
-        atomicAdd(px + 2, sb);
-        atomicAdd(px + 3, (double)cnt);
-    }
-    if (bad) atomicAdd(film.counters + 0, (unsigned long long)bad);
+    if (bad) atomicAdd(film.counters + 0, bad);
     atomicAdd(film.counters + 1, (unsigned long long)st.ext);
     atomicAdd(film.counters + 2, (unsigned long long)st.shd);
-    atomicAdd(film.counters + 3, (unsigned long long)(k1 - k0));
+    atomicAdd(film.counters + 3, npaths);
+}
+
+__global__ void k_zero_u64(unsigned long long* p, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+void zero_queue_counters(unsigned long long* qctr, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(64), 0, stream, qctr, kXcdParts);
+    ORF_HIP_CHECK(hipGetLastError());
+}
+
+int persistent_grid(long long items)
+{
+    static int cus[64] = {0};
+    int dev = 0;
+    ORF_HIP_CHECK(hipGetDevice(&dev));
+    int& n = cus[dev & 63];
+    if (n == 0) ORF_HIP_CHECK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
+    long long g = (long long)n * 4 * ORF_MEGA_WAVES;
+    return (int)std::max<long long>(1, std::min<long long>(g, items));
 }
 
 void launch_megakernel(const SceneGPU& sc, const CameraGPU& cam, const FilmGPU& film, const int* remap,
                        unsigned frame0, unsigned spp, hipStream_t stream)
 {
     if (spp == 0 || film.width <= 0 || film.height <= 0) return;
+    ORF_REQUIRE(film.qctr, "launch: film has no queue counters");
     int tiles_x = (film.width + 7) / 8, tiles_y = (film.height + 7) / 8;
-    int64_t tiles = (int64_t)tiles_x * tiles_y;
-    // enough lanes to fill 256 CUs several times over; each lane runs >= 4 samples
-    int64_t target_waves = 256 * 64;
-    int chunks = (int)std::max<int64_t>(1, std::min<int64_t>((target_waves + tiles - 1) / tiles, (spp + 3) / 4));
-    unsigned per = (spp + chunks - 1) / chunks;
-    chunks = (int)((spp + per - 1) / per);
-    int64_t blocks = tiles * chunks;
-    ORF_REQUIRE(blocks < 0x7fffffffLL, "launch: image too large");
-    hipLaunchKernelGGL(k_render_mega, dim3((unsigned)blocks), dim3(kMegaBlock), 0, stream, sc, cam, film, remap,
-                       frame0, spp, chunks, per, tiles_x);
+    int chunks = (int)((spp + kMegaChunk - 1) / kMegaChunk);
+    long long items = (long long)tiles_x * tiles_y * chunks;
+    ORF_REQUIRE(items / kXcdParts < 0xF0000000LL, "launch: too many work items");
+    zero_queue_counters(film.qctr, stream);
+    hipLaunchKernelGGL(k_render_mega, dim3((unsigned)persistent_grid(items)), dim3(kMegaBlock), 0, stream, sc, cam,
+                       film, remap, frame0, spp, tiles_x, chunks, items);
     ORF_HIP_CHECK(hipGetLastError());
 }
 

@@ -39,7 +39,7 @@ struct MaterialGPU {
     int kind;
     int microfacet_model;
     int normal_distribution;
-    int pad0;
+    int sss;            // translucent: index into SceneGPU::sss (-1: none)
     f3 diffuse_color;   // Lambert albedo / interface albedo
     f3 radiance;        // emitter radiance
     f3 glass_absorption;
@@ -87,6 +87,33 @@ struct CameraGPU {
     unsigned patch_dims[2];
 };
 
+// ScatteringMaterialProperties (ORF/structs.h:116-144), FP32, derived on the host
+// as ORF/RoughTranslucentMaterial.cpp:366-395 does (outside ior 1).
+struct ScatteringGPU {
+    float relative_ior;
+    int dipole_model;   // 0 standard, 1 directional (ORF/structs.h:311-316)
+    f3 absorption, scattering, meancosine;
+    f3 extinction, reducedExtinction, deltaEddExtinction, D, transport;
+    f3 C_phi, C_phi_inv, C_E, A, albedo, reducedAlbedo, de, three_D, two_A_de, global_coeff, one_over_three_ext;
+    float mean_transport;
+};
+
+// Surface probes of the translucent object: Poisson-disk positions fixed at
+// build time, per-launch irradiance from the probe pass, hashed uniform grid.
+struct ProbesGPU {
+    const float4* pos;   // xyz
+    const float4* nrm;   // xyz (shading normal at the probe)
+    float4* L;           // RGB: incident radiance estimate Le of this launch's probe sample
+    float4* w12;         // xyz: refracted incident direction, w: T12
+    const uint2* cells;  // hash slot -> (first probe, count); probes sorted by slot
+    unsigned table_mask; // slots - 1 (power of two)
+    float cell;          // grid cell size (>= the nearest-probe search radius)
+    int count;           // probes (0: none)
+    int samples;         // exit-radius samples per entering hit
+    int sss;             // ScatteringGPU index of the probes' material (-1: none)
+    int pad;
+};
+
 struct SceneGPU {
     const float4* bvh;        // 4 float4 per node
     const float4* tri;        // 3 float4 per leaf-ordered triangle
@@ -102,6 +129,8 @@ struct SceneGPU {
     int max_depth;
     float scene_epsilon;
     int pad;
+    const ScatteringGPU* sss;  // per translucent material
+    ProbesGPU probes;
 };
 
 constexpr float kRayMax = 1e27f;  // RT_DEFAULT_MAX

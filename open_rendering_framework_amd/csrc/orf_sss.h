@@ -1,0 +1,145 @@
+// orf_sss.h -- diffusion gather of the translucent material (device; also host-callable).
+//
+// The reference gathers a translucent hit from every probe of the frame
+// (CU/subsurface_scattering_shader.cu:120-167: sum over N probes of
+// T12 L S(d) / exp_term with Russian roulette on exp(-d sigma_tr,min), times
+// global_coeff / N).  With 2^20 Poisson-disk probes that O(N) loop is replaced
+// by importance sampling of the exit point (King et al. 2013, "BSSRDF importance
+// sampling", one projection axis):
+//   * channel c uniform, radius r ~ sigma_tr,c exp(-sigma_tr,c r), angle uniform
+//     on the tangent plane of the entry point xo;
+//   * a probe ray along -no through that plane point, clipped to the sphere of
+//     radius r_max, collects up to kSssHits crossings of the same translucent
+//     material; one is picked uniformly;
+//   * the exit point's incident light is the nearest probe's (within the grid
+//     cell size), its pdf on the surface is
+//         p = (1/3) sum_c sigma_c exp(-sigma_c r) / (2 pi r) * |n_i . no| / hits,
+//   * estimate = global_coeff * mean over samples of T12 Le S(xi, xo) / p.
+// This is the reference's integral (its L already carries the 1 / pdf of the
+// probe position) with a different Monte-Carlo estimator.
+#pragma once
+
+#include "orf_dipole.h"
+#include "orf_shade.h"
+#include "orf_trace.h"
+
+namespace orf {
+
+constexpr int kSssHits = 4;            // crossings kept per probe ray
+constexpr float kSssRadiusSigmas = 12.0f;  // r_max = 12 / sigma_tr,min
+
+ORF_HD unsigned probe_hash(int ix, int iy, int iz, unsigned mask)
+{
+    return ((unsigned)ix * 73856093u ^ (unsigned)iy * 19349663u ^ (unsigned)iz * 83492791u) & mask;
+}
+
+ORF_HD int probe_cell(float v, float inv_cell) { return (int)floorf(v * inv_cell); }
+
+// Nearest probe strictly within distance `cell` of y (ties: lower index), or -1.
+ORF_HD int nearest_probe(const ProbesGPU& pr, f3 y)
+{
+    float inv = 1.0f / pr.cell;
+    int cx = probe_cell(y.x, inv), cy = probe_cell(y.y, inv), cz = probe_cell(y.z, inv);
+    float best_d2 = pr.cell * pr.cell;
+    int best = -1;
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                uint2 c = pr.cells[probe_hash(cx + dx, cy + dy, cz + dz, pr.table_mask)];
+                for (unsigned j = c.x; j < c.x + c.y; ++j) {
+                    float4 q = pr.pos[j];
+                    f3 e = mk3(q.x, q.y, q.z) - y;
+                    float d2 = dot(e, e);
+                    if (d2 < best_d2 || (d2 == best_d2 && best >= 0 && (int)j < best)) {
+                        best_d2 = d2;
+                        best = (int)j;
+                    }
+                }
+            }
+    return best;
+}
+
+struct SssHit {
+    float t;
+    int prim, leaf;
+    float b, g;
+};
+
+// Crossings of the translucent material `sss` along o + t dir, 0 < t < tmax.
+ORF_HD int probe_ray_hits(const SceneGPU& sc, int sss, f3 o, f3 dir, float tmax, SssHit* out, unsigned* stk)
+{
+    int n = 0;
+    float tmin = 0.0f;
+    for (int it = 0; it < 2 * kSssHits && n < kSssHits; ++it) {
+        Hit h;
+        if (!trace<false>(sc, o, dir, tmin, tmax, h, stk, 1)) break;
+        int mid = sc.shade[h.prim].w;
+        if (sc.materials[mid].kind == MAT_TRANSLUCENT && sc.materials[mid].sss == sss) {
+            out[n].t = h.t;
+            out[n].prim = h.prim;
+            out[n].leaf = h.leaf;
+            out[n].b = h.b;
+            out[n].g = h.g;
+            ++n;
+        }
+        tmin = h.t;  // strict: the next crossing lies beyond this one
+    }
+    return n;
+}
+
+// Diffusion term entering at xo (normal no facing the incident side).
+ORF_HD f3 sss_gather(const SceneGPU& sc, const ScatteringGPU& P, int sss, f3 xo, f3 no, uint32_t& rng)
+{
+    const ProbesGPU& pr = sc.probes;
+    f3 acc = mk3(0.0f);
+    if (pr.count <= 0 || pr.samples <= 0) return acc;
+    unsigned stk[kLdsStack];
+    f3 t1, t2;
+    make_frame(no, t1, t2);
+    float smin = fminf(P.transport.x, fminf(P.transport.y, P.transport.z));
+    float r_max = kSssRadiusSigmas / smin;
+    for (int k = 0; k < pr.samples; ++k) {
+        float u = rnd_tea(rng), v = rnd_tea(rng), w = rnd_tea(rng), s = rnd_tea(rng);
+        int c = min(2, (int)(3.0f * u));
+        float sig = c == 0 ? P.transport.x : (c == 1 ? P.transport.y : P.transport.z);
+        float r = -orf_logf(1.0f - v) / sig;
+        if (!(r < r_max)) continue;
+        float sp, cp;
+        orf_sincosf(6.28318530717958647692f * w, &sp, &cp);
+        float hh = sqrtf(r_max * r_max - r * r);
+        f3 top = xo + (t1 * cp + t2 * sp) * r + no * hh;
+        SssHit hits[kSssHits];
+        int nh = probe_ray_hits(sc, sss, top, -no, 2.0f * hh, hits, stk);
+        if (nh == 0) continue;
+        const SssHit& hp = hits[min(nh - 1, (int)(s * (float)nh))];
+        f3 xi = top + (-no) * hp.t;
+        const float4* tg = sc.tri + 3 * (size_t)hp.leaf;
+        float4 q1 = tg[1], q2 = tg[2];
+        int4 sh;
+        f3 ni = shading_normal(sc, hp.prim, hp.b, hp.g, mk3(q1.x, q1.y, q1.z), mk3(q2.x, q2.y, q2.z), sh);
+        float cosn = fabsf(dot(ni, no));
+        f3 dx = xi - xo;
+        f3 inpl = dx - no * dot(dx, no);
+        float rp = sqrtf(dot(inpl, inpl));
+        float pr_r = (P.transport.x * orf_expf(-P.transport.x * rp) + P.transport.y * orf_expf(-P.transport.y * rp) +
+                      P.transport.z * orf_expf(-P.transport.z * rp)) /
+                     (3.0f * 6.28318530717958647692f * rp);
+        float pdf = pr_r * cosn / (float)nh;
+        if (!(pdf > 0.0f) || !__builtin_isfinite(pdf)) continue;
+        int j = nearest_probe(pr, xi);
+        if (j < 0) continue;
+        float4 Lj = pr.L[j], wj = pr.w12[j];
+        f3 E = mk3(Lj.x, Lj.y, Lj.z) * wj.w;
+        f3 S;
+        if (P.dipole_model == 1) {
+            float4 nj = pr.nrm[j];
+            S = dirpole_bssrdf(xi, mk3(nj.x, nj.y, nj.z), mk3(wj.x, wj.y, wj.z), xo, no, P);
+        } else {
+            S = dipole_bssrdf(sqrtf(dot(dx, dx)), P);
+        }
+        acc += E * S / pdf;
+    }
+    return acc * (1.0f / (float)pr.samples) * P.global_coeff;
+}
+
+}  // namespace orf

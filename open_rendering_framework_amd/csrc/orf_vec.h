@@ -32,6 +32,10 @@ ORF_HD f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
 ORF_HD f3 operator*(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
 ORF_HD f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
 ORF_HD f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+ORF_HD f3 operator+(f3 a, float s) { return f3{a.x + s, a.y + s, a.z + s}; }
+ORF_HD f3 operator+(float s, f3 a) { return f3{s + a.x, s + a.y, s + a.z}; }
+ORF_HD f3 operator-(f3 a, float s) { return f3{a.x - s, a.y - s, a.z - s}; }
+ORF_HD f3 operator-(float s, f3 a) { return f3{s - a.x, s - a.y, s - a.z}; }
 ORF_HD f3& operator+=(f3& a, f3 b) { a = a + b; return a; }
 ORF_HD f3& operator*=(f3& a, f3 b) { a = a * b; return a; }
 ORF_HD f3& operator*=(f3& a, float s) { a = a * s; return a; }
