@@ -242,6 +242,33 @@ orf_status orf_diff_adjoint(orf_context ctx, const orf_diff_desc* desc, const or
                             int32_t nviews, const float* albedo, const float* roughness, const float* d_images,
                             float* g_albedo, float* g_roughness, uint32_t frame_shift, void* stream);
 
+/* ------------------------------------------------------------------------
+ * Translucent materials ("subsurface_scattering_shader.cu").
+ *
+ * Material variables absorption, scattering, meancosine (3 floats, 1/mm), scale,
+ * ior, dipole_model (0 standard, 1 directional) as ScatteringMaterial /
+ * RoughTranslucentMaterial take them (ORF/RoughTranslucentMaterial.cpp:158-215);
+ * defaults are the Wholemilk preset with scale 100.  Context variables
+ * probe_count (0: no diffusion term), probe_samples (exit-point samples per
+ * entering hit, default 4) and probe_seed select the Poisson-disk surface probes
+ * that replace the reference's per-frame probe buffer (CU/sample_camera.cu);
+ * every launch first runs the probe pass with sample index frame0.
+ * ------------------------------------------------------------------------ */
+/* Derived coefficients (ScatteringMaterialProperties, ORF/structs.h:116-144, as
+ * ORF/RoughTranslucentMaterial.cpp:366-395 computes them): n >= 64 floats in the
+ * order relative_ior, dipole_model, then the 3-vectors absorption, scattering,
+ * meancosine, extinction, reducedExtinction, deltaEddExtinction, D, transport,
+ * C_phi, C_phi_inv, C_E, A, albedo, reducedAlbedo, de, three_D, two_A_de,
+ * global_coeff, one_over_three_ext, then mean_transport. */
+orf_status orf_scattering_properties(orf_context ctx, int material, float* out, int n);
+/* The probes (no reference counterpart).  Arrays of 4 floats per probe (xyz, w);
+ * any pointer may be NULL.  L: incident radiance estimate of the last probe pass;
+ * w12: refracted incident direction (xyz) and T12 (w). */
+orf_status orf_probes_read(orf_context ctx, int64_t* count, double* area, float* cell, float* rmin, float* pos,
+                           float* nrm, float* L, float* w12);
+/* Run the probe pass alone for sample index `frame`. */
+orf_status orf_probe_pass(orf_context ctx, uint32_t frame, void* stream);
+
 /* Procedural scene pieces (host).  Icosphere: call with verts = tris = normals =
  * NULL to get the sizes.  Env: sun_dir in env coordinates, sigma in degrees. */
 orf_status orf_gen_icosphere(int level, uint32_t noise_seed, float amplitude, float frequency,

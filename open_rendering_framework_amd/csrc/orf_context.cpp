@@ -54,6 +54,7 @@ static int program_kind(const std::string& prog)
     if (p == "arealight_shader") return MAT_EMITTER;
     if (p == "rough_transparent_shader") return MAT_DIELECTRIC;
     if (p == "lambertian_interface_shader") return MAT_INTERFACE;
+    if (p == "subsurface_scattering_shader" || p == "translucent_shader") return MAT_TRANSLUCENT;
     return -1;
 }
 
@@ -92,6 +93,13 @@ static void set_context_int(orf_context_t* c, const std::string& name, const int
         dst[0] = (unsigned)v[0];
         dst[1] = (unsigned)v[1];
         c->patch_set = true;
+    } else if (name == "probe_count" || name == "probe_samples" || name == "probe_seed") {
+        // surface probes of the translucent object (count, gather samples per hit, seed)
+        ORF_REQUIRE(n == 1 && v[0] >= 0 && v[0] <= (1 << 24), name + " takes one value in [0, 2^24]");
+        if (name == "probe_count") c->probe_count_req = (int)v[0];
+        else if (name == "probe_samples") c->probe_samples = (int)v[0];
+        else c->probe_seed = (unsigned)v[0];
+        if (name != "probe_samples") c->dirty_geo = true;
     } else {
         ORF_THROW(ORF_ERROR_INVALID_VALUE, "unknown integer context variable '" + name + "'");
     }
@@ -119,6 +127,18 @@ static void set_material_float(orf_context_t* c, int handle, const std::string& 
     } else if (name == "ior") {
         ORF_REQUIRE(n == 1, "float ior takes 1 float (conductors take MyComplex3 user data)");
         m.ior = v[0];
+        c->mat_sss[handle].ior = v[0];
+    } else if (name == "absorption" || name == "scattering" || name == "meancosine") {
+        // ScatteringMaterial / TranslucentMaterial parameters (ORF/RoughTranslucentMaterial.cpp:158-215)
+        ORF_REQUIRE(n == 3, "material variable '" + name + "' takes 3 floats");
+        f3 x = mk3(v[0], v[1], v[2]);
+        SssParamsHost& h = c->mat_sss[handle];
+        if (name == "absorption") h.absorption = x;
+        else if (name == "scattering") h.scattering = x;
+        else h.meancosine = x;
+    } else if (name == "scale") {
+        ORF_REQUIRE(n == 1 && v[0] > 0.0f, "scale takes one positive float");
+        c->mat_sss[handle].scale = v[0];
     } else {
         ORF_THROW(ORF_ERROR_INVALID_VALUE, "unknown float material variable '" + name + "'");
     }
@@ -134,6 +154,9 @@ static void set_material_int(orf_context_t* c, int handle, const std::string& na
     } else if (name == "normal_distribution") {
         ORF_REQUIRE(v[0] == 0 || v[0] == 1, "normal_distribution must be 0 or 1");
         m.normal_distribution = (int)v[0];
+    } else if (name == "dipole_model") {
+        ORF_REQUIRE(v[0] == 0 || v[0] == 1, "dipole_model must be 0 (standard) or 1 (directional)");
+        c->mat_sss[handle].dipole_model = (int)v[0];
     } else {
         ORF_THROW(ORF_ERROR_INVALID_VALUE, "unknown integer material variable '" + name + "'");
     }
@@ -198,6 +221,36 @@ static void build_geometry(orf_context_t* c, hipStream_t)
     c->d_tri.upload(bvh.tris);
     c->d_shade.upload(shade);
     c->d_normals.upload(normals);
+    // probes on the first mesh whose material is translucent
+    c->probe_sss = -1;
+    c->probe_count = 0;
+    if (c->probe_count_req > 0) {
+        for (const MeshRec& m : c->meshes)
+            if (m.material >= 0 && m.material < (int)c->mats.size() && c->mats[m.material].kind == MAT_TRANSLUCENT) {
+                c->probe_sss = m.material;
+                break;
+            }
+    }
+    if (c->probe_sss >= 0) {
+        ProbeSetHost ps = generate_probes(V.data(), I.data(), shade.data(), normals.data(), (int64_t)(I.size() / 3),
+                                          c->probe_sss, c->probe_count_req, c->probe_seed);
+        c->probe_count = (int)ps.pos.size();
+        c->probe_mask = ps.table_mask;
+        c->probe_cell = ps.cell;
+        c->probe_rmin = ps.rmin;
+        c->probe_area = ps.area;
+        if (c->probe_count > 0) {
+            c->d_probe_pos.upload(ps.pos);
+            c->d_probe_nrm.upload(ps.nrm);
+            c->d_probe_cells.upload(ps.cells);
+            c->d_probe_L.ensure(ps.pos.size() * sizeof(float4));
+            c->d_probe_w12.ensure(ps.pos.size() * sizeof(float4));
+            ORF_HIP_CHECK(hipMemset(c->d_probe_L.p, 0, c->d_probe_L.bytes));
+            ORF_HIP_CHECK(hipMemset(c->d_probe_w12.p, 0, c->d_probe_w12.bytes));
+        } else {
+            c->probe_sss = -1;
+        }
+    }
     c->dirty_geo = false;
 }
 
@@ -244,8 +297,15 @@ void prepare(orf_context_t* c, hipStream_t stream)
     if (c->dirty_geo) build_geometry(c, stream);
     if (c->dirty_mat) {
         std::vector<MaterialGPU> m = c->mats;
+        std::vector<ScatteringGPU> sss;
+        for (size_t i = 0; i < m.size(); ++i) {
+            sss.push_back(scattering_coefficients(c->mat_sss[i]));
+            m[i].sss = m[i].kind == MAT_TRANSLUCENT ? (int)i : -1;
+        }
         if (m.empty()) m.push_back(default_material(MAT_LAMBERT));
+        if (sss.empty()) sss.push_back(scattering_coefficients(SssParamsHost()));
         c->d_mats.upload(m);
+        c->d_sss.upload(sss);
         c->dirty_mat = false;
     }
     if (c->dirty_env) upload_env(c, stream);
@@ -275,7 +335,31 @@ SceneGPU scene_view(orf_context_t* c)
     s.env.sampled = c->env_w > 0 && c->env_is;
     s.max_depth = c->max_depth;
     s.scene_epsilon = c->scene_epsilon;
+    s.sss = c->d_sss.as<ScatteringGPU>();
+    s.probes.sss = -1;
+    if (c->probe_count > 0) {
+        s.probes.pos = c->d_probe_pos.as<float4>();
+        s.probes.nrm = c->d_probe_nrm.as<float4>();
+        s.probes.L = c->d_probe_L.as<float4>();
+        s.probes.w12 = c->d_probe_w12.as<float4>();
+        s.probes.cells = c->d_probe_cells.as<uint2>();
+        s.probes.table_mask = c->probe_mask;
+        s.probes.cell = c->probe_cell;
+        s.probes.count = c->probe_count;
+        s.probes.samples = c->probe_samples;
+        s.probes.sss = c->probe_sss;
+    }
     return s;
+}
+
+// Probe pass of the translucent object for sample index `frame` (no-op without probes).
+void run_probe_pass(orf_context_t* c, unsigned frame, hipStream_t stream)
+{
+    if (c->probe_count <= 0) return;
+    SceneGPU s = scene_view(c);
+    int sss = s.probes.sss;
+    s.probes.sss = -1;  // no gathers while the probes are being written
+    launch_probe_pass(s, c->d_sss.as<ScatteringGPU>(), sss, frame, stream);
 }
 
 static void film_clear(orf_context_t* c)
@@ -392,6 +476,8 @@ extern "C" orf_status orf_material_create(orf_context ctx, const char* program, 
     int kind = orf::program_kind(program);
     if (kind < 0) ORF_THROW(ORF_ERROR_NOT_FOUND, std::string("no radiance program for '") + program + "'");
     c->mats.push_back(orf::default_material(kind));
+    c->mat_sss.push_back(orf::SssParamsHost());
+    if (kind == orf::MAT_TRANSLUCENT) c->mats.back().ior = 1.3f;
     c->dirty_mat = true;
     *material = (int)c->mats.size() - 1;
     ORF_API_END
@@ -579,6 +665,7 @@ extern "C" orf_status orf_launch(orf_context ctx, int width, int height, uint32_
         cam.patch_dims[0] = (unsigned)width;
         cam.patch_dims[1] = (unsigned)height;
     }
+    orf::run_probe_pass(c, frame0, st);
     orf::FilmGPU film{c->d_film.as<double>(), c->d_counters.as<unsigned long long>(), width, height,
                       c->d_counters.as<unsigned long long>() + 8};
     orf::launch_megakernel(orf::scene_view(c), cam, film, nullptr, frame0, spp, st);
@@ -638,5 +725,61 @@ extern "C" orf_status orf_counters(orf_context ctx, uint64_t out[4])
     }
     ORF_HIP_CHECK(hipDeviceSynchronize());
     ORF_HIP_CHECK(hipMemcpy(out, c->d_counters.p, 4 * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    ORF_API_END
+}
+
+extern "C" orf_status orf_scattering_properties(orf_context ctx, int material, float* out, int n)
+{
+    ORF_API_BEGIN
+    orf_context_t* c = ctx_check(ctx);
+    ORF_REQUIRE(out && n >= 64, "orf_scattering_properties: need 64 floats");
+    ORF_REQUIRE(material >= 0 && material < (int)c->mats.size(), "orf_scattering_properties: bad material");
+    orf::ScatteringGPU p = orf::scattering_coefficients(c->mat_sss[material]);
+    const orf::f3* v[] = {&p.absorption, &p.scattering, &p.meancosine, &p.extinction, &p.reducedExtinction,
+                          &p.deltaEddExtinction, &p.D, &p.transport, &p.C_phi, &p.C_phi_inv, &p.C_E, &p.A,
+                          &p.albedo, &p.reducedAlbedo, &p.de, &p.three_D, &p.two_A_de, &p.global_coeff,
+                          &p.one_over_three_ext};
+    int k = 0;
+    out[k++] = p.relative_ior;
+    out[k++] = (float)p.dipole_model;
+    for (const orf::f3* x : v) {
+        out[k++] = x->x;
+        out[k++] = x->y;
+        out[k++] = x->z;
+    }
+    out[k++] = p.mean_transport;
+    while (k < n) out[k++] = 0.0f;
+    ORF_API_END
+}
+
+extern "C" orf_status orf_probes_read(orf_context ctx, int64_t* count, double* area, float* cell, float* rmin,
+                                      float* pos, float* nrm, float* L, float* w12)
+{
+    ORF_API_BEGIN
+    orf_context_t* c = ctx_check(ctx);
+    orf::prepare(c, nullptr);
+    if (count) *count = c->probe_count;
+    if (area) *area = c->probe_area;
+    if (cell) *cell = c->probe_cell;
+    if (rmin) *rmin = c->probe_rmin;
+    size_t bytes = (size_t)c->probe_count * sizeof(float4);
+    ORF_HIP_CHECK(hipDeviceSynchronize());
+    if (c->probe_count > 0) {
+        if (pos) ORF_HIP_CHECK(hipMemcpy(pos, c->d_probe_pos.p, bytes, hipMemcpyDeviceToHost));
+        if (nrm) ORF_HIP_CHECK(hipMemcpy(nrm, c->d_probe_nrm.p, bytes, hipMemcpyDeviceToHost));
+        if (L) ORF_HIP_CHECK(hipMemcpy(L, c->d_probe_L.p, bytes, hipMemcpyDeviceToHost));
+        if (w12) ORF_HIP_CHECK(hipMemcpy(w12, c->d_probe_w12.p, bytes, hipMemcpyDeviceToHost));
+    }
+    ORF_API_END
+}
+
+extern "C" orf_status orf_probe_pass(orf_context ctx, uint32_t frame, void* stream)
+{
+    ORF_API_BEGIN
+    orf_context_t* c = ctx_check(ctx);
+    hipStream_t st = (hipStream_t)stream;
+    orf::prepare(c, st);
+    orf::run_probe_pass(c, frame, st);
+    ORF_HIP_CHECK(hipStreamSynchronize(st));
     ORF_API_END
 }

@@ -6,6 +6,7 @@
 
 #include "orf_bvh.h"
 #include "orf_devbuf.h"
+#include "orf_probes_host.h"
 #include "orf_render.h"
 #include "orf_wavefront_host.h"
 
@@ -32,6 +33,7 @@ struct orf_context_t {
     float exception_color[3] = {1.0f, 0.0f, 0.0f};
     // scene records
     std::vector<orf::MaterialGPU> mats;
+    std::vector<orf::SssParamsHost> mat_sss;  // translucent parameters, per material handle
     std::vector<orf::MeshRec> meshes;
     std::vector<orf::LightGPU> lights;
     std::vector<orf::TriLightGPU> tri_lights;
@@ -44,6 +46,16 @@ struct orf_context_t {
     orf::DevBuf d_bvh, d_tri, d_shade, d_normals, d_mats, d_lights, d_trilights;
     orf::DevBuf d_env, d_mpdf, d_mcdf, d_cpdf, d_ccdf, d_envscratch;
     orf::DevBuf d_film, d_counters;
+    // translucent material: coefficients and surface probes
+    orf::DevBuf d_sss, d_probe_pos, d_probe_nrm, d_probe_L, d_probe_w12, d_probe_cells;
+    int probe_count_req = 0;       // context variable probe_count (0: no probes)
+    int probe_samples = 4;         // context variable probe_samples (gather samples per hit)
+    unsigned probe_seed = 11;      // context variable probe_seed
+    int probe_sss = -1;            // material handle whose mesh carries the probes
+    int probe_count = 0;           // probes generated
+    unsigned probe_mask = 0;
+    float probe_cell = 0.0f, probe_rmin = 0.0f;
+    double probe_area = 0.0;
     int film_w = 0, film_h = 0;
     int nlights_dev = 0;
     // batched renders
@@ -61,5 +73,6 @@ namespace orf {
 orf_context_t* ctx_check(orf_context ctx);
 void prepare(orf_context_t* c, hipStream_t stream);
 SceneGPU scene_view(orf_context_t* c);
+void run_probe_pass(orf_context_t* c, unsigned frame, hipStream_t stream);
 }  // namespace orf
 

@@ -81,6 +81,7 @@ extern "C" orf_status orf_diff_render(orf_context ctx, const orf_diff_desc* desc
     DiffParams dp = diff_params(c, desc, albedo, roughness);
     int w, h;
     stage_views(c, views, nviews, st, w, h);
+    run_probe_pass(c, views[0].frame0, st);
     launch_diff(scene_view(c), dp, c->d_diff_views.as<WfRender>(), nviews, w, h, false, 0, images, nullptr,
                 nullptr, nullptr, c->d_diff_stats.as<unsigned long long>(), st);
     ORF_API_END

@@ -1,0 +1,71 @@
+// orf_probe_sample.h -- one incident-light sample of a translucent object's probe
+// (device; also host-callable).
+//
+// As CU/sample_camera.cu:109-212 does for its per-frame probes: with probability
+// 1/2 a light-source sample (uniform light pick, shadow ray to the light), else a
+// cosine-sampled path whose first segment counts only light that has already
+// bounced (direct light is the other strategy's); Le is divided by 1/2.  The
+// probe keeps Le, the refracted incident direction w12 and T12 = 1 - R.  TEA
+// streams tea<16>(probe, frame) replace the reference's RND_64 (MD5) streams.
+#pragma once
+
+#include "orf_path.h"
+
+namespace orf {
+
+ORF_HD void probe_sample(const SceneGPU& sc, const ScatteringGPU& P, int j, unsigned frame, unsigned* stk, int stride,
+                         float4& L_out, float4& w12_out)
+{
+    const ProbesGPU& pr = sc.probes;
+    float4 pp = pr.pos[j], nn = pr.nrm[j];
+    f3 pos = mk3(pp.x, pp.y, pp.z), n = mk3(nn.x, nn.y, nn.z);
+    uint32_t t = tea<16>((unsigned)j, frame);
+    float recip = 1.0f / P.relative_ior;
+    f3 Le = mk3(0.0f), wi = n;
+    if (rnd_tea(t) < 0.5f) {
+        unsigned li = (unsigned)((float)sc.nlights * rnd_tea(t));
+        if (sc.nlights > 0) {
+            if (li >= (unsigned)sc.nlights) li = sc.nlights - 1;
+            float light_pdf = 1.0f / (float)sc.nlights;
+            f3 wl = mk3(0.0f), rad = mk3(0.0f);
+            float dist = 0.0f, penv;
+            sample_light(sc, sc.lights[li], pos, t, wl, rad, dist, penv);
+            wi = wl;
+            float cs = fmaxf(dot(wl, n), 0.0f);
+            if (cs > 0.0f) {
+                Hit hs;
+                if (!trace<true>(sc, pos, wl, sc.scene_epsilon, dist, hs, stk, stride)) Le = rad * cs / light_pdf;
+            }
+        }
+    } else {
+        wi = sample_cosine(n, t);
+        PathState ps;
+        path_begin(ps, pos, wi, t);
+        ps.emit = 0;  // an emitter seen directly is the light-source strategy's
+        for (bool first = true;; first = false) {
+            Hit h;
+            if (!trace<false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk, stride)) {
+                if (!first || !sc.env.sampled) shade_miss(sc, ps);
+                break;
+            }
+            ShadowRay sr;
+            bool more = shade_vertex(sc, nullptr, ps, h, sr);
+            if (sr.valid) {
+                Hit hs;
+                if (!trace<true>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) ps.L += sr.contrib;
+            }
+            if (!more) break;
+        }
+        Le = ps.L * kPi;
+    }
+    Le = Le / 0.5f;
+    float ci = fmaxf(dot(wi, n), 0.0f);
+    float st2 = recip * recip * (1.0f - ci * ci);
+    float ct = sqrtf(1.0f - st2);
+    f3 w12 = recip * (ci * n - wi) - n * ct;
+    float T12 = 1.0f - fresnel_R3(ci, ct, recip);
+    L_out = make_float4(Le.x, Le.y, Le.z, 0.0f);
+    w12_out = make_float4(w12.x, w12.y, w12.z, T12);
+}
+
+}  // namespace orf

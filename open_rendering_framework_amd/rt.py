@@ -75,6 +75,9 @@ _SIGS = {
     "orf_wavefront_autotune": [c_vp, c_vp, ctypes.c_int32, c_vp, ctypes.c_int32, c_u32, c_vp, ctypes.c_char_p,
                                ctypes.c_size_t],
     "orf_diff_render": [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp, c_vp, c_vp, c_vp],
+    "orf_scattering_properties": [c_vp, ctypes.c_int, c_vp, ctypes.c_int],
+    "orf_probes_read": [c_vp, ctypes.POINTER(c_i64), c_dblp, c_f32p, c_f32p, c_vp, c_vp, c_vp, c_vp],
+    "orf_probe_pass": [c_vp, c_u32, c_vp],
     "orf_diff_adjoint": [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp],
 }
 for _name, _args in _SIGS.items():
@@ -410,6 +413,38 @@ class Context:
         call("orf_diff_adjoint", self.handle, ctypes.byref(desc), arr, len(views), ptr(albedo), ptr(roughness),
              ptr(d_images), ptr(g_albedo), ptr(g_roughness), frame_shift, stream_ptr(stream))
         return g_albedo, g_roughness
+
+    SSS_FIELDS = ("absorption", "scattering", "meancosine", "extinction", "reducedExtinction",
+                  "deltaEddExtinction", "D", "transport", "C_phi", "C_phi_inv", "C_E", "A", "albedo",
+                  "reducedAlbedo", "de", "three_D", "two_A_de", "global_coeff", "one_over_three_ext")
+
+    def scattering_properties(self, material):
+        """Derived coefficients of a translucent material as a dict of floats / 3-tuples."""
+        import numpy as np
+
+        out = np.zeros(64, np.float32)
+        call("orf_scattering_properties", self.handle, material, _np_ptr(out), 64)
+        d = {"relative_ior": float(out[0]), "dipole_model": int(out[1])}
+        for i, k in enumerate(self.SSS_FIELDS):
+            d[k] = tuple(float(x) for x in out[2 + 3 * i: 5 + 3 * i])
+        d["mean_transport"] = float(out[2 + 3 * len(self.SSS_FIELDS)])
+        return d
+
+    def probes(self):
+        """Probe set: dict with count, area, cell, rmin and (n, 4) arrays pos, nrm, L, w12."""
+        import numpy as np
+
+        n, area = c_i64(0), ctypes.c_double()
+        cell, rmin = ctypes.c_float(), ctypes.c_float()
+        call("orf_probes_read", self.handle, ctypes.byref(n), ctypes.byref(area), ctypes.byref(cell),
+             ctypes.byref(rmin), None, None, None, None)
+        arrs = [np.zeros((n.value, 4), np.float32) for _ in range(4)]
+        call("orf_probes_read", self.handle, None, None, None, None, *[_np_ptr(a) for a in arrs])
+        return {"count": n.value, "area": area.value, "cell": cell.value, "rmin": rmin.value,
+                "pos": arrs[0], "nrm": arrs[1], "L": arrs[2], "w12": arrs[3]}
+
+    def probe_pass(self, frame, stream=None):
+        call("orf_probe_pass", self.handle, frame, stream_ptr(stream))
 
     def autotune(self, items, remap=None, spp_divisor=16):
         """(best WavefrontConfig, report dict) of the wavefront coordinate search."""

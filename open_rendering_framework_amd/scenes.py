@@ -68,6 +68,7 @@ class SceneDesc:
     height: int = 512
     max_depth: int = 8
     scene_epsilon: float = 1e-4
+    ctx_uints: dict = field(default_factory=dict)  # context variables, e.g. probe_count
 
     def flat(self):
         """World-space triangle soup: verts, tris, normals, nindex, tri_mat."""
@@ -218,6 +219,41 @@ def batch_workload(n=64, seed=1234, level=7, env_size=(1024, 512), max_depth=8, 
     return desc, items, np.array(remap, np.int32)
 
 
+def subsurface_scene(width=512, height=512, level=8, glass_level=6, env_size=(1024, 512), max_depth=8,
+                     probes=1 << 20, probe_samples=4, dipole_model=0):
+    """The subsurface + multiple-scattering scene: the path-tracer scene family with
+    a displaced icosphere (radius 0.9, at x = -1) of whole milk (dipole BSSRDF,
+    sigma_a = (0.0011, 0.0024, 0.014)/mm, sigma_s' = (2.55, 3.21, 3.77)/mm, eta 1.3,
+    scale 100: one scene unit is 100 mm) next to a rough glass sphere (radius 0.7,
+    at x = +1.1; GGX alpha 0.6, ior 1.5, microsurface multiple-scattering walk),
+    ground, area light and sky.  `probes` Poisson-disk surface probes on the milk."""
+    mats = [
+        Material("diffuse_shader.cu", floats={"diffuse_color": (0.5, 0.5, 0.5)}),
+        Material("subsurface_scattering_shader.cu",
+                 floats={"absorption": (0.0011, 0.0024, 0.014), "scattering": (2.55, 3.21, 3.77),
+                         "meancosine": (0.0, 0.0, 0.0), "scale": (100.0,), "ior": (1.3,)},
+                 uints={"dipole_model": (dipole_model,)}),
+        Material("arealight_shader.cu", floats={"radiance": (10.0, 10.0, 10.0)}),
+        Material("rough_transparent_shader.cu",
+                 floats={"roughness": (0.6,), "ior": (1.5,), "glass_absorption": (0.0, 0.0, 0.0)},
+                 uints={"microfacet_model": (2,), "normal_distribution": (1,)}),
+    ]
+    gv, gt = quad(-1.0, 10.0, facing_down=False)
+    ground = Mesh(gv, gt, 0, normals=np.tile([[0, 1, 0]], (4, 1)).astype(np.float32))
+    sv, sn, st = icosphere(level)
+    milk = Mesh((sv * 0.9 + np.array([-1.0, -0.1, 0.0], np.float32)).astype(np.float32), st, 1, normals=sn)
+    gv2, gn2, gt2 = icosphere(glass_level)
+    glass = Mesh((gv2 * 0.7 + np.array([1.1, -0.3, 0.0], np.float32)).astype(np.float32), gt2, 3, normals=gn2)
+    lv, lt = quad(3.0, 0.5, facing_down=True)
+    ln = np.tile([[0, -1, 0]], (4, 1)).astype(np.float32)
+    light_mesh = Mesh(lv, lt, 2, normals=ln)
+    lights = [Light("triangles", (10.0, 10.0, 10.0), tri_verts=lv[lt].reshape(-1, 9), tri_normals=ln[lt].reshape(-1, 9))]
+    eye, U, V, W = pinhole((0.0, 1.5, 4.5), (0.0, -0.2, 0.0), (0.0, 1.0, 0.0), 45.0, width, height)
+    envmap = sky_envmap(*env_size)
+    return SceneDesc([ground, milk, glass, light_mesh], mats, lights, envmap, True, eye, U, V, W, width, height,
+                     max_depth, 1e-4, ctx_uints={"probe_count": (probes,), "probe_samples": (probe_samples,)})
+
+
 def ring_views(n=8, size=256, spp=16, radius=4.0, height=1.5, frame0=0, vfov=40.0):
     """`n` pinhole views on a ring of `radius` at `height`, looking at the origin;
     view i starts at frame frame0 + i * spp."""
@@ -256,6 +292,8 @@ def to_context(desc, ctx=None):
 
     ctx = ctx or rt.Context()
     ctx.set_int("max_depth", desc.max_depth)
+    for k, v in desc.ctx_uints.items():
+        ctx.set_uint(k, *v)
     ctx.set_float("scene_epsilon", desc.scene_epsilon)
     for name in ("eye", "U", "V", "W"):
         ctx.set_float(name, *getattr(desc, name))

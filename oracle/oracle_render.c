@@ -33,6 +33,15 @@ typedef struct {
     int32_t first, count;
 } bnode;
 
+/* ScatteringMaterialProperties (ORF/structs.h:116-144) in R. */
+typedef struct {
+    R relative_ior;
+    int dipole_model;
+    v3 absorption, scattering, meancosine, extinction, reducedExtinction, deltaEddExtinction, D, transport;
+    v3 C_phi, C_phi_inv, C_E, A, albedo, reducedAlbedo, de, three_D, two_A_de, global_coeff, one_over_three_ext;
+    R mean_transport;
+} osss;
+
 typedef struct {
     /* geometry (world space) */
     int64_t ntris;
@@ -72,6 +81,15 @@ typedef struct {
     R* alb_frozen;
     R* rough_eval;
     R* rough_frozen;
+    /* translucent materials: coefficients per material, probes */
+    osss* sss;
+    int32_t nprobes, probe_samples, probe_mat;
+    R probe_cell;
+    v3* probe_pos;
+    v3* probe_nrm;
+    v3* probe_L;
+    v3* probe_w12;
+    R* probe_T12;
 } oscene;
 
 typedef struct {
@@ -787,6 +805,253 @@ static v3 shade_interface(const oscene* s, const orc_material* m, v3 o, v3 d, co
     return result;
 }
 
+/* ---------------------------------------------------------------- translucent (new estimator)
+ * Coefficients: ORF/RoughTranslucentMaterial.cpp:366-395 (outside ior 1); Fresnel
+ * moments ORF/Fresnel.h:93-124; dipoles ORF/dipoles/standard_dipole.h:8-29 and
+ * ORF/dipoles/directional_dipole.h:8-72; shader structure
+ * CU/subsurface_scattering_shader.cu:55-198; probe sample CU/sample_camera.cu:109-212
+ * (TEA streams instead of RND_64).  The gather replaces the reference's sum over
+ * all probes by exit-point importance sampling (probe rays along -n through
+ * exponentially distributed tangent-plane points, nearest probe within `cell`),
+ * the estimator the product uses. */
+
+static v3 vdiv(v3 a, v3 b) { return V3(a.x / b.x, a.y / b.y, a.z / b.z); }
+static v3 vsqrt3(v3 a) { return V3(sqrt(a.x), sqrt(a.y), sqrt(a.z)); }
+static v3 vexp3(v3 a) { return V3(F_EXP(a.x), F_EXP(a.y), F_EXP(a.z)); }
+static v3 vaddf(v3 a, R f) { return V3(a.x + f, a.y + f, a.z + f); }
+static v3 vfsub(R f, v3 a) { return V3(f - a.x, f - a.y, f - a.z); }
+
+static R three_C2(R n)
+{
+    R r;
+    if (n >= (R)1.0f) {
+        r = (R)-1641.1f + n * ((R)1213.67f + n * ((R)-568.556f + n * ((R)164.798f + n * ((R)-27.0181f + n * (R)1.91826f))));
+        r += ((((R)135.926f / n) - (R)656.175f) / n + (R)1376.53f) / n;
+    } else {
+        r = (R)0.828421f + n * ((R)-2.62051f + n * ((R)3.36231f + n * ((R)-1.95284f + n * ((R)0.236494f + n * (R)0.145787f))));
+    }
+    return r;
+}
+static R C_phi(R ni) { return (R)0.25f * ((R)1.0f - two_C1(ni)); }
+static R C_E(R ni) { return (R)0.5f * ((R)1.0f - three_C2(ni)); }
+
+static osss sss_coefficients(const orc_material* m)
+{
+    osss p;
+    memset(&p, 0, sizeof(p));
+    R scale = (R)m->sss_scale;
+    p.dipole_model = m->dipole_model;
+    p.absorption = vscale(V3(m->sss_absorption[0], m->sss_absorption[1], m->sss_absorption[2]), scale);
+    p.scattering = vscale(V3(m->sss_scattering[0], m->sss_scattering[1], m->sss_scattering[2]), scale);
+    p.meancosine = V3(m->sss_meancosine[0], m->sss_meancosine[1], m->sss_meancosine[2]);
+    p.relative_ior = (R)m->ior;
+    R ior = (R)m->ior;
+    v3 reducedScattering = vmul(p.scattering, vfsub((R)1.0f, p.meancosine));
+    p.relative_ior /= ((R)1.0f + (R)1.0f + (R)1.0f) / (R)3.0f;
+    p.extinction = vadd(p.scattering, p.absorption);
+    p.reducedExtinction = vadd(reducedScattering, p.absorption);
+    p.deltaEddExtinction = vadd(vmul(p.scattering, vfsub((R)1.0f, vmul(p.meancosine, p.meancosine))), p.absorption);
+    p.D = vdiv(V3(1, 1, 1), vscale(p.reducedExtinction, (R)3.0f));
+    p.transport = vsqrt3(vdiv(p.absorption, p.D));
+    p.C_phi = V3(C_phi(ior), C_phi(ior), C_phi(ior));
+    p.C_phi_inv = V3(C_phi((R)1.0f / ior), C_phi((R)1.0f / ior), C_phi((R)1.0f / ior));
+    p.C_E = V3(C_E(ior), C_E(ior), C_E(ior));
+    p.albedo = vdiv(p.scattering, p.extinction);
+    p.reducedAlbedo = vdiv(reducedScattering, p.reducedExtinction);
+    p.de = vdiv(vscale(p.D, (R)2.131f), vsqrt3(p.reducedAlbedo));
+    p.A = vdiv(vfsub((R)1.0f, p.C_E), vscale(p.C_phi, (R)2.0f));
+    p.three_D = vscale(p.D, (R)3.0f);
+    p.two_A_de = vmul(vscale(p.A, (R)2.0f), p.de);
+    v3 g = vscale(vdiv(V3(1, 1, 1), vscale(p.C_phi_inv, (R)4.0f)), (R)1.0f);
+    R q = (R)4.0f * PI_R * PI_R;
+    p.global_coeff = V3(g.x / q, g.y / q, g.z / q);
+    p.one_over_three_ext = vdiv(V3(1, 1, 1), vscale(p.extinction, (R)3.0f));
+    p.mean_transport = (p.transport.x + p.transport.y + p.transport.z) / (R)3.0f;
+    return p;
+}
+
+static v3 dipole_bssrdf(R dist, const osss* p)
+{
+    v3 real_source = vmul(p->three_D, p->three_D);
+    v3 extrapolation = vmul(vscale(p->A, (R)4.0f), p->D);
+    v3 virtual_source = vmul(extrapolation, extrapolation);
+    v3 corrected_mean_free = vadd(p->three_D, extrapolation);
+    R r2 = dist * dist;
+    v3 r_sqr = V3(r2, r2, r2);
+    v3 d_r_sqr = vadd(r_sqr, real_source);
+    v3 d_r = vsqrt3(d_r_sqr);
+    v3 d_v_sqr = vadd(r_sqr, virtual_source);
+    v3 d_v = vsqrt3(d_v_sqr);
+    v3 tr_r = vmul(p->transport, d_r);
+    v3 S_r = vdiv(vmul(p->three_D, vaddf(tr_r, (R)1.0f)), vmul(d_r_sqr, d_r));
+    S_r = vmul(S_r, vexp3(vneg(tr_r)));
+    v3 tr_v = vmul(p->transport, d_v);
+    v3 S_v = vdiv(vmul(corrected_mean_free, vaddf(tr_v, (R)1.0f)), vmul(d_v_sqr, d_v));
+    S_v = vmul(S_v, vexp3(vneg(tr_v)));
+    return vadd(S_r, S_v);
+}
+
+static v3 S_infinite(v3 r_sqr, v3 x_dot_w12, R no_dot_w12, v3 x_dot_no, const osss* p)
+{
+    v3 r = vsqrt3(r_sqr);
+    v3 r_tr = vmul(p->transport, r);
+    v3 r_tr_p1 = vaddf(r_tr, (R)1.0f);
+    v3 T = vexp3(vneg(r_tr));
+    v3 coeff = vdiv(T, vmul(r, r_sqr));
+    v3 first = vscale(vmul(p->C_phi, vadd(vmul(r_sqr, p->reducedExtinction), vmul(r_tr_p1, x_dot_w12))), (R)3.0f);
+    v3 inner = vadd(r_tr_p1, vmul(vdiv(vmul(p->three_D, vadd(vscale(r_tr_p1, (R)3.0f), vmul(r_tr, r_tr))), r_sqr), x_dot_w12));
+    v3 second = vmul(p->C_E, vsub(vscale(vmul(p->three_D, r_tr_p1), no_dot_w12), vmul(inner, x_dot_no)));
+    return vmul(coeff, vsub(first, second));
+}
+
+static v3 dirpole_bssrdf(v3 xi, v3 ni, v3 w12, v3 xo, v3 no, const osss* p)
+{
+    v3 x = vsub(xo, xi);
+    R r_sqr = vdot(x, x);
+    v3 dr_sqr = V3(r_sqr, r_sqr, r_sqr);
+    R dot_x_w12 = vdot(x, w12);
+    R num = r_sqr - dot_x_w12 * dot_x_w12;
+    v3 cos_beta = vneg(vsqrt3(vdiv(V3(num, num, num), vaddf(vmul(p->de, p->de), r_sqr))));
+    R mu0 = -vdot(no, w12);
+    R edge = mu0 > 0 ? (R)1.0f : (R)0.0f;
+    v3 D_prime = vadd(vscale(vscale(p->D, mu0), edge), vscale(p->one_over_three_ext, (R)1.0f - edge));
+    dr_sqr = vadd(dr_sqr, vmul(D_prime, vsub(D_prime, vscale(vmul(vscale(p->de, (R)2.0f), cos_beta), edge))));
+    v3 t = orc_normalize(vcross(ni, x));
+    v3 nistar = r_sqr < (R)1.0e-12f ? ni : vcross(orc_normalize(x), t);
+    v3 wv = vsub(w12, vscale(nistar, (R)2.0f * vdot(w12, nistar)));
+    v3 xr = vsub(x, vscale(nistar, p->two_A_de.x)), xg = vsub(x, vscale(nistar, p->two_A_de.y)),
+       xb = vsub(x, vscale(nistar, p->two_A_de.z));
+    v3 dv_sqr = V3(vdot(xr, xr), vdot(xg, xg), vdot(xb, xb));
+    v3 x_dot_wv = V3(vdot(xr, wv), vdot(xg, wv), vdot(xb, wv));
+    v3 x_dot_no = V3(vdot(xr, no), vdot(xg, no), vdot(xb, no));
+    R xn = vdot(x, no);
+    v3 Sr = S_infinite(dr_sqr, V3(dot_x_w12, dot_x_w12, dot_x_w12), -mu0, V3(xn, xn, xn), p);
+    v3 Sv = S_infinite(dv_sqr, x_dot_wv, vdot(no, wv), x_dot_no, p);
+    v3 Sd = vsub(Sr, Sv);
+    return V3(fmax(Sd.x, (R)0), fmax(Sd.y, (R)0), fmax(Sd.z, (R)0));
+}
+
+/* Nearest probe strictly within `cell` of y (ties: lower index), by brute force. */
+static int nearest_probe(const oscene* s, v3 y)
+{
+    R best = s->probe_cell * s->probe_cell;
+    int bi = -1;
+    for (int j = 0; j < s->nprobes; j++) {
+        v3 e = vsub(s->probe_pos[j], y);
+        R d2 = vdot(e, e);
+        if (d2 < best || (d2 == best && bi >= 0 && j < bi)) {
+            best = d2;
+            bi = j;
+        }
+    }
+    return bi;
+}
+
+#define SSS_HITS 4
+#define SSS_RADIUS_SIGMAS ((R)12.0f)
+
+static v3 sss_gather(const oscene* s, const osss* P, v3 xo, v3 no, uint32_t* t)
+{
+    v3 acc = V3(0, 0, 0);
+    if (s->nprobes <= 0 || s->probe_samples <= 0) return acc;
+    v3 t1, t2;
+    orc_create_onb(no, &t1, &t2);
+    R smin = fmin(P->transport.x, fmin(P->transport.y, P->transport.z));
+    R r_max = SSS_RADIUS_SIGMAS / smin;
+    for (int k = 0; k < s->probe_samples; k++) {
+        R u = orc_rnd_tea(t), v = orc_rnd_tea(t), w = orc_rnd_tea(t), sx = orc_rnd_tea(t);
+        int c = (int)((R)3.0f * u);
+        if (c > 2) c = 2;
+        R sig = c == 0 ? P->transport.x : (c == 1 ? P->transport.y : P->transport.z);
+        R r = -F_LOG((R)1.0f - v) / sig;
+        if (!(r < r_max)) continue;
+        R ang = (R)6.28318530717958647692f * w;
+        R sp = F_SIN(ang), cp = F_COS(ang);
+        R hh = sqrt(r_max * r_max - r * r);
+        v3 top = vadd(vadd(xo, vscale(vadd(vscale(t1, cp), vscale(t2, sp)), r)), vscale(no, hh));
+        v3 dir = vneg(no);
+        /* crossings of the probes' material along the probe ray */
+        ohit hits[SSS_HITS];
+        int nh = 0;
+        R tmin = 0;
+        for (int it = 0; it < 2 * SSS_HITS && nh < SSS_HITS; it++) {
+            ohit h;
+            if (!trace(s, top, dir, tmin, (R)2.0f * hh, 0, &h)) break;
+            int mi = s->tri_mat[h.prim];
+            if (s->mats[mi].kind == ORC_TRANSLUCENT && mi == s->probe_mat) hits[nh++] = h;
+            tmin = h.t;
+        }
+        if (nh == 0) continue;
+        int pick = (int)(sx * (R)nh);
+        if (pick > nh - 1) pick = nh - 1;
+        const ohit* hp = &hits[pick];
+        v3 xi = vadd(top, vscale(dir, hp->t));
+        v3 ni = hit_normal(s, hp);
+        R cosn = fabs(vdot(ni, no));
+        v3 dx = vsub(xi, xo);
+        v3 inpl = vsub(dx, vscale(no, vdot(dx, no)));
+        R rp = sqrt(vdot(inpl, inpl));
+        R pr_r = (P->transport.x * F_EXP(-P->transport.x * rp) + P->transport.y * F_EXP(-P->transport.y * rp) +
+                  P->transport.z * F_EXP(-P->transport.z * rp)) /
+                 ((R)3.0f * (R)6.28318530717958647692f * rp);
+        R pdf = pr_r * cosn / (R)nh;
+        if (!(pdf > 0) || !isfinite(pdf)) continue;
+        int j = nearest_probe(s, xi);
+        if (j < 0) continue;
+        v3 E = vscale(s->probe_L[j], s->probe_T12[j]);
+        v3 S = P->dipole_model == 1 ? dirpole_bssrdf(xi, s->probe_nrm[j], s->probe_w12[j], xo, no, P)
+                                    : dipole_bssrdf(sqrt(vdot(dx, dx)), P);
+        v3 es = vmul(E, S);
+        acc = vadd(acc, V3(es.x / pdf, es.y / pdf, es.z / pdf));
+    }
+    return vmul(vscale(acc, (R)1.0f / (R)s->probe_samples), P->global_coeff);
+}
+
+static v3 shade_hit(const oscene* s, v3 o, v3 d, const ohit* h, int depth, int emit_light, uint32_t* seed,
+                    R prev_pdf, ostats* st);
+
+/* CU/subsurface_scattering_shader.cu:55-198 with the gather before the refracted path. */
+static v3 shade_translucent(const oscene* s, int mi, v3 o, v3 d, const ohit* h, int depth, uint32_t* t, ostats* st)
+{
+    v3 zero = V3(0, 0, 0);
+    if (depth > s->max_depth) return zero;
+    const osss* P = &s->sss[mi];
+    v3 xo = vadd(o, vscale(d, h->t));
+    v3 no = hit_normal(s, h);
+    v3 wo = vneg(d);
+    R recip = (R)1.0f / P->relative_ior;
+    R reflect_xi = orc_rnd_tea(t);
+    R co = vdot(wo, no);
+    int inside = co < 0;
+    v3 beam = V3(1, 1, 1);
+    if (inside) {
+        v3 ext = P->dipole_model == 1 ? P->deltaEddExtinction : P->extinction;
+        beam = V3(F_EXP((-h->t) * ext.x), F_EXP((-h->t) * ext.y), F_EXP((-h->t) * ext.z));
+        R prob = (beam.x + beam.y + beam.z) / (R)3.0f;
+        if (orc_rnd_tea(t) >= prob) return zero;
+        beam = V3(beam.x / prob, beam.y / prob, beam.z / prob);
+        recip = P->relative_ior;
+        no = vneg(no);
+        co = -co;
+    }
+    R st2 = recip * recip * ((R)1.0f - co * co);
+    R ct = 1, Rf = 1;
+    if (st2 < (R)1.0f) {
+        ct = sqrt((R)1.0f - st2);
+        Rf = orc_fresnel_R3(co, ct, recip);
+    }
+    v3 result = zero;
+    if (reflect_xi >= Rf) {
+        if (!inside && mi == s->probe_mat) result = sss_gather(s, P, xo, no, t);
+        v3 wt = vsub(vscale(vsub(vscale(no, co), wo), recip), vscale(no, ct));
+        result = vadd(result, radiance(s, xo, wt, depth + 1, 1, t, (R)-1, st));
+    } else {
+        v3 wr = vsub(vscale(no, (R)2.0f * co), wo);
+        result = radiance(s, xo, wr, depth + 1, 1, t, (R)-1, st);
+    }
+    return vmul(result, beam);
+}
+
 static v3 radiance(const oscene* s, v3 o, v3 d, int depth, int emit_light, uint32_t* seed, R prev_pdf, ostats* st)
 {
     ohit h;
@@ -797,6 +1062,14 @@ static v3 radiance(const oscene* s, v3 o, v3 d, int depth, int emit_light, uint3
         if (prev_pdf > 0 && s->env_is) E = vscale(E, mis_power(prev_pdf, env_pdf(s, d) / (R)s->nlights));
         return E;
     }
+    return shade_hit(s, o, d, &h, depth, emit_light, seed, prev_pdf, st);
+}
+
+static v3 shade_hit(const oscene* s, v3 o, v3 d, const ohit* hp, int depth, int emit_light, uint32_t* seed,
+                    R prev_pdf, ostats* st)
+{
+    ohit h = *hp;
+    (void)prev_pdf;
     const orc_material* m = &s->mats[s->tri_mat[h.prim]];
     if (m->kind == ORC_EMITTER) {
         /* CU/arealight_shader.cu:36-50 */
@@ -808,6 +1081,7 @@ static v3 radiance(const oscene* s, v3 o, v3 d, int depth, int emit_light, uint3
     if (m->kind == ORC_CONDUCTOR) return shade_conductor(s, m, &s->eta[s->tri_mat[h.prim]], o, d, &h, depth, seed, st);
     if (m->kind == ORC_DIELECTRIC) return shade_dielectric(s, m, o, d, &h, depth, seed, st);
     if (m->kind == ORC_INTERFACE) return shade_interface(s, m, o, d, &h, depth, emit_light, seed, st);
+    if (m->kind == ORC_TRANSLUCENT) return shade_translucent(s, s->tri_mat[h.prim], o, d, &h, depth, seed, st);
     return V3(0, 0, 0);
 }
 
@@ -869,6 +1143,9 @@ void* orc_scene_create(const orc_scene_desc* d)
         s->eta[i].y = orc_cdiv_fc((R)1.0f, c1);
         s->eta[i].z = orc_cdiv_fc((R)1.0f, c2);
     }
+    s->sss = (osss*)calloc((size_t)(d->nmats + 1), sizeof(osss));
+    for (int i = 0; i < d->nmats; i++) s->sss[i] = sss_coefficients(&d->mats[i]);
+    s->probe_mat = -1;
     s->has_env = d->env != NULL && d->env_w > 0 && d->env_h > 0;
     s->env_is = s->has_env && d->env_is;
     int extra = s->env_is ? 1 : 0;
@@ -920,6 +1197,12 @@ void orc_scene_destroy(void* p)
     free(s->alb_frozen);
     free(s->rough_eval);
     free(s->rough_frozen);
+    free(s->sss);
+    free(s->probe_pos);
+    free(s->probe_nrm);
+    free(s->probe_L);
+    free(s->probe_w12);
+    free(s->probe_T12);
     free(s);
 }
 
@@ -1099,4 +1382,181 @@ double orc_diff_eval(void* p, int W, int H, int nviews, const float* cams, const
         loss += part;
     }
     return loss;
+}
+
+/* ---------------------------------------------------------------- translucent API */
+
+void orc_probes_set(void* p, int n, const float* pos, const float* nrm, float cell, int samples, int material)
+{
+    oscene* s = (oscene*)p;
+    free(s->probe_pos);
+    free(s->probe_nrm);
+    free(s->probe_L);
+    free(s->probe_w12);
+    free(s->probe_T12);
+    s->nprobes = n;
+    s->probe_cell = (R)cell;
+    s->probe_samples = samples;
+    s->probe_mat = material;
+    s->probe_pos = (v3*)calloc((size_t)n + 1, sizeof(v3));
+    s->probe_nrm = (v3*)calloc((size_t)n + 1, sizeof(v3));
+    s->probe_L = (v3*)calloc((size_t)n + 1, sizeof(v3));
+    s->probe_w12 = (v3*)calloc((size_t)n + 1, sizeof(v3));
+    s->probe_T12 = (R*)calloc((size_t)n + 1, sizeof(R));
+    for (int j = 0; j < n; j++) {
+        s->probe_pos[j] = V3(pos[4 * j], pos[4 * j + 1], pos[4 * j + 2]);
+        s->probe_nrm[j] = V3(nrm[4 * j], nrm[4 * j + 1], nrm[4 * j + 2]);
+    }
+}
+
+void orc_probes_values(void* p, const float* L, const float* w12)
+{
+    oscene* s = (oscene*)p;
+    for (int j = 0; j < s->nprobes; j++) {
+        s->probe_L[j] = V3(L[4 * j], L[4 * j + 1], L[4 * j + 2]);
+        s->probe_w12[j] = V3(w12[4 * j], w12[4 * j + 1], w12[4 * j + 2]);
+        s->probe_T12[j] = (R)w12[4 * j + 3];
+    }
+}
+
+/* Light that has bounced at least once before reaching `o` along `d`: the first
+ * segment's emitter and (when the environment is a sampled light) environment
+ * belong to the light-source strategy. */
+static v3 probe_indirect(const oscene* s, v3 o, v3 d, uint32_t* t, ostats* st)
+{
+    ohit h;
+    st->ext++;
+    if (!trace(s, o, d, s->eps, RAY_MAX, 0, &h)) {
+        if (!s->has_env || s->env_is) return V3(0, 0, 0);
+        return env_lookup(s, d);
+    }
+    return shade_hit(s, o, d, &h, 0, 0, t, (R)-1, st);
+}
+
+/* CU/sample_camera.cu:109-212 for one probe (TEA stream tea16(j, frame)). */
+static void probe_sample(const oscene* s, const osss* P, int j, uint32_t frame, v3* L_out, v3* w12_out, R* T12_out)
+{
+    v3 pos = s->probe_pos[j], n = s->probe_nrm[j];
+    uint32_t t = orc_tea16((uint32_t)j, frame);
+    R recip = (R)1.0f / P->relative_ior;
+    v3 Le = V3(0, 0, 0), wi = n;
+    ostats st = {0, 0};
+    if (orc_rnd_tea(&t) < (R)0.5f) {
+        uint32_t li = (uint32_t)((R)s->nlights * orc_rnd_tea(&t));
+        if (s->nlights > 0) {
+            if (li >= (uint32_t)s->nlights) li = (uint32_t)s->nlights - 1;
+            R light_pdf = (R)1 / (R)s->nlights;
+            v3 wl, rad;
+            R dist, penv;
+            direct_light(s, &s->lights[li], pos, &t, &wl, &rad, &dist, &penv);
+            wi = wl;
+            R cs = fmax(vdot(wl, n), (R)0);
+            if (cs > 0) {
+                ohit sh;
+                if (!trace(s, pos, wl, s->eps, dist, 1, &sh)) {
+                    v3 a = vscale(rad, cs);
+                    Le = V3(a.x / light_pdf, a.y / light_pdf, a.z / light_pdf);
+                }
+            }
+        }
+    } else {
+        wi = orc_sample_cosine_weighted(n, &t);
+        Le = vscale(probe_indirect(s, pos, wi, &t, &st), PI_R);
+    }
+    Le = V3(Le.x / (R)0.5f, Le.y / (R)0.5f, Le.z / (R)0.5f);
+    R ci = fmax(vdot(wi, n), (R)0);
+    R st2 = recip * recip * ((R)1.0f - ci * ci);
+    R ct = sqrt((R)1.0f - st2);
+    *w12_out = vsub(vscale(vsub(vscale(n, ci), wi), recip), vscale(n, ct));
+    *T12_out = (R)1.0f - orc_fresnel_R3(ci, ct, recip);
+    *L_out = Le;
+}
+
+void orc_probe_pass(void* p, uint32_t frame, float* L, float* w12, int threads)
+{
+    oscene* s = (oscene*)p;
+    if (s->nprobes <= 0 || s->probe_mat < 0) return;
+    oscene off = *s;
+    off.probe_mat = -1; /* no gathers while the probes are being written */
+    const osss* P = &s->sss[s->probe_mat];
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
+    for (int j = 0; j < s->nprobes; j++) {
+        v3 Lj, wj;
+        R Tj;
+        probe_sample(&off, P, j, frame, &Lj, &wj, &Tj);
+        s->probe_L[j] = Lj;
+        s->probe_w12[j] = wj;
+        s->probe_T12[j] = Tj;
+        if (L) {
+            L[4 * j] = (float)Lj.x;
+            L[4 * j + 1] = (float)Lj.y;
+            L[4 * j + 2] = (float)Lj.z;
+            L[4 * j + 3] = 0.0f;
+        }
+        if (w12) {
+            w12[4 * j] = (float)wj.x;
+            w12[4 * j + 1] = (float)wj.y;
+            w12[4 * j + 2] = (float)wj.z;
+            w12[4 * j + 3] = (float)Tj;
+        }
+    }
+}
+
+void orc_sss_coefficients(void* p, int m, float* out)
+{
+    const oscene* s = (const oscene*)p;
+    const osss* q = &s->sss[m];
+    const v3* v[] = {&q->absorption, &q->scattering, &q->meancosine, &q->extinction, &q->reducedExtinction,
+                     &q->deltaEddExtinction, &q->D, &q->transport, &q->C_phi, &q->C_phi_inv, &q->C_E, &q->A,
+                     &q->albedo, &q->reducedAlbedo, &q->de, &q->three_D, &q->two_A_de, &q->global_coeff,
+                     &q->one_over_three_ext};
+    int k = 0;
+    out[k++] = (float)q->relative_ior;
+    out[k++] = (float)q->dipole_model;
+    for (int i = 0; i < 19; i++) {
+        out[k++] = (float)v[i]->x;
+        out[k++] = (float)v[i]->y;
+        out[k++] = (float)v[i]->z;
+    }
+    out[k++] = (float)q->mean_transport;
+    while (k < 64) out[k++] = 0.0f;
+}
+
+void orc_dipole_eval(void* p, int m, int n, const float* r, float* out)
+{
+    const oscene* s = (const oscene*)p;
+    for (int i = 0; i < n; i++) {
+        v3 v = dipole_bssrdf((R)r[i], &s->sss[m]);
+        out[3 * i] = (float)v.x;
+        out[3 * i + 1] = (float)v.y;
+        out[3 * i + 2] = (float)v.z;
+    }
+}
+
+void orc_dirpole_eval(void* p, int m, int n, const float* rows, float* out)
+{
+    const oscene* s = (const oscene*)p;
+    for (int i = 0; i < n; i++) {
+        const float* q = rows + 15 * i;
+        v3 v = dirpole_bssrdf(V3(q[0], q[1], q[2]), V3(q[3], q[4], q[5]), V3(q[6], q[7], q[8]), V3(q[9], q[10], q[11]),
+                              V3(q[12], q[13], q[14]), &s->sss[m]);
+        out[3 * i] = (float)v.x;
+        out[3 * i + 1] = (float)v.y;
+        out[3 * i + 2] = (float)v.z;
+    }
+}
+
+/* Mean of `reps` gathers at (xo, no) on material m, gather seeds tea16(i, seed). */
+void orc_sss_gather_eval(void* p, int m, const float* xo, const float* no, int reps, uint32_t seed, double* out)
+{
+    const oscene* s = (const oscene*)p;
+    double acc[3] = {0, 0, 0};
+    for (int i = 0; i < reps; i++) {
+        uint32_t t = orc_tea16((uint32_t)i, seed);
+        v3 g = sss_gather(s, &s->sss[m], V3(xo[0], xo[1], xo[2]), V3(no[0], no[1], no[2]), &t);
+        acc[0] += (double)g.x;
+        acc[1] += (double)g.y;
+        acc[2] += (double)g.z;
+    }
+    for (int k = 0; k < 3; k++) out[k] = acc[k] / reps;
 }

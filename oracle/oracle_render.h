@@ -13,7 +13,7 @@
 
 #include "orf_oracle.h"
 
-enum { ORC_LAMBERT = 0, ORC_CONDUCTOR = 1, ORC_EMITTER = 2, ORC_DIELECTRIC = 3, ORC_INTERFACE = 4 };
+enum { ORC_LAMBERT = 0, ORC_CONDUCTOR = 1, ORC_EMITTER = 2, ORC_DIELECTRIC = 3, ORC_INTERFACE = 4, ORC_TRANSLUCENT = 5 };
 enum { ORC_POINT = 0, ORC_DIRECTIONAL = 1, ORC_TRIANGLES = 2, ORC_ENV = 5 };
 
 typedef struct {
@@ -27,6 +27,11 @@ typedef struct {
     float roughness[2];
     float ior;
     double ior_re[3], ior_im[3];
+    /* translucent (ScatteringMaterial parameters, 1/mm, and scale) */
+    float sss_absorption[3], sss_scattering[3], sss_meancosine[3];
+    float sss_scale;
+    int32_t dipole_model;
+    int32_t pad2;
 } orc_material;
 
 typedef struct {
@@ -79,6 +84,21 @@ int orc_env_bsearch(const double* cdf, int n, double xi);
 /* Closest hit of a batch of rays against the oracle BVH (t, prim) and by brute force. */
 void orc_closest_hits(void* s, int n, const float* o, const float* d, float tmin, float* t, int32_t* prim,
                       int brute_force);
+/* Translucent material (new estimator, see oracle_render.c): probes given by the
+ * caller (4 floats per probe), nearest-probe search radius `cell`, gather samples
+ * per entering hit; probe pass for sample index `frame` (outputs may be NULL);
+ * probe values may also be loaded directly. */
+void orc_probes_set(void* s, int n, const float* pos, const float* nrm, float cell, int samples, int material);
+void orc_probe_pass(void* s, uint32_t frame, float* L, float* w12, int threads);
+void orc_probes_values(void* s, const float* L, const float* w12);
+/* Derived coefficients of material m (64 floats, the product's field order). */
+void orc_sss_coefficients(void* s, int m, float* out);
+/* Standard / directional dipole of material m: standard at distances r[i];
+ * directional at (xi, ni, w12, xo, no) per row of 15 floats.  out: 3 per entry. */
+void orc_dipole_eval(void* s, int m, int n, const float* r, float* out);
+void orc_dirpole_eval(void* s, int m, int n, const float* rows, float* out);
+void orc_sss_gather_eval(void* s, int m, const float* xo, const float* no, int reps, uint32_t seed, double* out);
+
 /* Differentiable form (new): textured albedo / roughness with frozen sampling,
  * and per-view evaluation of images and of the linear loss sum(dimg * mean L). */
 void orc_diff_set(void* s, int tex_w, int tex_h, int alb_mat, int alb_uv, const double* alb_uvp, int rough_mat,

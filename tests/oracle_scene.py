@@ -13,7 +13,10 @@ class OrcMaterial(ctypes.Structure):
                 ("normal_distribution", ctypes.c_int32), ("pad", ctypes.c_int32),
                 ("diffuse", ctypes.c_float * 3), ("radiance", ctypes.c_float * 3),
                 ("absorption", ctypes.c_float * 3), ("roughness", ctypes.c_float * 2), ("ior", ctypes.c_float),
-                ("ior_re", ctypes.c_double * 3), ("ior_im", ctypes.c_double * 3)]
+                ("ior_re", ctypes.c_double * 3), ("ior_im", ctypes.c_double * 3),
+                ("sss_absorption", ctypes.c_float * 3), ("sss_scattering", ctypes.c_float * 3),
+                ("sss_meancosine", ctypes.c_float * 3), ("sss_scale", ctypes.c_float), ("dipole_model", ctypes.c_int32),
+                ("pad2", ctypes.c_int32)]
 
 
 class OrcLight(ctypes.Structure):
@@ -36,7 +39,10 @@ class OrcSceneDesc(ctypes.Structure):
 
 
 _KIND = {"diffuse_shader.cu": 0, "metallic_shader.cu": 1, "arealight_shader.cu": 2, "rough_transparent_shader.cu": 3,
-         "lambertian_interface_shader.cu": 4}
+         "lambertian_interface_shader.cu": 4, "subsurface_scattering_shader.cu": 5}
+# Wholemilk (ORF/RoughTranslucentMaterial.cpp:258-263) and the default scale
+SSS_DEFAULTS = {"absorption": (0.0011, 0.0024, 0.014), "scattering": (2.55, 3.21, 3.77),
+                "meancosine": (0.0, 0.0, 0.0), "scale": (100.0,)}
 
 
 def _p(a):
@@ -60,6 +66,13 @@ class OracleScene:
         L.orc_env_bsearch.restype = ctypes.c_int
         L.orc_env_bsearch.argtypes = [c_vp, ctypes.c_int, ctypes.c_double]
         L.orc_closest_hits.argtypes = [c_vp, ctypes.c_int, c_vp, c_vp, ctypes.c_float, c_vp, c_vp, ctypes.c_int]
+        L.orc_probes_set.argtypes = [c_vp, ctypes.c_int, c_vp, c_vp, ctypes.c_float, ctypes.c_int, ctypes.c_int]
+        L.orc_probe_pass.argtypes = [c_vp, ctypes.c_uint32, c_vp, c_vp, ctypes.c_int]
+        L.orc_probes_values.argtypes = [c_vp, c_vp, c_vp]
+        L.orc_sss_coefficients.argtypes = [c_vp, ctypes.c_int, c_vp]
+        L.orc_dipole_eval.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, c_vp, c_vp]
+        L.orc_dirpole_eval.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, c_vp, c_vp]
+        L.orc_sss_gather_eval.argtypes = [c_vp, ctypes.c_int, c_vp, c_vp, ctypes.c_int, ctypes.c_uint32, c_vp]
         L.orc_diff_set.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_vp, ctypes.c_int,
                                    ctypes.c_int, c_vp, c_vp, c_vp, c_vp, c_vp]
         L.orc_diff_eval.restype = ctypes.c_double
@@ -84,7 +97,12 @@ class OracleScene:
             r = m.floats.get("roughness", (0.1, 0.1))
             mm.roughness[0] = r[0]
             mm.roughness[1] = r[1] if len(r) > 1 else r[0]
-            mm.ior = m.floats.get("ior", (1.5,))[0]
+            mm.ior = m.floats.get("ior", (1.3,) if mm.kind == 5 else (1.5,))[0]
+            for key, field in (("absorption", "sss_absorption"), ("scattering", "sss_scattering"),
+                               ("meancosine", "sss_meancosine")):
+                getattr(mm, field)[:] = [float(x) for x in m.floats.get(key, SSS_DEFAULTS[key])]
+            mm.sss_scale = m.floats.get("scale", SSS_DEFAULTS["scale"])[0]
+            mm.dipole_model = m.uints.get("dipole_model", (0,))[0]
             ior = m.ior_complex or ((1.5, 0.0),) * 3
             for k in range(3):
                 mm.ior_re[k] = ior[k][0]
@@ -212,3 +230,47 @@ class OracleScene:
         loss = self.lib.orc_diff_eval(self.handle, W, H, len(views), _p(cams), _p(f0), spp, shift, _p(dd), _p(img),
                                       threads)
         return loss, img
+
+    # ------------------------------------------------ translucent material
+    def probes_set(self, probes, samples, material):
+        """Use the product's probe set (dict from rt.Context.probes()) with `samples`
+        gather samples per hit on material handle `material`."""
+        self._probe_keep = [np.ascontiguousarray(probes["pos"], np.float32),
+                            np.ascontiguousarray(probes["nrm"], np.float32)]
+        pos, nrm = self._probe_keep
+        self.lib.orc_probes_set(self.handle, len(pos), _p(pos), _p(nrm), probes["cell"], samples, material)
+
+    def probe_pass(self, frame, threads=8):
+        n = len(self._probe_keep[0])
+        L = np.zeros((n, 4), np.float32)
+        w = np.zeros((n, 4), np.float32)
+        self.lib.orc_probe_pass(self.handle, frame, _p(L), _p(w), threads)
+        return L, w
+
+    def probes_values(self, L, w12):
+        self._probe_vals = [np.ascontiguousarray(L, np.float32), np.ascontiguousarray(w12, np.float32)]
+        self.lib.orc_probes_values(self.handle, *[_p(a) for a in self._probe_vals])
+
+    def sss_coefficients(self, material):
+        out = np.zeros(64, np.float32)
+        self.lib.orc_sss_coefficients(self.handle, material, _p(out))
+        return out
+
+    def dipole(self, material, r):
+        r = np.ascontiguousarray(r, np.float32)
+        out = np.zeros((len(r), 3), np.float32)
+        self.lib.orc_dipole_eval(self.handle, material, len(r), _p(r), _p(out))
+        return out
+
+    def dirpole(self, material, rows):
+        rows = np.ascontiguousarray(rows, np.float32)
+        out = np.zeros((len(rows), 3), np.float32)
+        self.lib.orc_dirpole_eval(self.handle, material, len(rows), _p(rows), _p(out))
+        return out
+
+    def sss_gather(self, material, xo, no, reps, seed=1):
+        xo = np.ascontiguousarray(xo, np.float32)
+        no = np.ascontiguousarray(no, np.float32)
+        out = np.zeros(3)
+        self.lib.orc_sss_gather_eval(self.handle, material, _p(xo), _p(no), reps, seed, _p(out))
+        return out

@@ -1,0 +1,165 @@
+"""Pins the oracle's translucent-material restatement (coefficients of
+ORF/RoughTranslucentMaterial.cpp:366-395, Fresnel moments of ORF/Fresnel.h:93-124,
+the dipoles of ORF/dipoles/, and the importance-sampled gather):
+
+- derived coefficients against the reference's formulas evaluated in numpy (FP64);
+- the C_phi / C_E polynomial fits against quadrature of the Fresnel moments
+  2 C1 = 2 int R mu dmu and 3 C2 = 3 int R mu^2 dmu (fit error < 2e-3);
+- the standard dipole against the closed form of its total reflectance: a
+  source term z_n (1 + s d) exp(-s d) / d^3 with d^2 = r^2 + z_p^2 integrates
+  over the plane to 2 pi (z_n / z_p) exp(-s z_p).  ORF/dipoles/standard_dipole.h
+  puts the virtual source at depth z_p = 4AD with weight z_n = 3D + 4AD (Jensen
+  et al. 2001 use z_p = z_n = 3D + 4AD); the restatement keeps that, and with
+  z_p = z_n the same form gives Jensen's Rd = alpha'/2 (1 + exp(-4/3 A
+  sqrt(3(1-alpha')))) exp(-sqrt(3(1-alpha'))), checked on the real source;
+- the gather estimator on a flat translucent slab with constant probe light:
+  its mean equals global_coeff E T12 times that plane integral (exit-point pdf,
+  one probe-ray crossing); with the entry normal tilted 45 degrees from the
+  slab's the probe rays cross the slab obliquely, so a dropped |n_i . n_o|
+  factor moves the mean by sqrt(2), far beyond the Monte-Carlo error;
+- the directional dipole is non-negative and rotationally symmetric about the
+  normal at normal incidence (its values are otherwise parity unpinned).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from open_rendering_framework_amd import scenes
+from tests.oracle_scene import OracleScene
+
+MILK = dict(absorption=(0.0011, 0.0024, 0.014), scattering=(2.55, 3.21, 3.77), meancosine=(0.0, 0.0, 0.0),
+            scale=100.0, ior=1.3)
+
+
+def _slab_scene(dipole_model=0, half=5.0):
+    v = np.array([[-half, 0.0, -half], [half, 0.0, -half], [half, 0.0, half], [-half, 0.0, half]], np.float32)
+    t = np.array([[0, 2, 1], [0, 3, 2]], np.int32)
+    slab = scenes.Mesh(v, t, 0, normals=np.tile([[0, 1, 0]], (4, 1)).astype(np.float32))
+    mat = scenes.Material("subsurface_scattering_shader.cu",
+                          floats={"absorption": MILK["absorption"], "scattering": MILK["scattering"],
+                                  "meancosine": MILK["meancosine"], "scale": (MILK["scale"],), "ior": (MILK["ior"],)},
+                          uints={"dipole_model": (dipole_model,)})
+    return scenes.SceneDesc([slab], [mat], [], None, False, (0, 1, 0), (1, 0, 0), (0, 0, 1), (0, -1, 0), 8, 8, 8, 1e-4)
+
+
+def _fresnel_R(c, eta):
+    st2 = eta * eta * (1.0 - c * c)
+    ct = np.sqrt(np.maximum(0.0, 1.0 - st2))
+    rs = (eta * c - ct) / (eta * c + ct)
+    rp = (c - eta * ct) / (c + eta * ct)
+    return np.where(st2 >= 1.0, 1.0, 0.5 * (rs * rs + rp * rp))
+
+
+def _fields(v):
+    names = ("absorption", "scattering", "meancosine", "extinction", "reducedExtinction", "deltaEddExtinction", "D",
+             "transport", "C_phi", "C_phi_inv", "C_E", "A", "albedo", "reducedAlbedo", "de", "three_D", "two_A_de",
+             "global_coeff", "one_over_three_ext")
+    d = {"relative_ior": v[0], "dipole_model": v[1]}
+    for i, k in enumerate(names):
+        d[k] = np.array(v[2 + 3 * i: 5 + 3 * i], np.float64)
+    d["mean_transport"] = v[2 + 3 * len(names)]
+    return d
+
+
+def test_coefficients_follow_the_reference_formulas():
+    c = _fields(OracleScene(_slab_scene()).sss_coefficients(0))
+    sa = np.array(MILK["absorption"]) * MILK["scale"]
+    ss = np.array(MILK["scattering"]) * MILK["scale"]
+    g = np.array(MILK["meancosine"])
+    red_s = ss * (1 - g)
+    ext = ss + sa
+    red_ext = red_s + sa
+    D = 1.0 / (3.0 * red_ext)
+    tr = np.sqrt(sa / D)
+    want = {"extinction": ext, "reducedExtinction": red_ext, "D": D, "transport": tr, "albedo": ss / ext,
+            "reducedAlbedo": red_s / red_ext, "three_D": 3 * D, "one_over_three_ext": 1 / (3 * ext),
+            "de": 2.131 * D / np.sqrt(red_s / red_ext)}
+    for k, v in want.items():
+        np.testing.assert_allclose(c[k], v, rtol=2e-6, err_msg=k)
+    np.testing.assert_allclose(c["A"], (1 - c["C_E"]) / (2 * c["C_phi"]), rtol=2e-6)
+    np.testing.assert_allclose(c["global_coeff"], 1 / (4 * c["C_phi_inv"]) / (4 * math.pi**2), rtol=2e-6)
+    assert c["relative_ior"] == pytest.approx(1.3)
+
+
+def test_fresnel_moment_fits_against_quadrature():
+    c = _fields(OracleScene(_slab_scene()).sss_coefficients(0))
+    x, w = np.polynomial.legendre.leggauss(4000)
+    mu, w = 0.5 * (x + 1), 0.5 * w
+    for eta, cphi in ((1.3, c["C_phi"][0]), (1 / 1.3, c["C_phi_inv"][0])):
+        two_c1 = 2 * np.sum(w * mu * _fresnel_R(mu, eta))
+        assert abs(cphi - 0.25 * (1 - two_c1)) < 2e-3
+    three_c2 = 3 * np.sum(w * mu * mu * _fresnel_R(mu, 1.3))
+    assert abs(c["C_E"][0] - 0.5 * (1 - three_c2)) < 2e-3
+
+
+def _rd_total(alpha_p, A):
+    s = np.sqrt(3 * (1 - alpha_p))
+    return alpha_p / 2 * (1 + np.exp(-4.0 / 3.0 * A * s)) * np.exp(-s)
+
+
+def _plane_integral(c):
+    """2 pi int_0^inf S(r) r dr of the standard dipole as the reference places its sources."""
+    z_r, z_p, z_n = c["three_D"], 4 * c["A"] * c["D"], c["three_D"] + 4 * c["A"] * c["D"]
+    s = c["transport"]
+    return 2 * math.pi * (np.exp(-s * z_r) + z_n / z_p * np.exp(-s * z_p))
+
+
+def test_standard_dipole_total_reflectance():
+    orc = OracleScene(_slab_scene())
+    c = _fields(orc.sss_coefficients(0))
+    # 2 pi int_0^inf S(r) r dr by quadrature on a log grid (S is smooth, ~1/r^3 tail)
+    r = np.geomspace(1e-6, 60.0 / c["transport"].min(), 200001)
+    S = orc.dipole(0, r).astype(np.float64)
+    integral = 2 * math.pi * np.trapezoid(S * r[:, None], r, axis=0)
+    np.testing.assert_allclose(integral, _plane_integral(c), rtol=2e-3)
+    # the real-source half alone is Jensen's first term: (4 pi / alpha') alpha'/2 exp(-sqrt(3(1-alpha')))
+    a = c["reducedAlbedo"]
+    np.testing.assert_allclose(2 * math.pi * np.exp(-c["transport"] * c["three_D"]),
+                               4 * math.pi / a * (a / 2) * np.exp(-np.sqrt(3 * (1 - a))), rtol=1e-5)
+    # and with z_p = z_n the closed form is Jensen's total diffuse reflectance
+    z_v = c["three_D"] + 4 * c["A"] * c["D"]
+    np.testing.assert_allclose(2 * math.pi * (np.exp(-c["transport"] * c["three_D"]) + np.exp(-c["transport"] * z_v)),
+                               4 * math.pi / a * _rd_total(a, c["A"]), rtol=1e-5)
+
+
+@pytest.mark.parametrize("tilt", [0.0, 45.0])
+def test_gather_on_a_slab_is_unbiased(tilt):
+    d = _slab_scene(0)
+    orc = OracleScene(d)
+    c = _fields(orc.sss_coefficients(0))
+    # constant probe light on a coarse grid (every exit point finds a probe within `cell`)
+    g = np.arange(-2.0, 2.0 + 1e-9, 0.05, dtype=np.float32)
+    X, Z = np.meshgrid(g, g)
+    pos = np.stack([X.ravel(), np.zeros(X.size, np.float32), Z.ravel(), np.zeros(X.size, np.float32)], 1)
+    nrm = np.tile(np.array([[0, 1, 0, 0]], np.float32), (len(pos), 1))
+    orc.probes_set({"pos": pos, "nrm": nrm, "cell": 0.1}, 4, 0)
+    E0, T12 = np.array([1.0, 0.5, 0.25]), 0.9
+    L = np.tile(np.array([[1.0, 0.5, 0.25, 0.0]], np.float32), (len(pos), 1))
+    w12 = np.tile(np.array([[0.0, -1.0, 0.0, T12]], np.float32), (len(pos), 1))
+    orc.probes_values(L, w12)
+    reps = 20000
+    t = math.radians(tilt)
+    no = [math.sin(t), math.cos(t), 0.0]
+    got = orc.sss_gather(0, [0.0, 0.0, 0.0], no, reps, seed=3)
+    want = c["global_coeff"] * E0 * T12 * _plane_integral(c)
+    # two independent halves give the Monte-Carlo error
+    a = orc.sss_gather(0, [0.0, 0.0, 0.0], no, reps, seed=11)
+    spread = np.abs(a - got) / np.abs(want)
+    rel = np.abs(got - want) / np.abs(want)
+    assert np.all(rel < np.maximum(0.02, 4 * spread)), (got, want, spread)
+    assert np.all(rel < 0.05)
+
+
+def test_directional_dipole_symmetry_and_sign():
+    orc = OracleScene(_slab_scene(1))
+    n = np.array([0.0, 1.0, 0.0])
+    rows = []
+    for r in (0.002, 0.01, 0.05, 0.2):
+        for phi in (0.0, 1.1, 2.5, 4.0):
+            xi = np.array([r * math.cos(phi), 0.0, r * math.sin(phi)])
+            rows.append(np.concatenate([xi, n, -n, np.zeros(3), n]))
+    S = orc.dirpole(0, np.array(rows, np.float32)).reshape(4, 4, 3)
+    assert np.all(S >= 0) and np.all(np.isfinite(S))
+    np.testing.assert_allclose(S, np.repeat(S[:, :1], 4, axis=1), rtol=1e-4)
+    assert np.all(S[:-1].sum(-1) > S[1:].sum(-1))  # decreasing with distance
