@@ -13,6 +13,11 @@ Workloads (--workload):
          megakernel launch per render.
   diff : inverse rendering step (primal + L2 loss + path-replay adjoint +
          texture update) over 8 ring views at 256x256, 16 + 16 spp.
+  sss  : subsurface + multiple scattering, 512x512, 128 spp: whole-milk
+         displaced icosphere (1,310,720 triangles, standard dipole, 2^20
+         Poisson-disk probes, 4 gather samples per hit) beside a GGX rough
+         glass sphere with the microsurface multiple-scattering walk.  One step
+         = the probe pass plus one launch of all 128 samples (33.6 M paths).
 
 A step is one pass of the hot path over one batch of synthetic input that is
 already resident in HBM.  The timed region is bracketed by
@@ -172,6 +177,8 @@ def bench_path(args):
         line["batched"] = batch_measure(1, 1)
     if not args.no_diff:
         line["differentiable"] = diff_measure(5, 1)
+    if not args.no_sss:
+        line["subsurface"] = sss_measure(2, 1)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline_path(desc)
     return line
@@ -253,7 +260,7 @@ DIFF_SIZE, DIFF_VIEWS, DIFF_SPP, DIFF_TEX = 256, 8, 16, 64
 
 
 def diff_setup(size=DIFF_SIZE, views_n=DIFF_VIEWS, spp=DIFF_SPP, tex=DIFF_TEX, level=7):
-    """Config-5 inverse rendering: the path-tracer scene with a roughness texture on
+    """Inverse-rendering workload: the path-tracer scene with a roughness texture on
     the conductor sphere and an albedo texture on the ground, 8 ring views, targets
     rendered from the seed-99 ground-truth textures; the optimisation starts from
     constant textures (albedo 0.5, alpha 0.3)."""
@@ -328,7 +335,59 @@ def bench_diff(args):
     }
 
 
-WORKLOADS = {"path": bench_path, "bsdf": bench_bsdf, "batch": bench_batch, "diff": bench_diff}
+# --------------------------------------------------------------------- sss
+SSS_W, SSS_H, SSS_SPP = 512, 512, 128
+
+
+def sss_measure(steps, warmup):
+    """Subsurface scene at 512x512, 128 spp; per step one probe pass + one launch."""
+    import torch
+
+    from open_rendering_framework_amd import scenes
+
+    desc = scenes.subsurface_scene(width=SSS_W, height=SSS_H)
+    ctx = scenes.to_context(desc)
+    stats = ctx.accel_build()
+    pr = ctx.probes()
+    stream = torch.cuda.current_stream()
+    for _ in range(max(1, warmup)):
+        ctx.launch(SSS_W, SSS_H, 0, SSS_SPP, stream=stream)
+    torch.cuda.synchronize()
+    ctx.film_clear()
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for s in range(steps):
+        ev[s][0].record(stream)
+        ctx.launch(SSS_W, SSS_H, 0, SSS_SPP, stream=stream)
+        ev[s][1].record(stream)
+    torch.cuda.synchronize()
+    wall = (time.perf_counter() - t0) / steps
+    kms = sum(a.elapsed_time(b) for a, b in ev) / steps
+    c = ctx.counters()
+    paths = SSS_W * SSS_H * SSS_SPP
+    assert c["paths"] == paths * steps, c
+    return {"mpaths_s": paths / wall / 1e6, "s_per_launch": wall, "launch_ms_events": kms, "paths": paths,
+            "spp": SSS_SPP, "triangles": int(stats["tris"]), "probes": pr["count"],
+            "ext_rays_per_path": c["ext_rays"] / c["paths"], "shadow_rays_per_path": c["shadow_rays"] / c["paths"],
+            "nonfinite_samples": c["nonfinite"]}
+
+
+def bench_sss(args):
+    d = sss_measure(args.steps, args.warmup)
+    return {
+        "metric": "Mpaths/s at 512^2 128spp (subsurface + multiple scattering)",
+        "value": d["mpaths_s"],
+        "unit": "Mpaths/s",
+        "ms_per_step": d["s_per_launch"] * 1e3,
+        "dtype": "f32",
+        "config": {"workload": "subsurface_multiscatter_512x512_128spp", "width": SSS_W, "height": SSS_H,
+                   "spp": SSS_SPP, "probes": d["probes"], "triangles": d["triangles"]},
+        "subsurface": d,
+    }
+
+
+WORKLOADS = {"path": bench_path, "bsdf": bench_bsdf, "batch": bench_batch, "diff": bench_diff, "sss": bench_sss}
 
 
 def main():
@@ -340,10 +399,11 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched-wavefront measurement")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable-render measurement")
+    ap.add_argument("--no-sss", action="store_true", help="skip the subsurface-scene measurement")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
-              "(the render path shards by image tile or sample range, see DESIGN.md)", file=sys.stderr)
+        print("bench.py: measures one GPU only; multi-GPU runs are not supported "
+              "(the render path would shard by image tile or sample range)", file=sys.stderr)
         sys.exit(2)
     import torch
 

@@ -335,6 +335,8 @@ SceneGPU scene_view(orf_context_t* c)
     s.env.sampled = c->env_w > 0 && c->env_is;
     s.max_depth = c->max_depth;
     s.scene_epsilon = c->scene_epsilon;
+    s.features = 0u;
+    for (const MaterialGPU& m : c->mats) s.features |= material_features(m.kind);
     s.sss = c->d_sss.as<ScatteringGPU>();
     s.probes.sss = -1;
     if (c->probe_count > 0) {

@@ -34,6 +34,7 @@ __device__ __forceinline__ long long mega_next(unsigned long long* qctr, int par
 // radiances into px (plain read-modify-write when this item is the pixel's only
 // writer in the launch, FP64 atomics otherwise).  Pixels outside the camera
 // patch (CU/path_tracer.cu:36-43) record zero radiance.
+template <unsigned kFeat>
 __device__ __forceinline__ void mega_pixel(const SceneGPU& sc, const CameraGPU& cam, int w, int h, int x, int y,
                                            const int* remap, unsigned frame0, unsigned k0, unsigned k1, bool owner,
                                            double* px, unsigned* stk, PathStats& st, unsigned long long& bad)
@@ -47,7 +48,7 @@ __device__ __forceinline__ void mega_pixel(const SceneGPU& sc, const CameraGPU& 
         if (inside) {
             f3 dir;
             uint32_t rng = camera_ray(cam, w, h, x, y, frame0 + k, dir);
-            L = path_radiance(sc, remap, cam.eye, dir, rng, stk, kMegaBlock, st);
+            L = path_radiance<kFeat>(sc, remap, cam.eye, dir, rng, stk, kMegaBlock, st);
         }
         if (finite3(L)) {
             sr += (double)L.x;

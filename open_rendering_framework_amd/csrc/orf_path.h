@@ -102,21 +102,27 @@ ORF_HD HitGeom hit_geometry(const SceneGPU& sc, const PathState& ps, const Hit& 
     return g;
 }
 
+template <unsigned kFeat = kFeatAll>
 ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps, const Hit& h, const HitGeom& g,
                        ShadowRay& sr);
 
 // Shade one surface vertex.  Returns true when the path continues with ps.o/ps.d
-// as its next ray; `sr` receives the NEE shadow segment, if any.
+// as its next ray; `sr` receives the NEE shadow segment, if any.  kFeat: the
+// material code compiled in (kFeat* bits); the caller guarantees it covers the
+// scene's materials.
+template <unsigned kFeat = kFeatAll>
 ORF_HD bool shade_vertex(const SceneGPU& sc, const int* remap, PathState& ps, const Hit& h, ShadowRay& sr)
 {
     HitGeom g = hit_geometry(sc, ps, h);
-    return shade_with(sc, hit_material(sc, remap, g.sh.w), ps, h, g, sr);
+    return shade_with<kFeat>(sc, hit_material(sc, remap, g.sh.w), ps, h, g, sr);
 }
 
 // Shade a vertex with material `mat` (the material programs' closest hits).
+template <unsigned kFeat>
 ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps, const Hit& h, const HitGeom& g,
                        ShadowRay& sr)
 {
+    constexpr bool kDiel = (kFeat & kFeatDielectric) != 0, kSss = (kFeat & kFeatTranslucent) != 0;
     sr.valid = false;
     const float eps = sc.scene_epsilon;
     f3 d = ps.d;
@@ -175,7 +181,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         ps.prev_bsdf_pdf = -1.0f;
         return true;
     }
-    if (mat.kind == MAT_DIELECTRIC) {
+    if (kDiel && mat.kind == MAT_DIELECTRIC) {
         if (ps.depth >= sc.max_depth) return false;
         f3 nd, f;
         if (!dielectric_step(d, h.t, normal, ffn, mat, ps.rng, nd, f)) return false;
@@ -187,7 +193,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         ps.prev_bsdf_pdf = -1.0f;
         return true;
     }
-    if (mat.kind == MAT_INTERFACE) {
+    if (kDiel && mat.kind == MAT_INTERFACE) {
         // Lambertian interface (CU/lambertian_interface_shader.cu:51-173): a Lambert
         // base under a smooth dielectric of index ior.  NEE through both Fresnel
         // transmissions; the continuation is the specular reflection with the
@@ -241,7 +247,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         ps.prev_bsdf_pdf = -1.0f;
         return true;
     }
-    if (mat.kind == MAT_TRANSLUCENT) {
+    if (kSss && mat.kind == MAT_TRANSLUCENT) {
         // CU/subsurface_scattering_shader.cu:55-198: Fresnel choice between the
         // specular reflection and the refraction; entering from outside also adds
         // the diffusion term (sss_gather, before the path continues); inside, Beer
@@ -289,6 +295,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
 }
 
 // The whole path of one camera sample (megakernel form).
+template <unsigned kFeat = kFeatAll>
 ORF_HD f3 path_radiance(const SceneGPU& sc, const int* remap, f3 o, f3 d, uint32_t rng, unsigned* stk, int stride,
                         PathStats& st)
 {
@@ -302,7 +309,7 @@ ORF_HD f3 path_radiance(const SceneGPU& sc, const int* remap, f3 o, f3 d, uint32
             break;
         }
         ShadowRay sr;
-        bool more = shade_vertex(sc, remap, ps, h, sr);
+        bool more = shade_vertex<kFeat>(sc, remap, ps, h, sr);
         if (sr.valid) {
             Hit hs;
             st.shd++;

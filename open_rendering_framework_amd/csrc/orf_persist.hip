@@ -20,6 +20,7 @@ namespace orf {
 
 constexpr int kPersistBlock = kMegaBlock;
 
+template <unsigned kFeat>
 __global__ __launch_bounds__(kPersistBlock, ORF_MEGA_WAVES) void k_batch_persistent(
     SceneGPU sc, const WfRender* __restrict__ renders, const PersistRender* __restrict__ pr, int nrenders,
     const int* __restrict__ remap, double* film, unsigned long long* stats, unsigned long long* qctr,
@@ -50,7 +51,7 @@ __global__ __launch_bounds__(kPersistBlock, ORF_MEGA_WAVES) void k_batch_persist
             const int x = (tile % P.tiles_x) * 8 + (int)(lane & 7), y = (tile / P.tiles_x) * 8 + (int)(lane >> 3);
             if (x < R.width && y < R.height) {  // every lane reaches the next item fetch
                 const unsigned k0 = (unsigned)chunk * P.per, k1 = min(k0 + P.per, R.spp);
-                mega_pixel(sc, R.cam, R.width, R.height, x, y, remap + R.remap_offset, R.frame0, k0, k1,
+                mega_pixel<kFeat>(sc, R.cam, R.width, R.height, x, y, remap + R.remap_offset, R.frame0, k0, k1,
                            P.chunks == 1, film + 4 * ((size_t)R.film_offset + (size_t)y * R.width + x), stk, st, bad);
                 npaths += k1 - k0;
             }
@@ -71,8 +72,16 @@ void launch_batch_persistent(const SceneGPU& sc, const WfRender* renders, const 
     ORF_REQUIRE(total_items / kXcdParts < 0xF0000000LL, "persistent batch: too many work items");
     if (total_items == 0) return;
     zero_queue_counters(qctr, stream);
-    hipLaunchKernelGGL(k_batch_persistent, dim3((unsigned)grid), dim3(kPersistBlock), 0, stream, sc, renders, pr,
-                       nrenders, remap, film, stats, qctr, total_items);
+    const dim3 g((unsigned)grid), b(kPersistBlock);
+    if (sc.features == 0u)
+        hipLaunchKernelGGL(k_batch_persistent<0u>, g, b, 0, stream, sc, renders, pr, nrenders, remap, film, stats, qctr,
+                           total_items);
+    else if (sc.features == kFeatDielectric)
+        hipLaunchKernelGGL(k_batch_persistent<kFeatDielectric>, g, b, 0, stream, sc, renders, pr, nrenders, remap, film,
+                           stats, qctr, total_items);
+    else
+        hipLaunchKernelGGL(k_batch_persistent<kFeatAll>, g, b, 0, stream, sc, renders, pr, nrenders, remap, film,
+                           stats, qctr, total_items);
     ORF_HIP_CHECK(hipGetLastError());
 }
 

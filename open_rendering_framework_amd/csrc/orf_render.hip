@@ -12,6 +12,7 @@
 
 namespace orf {
 
+template <unsigned kFeat>
 __global__ __launch_bounds__(kMegaBlock, ORF_MEGA_WAVES) void k_render_mega(SceneGPU sc, CameraGPU cam, FilmGPU film,
                                                                              const int* remap, unsigned frame0,
                                                                              unsigned spp, int tiles_x, int chunks,
@@ -32,7 +33,7 @@ __global__ __launch_bounds__(kMegaBlock, ORF_MEGA_WAVES) void k_render_mega(Scen
             const int x = (tile % tiles_x) * 8 + (int)(threadIdx.x & 7), y = (tile / tiles_x) * 8 + (int)(threadIdx.x >> 3);
             if (x < film.width && y < film.height) {
                 const unsigned k0 = (unsigned)chunk * kMegaChunk, k1 = min(k0 + kMegaChunk, spp);
-                mega_pixel(sc, cam, film.width, film.height, x, y, remap, frame0, k0, k1, chunks == 1,
+                mega_pixel<kFeat>(sc, cam, film.width, film.height, x, y, remap, frame0, k0, k1, chunks == 1,
                            film.sums + 4 * ((size_t)y * film.width + x), stk, st, bad);
                 npaths += k1 - k0;
             }
@@ -77,8 +78,16 @@ void launch_megakernel(const SceneGPU& sc, const CameraGPU& cam, const FilmGPU& 
     long long items = (long long)tiles_x * tiles_y * chunks;
     ORF_REQUIRE(items / kXcdParts < 0xF0000000LL, "launch: too many work items");
     zero_queue_counters(film.qctr, stream);
-    hipLaunchKernelGGL(k_render_mega, dim3((unsigned)persistent_grid(items)), dim3(kMegaBlock), 0, stream, sc, cam,
-                       film, remap, frame0, spp, tiles_x, chunks, items);
+    const dim3 grid((unsigned)persistent_grid(items)), block(kMegaBlock);
+    if (sc.features == 0u)
+        hipLaunchKernelGGL(k_render_mega<0u>, grid, block, 0, stream, sc, cam, film, remap, frame0, spp, tiles_x,
+                           chunks, items);
+    else if (sc.features == kFeatDielectric)
+        hipLaunchKernelGGL(k_render_mega<kFeatDielectric>, grid, block, 0, stream, sc, cam, film, remap, frame0, spp,
+                           tiles_x, chunks, items);
+    else
+        hipLaunchKernelGGL(k_render_mega<kFeatAll>, grid, block, 0, stream, sc, cam, film, remap, frame0, spp,
+                           tiles_x, chunks, items);
     ORF_HIP_CHECK(hipGetLastError());
 }
 

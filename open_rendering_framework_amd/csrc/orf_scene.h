@@ -128,10 +128,23 @@ struct SceneGPU {
     int has_env;
     int max_depth;
     float scene_epsilon;
-    int pad;
+    unsigned features;        // OR of material_features() over the material table
     const ScatteringGPU* sss;  // per translucent material
     ProbesGPU probes;
 };
+
+// Material code a megakernel variant is compiled with: Lambert, conductor and
+// emitter always; kFeatDielectric adds the rough dielectric (with the
+// microsurface walk) and the Lambertian interface, kFeatTranslucent the
+// translucent material.  Launches pick the smallest variant covering the scene,
+// so simple scenes do not pay the register pressure of unused materials.
+constexpr unsigned kFeatDielectric = 1u, kFeatTranslucent = 2u, kFeatAll = 3u;
+
+inline unsigned material_features(int kind)
+{
+    return kind == MAT_DIELECTRIC || kind == MAT_INTERFACE ? kFeatDielectric
+                                                           : (kind == MAT_TRANSLUCENT ? kFeatTranslucent : 0u);
+}
 
 constexpr float kRayMax = 1e27f;  // RT_DEFAULT_MAX
 constexpr int kStackDepth = 48;   // builder caps BVH depth below this

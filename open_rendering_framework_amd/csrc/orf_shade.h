@@ -50,13 +50,16 @@ ORF_HD f3 env_lookup(const EnvGPU& env, f3 dir)
     return tex_bilinear(env.tex, env.w, env.h, u, v);
 }
 
-// first index i in [0, n) with cdf[i] >= xi (n - 1 if none)
-ORF_HD int cdf_lower_bound(const float* cdf, int n, float xi)
+// First index i in [0, n) with cdf[i] > xi (n - 1 if none): the cell of xi,
+// never one of zero width (the reference's bisection, ORF/Envmap.h:23-55, can
+// return a zero-width cell when xi equals a flat run's value, e.g. xi = 0 with
+// zero-density leading cells, and the sampler then divides 0/0).
+ORF_HD int cdf_upper_bound(const float* cdf, int n, float xi)
 {
     int lo = 0, hi = n - 1;
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
-        if (cdf[mid] >= xi) hi = mid;
+        if (cdf[mid] > xi) hi = mid;
         else lo = mid + 1;
     }
     return lo;
@@ -81,13 +84,13 @@ ORF_HD void env_sample(const EnvGPU& env, uint32_t& rng, f3& dir, f3& L, float& 
 {
     const float two_pi2 = 2.0f * kPi * kPi;
     float xi1 = rnd_tea(rng), xi2 = rnd_tea(rng);
-    int vi = cdf_lower_bound(env.marg_cdf, env.h, xi1);
+    int vi = cdf_upper_bound(env.marg_cdf, env.h, xi1);
     float c0 = vi > 0 ? env.marg_cdf[vi - 1] : 0.0f;
     float dv = vi > 0 ? (xi1 - c0) / (env.marg_cdf[vi] - c0) : xi1 / env.marg_cdf[vi];
     float pdf_m = env.marg_pdf[vi];
     float v = ((float)vi + dv) / (float)env.h;
     const float* ccdf = env.cond_cdf + (size_t)vi * env.w;
-    int ui = cdf_lower_bound(ccdf, env.w, xi2);
+    int ui = cdf_upper_bound(ccdf, env.w, xi2);
     float c1 = ui > 0 ? ccdf[ui - 1] : 0.0f;
     float du = ui > 0 ? (xi2 - c1) / (ccdf[ui] - c1) : xi2 / ccdf[ui];
     float pdf_c = env.cond_pdf[(size_t)vi * env.w + ui];
@@ -101,7 +104,9 @@ ORF_HD void env_sample(const EnvGPU& env, uint32_t& rng, f3& dir, f3& L, float& 
     dir = mk3(st * sp, ct, st * cp);
     f3 E = tex_bilinear(env.tex, env.w, env.h, u, v);
     L = E * (st * two_pi2 / prob);
-    pdf_sa = prob / (two_pi2 * st);
+    // a draw that rounds onto a pole (st = 0) carries L = 0; its density is
+    // reported as 0 (not +inf) so the MIS weight stays finite
+    pdf_sa = st > 0.0f ? prob / (two_pi2 * st) : 0.0f;
 }
 
 // evaluate_direct_illumination (ORF/LightSampler.h:143-166) for one picked light.

@@ -357,22 +357,21 @@ static v3 env_lookup(const oscene* s, v3 dir)
     return tex_bilinear(s, u, v);
 }
 
-/* cdf_bsearch_marginal / cdf_bsearch_conditional (ORF/Envmap.h:23-55), with the
- * out-of-range reads at index -1 and n guarded. */
+/* cdf_bsearch_marginal / cdf_bsearch_conditional (ORF/Envmap.h:23-55) pick the
+ * cell i with cdf[i-1] <= xi <= cdf[i].  Where xi equals the value of a flat
+ * run of the cdf (always so for xi = 0 when the first cells have zero density)
+ * that bisection can land on a zero-width cell and the sampler then divides 0/0.
+ * Deviation: the first i with cdf[i] > xi (n - 1 if none), which never has zero
+ * width; for every other xi it is the same cell. */
 static int env_bsearch(const R* cdf, int n, R xi)
 {
-    unsigned ts = (unsigned)n;
-    unsigned middle = ts = ts >> 1;
-    while (ts > 0) {
-        unsigned odd = ts & 1u;
-        ts >>= 1;
-        unsigned tmp = ts + odd;
-        R left = middle > 0 ? cdf[middle - 1] : (R)-1;
-        if (xi > cdf[middle]) middle = middle + tmp;
-        else if (xi < left) middle = middle - tmp;
-        if (middle >= (unsigned)n) middle = (unsigned)n - 1;
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cdf[mid] > xi) hi = mid;
+        else lo = mid + 1;
     }
-    return (int)middle;
+    return lo;
 }
 
 int orc_env_bsearch(const double* cdf, int n, double xi)
@@ -468,7 +467,9 @@ static void env_sample(const oscene* s, uint32_t* t, v3* dir, v3* L, R* pdf_sa)
     *dir = V3(st * sp, ct, st * cp);
     v3 E = tex_bilinear(s, u, v);
     *L = vscale(E, st * two_pi2 / prob);
-    *pdf_sa = prob / (two_pi2 * st);
+    /* new: a draw that rounds onto a pole (st = 0) has L = 0 and density 0 here
+     * (not +inf, which made the MIS weight inf / inf) */
+    *pdf_sa = st > 0 ? prob / (two_pi2 * st) : 0;
 }
 
 /* ---------------------------------------------------------------- lights */

@@ -46,7 +46,9 @@ def test_bvh_matches_brute_force():
     np.testing.assert_array_equal(t1, t2)
 
 
-def test_env_bsearch_is_lower_bound():
+def test_env_bsearch_is_upper_bound():
+    """The cell of xi is the first i with cdf[i] > xi: exact ties with a cdf value
+    (flat runs, xi = 0 ahead of zero-density cells) never give a zero-width cell."""
     d = _small_scene()
     s = OracleScene(d)
     rng = np.random.default_rng(2)
@@ -55,10 +57,14 @@ def test_env_bsearch_is_lower_bound():
         pdf[rng.random(n) < 0.1] = 0.0  # flat cdf steps
         cdf = np.cumsum(pdf) / pdf.sum()
         cdf[-1] = 1.0
-        xs = rng.random(2000)
-        got = [s.lib.orc_env_bsearch(cdf.ctypes.data, n, float(x)) for x in xs]
-        want = np.searchsorted(cdf, xs, side="left")
+        cdf = cdf.astype(np.float32).astype(np.float64)  # the oracle searches FP32 tables
+        xs = np.concatenate([rng.random(2000), cdf[:-1], [0.0]]).astype(np.float32).astype(np.float64)
+        xs = xs[xs < 1.0]  # the samplers draw xi in [0, 1)
+        got = np.array([s.lib.orc_env_bsearch(cdf.ctypes.data, n, float(x)) for x in xs])
+        want = np.minimum(np.searchsorted(cdf, xs, side="right"), n - 1)
         assert np.array_equal(got, want), n
+        width = cdf[got] - np.where(got > 0, cdf[np.maximum(got - 1, 0)], 0.0)
+        assert np.all(width > 0), n
 
 
 def test_env_tables_are_densities_and_sampling_follows_them():
