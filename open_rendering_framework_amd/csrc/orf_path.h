@@ -104,23 +104,25 @@ ORF_HD HitGeom hit_geometry(const SceneGPU& sc, const PathState& ps, const Hit& 
 
 template <unsigned kFeat = kFeatAll>
 ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps, const Hit& h, const HitGeom& g,
-                       ShadowRay& sr);
+                       ShadowRay& sr, unsigned* stk = nullptr, int stride = 1);
 
 // Shade one surface vertex.  Returns true when the path continues with ps.o/ps.d
 // as its next ray; `sr` receives the NEE shadow segment, if any.  kFeat: the
 // material code compiled in (kFeat* bits); the caller guarantees it covers the
-// scene's materials.
+// scene's materials.  (stk, stride): a traversal stack the shading may use for
+// rays of its own (the translucent gather), or null.
 template <unsigned kFeat = kFeatAll>
-ORF_HD bool shade_vertex(const SceneGPU& sc, const int* remap, PathState& ps, const Hit& h, ShadowRay& sr)
+ORF_HD bool shade_vertex(const SceneGPU& sc, const int* remap, PathState& ps, const Hit& h, ShadowRay& sr,
+                         unsigned* stk = nullptr, int stride = 1)
 {
     HitGeom g = hit_geometry(sc, ps, h);
-    return shade_with<kFeat>(sc, hit_material(sc, remap, g.sh.w), ps, h, g, sr);
+    return shade_with<kFeat>(sc, hit_material(sc, remap, g.sh.w), ps, h, g, sr, stk, stride);
 }
 
 // Shade a vertex with material `mat` (the material programs' closest hits).
 template <unsigned kFeat>
 ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps, const Hit& h, const HitGeom& g,
-                       ShadowRay& sr)
+                       ShadowRay& sr, unsigned* stk, int stride)
 {
     constexpr bool kDiel = (kFeat & kFeatDielectric) != 0, kSss = (kFeat & kFeatTranslucent) != 0;
     sr.valid = false;
@@ -279,7 +281,8 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         ps.thr = ps.thr * beam;
         f3 nd;
         if (reflect_xi >= R) {
-            if (!inside && mat.sss == sc.probes.sss) ps.L += ps.thr * sss_gather(sc, P, mat.sss, p, no, ps.rng);
+            if (!inside && mat.sss == sc.probes.sss)
+                ps.L += ps.thr * sss_gather(sc, P, mat.sss, p, no, ps.rng, stk, stride);
             nd = recip * (co * no - wo) - no * ct;
         } else {
             nd = 2.0f * co * no - wo;
@@ -309,7 +312,7 @@ ORF_HD f3 path_radiance(const SceneGPU& sc, const int* remap, f3 o, f3 d, uint32
             break;
         }
         ShadowRay sr;
-        bool more = shade_vertex<kFeat>(sc, remap, ps, h, sr);
+        bool more = shade_vertex<kFeat>(sc, remap, ps, h, sr, stk, stride);
         if (sr.valid) {
             Hit hs;
             st.shd++;

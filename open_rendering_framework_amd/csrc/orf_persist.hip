@@ -21,7 +21,7 @@ namespace orf {
 constexpr int kPersistBlock = kMegaBlock;
 
 template <unsigned kFeat>
-__global__ __launch_bounds__(kPersistBlock, ORF_MEGA_WAVES) void k_batch_persistent(
+__global__ __launch_bounds__(kPersistBlock, mega_waves(kFeat)) void k_batch_persistent(
     SceneGPU sc, const WfRender* __restrict__ renders, const PersistRender* __restrict__ pr, int nrenders,
     const int* __restrict__ remap, double* film, unsigned long long* stats, unsigned long long* qctr,
     long long total)

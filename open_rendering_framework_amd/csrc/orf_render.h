@@ -7,10 +7,18 @@
 
 namespace orf {
 
-// Minimum waves per SIMD the register allocator must allow (occupancy vs spills).
+// Minimum waves per SIMD the register allocator must allow (occupancy vs
+// spills) in the megakernels without / with the translucent material.
 #ifndef ORF_MEGA_WAVES
-#define ORF_MEGA_WAVES 6
+#define ORF_MEGA_WAVES 8
 #endif
+#ifndef ORF_MEGA_WAVES_SSS
+#define ORF_MEGA_WAVES_SSS 8
+#endif
+constexpr int mega_waves(unsigned features)
+{
+    return (features & kFeatTranslucent) ? ORF_MEGA_WAVES_SSS : ORF_MEGA_WAVES;
+}
 
 // Workgroups are dealt to the 8 XCDs round-robin (workgroup b runs on XCD b % 8),
 // and each XCD has its own L2.  Renumbering so that XCD x receives one contiguous
@@ -41,7 +49,9 @@ struct FilmGPU {
 };
 
 // Workgroups that fill the current device at the megakernels' occupancy (at most `items`).
-int persistent_grid(long long items);
+// Workgroups of a persistent megakernel launch: CUs x 4 SIMDs x waves, capped at
+// the item count.
+int persistent_grid(long long items, int waves);
 // Zero 8 queue counters with a kernel (safe inside captured streams).
 void zero_queue_counters(unsigned long long* qctr, hipStream_t stream);
 

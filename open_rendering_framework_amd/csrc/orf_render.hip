@@ -13,7 +13,7 @@
 namespace orf {
 
 template <unsigned kFeat>
-__global__ __launch_bounds__(kMegaBlock, ORF_MEGA_WAVES) void k_render_mega(SceneGPU sc, CameraGPU cam, FilmGPU film,
+__global__ __launch_bounds__(kMegaBlock, mega_waves(kFeat)) void k_render_mega(SceneGPU sc, CameraGPU cam, FilmGPU film,
                                                                              const int* remap, unsigned frame0,
                                                                              unsigned spp, int tiles_x, int chunks,
                                                                              long long items)
@@ -57,14 +57,14 @@ void zero_queue_counters(unsigned long long* qctr, hipStream_t stream)
     ORF_HIP_CHECK(hipGetLastError());
 }
 
-int persistent_grid(long long items)
+int persistent_grid(long long items, int waves)
 {
     static int cus[64] = {0};
     int dev = 0;
     ORF_HIP_CHECK(hipGetDevice(&dev));
     int& n = cus[dev & 63];
     if (n == 0) ORF_HIP_CHECK(hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev));
-    long long g = (long long)n * 4 * ORF_MEGA_WAVES;
+    long long g = (long long)n * 4 * waves;
     return (int)std::max<long long>(1, std::min<long long>(g, items));
 }
 
@@ -78,7 +78,7 @@ void launch_megakernel(const SceneGPU& sc, const CameraGPU& cam, const FilmGPU& 
     long long items = (long long)tiles_x * tiles_y * chunks;
     ORF_REQUIRE(items / kXcdParts < 0xF0000000LL, "launch: too many work items");
     zero_queue_counters(film.qctr, stream);
-    const dim3 grid((unsigned)persistent_grid(items)), block(kMegaBlock);
+    const dim3 grid((unsigned)persistent_grid(items, mega_waves(sc.features))), block(kMegaBlock);
     if (sc.features == 0u)
         hipLaunchKernelGGL(k_render_mega<0u>, grid, block, 0, stream, sc, cam, film, remap, frame0, spp, tiles_x,
                            chunks, items);

@@ -9,8 +9,8 @@
 //   * channel c uniform, radius r ~ sigma_tr,c exp(-sigma_tr,c r), angle uniform
 //     on the tangent plane of the entry point xo;
 //   * a probe ray along -no through that plane point, clipped to the sphere of
-//     radius r_max, collects up to kSssHits crossings of the same translucent
-//     material; one is picked uniformly;
+//     radius r_max, collects in one traversal the first kSssHits crossings of
+//     the same translucent material; one is picked uniformly;
 //   * the exit point's incident light is the nearest probe's (within the grid
 //     cell size), its pdf on the surface is
 //         p = (1/3) sum_c sigma_c exp(-sigma_c r) / (2 pi r) * |n_i . no| / hits,
@@ -65,35 +65,105 @@ struct SssHit {
     float b, g;
 };
 
-// Crossings of the translucent material `sss` along o + t dir, 0 < t < tmax.
-ORF_HD int probe_ray_hits(const SceneGPU& sc, int sss, f3 o, f3 dir, float tmax, SssHit* out, unsigned* stk)
+// Keeps out[0..n) sorted by t with distinct t values, at most kSssHits: a
+// crossing at an existing t replaces it when its triangle id is lower.
+ORF_HD void sss_insert(SssHit* out, int& n, const SssHit& c)
 {
-    int n = 0;
-    float tmin = 0.0f;
-    for (int it = 0; it < 2 * kSssHits && n < kSssHits; ++it) {
-        Hit h;
-        if (!trace<false>(sc, o, dir, tmin, tmax, h, stk, 1)) break;
-        int mid = sc.shade[h.prim].w;
-        if (sc.materials[mid].kind == MAT_TRANSLUCENT && sc.materials[mid].sss == sss) {
-            out[n].t = h.t;
-            out[n].prim = h.prim;
-            out[n].leaf = h.leaf;
-            out[n].b = h.b;
-            out[n].g = h.g;
-            ++n;
+    for (int i = 0; i < n; ++i)
+        if (out[i].t == c.t) {
+            if (c.prim < out[i].prim) out[i] = c;
+            return;
         }
-        tmin = h.t;  // strict: the next crossing lies beyond this one
+    if (n == kSssHits && !(c.t < out[n - 1].t)) return;
+    int i = n < kSssHits ? n++ : kSssHits - 1;
+    while (i > 0 && out[i - 1].t > c.t) {
+        out[i] = out[i - 1];
+        --i;
+    }
+    out[i] = c;
+}
+
+// Crossings of the translucent material `sss` along o + t dir, 0 < t < tmax, in
+// one BVH traversal: the kSssHits smallest distinct t, ascending (at equal t
+// the lower triangle id), whatever the traversal order.  Once the list is full,
+// boxes and triangles beyond its last t (plus one ulp, so equal-t ties still
+// resolve) are culled.
+ORF_HD int probe_ray_hits(const SceneGPU& sc, int sss, f3 o, f3 dir, float tmax, SssHit* out, unsigned* stk,
+                          int stride)
+{
+    RayPre r = ray_pre(o, dir);
+    int n = 0;
+    float tcull = tmax;
+    unsigned node = 0;
+    int sp = 0;
+    unsigned ovf[kStackDepth - kLdsStack + 1];
+    auto push = [&](unsigned v) {
+        if (sp < kLdsStack) stk[stride * sp] = v;
+        else ovf[sp - kLdsStack] = v;
+        ++sp;
+    };
+    auto pop = [&]() -> unsigned {
+        --sp;
+        return sp < kLdsStack ? stk[stride * sp] : ovf[sp - kLdsStack];
+    };
+    for (;;) {
+        if (!link_is_leaf(node)) {
+            const float4* nd = sc.bvh + 4 * (size_t)node;
+            float4 a = nd[0], b = nd[1], c = nd[2], l = nd[3];
+            float e0 = box_enter(r, a.x, a.y, a.z, a.w, b.x, b.y, 0.0f, tcull);
+            float e1 = box_enter(r, b.z, b.w, c.x, c.y, c.z, c.w, 0.0f, tcull);
+            unsigned l0 = f2u(l.x), l1 = f2u(l.y);
+            bool h0 = e0 != __builtin_huge_valf(), h1 = e1 != __builtin_huge_valf();
+            if (h0 && h1) {
+                unsigned nearc = e0 <= e1 ? l0 : l1;
+                unsigned farc = e0 <= e1 ? l1 : l0;
+                if (sp < kStackDepth) push(farc);
+                node = nearc;
+            } else if (h0) {
+                node = l0;
+            } else if (h1) {
+                node = l1;
+            } else {
+                if (sp == 0) break;
+                node = pop();
+            }
+            continue;
+        }
+        unsigned first = leaf_first(node), cnt = leaf_count(node);
+        for (unsigned k = 0; k < cnt; ++k) {
+            const float4* tp = sc.tri + 3 * (size_t)(first + k);
+            float4 q0 = tp[0], q1 = tp[1], q2 = tp[2];
+            SssHit c;
+            if (!intersect_tri(o, dir, mk3(q0.x, q0.y, q0.z), mk3(q1.x, q1.y, q1.z), mk3(q2.x, q2.y, q2.z), 0.0f,
+                               tcull, c.t, c.b, c.g))
+                continue;
+            c.prim = (int)f2u(q0.w);
+            const MaterialGPU& m = sc.materials[sc.shade[c.prim].w];
+            if (m.kind != MAT_TRANSLUCENT || m.sss != sss) continue;
+            c.leaf = (int)(first + k);
+            sss_insert(out, n, c);
+            if (n == kSssHits) tcull = orf_u2f(f2u(out[n - 1].t) + 1u);
+        }
+        if (sp == 0) break;
+        node = pop();
     }
     return n;
 }
 
-// Diffusion term entering at xo (normal no facing the incident side).
-ORF_HD f3 sss_gather(const SceneGPU& sc, const ScatteringGPU& P, int sss, f3 xo, f3 no, uint32_t& rng)
+// Diffusion term entering at xo (normal no facing the incident side).  The probe
+// rays use the caller's traversal stack (stk, stride) -- free between traces --
+// or a private one when stk is null.
+ORF_HD f3 sss_gather(const SceneGPU& sc, const ScatteringGPU& P, int sss, f3 xo, f3 no, uint32_t& rng,
+                     unsigned* stk = nullptr, int stride = 1)
 {
     const ProbesGPU& pr = sc.probes;
     f3 acc = mk3(0.0f);
     if (pr.count <= 0 || pr.samples <= 0) return acc;
-    unsigned stk[kLdsStack];
+    unsigned own[kLdsStack];
+    if (!stk) {
+        stk = own;
+        stride = 1;
+    }
     f3 t1, t2;
     make_frame(no, t1, t2);
     float smin = fminf(P.transport.x, fminf(P.transport.y, P.transport.z));
@@ -109,7 +179,7 @@ ORF_HD f3 sss_gather(const SceneGPU& sc, const ScatteringGPU& P, int sss, f3 xo,
         float hh = sqrtf(r_max * r_max - r * r);
         f3 top = xo + (t1 * cp + t2 * sp) * r + no * hh;
         SssHit hits[kSssHits];
-        int nh = probe_ray_hits(sc, sss, top, -no, 2.0f * hh, hits, stk);
+        int nh = probe_ray_hits(sc, sss, top, -no, 2.0f * hh, hits, stk, stride);
         if (nh == 0) continue;
         const SssHit& hp = hits[min(nh - 1, (int)(s * (float)nh))];
         f3 xi = top + (-no) * hp.t;

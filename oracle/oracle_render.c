@@ -951,6 +951,58 @@ static int nearest_probe(const oscene* s, v3 y)
 #define SSS_HITS 4
 #define SSS_RADIUS_SIGMAS ((R)12.0f)
 
+/* Keeps out[0..n) sorted by t with distinct t values, at most SSS_HITS: a
+ * crossing at an existing t replaces it when its triangle index is lower. */
+static void hits_insert(ohit* out, int* n, ohit c)
+{
+    for (int i = 0; i < *n; i++)
+        if (out[i].t == c.t) {
+            if (c.prim < out[i].prim) out[i] = c;
+            return;
+        }
+    if (*n == SSS_HITS && !(c.t < out[*n - 1].t)) return;
+    int i = *n < SSS_HITS ? (*n)++ : SSS_HITS - 1;
+    while (i > 0 && out[i - 1].t > c.t) {
+        out[i] = out[i - 1];
+        i--;
+    }
+    out[i] = c;
+}
+
+/* Crossings of material m along o + t d, 0 < t < tmax (new: the importance-
+ * sampled gather's probe ray): the SSS_HITS smallest distinct t, ascending, at
+ * equal t the lower triangle index.  Every box overlapping the segment is
+ * visited, so the result does not depend on traversal order. */
+static int probe_collect(const oscene* s, v3 o, v3 d, R tmax, int m, ohit* out)
+{
+    int n = 0;
+    if (s->nnodes == 0) return 0;
+    double od[3] = {o.x, o.y, o.z};
+    double inv[3] = {1.0 / (double)d.x, 1.0 / (double)d.y, 1.0 / (double)d.z};
+    int32_t stack[128];
+    int sp = 0;
+    stack[sp++] = 0;
+    while (sp > 0) {
+        const bnode* b = &s->nodes[stack[--sp]];
+        double tn;
+        if (!box_hit(b, od, inv, 0.0, (double)tmax * (1.0 + 1e-6), &tn)) continue;
+        if (b->left < 0) {
+            for (int32_t k = 0; k < b->count; k++) {
+                int32_t id = s->order[b->first + k];
+                if (s->tri_mat[id] != m) continue;
+                ohit c;
+                if (!isect(o, d, s->p0[id], s->e1[id], s->e2[id], (R)0, tmax, &c.t, &c.b, &c.g)) continue;
+                c.prim = id;
+                hits_insert(out, &n, c);
+            }
+            continue;
+        }
+        stack[sp++] = b->right;
+        stack[sp++] = b->left;
+    }
+    return n;
+}
+
 static v3 sss_gather(const oscene* s, const osss* P, v3 xo, v3 no, uint32_t* t)
 {
     v3 acc = V3(0, 0, 0);
@@ -973,15 +1025,7 @@ static v3 sss_gather(const oscene* s, const osss* P, v3 xo, v3 no, uint32_t* t)
         v3 dir = vneg(no);
         /* crossings of the probes' material along the probe ray */
         ohit hits[SSS_HITS];
-        int nh = 0;
-        R tmin = 0;
-        for (int it = 0; it < 2 * SSS_HITS && nh < SSS_HITS; it++) {
-            ohit h;
-            if (!trace(s, top, dir, tmin, (R)2.0f * hh, 0, &h)) break;
-            int mi = s->tri_mat[h.prim];
-            if (s->mats[mi].kind == ORC_TRANSLUCENT && mi == s->probe_mat) hits[nh++] = h;
-            tmin = h.t;
-        }
+        int nh = probe_collect(s, top, dir, (R)2.0f * hh, s->probe_mat, hits);
         if (nh == 0) continue;
         int pick = (int)(sx * (R)nh);
         if (pick > nh - 1) pick = nh - 1;
