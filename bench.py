@@ -387,7 +387,27 @@ def bench_sss(args):
     }
 
 
-WORKLOADS = {"path": bench_path, "bsdf": bench_bsdf, "batch": bench_batch, "diff": bench_diff, "sss": bench_sss}
+# --------------------------------------------------------------------- copy
+def bench_copy(args):
+    """Plain 1 GiB device copy (torch copy_): the known-size kernel whose
+    FETCH_SIZE / WRITE_SIZE readings scale the counters of the render kernels."""
+    import torch
+
+    n = 1 << 28
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / args.steps
+    return {"metric": "device copy GB/s (1 GiB, counter calibration)", "value": 2 * 4 * n / dt / 1e9, "unit": "GB/s",
+            "ms_per_step": dt * 1e3, "dtype": "f32", "config": {"workload": "copy_1GiB", "bytes_each_way": 4 * n}}
+
+
+WORKLOADS = {"copy": bench_copy, "path": bench_path, "bsdf": bench_bsdf, "batch": bench_batch, "diff": bench_diff,
+             "sss": bench_sss}
 
 
 def main():

@@ -18,7 +18,8 @@ constexpr int kXcdParts = 8;
 #ifndef ORF_MEGA_CHUNK
 #define ORF_MEGA_CHUNK 4
 #endif
-constexpr unsigned kMegaChunk = ORF_MEGA_CHUNK;  // samples per work item
+constexpr unsigned kMegaChunk = ORF_MEGA_CHUNK;  // most samples per work item
+constexpr long long kMegaItemsPerWave = 16;      // fewest items per wave before chunks shrink
 
 // Next item of partition `part` ([lo, hi)) for the whole wave, or -1 when drained.
 __device__ __forceinline__ long long mega_next(unsigned long long* qctr, int part, long long lo, long long hi)
@@ -34,29 +35,62 @@ __device__ __forceinline__ long long mega_next(unsigned long long* qctr, int par
 // radiances into px (plain read-modify-write when this item is the pixel's only
 // writer in the launch, FP64 atomics otherwise).  Pixels outside the camera
 // patch (CU/path_tracer.cu:36-43) record zero radiance.
+//
+// The lane runs its samples as one loop of bounces (extension ray, shading,
+// shadow ray); a finished sample starts the next at once, so a wave waits for
+// its longest run of samples, not for its longest path at every sample.  The
+// per-sample order of operations is path_radiance()'s, so is the radiance.
 template <unsigned kFeat>
 __device__ __forceinline__ void mega_pixel(const SceneGPU& sc, const CameraGPU& cam, int w, int h, int x, int y,
                                            const int* remap, unsigned frame0, unsigned k0, unsigned k1, bool owner,
                                            double* px, unsigned* stk, PathStats& st, unsigned long long& bad)
 {
-    bool inside = !((unsigned)x < cam.patch_origin[0] || (unsigned)x > cam.patch_origin[0] + cam.patch_dims[0] ||
-                    (unsigned)y < cam.patch_origin[1] || (unsigned)y > cam.patch_origin[1] + cam.patch_dims[1]);
+    const bool inside = !((unsigned)x < cam.patch_origin[0] || (unsigned)x > cam.patch_origin[0] + cam.patch_dims[0] ||
+                          (unsigned)y < cam.patch_origin[1] || (unsigned)y > cam.patch_origin[1] + cam.patch_dims[1]);
     double sr = 0.0, sg = 0.0, sb = 0.0;
     unsigned cnt = 0;
-    for (unsigned k = k0; k < k1; ++k) {
-        f3 L = mk3(0.0f);
-        if (inside) {
+    unsigned k = k0;
+    if (!inside) {  // zero radiance for every sample
+        cnt = k1 - k0;
+        k = k1;
+    }
+    PathState ps;
+    bool live = false;
+    for (;;) {
+        if (!live) {
+            if (k >= k1) break;
             f3 dir;
             uint32_t rng = camera_ray(cam, w, h, x, y, frame0 + k, dir);
-            L = path_radiance<kFeat>(sc, remap, cam.eye, dir, rng, stk, kMegaBlock, st);
+            path_begin(ps, cam.eye, dir, rng);
+            live = true;
         }
-        if (finite3(L)) {
-            sr += (double)L.x;
-            sg += (double)L.y;
-            sb += (double)L.z;
-            cnt++;
+        Hit hit;
+        st.ext++;
+        bool more = false;
+        if (!trace<false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, hit, stk, kMegaBlock)) {
+            shade_miss(sc, ps);
         } else {
-            bad++;
+            ShadowRay shadow;
+            more = shade_vertex<kFeat>(sc, remap, ps, hit, shadow, stk, kMegaBlock);
+            if (shadow.valid) {
+                Hit hs;
+                st.shd++;
+                if (!trace<true>(sc, shadow.o, shadow.d, sc.scene_epsilon, shadow.tmax, hs, stk, kMegaBlock))
+                    ps.L += shadow.contrib;
+            }
+        }
+        if (!more) {
+            f3 L = ps.L;
+            if (finite3(L)) {
+                sr += (double)L.x;
+                sg += (double)L.y;
+                sb += (double)L.z;
+                cnt++;
+            } else {
+                bad++;
+            }
+            ++k;
+            live = false;
         }
     }
     if (owner) {

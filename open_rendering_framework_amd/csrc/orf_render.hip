@@ -1,8 +1,8 @@
 // orf_render.hip -- plain megakernel path tracer and environment CDF build (gfx950).
 //
-// Megakernel: persistent wave64 workgroups take (8x8 pixel tile, chunk of
-// kMegaChunk samples) work items from per-XCD queues (orf_mega.h); each lane
-// walks its pixel's samples of the chunk (path loop in orf_path.h) and adds FP64
+// Megakernel: persistent wave64 workgroups take (8x8 pixel tile, chunk of up
+// to kMegaChunk samples, sized per launch) work items from per-XCD queues
+// (orf_mega.h); each lane walks its pixel's samples of the chunk and adds FP64
 // sums of its finite samples to the film (plain RMW when one chunk covers all
 // samples, else FP64 atomics).
 #include <algorithm>
@@ -15,8 +15,8 @@ namespace orf {
 template <unsigned kFeat>
 __global__ __launch_bounds__(kMegaBlock, mega_waves(kFeat)) void k_render_mega(SceneGPU sc, CameraGPU cam, FilmGPU film,
                                                                              const int* remap, unsigned frame0,
-                                                                             unsigned spp, int tiles_x, int chunks,
-                                                                             long long items)
+                                                                             unsigned spp, unsigned per, int tiles_x,
+                                                                             int chunks, long long items)
 {
     __shared__ unsigned stack_lds[kLdsStack * kMegaBlock];
     unsigned* stk = stack_lds + threadIdx.x;
@@ -32,9 +32,9 @@ __global__ __launch_bounds__(kMegaBlock, mega_waves(kFeat)) void k_render_mega(S
             const int tile = (int)(item / chunks), chunk = (int)(item - (long long)tile * chunks);
             const int x = (tile % tiles_x) * 8 + (int)(threadIdx.x & 7), y = (tile / tiles_x) * 8 + (int)(threadIdx.x >> 3);
             if (x < film.width && y < film.height) {
-                const unsigned k0 = (unsigned)chunk * kMegaChunk, k1 = min(k0 + kMegaChunk, spp);
+                const unsigned k0 = (unsigned)chunk * per, k1 = min(k0 + per, spp);
                 mega_pixel<kFeat>(sc, cam, film.width, film.height, x, y, remap, frame0, k0, k1, chunks == 1,
-                           film.sums + 4 * ((size_t)y * film.width + x), stk, st, bad);
+                                  film.sums + 4 * ((size_t)y * film.width + x), stk, st, bad);
                 npaths += k1 - k0;
             }
         }
@@ -73,20 +73,28 @@ void launch_megakernel(const SceneGPU& sc, const CameraGPU& cam, const FilmGPU& 
 {
     if (spp == 0 || film.width <= 0 || film.height <= 0) return;
     ORF_REQUIRE(film.qctr, "launch: film has no queue counters");
-    int tiles_x = (film.width + 7) / 8, tiles_y = (film.height + 7) / 8;
-    int chunks = (int)((spp + kMegaChunk - 1) / kMegaChunk);
-    long long items = (long long)tiles_x * tiles_y * chunks;
+    const int tiles_x = (film.width + 7) / 8, tiles_y = (film.height + 7) / 8;
+    const int waves = mega_waves(sc.features);
+    // Samples per item: the largest of kMegaChunk, /2, ... that still gives every
+    // wave of a full persistent grid kMegaItemsPerWave items (the last items
+    // of a launch run alone, so a few big ones leave the chip idle).
+    const long long full = (long long)persistent_grid(1LL << 40, waves);
+    unsigned per = kMegaChunk;
+    while (per > 1 && (long long)tiles_x * tiles_y * ((spp + per - 1) / per) < kMegaItemsPerWave * full) per >>= 1;
+    per = std::min(per, spp);
+    const int chunks = (int)((spp + per - 1) / per);
+    const long long items = (long long)tiles_x * tiles_y * chunks;
     ORF_REQUIRE(items / kXcdParts < 0xF0000000LL, "launch: too many work items");
     zero_queue_counters(film.qctr, stream);
-    const dim3 grid((unsigned)persistent_grid(items, mega_waves(sc.features))), block(kMegaBlock);
+    const dim3 grid((unsigned)persistent_grid(items, waves)), block(kMegaBlock);
     if (sc.features == 0u)
-        hipLaunchKernelGGL(k_render_mega<0u>, grid, block, 0, stream, sc, cam, film, remap, frame0, spp, tiles_x,
+        hipLaunchKernelGGL(k_render_mega<0u>, grid, block, 0, stream, sc, cam, film, remap, frame0, spp, per, tiles_x,
                            chunks, items);
     else if (sc.features == kFeatDielectric)
         hipLaunchKernelGGL(k_render_mega<kFeatDielectric>, grid, block, 0, stream, sc, cam, film, remap, frame0, spp,
-                           tiles_x, chunks, items);
+                           per, tiles_x, chunks, items);
     else
-        hipLaunchKernelGGL(k_render_mega<kFeatAll>, grid, block, 0, stream, sc, cam, film, remap, frame0, spp,
+        hipLaunchKernelGGL(k_render_mega<kFeatAll>, grid, block, 0, stream, sc, cam, film, remap, frame0, spp, per,
                            tiles_x, chunks, items);
     ORF_HIP_CHECK(hipGetLastError());
 }

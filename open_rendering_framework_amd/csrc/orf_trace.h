@@ -74,11 +74,14 @@ ORF_HD float box_enter(const RayPre& r, float lx, float ly, float lz, float hx, 
     return tn <= tf ? tn : __builtin_huge_valf();
 }
 
-// Closest hit (any_hit == false) or first hit (any_hit == true) in (tmin, tmax).
-template <bool ANY>
-ORF_HD bool trace(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, Hit& h,
-                                      unsigned* stk, int stride)
+// Closest hit (any-hit off) or first hit (any-hit on) in (tmin, tmax).  kMode 0:
+// closest, 1: any, 2: chosen per lane by `any` (one traversal loop serves
+// extension and shadow rays of different lanes at once).
+template <int kMode>
+ORF_HD bool trace_impl(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, bool any, Hit& h, unsigned* stk,
+                       int stride)
 {
+    const bool ANY = kMode == 1 || (kMode == 2 && any);
     RayPre r = ray_pre(o, d);
     float tbest = tmax;
     int prim = -1, leaf = -1;
@@ -150,6 +153,18 @@ ORF_HD bool trace(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, Hit& h
     h.prim = prim;
     h.leaf = leaf;
     return prim >= 0;
+}
+
+template <bool ANY>
+ORF_HD bool trace(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, Hit& h, unsigned* stk, int stride)
+{
+    return trace_impl<ANY ? 1 : 0>(sc, o, d, tmin, tmax, ANY, h, stk, stride);
+}
+
+ORF_HD bool trace_dyn(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, bool any, Hit& h, unsigned* stk,
+                      int stride)
+{
+    return trace_impl<2>(sc, o, d, tmin, tmax, any, h, stk, stride);
 }
 
 }  // namespace orf
