@@ -155,7 +155,8 @@ constexpr int kStackDepth = 48;   // builder caps BVH depth below this
 #define ORF_LDS_STACK 16
 #endif
 constexpr int kLdsStack = ORF_LDS_STACK;
-static_assert(kLdsStack >= 1 && kLdsStack <= kStackDepth, "bad LDS stack size");
+static_assert(kLdsStack >= 1 && kLdsStack <= kStackDepth && (kLdsStack & (kLdsStack - 1)) == 0,
+              "LDS stack size must be a power of two no larger than kStackDepth");
 
 // BVH child link: internal node index, or leaf = 0x80000000 | (count-1) << 27 | first
 ORF_HD bool link_is_leaf(unsigned c) { return (c & 0x80000000u) != 0; }

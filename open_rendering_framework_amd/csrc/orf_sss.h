@@ -97,15 +97,8 @@ ORF_HD int probe_ray_hits(const SceneGPU& sc, int sss, f3 o, f3 dir, float tmax,
     unsigned node = 0;
     int sp = 0;
     unsigned ovf[kStackDepth - kLdsStack + 1];
-    auto push = [&](unsigned v) {
-        if (sp < kLdsStack) stk[stride * sp] = v;
-        else ovf[sp - kLdsStack] = v;
-        ++sp;
-    };
-    auto pop = [&]() -> unsigned {
-        --sp;
-        return sp < kLdsStack ? stk[stride * sp] : ovf[sp - kLdsStack];
-    };
+    auto push = [&](unsigned v) { stack_push(stk, stride, ovf, sp, v); };
+    auto pop = [&]() -> unsigned { return stack_pop(stk, stride, ovf, sp); };
     for (;;) {
         if (!link_is_leaf(node)) {
             const float4* nd = sc.bvh + 4 * (size_t)node;

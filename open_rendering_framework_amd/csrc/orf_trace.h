@@ -11,9 +11,9 @@
 // e1 x e2.  A hit is accepted when tmin < t < t_best (strict).
 // Box test: slabs with precomputed 1/d and o/d (explicit fma), far distance
 // inflated by 1 + 2^-21 so rounding never culls a box containing a hit.
-// Traversal stack: the first kLdsStack entries of a lane live in a column of
-// the workgroup's LDS array (stride = block size), entries up to kStackDepth in
-// a private array (the builder bounds the tree depth below kStackDepth).
+// Traversal stack: the top kLdsStack entries of a lane live in a column of the
+// workgroup's LDS array (stride = block size), deeper ones (up to kStackDepth)
+// in a private array (the builder bounds the tree depth below kStackDepth).
 #pragma once
 
 #include "orf_scene.h"
@@ -74,6 +74,28 @@ ORF_HD float box_enter(const RayPre& r, float lx, float ly, float lz, float hx, 
     return tn <= tf ? tn : __builtin_huge_valf();
 }
 
+// Traversal stack: entry i lives in LDS slot i mod kLdsStack while it is among
+// the top kLdsStack entries; a push that reuses a slot first moves the deeper
+// entry it holds to the private array `ovf`, and the pop that frees the slot
+// moves it back.  Every pop and push is an LDS access; the private array is
+// touched only past kLdsStack entries.
+ORF_HD void stack_push(unsigned* stk, int stride, unsigned* ovf, int& sp, unsigned v)
+{
+    const int slot = stride * (sp & (kLdsStack - 1));
+    if (sp >= kLdsStack) ovf[sp - kLdsStack] = stk[slot];
+    stk[slot] = v;
+    ++sp;
+}
+
+ORF_HD unsigned stack_pop(unsigned* stk, int stride, const unsigned* ovf, int& sp)
+{
+    --sp;
+    const int slot = stride * (sp & (kLdsStack - 1));
+    const unsigned v = stk[slot];
+    if (sp >= kLdsStack) stk[slot] = ovf[sp - kLdsStack];
+    return v;
+}
+
 // Closest hit (any-hit off) or first hit (any-hit on) in (tmin, tmax).  kMode 0:
 // closest, 1: any, 2: chosen per lane by `any` (one traversal loop serves
 // extension and shadow rays of different lanes at once).
@@ -89,15 +111,8 @@ ORF_HD bool trace_impl(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, b
     unsigned node = 0;
     int sp = 0;
     unsigned ovf[kStackDepth - kLdsStack + 1];
-    auto push = [&](unsigned v) {
-        if (sp < kLdsStack) stk[stride * sp] = v;
-        else ovf[sp - kLdsStack] = v;
-        ++sp;
-    };
-    auto pop = [&]() -> unsigned {
-        --sp;
-        return sp < kLdsStack ? stk[stride * sp] : ovf[sp - kLdsStack];
-    };
+    auto push = [&](unsigned v) { stack_push(stk, stride, ovf, sp, v); };
+    auto pop = [&]() -> unsigned { return stack_pop(stk, stride, ovf, sp); };
     bool have = true;  // `node` holds a live internal node or leaf link
     while (have) {
         if (!link_is_leaf(node)) {
