@@ -2,8 +2,9 @@
 //
 // Top-down build over triangle references: at each node, centroids are binned
 // into `bins` buckets on all three axes and the split with the lowest surface-
-// area-heuristic cost (C_trav = 1, C_isect = 1) is taken; a node becomes a leaf
-// when that is cheaper and it holds at most `max_leaf` triangles.  Where SAH
+// area-heuristic cost (C_isect = 1 per triangle, C_trav = kBvhTravCost per
+// node) is taken; a node becomes a leaf when that is cheaper and it holds at
+// most `max_leaf` triangles.  Where SAH
 // splits could push the depth past the traversal stack, the builder switches to
 // object-median splits, so the tree depth stays below kStackDepth = 48.  Nodes are emitted in
 // depth-first order (the near child follows its parent), triangles in leaf order.
@@ -199,7 +200,7 @@ BvhResult build_bvh(const float* verts, int64_t nverts, const int32_t* idx, int6
         }
 
         double leaf_cost = (double)n * parea;
-        double split_cost = parea + best_cost;
+        double split_cost = kBvhTravCost * parea + best_cost;
         bool make_leaf = n <= 1 || (n <= max_leaf && (best_axis < 0 || leaf_cost <= split_cost));
         if (make_leaf) {
             tn[tk.node].first = (uint32_t)tk.begin;

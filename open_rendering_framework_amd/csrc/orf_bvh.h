@@ -32,8 +32,20 @@ struct BvhResult {
     BvhStats stats;
 };
 
+// SAH cost of a node visit relative to one triangle test, largest leaf, bins.
+#ifndef ORF_BVH_TRAV_COST
+#define ORF_BVH_TRAV_COST 1.5
+#endif
+#ifndef ORF_BVH_MAX_LEAF
+#define ORF_BVH_MAX_LEAF 4
+#endif
+#ifndef ORF_BVH_BINS
+#define ORF_BVH_BINS 16
+#endif
+constexpr double kBvhTravCost = ORF_BVH_TRAV_COST;
+
 // verts: 3 floats per vertex; idx: 3 ints per triangle (object = world space).
 BvhResult build_bvh(const float* verts, int64_t nverts, const int32_t* idx, int64_t ntris,
-                    int max_leaf = 4, int bins = 16);
+                    int max_leaf = ORF_BVH_MAX_LEAF, int bins = ORF_BVH_BINS);
 
 }  // namespace orf
