@@ -36,7 +36,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 # HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE passes of these exact
 # workloads, scaled by a 1 GiB copy kernel (profiles/hbm_traffic_pmc.md)
-TRAFFIC_PATH_512_64SPP = 8.76e9
+TRAFFIC_PATH_512_64SPP = 13.81e9
+TRAFFIC_SSS_512_128SPP = 229.8e9
 TRAFFIC_BSDF_2E22 = 218.4e6
 
 
@@ -368,6 +369,7 @@ def sss_measure(steps, warmup):
     paths = SSS_W * SSS_H * SSS_SPP
     assert c["paths"] == paths * steps, c
     return {"mpaths_s": paths / wall / 1e6, "s_per_launch": wall, "launch_ms_events": kms, "paths": paths,
+            "hbm_traffic_per_launch": TRAFFIC_SSS_512_128SPP, "hbm_gbs": TRAFFIC_SSS_512_128SPP / (kms * 1e-3) / 1e9,
             "spp": SSS_SPP, "triangles": int(stats["tris"]), "probes": pr["count"],
             "ext_rays_per_path": c["ext_rays"] / c["paths"], "shadow_rays_per_path": c["shadow_rays"] / c["paths"],
             "nonfinite_samples": c["nonfinite"]}

@@ -266,7 +266,9 @@ orf_status orf_scattering_properties(orf_context ctx, int material, float* out, 
  * w12: refracted incident direction (xyz) and T12 (w). */
 orf_status orf_probes_read(orf_context ctx, int64_t* count, double* area, float* cell, float* rmin, float* pos,
                            float* nrm, float* L, float* w12);
-/* Run the probe pass alone for sample index `frame`. */
+/* Run the probe pass alone for sample index `frame`: replaces the reference's
+ * per-frame launch of the sample_camera ray-generation program
+ * (ORF/OptixSceneLoader.cpp:152-153, CU/sample_camera.cu:48-268). */
 orf_status orf_probe_pass(orf_context ctx, uint32_t frame, void* stream);
 
 /* Procedural scene pieces (host).  Icosphere: call with verts = tris = normals =
