@@ -243,7 +243,11 @@ orf_status orf_diff_adjoint(orf_context ctx, const orf_diff_desc* desc, const or
                             float* g_albedo, float* g_roughness, uint32_t frame_shift, void* stream);
 
 /* ------------------------------------------------------------------------
- * Translucent materials ("subsurface_scattering_shader.cu").
+ * Translucent materials ("subsurface_scattering_shader.cu", smooth boundary;
+ * "rough_subsurface_scattering_shader.cu", microfacet boundary with the
+ * material variables roughness, microfacet_model and normal_distribution as
+ * RoughTranslucentMaterial sets them, ORF/RoughTranslucentMaterial.cpp:14-28;
+ * defaults Beckmann, visible normals, roughness 0.1).
  *
  * Material variables absorption, scattering, meancosine (3 floats, 1/mm), scale,
  * ior, dipole_model (0 standard, 1 directional) as ScatteringMaterial /
@@ -263,9 +267,13 @@ orf_status orf_diff_adjoint(orf_context ctx, const orf_diff_desc* desc, const or
 orf_status orf_scattering_properties(orf_context ctx, int material, float* out, int n);
 /* The probes (no reference counterpart).  Arrays of 4 floats per probe (xyz, w);
  * any pointer may be NULL.  L: incident radiance estimate of the last probe pass;
- * w12: refracted incident direction (xyz) and T12 (w). */
+ * w12: refracted incident direction (xyz) and T12 (w; times the microfacet
+ * boundary's transmission weight for "rough_subsurface_scattering_shader.cu");
+ * sn: the normal the light entered through (the probe normal, or the sampled
+ * microfacet normal of a rough boundary; PositionSample::normal after
+ * CU/sample_camera.cu:130-151). */
 orf_status orf_probes_read(orf_context ctx, int64_t* count, double* area, float* cell, float* rmin, float* pos,
-                           float* nrm, float* L, float* w12);
+                           float* nrm, float* L, float* w12, float* sn);
 /* Run the probe pass alone for sample index `frame`: replaces the reference's
  * per-frame launch of the sample_camera ray-generation program
  * (ORF/OptixSceneLoader.cpp:152-153, CU/sample_camera.cu:48-268). */

@@ -55,6 +55,7 @@ static int program_kind(const std::string& prog)
     if (p == "rough_transparent_shader") return MAT_DIELECTRIC;
     if (p == "lambertian_interface_shader") return MAT_INTERFACE;
     if (p == "subsurface_scattering_shader" || p == "translucent_shader") return MAT_TRANSLUCENT;
+    if (p == "rough_subsurface_scattering_shader") return MAT_ROUGH_TRANSLUCENT;
     return -1;
 }
 
@@ -226,7 +227,7 @@ static void build_geometry(orf_context_t* c, hipStream_t)
     c->probe_count = 0;
     if (c->probe_count_req > 0) {
         for (const MeshRec& m : c->meshes)
-            if (m.material >= 0 && m.material < (int)c->mats.size() && c->mats[m.material].kind == MAT_TRANSLUCENT) {
+            if (m.material >= 0 && m.material < (int)c->mats.size() && is_translucent(c->mats[m.material].kind)) {
                 c->probe_sss = m.material;
                 break;
             }
@@ -245,8 +246,10 @@ static void build_geometry(orf_context_t* c, hipStream_t)
             c->d_probe_cells.upload(ps.cells);
             c->d_probe_L.ensure(ps.pos.size() * sizeof(float4));
             c->d_probe_w12.ensure(ps.pos.size() * sizeof(float4));
+            c->d_probe_sn.ensure(ps.pos.size() * sizeof(float4));
             ORF_HIP_CHECK(hipMemset(c->d_probe_L.p, 0, c->d_probe_L.bytes));
             ORF_HIP_CHECK(hipMemset(c->d_probe_w12.p, 0, c->d_probe_w12.bytes));
+            ORF_HIP_CHECK(hipMemset(c->d_probe_sn.p, 0, c->d_probe_sn.bytes));
         } else {
             c->probe_sss = -1;
         }
@@ -300,7 +303,13 @@ void prepare(orf_context_t* c, hipStream_t stream)
         std::vector<ScatteringGPU> sss;
         for (size_t i = 0; i < m.size(); ++i) {
             sss.push_back(scattering_coefficients(c->mat_sss[i]));
-            m[i].sss = m[i].kind == MAT_TRANSLUCENT ? (int)i : -1;
+            m[i].sss = is_translucent(m[i].kind) ? (int)i : -1;
+            const bool ms = m[i].microfacet_model == MULTISCATTERING_MODEL;
+            ORF_REQUIRE(!(ms && m[i].kind == MAT_CONDUCTOR),
+                        "material " + std::to_string(i) + ": the conductor multiple-scattering walk is not built");
+            ORF_REQUIRE(!(ms && m[i].normal_distribution == BECKMANN_DISTRIBUTION &&
+                          (m[i].kind == MAT_DIELECTRIC || m[i].kind == MAT_ROUGH_TRANSLUCENT)),
+                        "material " + std::to_string(i) + ": the Beckmann multiple-scattering walk is not built");
         }
         if (m.empty()) m.push_back(default_material(MAT_LAMBERT));
         if (sss.empty()) sss.push_back(scattering_coefficients(SssParamsHost()));
@@ -344,6 +353,7 @@ SceneGPU scene_view(orf_context_t* c)
         s.probes.nrm = c->d_probe_nrm.as<float4>();
         s.probes.L = c->d_probe_L.as<float4>();
         s.probes.w12 = c->d_probe_w12.as<float4>();
+        s.probes.sn = c->d_probe_sn.as<float4>();
         s.probes.cells = c->d_probe_cells.as<uint2>();
         s.probes.table_mask = c->probe_mask;
         s.probes.cell = c->probe_cell;
@@ -479,7 +489,11 @@ extern "C" orf_status orf_material_create(orf_context ctx, const char* program, 
     if (kind < 0) ORF_THROW(ORF_ERROR_NOT_FOUND, std::string("no radiance program for '") + program + "'");
     c->mats.push_back(orf::default_material(kind));
     c->mat_sss.push_back(orf::SssParamsHost());
-    if (kind == orf::MAT_TRANSLUCENT) c->mats.back().ior = 1.3f;
+    if (orf::is_translucent(kind)) c->mats.back().ior = 1.3f;
+    if (kind == orf::MAT_ROUGH_TRANSLUCENT) {  // ORF/RoughTranslucentMaterial.cpp:14-18
+        c->mats.back().normal_distribution = orf::BECKMANN_DISTRIBUTION;
+        c->mats.back().microfacet_model = orf::VISIBLE_NORMALS_MODEL;
+    }
     c->dirty_mat = true;
     *material = (int)c->mats.size() - 1;
     ORF_API_END
@@ -755,7 +769,7 @@ extern "C" orf_status orf_scattering_properties(orf_context ctx, int material, f
 }
 
 extern "C" orf_status orf_probes_read(orf_context ctx, int64_t* count, double* area, float* cell, float* rmin,
-                                      float* pos, float* nrm, float* L, float* w12)
+                                      float* pos, float* nrm, float* L, float* w12, float* sn)
 {
     ORF_API_BEGIN
     orf_context_t* c = ctx_check(ctx);
@@ -771,6 +785,7 @@ extern "C" orf_status orf_probes_read(orf_context ctx, int64_t* count, double* a
         if (nrm) ORF_HIP_CHECK(hipMemcpy(nrm, c->d_probe_nrm.p, bytes, hipMemcpyDeviceToHost));
         if (L) ORF_HIP_CHECK(hipMemcpy(L, c->d_probe_L.p, bytes, hipMemcpyDeviceToHost));
         if (w12) ORF_HIP_CHECK(hipMemcpy(w12, c->d_probe_w12.p, bytes, hipMemcpyDeviceToHost));
+        if (sn) ORF_HIP_CHECK(hipMemcpy(sn, c->d_probe_sn.p, bytes, hipMemcpyDeviceToHost));
     }
     ORF_API_END
 }

@@ -47,7 +47,7 @@ struct orf_context_t {
     orf::DevBuf d_env, d_mpdf, d_mcdf, d_cpdf, d_ccdf, d_envscratch;
     orf::DevBuf d_film, d_counters;
     // translucent material: coefficients and surface probes
-    orf::DevBuf d_sss, d_probe_pos, d_probe_nrm, d_probe_L, d_probe_w12, d_probe_cells;
+    orf::DevBuf d_sss, d_probe_pos, d_probe_nrm, d_probe_L, d_probe_w12, d_probe_sn, d_probe_cells;
     int probe_count_req = 0;       // context variable probe_count (0: no probes)
     int probe_samples = 4;         // context variable probe_samples (gather samples per hit)
     unsigned probe_seed = 11;      // context variable probe_seed

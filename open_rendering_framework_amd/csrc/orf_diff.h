@@ -126,7 +126,7 @@ ORF_HD f3 diff_path(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32
         bool more = diff_vertex(sc, dp, ps, h, sr, dv);
         if (sr.valid) {
             Hit hs;
-            if (!trace<true>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) ps.L += sr.contrib;
+            if (!trace<true>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) shadow_unoccluded(ps, sr);
         }
         if (!more) break;
     }
@@ -151,7 +151,7 @@ ORF_HD void diff_replay(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, ui
         bool more = diff_vertex(sc, dp, ps, h, sr, dv);
         if (sr.valid) {
             Hit hs;
-            if (!trace<true>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) ps.L += sr.contrib;
+            if (!trace<true>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) shadow_unoccluded(ps, sr);
         }
         if (dv.kind == 1) {
             if (dv.rho.x != 0.0f) sink.albedo(dv.texel, 0, delta.x * rem_before.x / dv.rho.x);

@@ -76,7 +76,7 @@ __device__ __forceinline__ void mega_pixel(const SceneGPU& sc, const CameraGPU& 
                 Hit hs;
                 st.shd++;
                 if (!trace<true>(sc, shadow.o, shadow.d, sc.scene_epsilon, shadow.tmax, hs, stk, kMegaBlock))
-                    ps.L += shadow.contrib;
+                    shadow_unoccluded(ps, shadow);
             }
         }
         if (!more) {

@@ -313,4 +313,93 @@ ORF_HD float masking_g1(f3 v, f3 m, f3 n, float ax, float ay, int dist)
     return dist == 1 ? ggx_masking(v, m, n, ax, ay) : beckmann_masking(v, m, n, ax, ay);
 }
 
+// ---------------------------------------------------------------- Walter et al. 2007 model
+
+// Isotropic masking with roughness a: ggx_G1(v, m, n, a_g) / beckmann_G1(v, m,
+// n, width) (ORF/MicrofacetGGX.h:77-83, ORF/MicrofacetBeckmann.h:78-86), the
+// form the reference's Walter-model branches call (Microfacet.h:16-22).
+ORF_HD float masking_g1_iso(f3 v, f3 m, f3 n, float a, int dist)
+{
+    float cv = dot(v, n);
+    if (dist == 1) {
+        if (cv * dot(v, m) <= 0.0f) return 0.0f;
+        return ggx_smith_g1(cv, a * a);
+    }
+    if (dot(v, m) * cv <= 0.0f) return 0.0f;
+    return beckmann_smith_g1(cv, a * a);
+}
+
+// microfacet_sample_normal (ORF/Microfacet.h:50-60) -> ggx_sample_hemisphere /
+// beckmann_sample_hemisphere (ORF/MicrofacetGGX.h:123-139,
+// ORF/MicrofacetBeckmann.h:144-158): a normal with density D(m) cos(theta_m)
+// about n (tan^2 theta = a^2 z1 / (1 - z1) for GGX, -a^2 log(1 - z1) for
+// Beckmann, phi = 2 pi z2), placed with the create_onb frame, not renormalised.
+ORF_HD f3 sample_normal_walter(f3 n, float z1, float z2, float a, int dist)
+{
+    float phi = 2.0f * kPi * z2;
+    float theta;
+    if (dist == 1) {
+        float tan_theta = a * sqrtf(z1) / sqrtf(1.0f - z1);
+        theta = orf_atanf(tan_theta);
+    } else {
+        float tan_theta_sqr = -a * a * orf_logf(1.0f - z1);
+        theta = orf_atanf(sqrtf(tan_theta_sqr));
+    }
+    float ct = orf_cosf(theta), st = orf_sinf(theta);
+    f3 l = mk3(st * orf_cosf(phi), st * orf_sinf(phi), ct);
+    f3 u, v;
+    make_frame(n, u, v);
+    return u * l.x + v * l.y + n * l.z;
+}
+
+// Walter-model weight f cos / pdf of a direction scattered off the sampled
+// normal m: |i.m| G1(i) G1(o) / (|i.n| |n.m|) (CU/metallic_shader.cu:148,
+// CU/rough_transparent_shader.cu:184-200).
+ORF_HD float walter_weight(f3 w_i, f3 m, f3 ffn, f3 normal, float g_i, float g_o)
+{
+    float abs_i_m = fabsf(dot(w_i, m));
+    float abs_i_n = fabsf(dot(w_i, ffn));
+    float abs_n_m = fabsf(dot(normal, m));
+    return abs_i_m * g_i * g_o / (abs_i_n * abs_n_m);
+}
+
+// ---------------------------------------------------------------- Beckmann evaluation
+
+// beckmann_projected_area (ORF/MicrofacetBeckmann.h:45-58); wi local.
+ORF_HD float beckmann_proj_area(f3 wi, float ax, float ay)
+{
+    if (wi.z > 0.9999f) return 1.0f;
+    if (wi.z < -0.9999f) return 0.0f;
+    float alpha = ggx_alpha_proj(wi, ax, ay);  // beckmann_alpha_i is the same expression
+    float theta = orf_acosf(wi.z);
+    float a = 1.0f / orf_tanf(theta) / alpha;
+    return 0.5f * (orf_erff(a) + 1.0f) * wi.z + 0.25f * k2SqrtPi * alpha * orf_sinf(theta) * orf_expf(-a * a);
+}
+
+// beckmann_slope_pdf / beckmann_eval_NDF (ORF/MicrofacetBeckmann.h:61-65, 249-261).
+ORF_HD float beckmann_ndf(f3 wm, float ax, float ay)
+{
+    if (wm.z <= 0.0f) return 0.0f;
+    float sx = -wm.x / wm.z, sy = -wm.y / wm.z;
+    float p22 = 1.0f / (kPi * ax * ay) * orf_expf(-sx * sx / (ax * ax) - sy * sy / (ay * ay));
+    return p22 / (wm.z * wm.z * wm.z * wm.z);
+}
+
+// beckmann_eval_VNDF (ORF/MicrofacetBeckmann.h:264-281).
+ORF_HD float beckmann_vndf(f3 wi, f3 wm, float ax, float ay)
+{
+    if (wm.z <= 0.0f) return 0.0f;
+    float area = beckmann_proj_area(wi, ax, ay);
+    if (area == 0.0f) return 0.0f;
+    float c = 1.0f / area;
+    return c * fmaxf(0.0f, dot(wi, wm)) * beckmann_ndf(wm, ax, ay);
+}
+
+// microfacet_eval_visible_normal (ORF/Microfacet.h:85-97): D_wi(wm) about n.
+ORF_HD float eval_visible_normal(f3 wi, f3 wm, f3 n, float ax, float ay, int dist)
+{
+    f3 li = to_local(wi, n), lm = to_local(wm, n);
+    return dist == 1 ? ggx_vndf(li, lm, ax, ay) : beckmann_vndf(li, lm, ax, ay);
+}
+
 }  // namespace orf

@@ -40,8 +40,16 @@ struct ShadowRay {
     f3 o, d;
     float tmax;
     f3 contrib;  // added to L when the segment is unoccluded
+    int emit;    // >= 0: the path's emit flag becomes this when unoccluded
     bool valid;
 };
+
+// Applies an unoccluded shadow segment to the path.
+ORF_HD void shadow_unoccluded(PathState& ps, const ShadowRay& sr)
+{
+    ps.L += sr.contrib;
+    if (sr.emit >= 0) ps.emit = sr.emit;
+}
 
 // Camera ray of pixel (x, y) of a w x h film for sample `frame`
 // (CU/path_tracer.cu:45-55); returns the seeded TEA state.
@@ -126,6 +134,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
 {
     constexpr bool kDiel = (kFeat & kFeatDielectric) != 0, kSss = (kFeat & kFeatTranslucent) != 0;
     sr.valid = false;
+    sr.emit = -1;
     const float eps = sc.scene_epsilon;
     f3 d = ps.d;
     f3 p = g.p, normal = g.normal, ffn = g.ffn;
@@ -294,6 +303,122 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         ps.prev_bsdf_pdf = -1.0f;
         return true;
     }
+    if (kSss && mat.kind == MAT_ROUGH_TRANSLUCENT) {
+        // CU/rough_subsurface_scattering_shader.cu:64-455: a microfacet boundary
+        // (Walter, visible-normal or multiple-scattering model) over the
+        // diffusion term.  NEE from outside through the boundary's reflection
+        // (the reference's weight G_i G_o D_wi / (4 |i.n| |l.n|) times Fresnel,
+        // or a multiple-scattering BSDF estimate); the continuation is the
+        // Fresnel choice between refraction (gathering the diffusion term when
+        // entering) and reflection; inside, Beer extinction with roulette.
+        if (ps.depth > sc.max_depth) return false;
+        const ScatteringGPU& P = sc.sss[mat.sss];
+        const f3 w_i = -d;
+        const float ax = mat.roughness.x, ay = mat.roughness.y;
+        const int dist = mat.normal_distribution, model = mat.microfacet_model;
+        float recip = 1.0f / P.relative_ior;
+        const float reflect_xi = rnd_tea(ps.rng);
+        const float z1 = rnd_tea(ps.rng), z2 = rnd_tea(ps.rng);
+        // the reference passes (a_x, z1, z2) to microfacet_sample_normal's
+        // (z1, z2, alpha) parameters (:96); kept as written
+        f3 mn = model == WALTER_MODEL ? sample_normal_walter(ffn, ax, z1, z2, dist)
+                                      : sample_visible_normal(w_i, ffn, ax, ay, z1, z2, dist);
+        const float cos_i = dot(w_i, mn);
+        const bool inside = dot(w_i, normal) < 0.0f;
+        // NEE over the reference's lights (the env, a light here, is not one there)
+        const int nreal = sc.nlights - (sc.env.sampled ? 1 : 0);
+        unsigned li = (unsigned)((float)nreal * rnd_tea(ps.rng));
+        bool nee_reflect = false;
+        if (nreal > 0) {
+            if (li >= (unsigned)nreal) li = nreal - 1;
+            const float light_pdf = 1.0f / (float)nreal;
+            f3 wl = mk3(0.0f), rad = mk3(0.0f);
+            float ldist = 0.0f, penv;
+            sample_light(sc, sc.lights[li], p, ps.rng, wl, rad, ldist, penv);
+            rad = rad / light_pdf;
+            const float cl = dot(ffn, wl);
+            if (!inside && cl > 0.0f) {
+                f3 c;
+                if (model == MULTISCATTERING_MODEL) {
+                    float f = ggx_ms_dielectric_eval(to_local(w_i, ffn), to_local(wl, ffn), recip, ax, ay, ps.rng);
+                    c = rad * f;
+                } else {
+                    f3 rn = normalize(w_i + wl);
+                    float cr = dot(rn, wl);
+                    float Rr = 1.0f;
+                    float st2 = recip * recip * (1.0f - cr * cr);
+                    if (st2 < 1.0f) Rr = fresnel_R3(cr, sqrtf(1.0f - st2), recip);
+                    float gi = masking_g1(w_i, rn, ffn, ax, ay, dist);
+                    float go = masking_g1(wl, rn, ffn, ax, ay, dist);
+                    float D = eval_visible_normal(w_i, rn, ffn, ax, ay, dist);
+                    float wr = gi * go * D / (4.0f * fabsf(dot(w_i, ffn) * dot(wl, ffn)));
+                    c = rad * Rr * wr;
+                    nee_reflect = true;
+                }
+                sr.o = p;
+                sr.d = wl;
+                sr.tmax = ldist;  // :130 (no epsilon)
+                sr.contrib = ps.thr * c;
+                sr.valid = true;
+            }
+        }
+        f3 beam = mk3(1.0f);
+        f3 nd, weight;
+        bool refract;
+        if (model == MULTISCATTERING_MODEL) {
+            // :360-455: the walk from the un-flipped normal's frame, weight 1
+            f3 lo;
+            if (!ggx_ms_dielectric_sample(to_local(w_i, normal), 1.0f / P.relative_ior, ax, ay, ps.rng, lo))
+                return false;
+            nd = to_world(lo, normal);
+            refract = dot(nd, ffn) < 0.0f;
+            weight = mk3(1.0f);
+        }
+        if (inside) {
+            f3 ext = P.dipole_model == 1 ? P.deltaEddExtinction : P.extinction;
+            beam = exp3((-h.t) * ext);
+            float prob = (beam.x + beam.y + beam.z) / 3.0f;
+            if (rnd_tea(ps.rng) >= prob) return false;
+            beam = beam / prob;
+            recip = P.relative_ior;
+        }
+        const f3 no = inside ? -normal : normal;
+        if (model != MULTISCATTERING_MODEL) {
+            float st2 = recip * recip * (1.0f - cos_i * cos_i);
+            float cos_t = 1.0f, R = 1.0f;
+            if (st2 < 1.0f) {
+                cos_t = sqrtf(1.0f - st2);
+                R = fresnel_R3(cos_i, cos_t, recip);
+            }
+            refract = reflect_xi >= R;
+            nd = refract ? recip * (cos_i * mn - w_i) - mn * cos_t : 2.0f * cos_i * mn - w_i;
+            float w;
+            if (model == WALTER_MODEL) {
+                float gi = masking_g1_iso(w_i, mn, ffn, ax, dist);
+                float go = masking_g1_iso(nd, mn, ffn, ax, dist);
+                w = fabsf(dot(w_i, mn)) * gi * go / (fabsf(dot(w_i, ffn)) * fabsf(dot(ffn, mn)));
+            } else {
+                w = masking_g1(nd, mn, ffn, ax, ay, dist);
+            }
+            weight = mk3(w);
+        }
+        ps.thr = ps.thr * beam;
+        if (refract) {
+            if (!inside && mat.sss == sc.probes.sss)
+                ps.L += ps.thr * weight * sss_gather(sc, P, mat.sss, p, no, ps.rng, stk, stride);
+            ps.emit = 1;
+        } else if (model == MULTISCATTERING_MODEL) {
+            ps.emit = 1;
+        } else if (nee_reflect) {
+            sr.emit = 1;  // :165: a visible light sample re-arms emission on the reflection
+        }
+        ps.thr = ps.thr * weight;
+        ps.o = p;
+        ps.d = nd;
+        ps.depth++;
+        ps.prev_bsdf_pdf = -1.0f;
+        return true;
+    }
     return false;  // materials without a radiance program here
 }
 
@@ -316,7 +441,7 @@ ORF_HD f3 path_radiance(const SceneGPU& sc, const int* remap, f3 o, f3 d, uint32
         if (sr.valid) {
             Hit hs;
             st.shd++;
-            if (!trace<true>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) ps.L += sr.contrib;
+            if (!trace<true>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) shadow_unoccluded(ps, sr);
         }
         if (!more) break;
     }

@@ -15,10 +15,11 @@ __global__ __launch_bounds__(kProbeBlock) void k_probe_pass(SceneGPU sc, const S
     __shared__ unsigned stack_lds[kLdsStack * kProbeBlock];
     int j = blockIdx.x * kProbeBlock + threadIdx.x;
     if (j >= sc.probes.count) return;
-    float4 L, w;
-    probe_sample(sc, props[sss], j, frame, stack_lds + threadIdx.x, kProbeBlock, L, w);
+    float4 L, w, n;
+    probe_sample(sc, props[sss], sc.materials[sss], j, frame, stack_lds + threadIdx.x, kProbeBlock, L, w, n);
     sc.probes.L[j] = L;
     sc.probes.w12[j] = w;
+    sc.probes.sn[j] = n;
 }
 
 void launch_probe_pass(const SceneGPU& sc, const ScatteringGPU* d_props, int sss, unsigned frame, hipStream_t stream)

@@ -27,7 +27,8 @@ enum MaterialKind : int {
     MAT_EMITTER = 2,        // arealight_shader.cu
     MAT_DIELECTRIC = 3,     // rough_transparent_shader.cu
     MAT_INTERFACE = 4,      // lambertian_interface_shader.cu
-    MAT_TRANSLUCENT = 5,    // subsurface_scattering_shader.cu / rough_subsurface_scattering_shader.cu
+    MAT_TRANSLUCENT = 5,    // subsurface_scattering_shader.cu (smooth boundary)
+    MAT_ROUGH_TRANSLUCENT = 6,  // rough_subsurface_scattering_shader.cu (microfacet boundary)
     MAT_COUNT
 };
 
@@ -104,7 +105,9 @@ struct ProbesGPU {
     const float4* pos;   // xyz
     const float4* nrm;   // xyz (shading normal at the probe)
     float4* L;           // RGB: incident radiance estimate Le of this launch's probe sample
-    float4* w12;         // xyz: refracted incident direction, w: T12
+    float4* w12;         // xyz: refracted incident direction, w: T12 (times the rough boundary's weight)
+    float4* sn;          // xyz: normal the light entered through (shading normal, or the
+                         //      sampled microfacet normal of a rough boundary)
     const uint2* cells;  // hash slot -> (first probe, count); probes sorted by slot
     unsigned table_mask; // slots - 1 (power of two)
     float cell;          // grid cell size (>= the nearest-probe search radius)
@@ -140,10 +143,13 @@ struct SceneGPU {
 // so simple scenes do not pay the register pressure of unused materials.
 constexpr unsigned kFeatDielectric = 1u, kFeatTranslucent = 2u, kFeatAll = 3u;
 
+inline bool is_translucent(int kind) { return kind == MAT_TRANSLUCENT || kind == MAT_ROUGH_TRANSLUCENT; }
+
 inline unsigned material_features(int kind)
 {
-    return kind == MAT_DIELECTRIC || kind == MAT_INTERFACE ? kFeatDielectric
-                                                           : (kind == MAT_TRANSLUCENT ? kFeatTranslucent : 0u);
+    return kind == MAT_DIELECTRIC || kind == MAT_INTERFACE
+               ? kFeatDielectric
+               : (kind == MAT_TRANSLUCENT || kind == MAT_ROUGH_TRANSLUCENT ? kFeatTranslucent : 0u);
 }
 
 constexpr float kRayMax = 1e27f;  // RT_DEFAULT_MAX

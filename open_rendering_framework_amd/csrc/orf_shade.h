@@ -171,10 +171,26 @@ ORF_HD void ggx_conductor_sample(f3 w_i, f3 n, float ax, float ay, float z1, flo
     weight = (g_i <= 0.0f || g_o <= 0.0f) ? mk3(0.0f) : F * g_o;
 }
 
-// Conductor step for either distribution (CU/metallic_shader.cu:84-158, visible
-// normals): weight F G1(w_o), 0 where the reference stops.
+// Conductor step for either distribution (CU/metallic_shader.cu:84-158): visible
+// normals with weight F G1(w_o), or the Walter model (normal ~ D cos) with weight
+// F |i.m| G1(i) G1(o) / (|i.n| |n.m|); 0 where the reference stops.
 ORF_HD void conductor_step(f3 w_i, f3 n, const MaterialGPU& m, float z1, float z2, f3& w_o, f3& weight)
 {
+    if (m.microfacet_model == WALTER_MODEL) {
+        const int dist = m.normal_distribution;
+        const float a = m.roughness.x;
+        f3 mn = sample_normal_walter(n, z1, z2, a, dist);
+        float g_i = masking_g1_iso(w_i, mn, n, a, dist);
+        w_o = -w_i + (mn * 2.0f) * dot(w_i, mn);
+        if (g_i <= 0.0f) {
+            weight = mk3(0.0f);
+            return;
+        }
+        f3 F = fresnel_conductor(w_i, mn, m.eta);
+        float g_o = masking_g1_iso(w_o, mn, n, a, dist);
+        weight = g_o <= 0.0f ? mk3(0.0f) : F * walter_weight(w_i, mn, n, n, g_i, g_o);
+        return;
+    }
     if (m.normal_distribution == GGX_DISTRIBUTION) {
         float pdf;
         ggx_conductor_sample(w_i, n, m.roughness.x, m.roughness.y, z1, z2, m.eta, w_o, pdf, weight);
@@ -194,6 +210,15 @@ ORF_HD float fresnel_R3(float c1, float c2, float eta)
     float rs = (eta * c1 - c2) / (eta * c1 + c2);
     float rp = (c1 - eta * c2) / (c1 + eta * c2);
     return (rs * rs + rp * rp) * 0.5f;
+}
+
+// fresnel_R(w_i, w_m, ior1/ior2) (ORF/Fresnel.h:42-53).
+ORF_HD float fresnel_R_dir(f3 w_i, f3 w_m, float eta)
+{
+    float c1 = dot(w_i, w_m);
+    float c22 = 1.0f - eta * eta * (1.0f - c1 * c1);
+    if (c22 <= 0.0f) return 1.0f;
+    return fresnel_R3(c1, sqrtf(c22), eta);
 }
 
 // Fresnel reflectance through an interface of relative index eta = n1/n2 at
@@ -271,6 +296,31 @@ ORF_HD f3 ggx_sample_dielectric_phase(f3 w_i, float m_eta, uint32_t& rng, bool& 
     return normalize((-eta) * w_i + w_m * (eta * c - ct));
 }
 
+// ggx_eval_dielectric_phase (ORF/MicrofacetGGX.h:308-339): the dielectric phase
+// function p(wi, wo) of the microsurface, reflection (same side) or refraction.
+ORF_HD float ggx_eval_dielectric_phase(f3 w_i, f3 w_o, bool wi_out, bool wo_out, float m_eta, float ax, float ay)
+{
+    float eta = wi_out ? m_eta : 1.0f / m_eta;
+    if (wi_out == wo_out) {
+        f3 w_h = normalize(w_i + w_o);
+        float D = wi_out ? ggx_vndf(w_i, w_h, ax, ay) : ggx_vndf(-w_i, -w_h, ax, ay);
+        float F = wi_out ? fresnel_R_dir(w_i, w_h, eta) : fresnel_R_dir(-w_i, -w_h, eta);
+        float a = wi_out ? 0.25f / dot(w_i, w_h) : 0.25f / dot(-w_i, -w_h);
+        return F * D * a;
+    }
+    f3 w_h = -normalize(w_i + w_o / eta);
+    w_h = w_h * (wi_out ? w_h.z / fabsf(w_h.z) : -w_h.z / fabsf(w_h.z));
+    if (dot(w_h, w_i) < 0.0f) return 0.0f;
+    float D = wi_out ? ggx_vndf(w_i, w_h, ax, ay) : ggx_vndf(-w_i, -w_h, ax, ay);
+    float F = wi_out ? fresnel_R_dir(w_i, w_h, eta) : fresnel_R_dir(-w_i, -w_h, eta);
+    float b = 1.0f / (eta * eta);
+    float c = wi_out ? fmaxf(0.0f, -dot(w_o, w_h)) : fmaxf(0.0f, -dot(-w_o, -w_h));
+    float e = wi_out ? dot(w_i, w_h) + dot(w_o, w_h) / eta : dot(-w_i, -w_h) + dot(-w_o, -w_h) / eta;
+    float d = e * e;
+    float a = b * c / d;
+    return a * D * (1.0f - F);
+}
+
 // Scattering events allowed per walk before the path is ended (the reference's
 // loop has no bound).
 constexpr int kMsWalkCap = 10;
@@ -299,8 +349,39 @@ ORF_HD bool ggx_ms_dielectric_sample(f3 local_wi, float eta, float ax, float ay,
     return true;
 }
 
+// ggx_multiscattering_dielectric_BSDF_eval (ORF/MicrofacetGGX.h:444-478): one
+// random-walk estimate of the multiple-scattering BSDF times cos for (wi, wo),
+// local frame: the sum over the walk's events of phase(wo) x masking of wo at
+// the event's height.  Truncated after kMsWalkCap events, the orders the
+// sampler keeps (the reference's loop has no bound).
+ORF_HD float ggx_ms_dielectric_eval(f3 w_i, f3 w_o, float eta, float ax, float ay, uint32_t& rng)
+{
+    f3 w_r = -w_i;
+    float hr = 1.0f + height_inv_cdf(0.9999f);
+    int order = 0;
+    float sum = 0.0f;
+    bool wr_out = (-w_r.z) > 0.0f;
+    const bool wo_out = w_o.z > 0.0f;
+    for (;;) {
+        float z = rnd_tea(rng);
+        hr = wr_out ? ggx_sample_height(w_r, hr, z, ax, ay) : -ggx_sample_height(-w_r, -hr, z, ax, ay);
+        if (hr == kRayMax || hr == -kRayMax || __builtin_isinf(hr)) break;
+        ++order;
+        float phase = ggx_eval_dielectric_phase(-w_r, w_o, wr_out, wo_out, eta, ax, ay);
+        float shadowing = wo_out ? ggx_masking_height(w_o, hr, ax, ay) : ggx_masking_height(-w_o, -hr, ax, ay);
+        sum += phase * shadowing;
+        w_r = ggx_sample_dielectric_phase(-w_r, eta, rng, wr_out, ax, ay);
+        if (hr != hr || w_r.z != w_r.z) {
+            sum = 0.0f;
+            break;
+        }
+        if (order >= kMsWalkCap) break;
+    }
+    return sum;
+}
+
 // One step of the rough dielectric (CU/rough_transparent_shader.cu:66-225,
-// visible-normal model): Beer absorption with Russian roulette when leaving the
+// visible-normal, Walter or multiple-scattering model): Beer absorption with Russian roulette when leaving the
 // medium, a visible microfacet normal, Fresnel-weighted choice between
 // reflection and refraction (get_rough_glass_rays, :39-55), weight G1(w_o).
 // Returns false where the reference returns 0; `factor` multiplies the throughput.
@@ -329,9 +410,10 @@ ORF_HD bool dielectric_step(f3 d, float t_hit, f3 normal, f3 ffn, const Material
         factor = beam;
         return true;
     }
-    f3 mn = sample_visible_normal(w_i, ffn, a, a, z1, z2, dist);
+    const bool walter = m.microfacet_model == WALTER_MODEL;
+    f3 mn = walter ? sample_normal_walter(ffn, z1, z2, a, dist) : sample_visible_normal(w_i, ffn, a, a, z1, z2, dist);
     float eta = dot(normal, w_i) < 0.0f ? m.ior : 1.0f / m.ior;
-    float g_i = masking_g1(w_i, mn, ffn, a, a, dist);
+    float g_i = walter ? masking_g1_iso(w_i, mn, ffn, a, dist) : masking_g1(w_i, mn, ffn, a, a, dist);
     if (g_i <= 0.0f) return false;
     float ct = fabsf(dot(mn, -d));
     float st2 = eta * eta * (1.0f - ct * ct);
@@ -339,10 +421,10 @@ ORF_HD bool dielectric_step(f3 d, float t_hit, f3 normal, f3 ffn, const Material
     float R = st2 < 1.0f ? fresnel_R3(ct, ctt, eta) : 1.0f;
     float rr = rnd_tea(rng);
     f3 dir = rr > R ? eta * d + mn * (eta * ct - ctt) : d - (mn * 2.0f) * dot(mn, d);
-    float g_o = masking_g1(dir, mn, ffn, a, a, dist);
+    float g_o = walter ? masking_g1_iso(dir, mn, ffn, a, dist) : masking_g1(dir, mn, ffn, a, a, dist);
     if (g_o <= 0.0f) return false;
     new_dir = dir;
-    factor = beam * g_o;
+    factor = beam * (walter ? walter_weight(w_i, mn, ffn, normal, g_i, g_o) : g_o);
     return true;
 }
 

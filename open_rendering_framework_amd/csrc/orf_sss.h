@@ -132,7 +132,7 @@ ORF_HD int probe_ray_hits(const SceneGPU& sc, int sss, f3 o, f3 dir, float tmax,
                 continue;
             c.prim = (int)f2u(q0.w);
             const MaterialGPU& m = sc.materials[sc.shade[c.prim].w];
-            if (m.kind != MAT_TRANSLUCENT || m.sss != sss) continue;
+            if (m.sss != sss) continue;  // the probes' own translucent material only
             c.leaf = (int)(first + k);
             sss_insert(out, n, c);
             if (n == kSssHits) tcull = orf_u2f(f2u(out[n - 1].t) + 1u);
@@ -195,7 +195,7 @@ ORF_HD f3 sss_gather(const SceneGPU& sc, const ScatteringGPU& P, int sss, f3 xo,
         f3 E = mk3(Lj.x, Lj.y, Lj.z) * wj.w;
         f3 S;
         if (P.dipole_model == 1) {
-            float4 nj = pr.nrm[j];
+            float4 nj = pr.sn[j];
             S = dirpole_bssrdf(xi, mk3(nj.x, nj.y, nj.z), mk3(wj.x, wj.y, wj.z), xo, no, P);
         } else {
             S = dipole_bssrdf(sqrtf(dot(dx, dx)), P);

@@ -121,7 +121,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfView v, int parity, int kind
         if (sr.valid) {
             v.sh_o[s] = make_float4(sr.o.x, sr.o.y, sr.o.z, sr.tmax);
             v.sh_d[s] = make_float4(sr.d.x, sr.d.y, sr.d.z, __uint_as_float((unsigned)slot));
-            v.sh_c[s] = make_float4(sr.contrib.x, sr.contrib.y, sr.contrib.z, 0.0f);
+            v.sh_c[s] = make_float4(sr.contrib.x, sr.contrib.y, sr.contrib.z, (float)sr.emit);
         }
     }
 }
@@ -146,6 +146,11 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfView v)
                 float4 l = v.L[slot];
                 f3 L = mk3(l.x, l.y, l.z) + mk3(c.x, c.y, c.z);
                 v.L[slot] = make_float4(L.x, L.y, L.z, 0.0f);
+                if (c.w >= 0.0f) {  // the segment re-arms (or clears) the path's emission flag
+                    uint4 st = v.st[slot];
+                    st.y = (st.y & 0x7FFFFFFFu) | ((c.w > 0.5f ? 1u : 0u) << 31);
+                    v.st[slot] = st;
+                }
             }
             rays++;
         }

@@ -76,7 +76,7 @@ _SIGS = {
                                ctypes.c_size_t],
     "orf_diff_render": [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp, c_vp, c_vp, c_vp],
     "orf_scattering_properties": [c_vp, ctypes.c_int, c_vp, ctypes.c_int],
-    "orf_probes_read": [c_vp, ctypes.POINTER(c_i64), c_dblp, c_f32p, c_f32p, c_vp, c_vp, c_vp, c_vp],
+    "orf_probes_read": [c_vp, ctypes.POINTER(c_i64), c_dblp, c_f32p, c_f32p, c_vp, c_vp, c_vp, c_vp, c_vp],
     "orf_probe_pass": [c_vp, c_u32, c_vp],
     "orf_diff_adjoint": [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp],
 }
@@ -431,17 +431,17 @@ class Context:
         return d
 
     def probes(self):
-        """Probe set: dict with count, area, cell, rmin and (n, 4) arrays pos, nrm, L, w12."""
+        """Probe set: dict with count, area, cell, rmin and (n, 4) arrays pos, nrm, L, w12, sn."""
         import numpy as np
 
         n, area = c_i64(0), ctypes.c_double()
         cell, rmin = ctypes.c_float(), ctypes.c_float()
         call("orf_probes_read", self.handle, ctypes.byref(n), ctypes.byref(area), ctypes.byref(cell),
-             ctypes.byref(rmin), None, None, None, None)
-        arrs = [np.zeros((n.value, 4), np.float32) for _ in range(4)]
+             ctypes.byref(rmin), None, None, None, None, None)
+        arrs = [np.zeros((n.value, 4), np.float32) for _ in range(5)]
         call("orf_probes_read", self.handle, None, None, None, None, *[_np_ptr(a) for a in arrs])
         return {"count": n.value, "area": area.value, "cell": cell.value, "rmin": rmin.value,
-                "pos": arrs[0], "nrm": arrs[1], "L": arrs[2], "w12": arrs[3]}
+                "pos": arrs[0], "nrm": arrs[1], "L": arrs[2], "w12": arrs[3], "sn": arrs[4]}
 
     def probe_pass(self, frame, stream=None):
         call("orf_probe_pass", self.handle, frame, stream_ptr(stream))

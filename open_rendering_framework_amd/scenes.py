@@ -220,19 +220,27 @@ def batch_workload(n=64, seed=1234, level=7, env_size=(1024, 512), max_depth=8, 
 
 
 def subsurface_scene(width=512, height=512, level=8, glass_level=6, env_size=(1024, 512), max_depth=8,
-                     probes=1 << 20, probe_samples=4, dipole_model=0):
+                     probes=1 << 20, probe_samples=4, dipole_model=0, rough_boundary=None):
     """The subsurface + multiple-scattering scene: the path-tracer scene family with
     a displaced icosphere (radius 0.9, at x = -1) of whole milk (dipole BSSRDF,
     sigma_a = (0.0011, 0.0024, 0.014)/mm, sigma_s' = (2.55, 3.21, 3.77)/mm, eta 1.3,
     scale 100: one scene unit is 100 mm) next to a rough glass sphere (radius 0.7,
     at x = +1.1; GGX alpha 0.6, ior 1.5, microsurface multiple-scattering walk),
-    ground, area light and sky.  `probes` Poisson-disk surface probes on the milk."""
+    ground, area light and sky.  `probes` Poisson-disk surface probes on the milk.
+    rough_boundary = (microfacet_model, normal_distribution, roughness) gives the
+    milk a microfacet boundary (rough_subsurface_scattering_shader.cu)."""
+    milk_floats = {"absorption": (0.0011, 0.0024, 0.014), "scattering": (2.55, 3.21, 3.77),
+                   "meancosine": (0.0, 0.0, 0.0), "scale": (100.0,), "ior": (1.3,)}
+    milk_uints = {"dipole_model": (dipole_model,)}
+    milk_program = "subsurface_scattering_shader.cu"
+    if rough_boundary is not None:
+        model, dist, alpha = rough_boundary
+        milk_program = "rough_subsurface_scattering_shader.cu"
+        milk_floats["roughness"] = (alpha, alpha)
+        milk_uints.update({"microfacet_model": (model,), "normal_distribution": (dist,)})
     mats = [
         Material("diffuse_shader.cu", floats={"diffuse_color": (0.5, 0.5, 0.5)}),
-        Material("subsurface_scattering_shader.cu",
-                 floats={"absorption": (0.0011, 0.0024, 0.014), "scattering": (2.55, 3.21, 3.77),
-                         "meancosine": (0.0, 0.0, 0.0), "scale": (100.0,), "ior": (1.3,)},
-                 uints={"dipole_model": (dipole_model,)}),
+        Material(milk_program, floats=milk_floats, uints=milk_uints),
         Material("arealight_shader.cu", floats={"radiance": (10.0, 10.0, 10.0)}),
         Material("rough_transparent_shader.cu",
                  floats={"roughness": (0.6,), "ior": (1.5,), "glass_absorption": (0.0, 0.0, 0.0)},

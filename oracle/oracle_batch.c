@@ -228,3 +228,56 @@ void orc_ggx_ms_sample_batch(int n, const float* wi, float eta, float alpha, uin
         wo[3 * i + 2] = (float)out.z;
     }
 }
+
+/* One multiple-scattering BSDF x cos estimate per direction pair wo[i] for the
+ * fixed local wi (stream tea16(i, seed)), at most `cap` events. */
+void orc_ggx_ms_eval_batch(int n, const float* wi, const float* wo, float eta, float alpha, uint32_t seed, int cap,
+                           float* out)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        uint32_t t = orc_tea16((uint32_t)i, seed);
+        out[i] = (float)orc_ggx_ms_dielectric_eval(V3(wi[0], wi[1], wi[2]), V3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]),
+                                                   eta, alpha, alpha, &t, cap);
+    }
+}
+
+/* Walter-model microfacet normals about +z (microfacet_sample_normal). */
+void orc_walter_sample_batch(int n, const float* z1, const float* z2, float alpha, int dist, float* m)
+{
+    for (int i = 0; i < n; i++) {
+        v3 r = orc_sample_normal_walter(V3(0, 0, 1), z1[i], z2[i], alpha, dist);
+        m[3 * i] = (float)r.x;
+        m[3 * i + 1] = (float)r.y;
+        m[3 * i + 2] = (float)r.z;
+    }
+}
+
+/* Reflection weight f cos / pdf with F = 1 for incidence wi (local, +z normal)
+ * under the Walter model (normal ~ D cos, weight |i.m| G1(i) G1(o) / (|i.n|
+ * |n.m|)) or visible normals (weight G1(o)); 0 where the shaders stop the path
+ * (G1 <= 0).  Both estimate the same directional albedo. */
+void orc_reflect_weight_batch(int n, const float* wi, const float* z1, const float* z2, float alpha, int dist,
+                              int model, float* out)
+{
+    v3 w_i = V3(wi[0], wi[1], wi[2]), nz = V3(0, 0, 1);
+    for (int i = 0; i < n; i++) {
+        R w = 0;
+        if (model == 0) {
+            v3 m = orc_sample_normal_walter(nz, z1[i], z2[i], alpha, dist);
+            R gi = orc_masking_iso(w_i, m, nz, alpha, dist);
+            v3 wo = vadd(vneg(w_i), vscale(vscale(m, (R)2.0f), vdot(w_i, m)));
+            R go = orc_masking_iso(wo, m, nz, alpha, dist);
+            if (gi > 0 && go > 0) w = fabs(vdot(w_i, m)) * gi * go / (fabs(vdot(w_i, nz)) * fabs(vdot(nz, m)));
+        } else {
+            v3 l = w_i;
+            v3 m = dist == 1 ? orc_ggx_sample_vndf(l, alpha, alpha, z1[i], z2[i])
+                             : orc_beckmann_sample_vndf(l, alpha, alpha, z1[i], z2[i]);
+            R gi = dist == 1 ? orc_ggx_G1(w_i, m, nz, alpha, alpha) : orc_beckmann_G1(w_i, m, nz, alpha, alpha);
+            v3 wo = vadd(vneg(w_i), vscale(vscale(m, (R)2.0f), vdot(w_i, m)));
+            R go = dist == 1 ? orc_ggx_G1(wo, m, nz, alpha, alpha) : orc_beckmann_G1(wo, m, nz, alpha, alpha);
+            if (gi > 0 && go > 0) w = go;
+        }
+        out[i] = (float)w;
+    }
+}

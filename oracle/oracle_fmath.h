@@ -20,6 +20,7 @@
 #define F_TAN(x) tan(x)
 #define F_ACOS(x) acos(x)
 #define F_ATAN2(y, x) atan2(y, x)
+#define F_ATAN(x) atan(x)
 #define F_LOG(x) log(x)
 #define F_EXP(x) exp(x)
 #define F_ERF(x) erf(x)
@@ -31,6 +32,7 @@
 #define F_TAN(x) orf_tanf(x)
 #define F_ACOS(x) orf_acosf(x)
 #define F_ATAN2(y, x) orf_atan2f(y, x)
+#define F_ATAN(x) orf_atanf(x)
 #define F_LOG(x) orf_logf(x)
 #define F_EXP(x) orf_expf(x)
 #define F_ERF(x) orf_erff(x)

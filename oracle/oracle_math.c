@@ -737,6 +737,112 @@ int orc_ggx_ms_dielectric_sample(v3 local_wi, R eta, R ax, R ay, uint32_t* seed,
     return 1;
 }
 
+/* fresnel_R(w_i, w_m, ior1/ior2) (ORF/Fresnel.h:42-53). */
+R orc_fresnel_R_dir(v3 w_i, v3 w_m, R eta)
+{
+    R c1 = vdot(w_i, w_m);
+    R c22 = (R)1.0f - eta * eta * ((R)1.0f - c1 * c1);
+    if (c22 <= 0) return 1;
+    return orc_fresnel_R3(c1, sqrt(c22), eta);
+}
+
+/* ggx_eval_dielectric_phase (ORF/MicrofacetGGX.h:308-339). */
+R orc_ggx_eval_dielectric_phase(v3 w_i, v3 w_o, int wi_out, int wo_out, R m_eta, R ax, R ay)
+{
+    R eta = wi_out ? m_eta : (R)1.0f / m_eta;
+    if (wi_out == wo_out) {
+        v3 w_h = orc_normalize(vadd(w_i, w_o));
+        R D = wi_out ? orc_ggx_eval_vndf(w_i, w_h, ax, ay) : orc_ggx_eval_vndf(vneg(w_i), vneg(w_h), ax, ay);
+        R F = wi_out ? orc_fresnel_R_dir(w_i, w_h, eta) : orc_fresnel_R_dir(vneg(w_i), vneg(w_h), eta);
+        R a = wi_out ? (R)0.25f / vdot(w_i, w_h) : (R)0.25f / vdot(vneg(w_i), vneg(w_h));
+        return F * D * a;
+    }
+    v3 w_h = vneg(orc_normalize(vadd(w_i, V3(w_o.x / eta, w_o.y / eta, w_o.z / eta))));
+    w_h = vscale(w_h, wi_out ? w_h.z / fabs(w_h.z) : -w_h.z / fabs(w_h.z));
+    if (vdot(w_h, w_i) < 0) return 0;
+    R D = wi_out ? orc_ggx_eval_vndf(w_i, w_h, ax, ay) : orc_ggx_eval_vndf(vneg(w_i), vneg(w_h), ax, ay);
+    R F = wi_out ? orc_fresnel_R_dir(w_i, w_h, eta) : orc_fresnel_R_dir(vneg(w_i), vneg(w_h), eta);
+    R b = (R)1.0f / (eta * eta);
+    R cc = wi_out ? -vdot(w_o, w_h) : -vdot(vneg(w_o), vneg(w_h));
+    R c = cc > 0 ? cc : 0;
+    R e = wi_out ? vdot(w_i, w_h) + vdot(w_o, w_h) / eta : vdot(vneg(w_i), vneg(w_h)) + vdot(vneg(w_o), vneg(w_h)) / eta;
+    R d = e * e;
+    R a = b * c / d;
+    return a * D * ((R)1.0f - F);
+}
+
+/* ggx_multiscattering_dielectric_BSDF_eval (ORF/MicrofacetGGX.h:444-478): one
+ * random-walk estimate of BSDF x cos, summed over the walk's events; truncated
+ * after `cap` events (the orders the sampler keeps; the reference's loop has
+ * no bound). */
+R orc_ggx_ms_dielectric_eval(v3 w_i, v3 w_o, R eta, R ax, R ay, uint32_t* seed, int cap)
+{
+    v3 w_r = vneg(w_i);
+    R hr = (R)1.0f + orc_height_inv_cdf((R)0.9999f);
+    int order = 0;
+    R sum = 0;
+    int wr_out = (-w_r.z) > 0;
+    int wo_out = w_o.z > 0;
+    for (;;) {
+        R z = orc_rnd_tea(seed);
+        hr = wr_out ? orc_ggx_sample_height(w_r, hr, z, ax, ay) : -orc_ggx_sample_height(vneg(w_r), -hr, z, ax, ay);
+        if (hr == RAY_MAX_R || hr == -RAY_MAX_R || isinf(hr)) break;
+        order++;
+        R phase = orc_ggx_eval_dielectric_phase(vneg(w_r), w_o, wr_out, wo_out, eta, ax, ay);
+        R shadowing = wo_out ? orc_ggx_masking_height(w_o, hr, ax, ay) : orc_ggx_masking_height(vneg(w_o), -hr, ax, ay);
+        sum += phase * shadowing;
+        w_r = ggx_sample_dielectric_phase(vneg(w_r), eta, seed, &wr_out, ax, ay);
+        if (hr != hr || w_r.z != w_r.z) {
+            sum = 0;
+            break;
+        }
+        if (order >= cap) break;
+    }
+    return sum;
+}
+
+/* masking_G1(v, m, n, alpha, distribution) (ORF/Microfacet.h:16-22): ggx_G1(v,
+ * m, n, a_g) / beckmann_G1(v, m, n, width) (ORF/MicrofacetGGX.h:77-83,
+ * ORF/MicrofacetBeckmann.h:78-86). */
+R orc_masking_iso(v3 v, v3 m, v3 n, R a, int dist)
+{
+    R cv = vdot(v, n);
+    if (dist == 1) {
+        if (cv * vdot(v, m) <= 0) return 0;
+        return orc_ggx_G1_cos(cv, a * a);
+    }
+    if (vdot(v, m) * cv <= 0) return 0;
+    return orc_beckmann_G1_cos(cv, a * a);
+}
+
+/* microfacet_sample_normal (ORF/Microfacet.h:50-60) -> ggx_sample_hemisphere /
+ * beckmann_sample_hemisphere (ORF/MicrofacetGGX.h:123-139,
+ * ORF/MicrofacetBeckmann.h:144-158). */
+v3 orc_sample_normal_walter(v3 n, R z1, R z2, R a, int dist)
+{
+    R phi = (R)2.0f * PI_R * z2;
+    R theta;
+    if (dist == 1) {
+        R tan_theta = a * sqrt(z1) / sqrt((R)1.0f - z1);
+        theta = F_ATAN(tan_theta);
+    } else {
+        R tan_theta_sqr = -a * a * F_LOG((R)1.0f - z1);
+        theta = F_ATAN(sqrt(tan_theta_sqr));
+    }
+    R ct = F_COS(theta), st = F_SIN(theta);
+    v3 l = V3(st * F_COS(phi), st * F_SIN(phi), ct);
+    v3 u, v;
+    orc_create_onb(n, &u, &v);
+    return vadd(vadd(vscale(u, l.x), vscale(v, l.y)), vscale(n, l.z));
+}
+
+/* microfacet_eval_visible_normal (ORF/Microfacet.h:85-97). */
+R orc_eval_visible_normal(v3 wi, v3 wm, v3 n, R ax, R ay, int dist)
+{
+    v3 li = orc_to_local(wi, n), lm = orc_to_local(wm, n);
+    return dist == 1 ? orc_ggx_eval_vndf(li, lm, ax, ay) : orc_beckmann_eval_vndf(li, lm, ax, ay);
+}
+
 /* The shared FP32 transcendentals, exported for their accuracy tests:
  * fn 0 sin, 1 cos, 2 tan, 3 asin, 4 acos, 5 atan, 6 atan2(x, y), 7 log, 8 exp, 9 erf. */
 void orc_fmath_eval(int fn, int64_t n, const float* x, const float* y, float* out)

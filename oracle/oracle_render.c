@@ -90,6 +90,7 @@ typedef struct {
     v3* probe_L;
     v3* probe_w12;
     R* probe_T12;
+    v3* probe_sn; /* normal the probe's light entered through (microfacet normal of a rough boundary) */
 } oscene;
 
 typedef struct {
@@ -643,6 +644,33 @@ static v3 shade_conductor(const oscene* s, const orc_material* m, const cplx3* e
     }
     R z1 = orc_rnd_tea(seed);
     R z2 = orc_rnd_tea(seed);
+    int dist = m->normal_distribution;
+    if (m->microfacet_model == 0) {
+        /* Walter model (CU/metallic_shader.cu:92-148): normal ~ D cos, weight
+         * F |i.m| G1(i) G1(o) / (|i.n| |n.m|) */
+        v3 mw = orc_sample_normal_walter(ffn, z1, z2, ax, dist);
+        R gi = orc_masking_iso(w_i, mw, ffn, ax, dist);
+        if (gi <= 0) return V3(0, 0, 0);
+        v3 Fw = orc_fresnel_conductor(w_i, mw, eta);
+        v3 wo = vadd(vneg(w_i), vscale(vscale(mw, (R)2.0f), vdot(w_i, mw)));
+        R go = orc_masking_iso(wo, mw, ffn, ax, dist);
+        if (go <= 0) return V3(0, 0, 0);
+        v3 cw = radiance(s, hit_point, wo, depth + 1, 1, seed, (R)-1, st);
+        R w = fabs(vdot(w_i, mw)) * gi * go / (fabs(vdot(w_i, ffn)) * fabs(vdot(normal, mw)));
+        return vmul(cw, vscale(Fw, w));
+    }
+    if (dist == 0) {
+        /* Beckmann visible normals (CU/metallic_shader.cu:96-146) */
+        v3 mb = orc_to_world(orc_beckmann_sample_vndf(orc_to_local(w_i, ffn), ax, ay, z1, z2), ffn);
+        R gi = orc_beckmann_G1(w_i, mb, ffn, ax, ay);
+        if (gi <= 0) return V3(0, 0, 0);
+        v3 Fb = orc_fresnel_conductor(w_i, mb, eta);
+        v3 wo = vadd(vneg(w_i), vscale(vscale(mb, (R)2.0f), vdot(w_i, mb)));
+        R go = orc_beckmann_G1(wo, mb, ffn, ax, ay);
+        if (go <= 0) return V3(0, 0, 0);
+        v3 cb = radiance(s, hit_point, wo, depth + 1, 1, seed, (R)-1, st);
+        return vmul(cb, vscale(Fb, go));
+    }
     v3 mn = orc_to_world(orc_ggx_sample_vndf(orc_to_local(w_i, ffn), ax, ay, z1, z2), ffn);
     R G_i = orc_ggx_G1(w_i, mn, ffn, ax, ay);
     if (G_i <= 0) return V3(0, 0, 0);
@@ -662,6 +690,12 @@ static v3 shade_conductor(const oscene* s, const orc_material* m, const cplx3* e
         weight = vscale(F, num / den);
     }
     return vmul(child, weight);
+}
+
+/* masking_G1(v, m, n, a_x, a_y, distribution) (ORF/Microfacet.h:24-32). */
+static R masking_xy(v3 v, v3 m, v3 n, R ax, R ay, int dist)
+{
+    return dist == 1 ? orc_ggx_G1(v, m, n, ax, ay) : orc_beckmann_G1(v, m, n, ax, ay);
 }
 
 /* masking_G1 / microfacet_sample_visible_normal dispatch (ORF/Microfacet.h:24-32, 72-83). */
@@ -712,9 +746,10 @@ static v3 shade_dielectric(const oscene* s, const orc_material* m, v3 o, v3 d, c
         v3 child = radiance(s, hit_point, dir, depth + 1, 1, t, (R)-1, st);
         return vmul(child, beam);
     }
-    v3 mn = sample_visible(w_i, ffn, a, z1, z2, dist);
+    int walter = m->microfacet_model == 0;
+    v3 mn = walter ? orc_sample_normal_walter(ffn, z1, z2, a, dist) : sample_visible(w_i, ffn, a, z1, z2, dist);
     R eta = vdot(normal, w_i) < 0 ? (R)m->ior : (R)1.0f / (R)m->ior;
-    R G_i = masking(w_i, mn, ffn, a, dist);
+    R G_i = walter ? orc_masking_iso(w_i, mn, ffn, a, dist) : masking(w_i, mn, ffn, a, dist);
     if (G_i <= 0) return zero;
     R ct = fabs(vdot(mn, vneg(d)));
     R st2 = eta * eta * ((R)1.0f - ct * ct);
@@ -724,10 +759,12 @@ static v3 shade_dielectric(const oscene* s, const orc_material* m, v3 o, v3 d, c
     v3 refl = vsub(d, vscale(vscale(mn, (R)2.0f), vdot(mn, d)));
     R rr = orc_rnd_tea(t);
     v3 dir = rr > Rf ? refr : refl;
-    R G_o = masking(dir, mn, ffn, a, dist);
+    R G_o = walter ? orc_masking_iso(dir, mn, ffn, a, dist) : masking(dir, mn, ffn, a, dist);
     if (G_o <= 0) return zero;
     v3 child = radiance(s, hit_point, dir, depth + 1, 1, t, (R)-1, st);
-    v3 result = vscale(child, G_o);
+    /* Walter weight |i.m| G1(i) G1(o) / (|i.n| |n.m|) (CU/rough_transparent_shader.cu:184-200) */
+    R w = walter ? fabs(vdot(w_i, mn)) * G_i * G_o / (fabs(vdot(w_i, ffn)) * fabs(vdot(normal, mn))) : G_o;
+    v3 result = vscale(child, w);
     return vmul(result, beam);
 }
 
@@ -1044,7 +1081,7 @@ static v3 sss_gather(const oscene* s, const osss* P, v3 xo, v3 no, uint32_t* t)
         int j = nearest_probe(s, xi);
         if (j < 0) continue;
         v3 E = vscale(s->probe_L[j], s->probe_T12[j]);
-        v3 S = P->dipole_model == 1 ? dirpole_bssrdf(xi, s->probe_nrm[j], s->probe_w12[j], xo, no, P)
+        v3 S = P->dipole_model == 1 ? dirpole_bssrdf(xi, s->probe_sn[j], s->probe_w12[j], xo, no, P)
                                     : dipole_bssrdf(sqrt(vdot(dx, dx)), P);
         v3 es = vmul(E, S);
         acc = vadd(acc, V3(es.x / pdf, es.y / pdf, es.z / pdf));
@@ -1097,6 +1134,128 @@ static v3 shade_translucent(const oscene* s, int mi, v3 o, v3 d, const ohit* h, 
     return vmul(result, beam);
 }
 
+/* CU/rough_subsurface_scattering_shader.cu:64-455: microfacet boundary (Walter,
+ * visible-normal or multiple-scattering model) over the diffusion term.  The
+ * NEE BSDF factor is evaluated before the shadow test (the product traces that
+ * segment after shading), so a multiple-scattering estimate draws its random
+ * numbers whether or not the light is visible. */
+static v3 shade_rough_translucent(const oscene* s, int mi, v3 o, v3 d, const ohit* h, int depth, int emit_light,
+                                  uint32_t* t, ostats* st)
+{
+    v3 zero = V3(0, 0, 0);
+    if (depth > s->max_depth) return zero;
+    const osss* P = &s->sss[mi];
+    const orc_material* M = &s->mats[mi];
+    v3 p = vadd(o, vscale(d, h->t));
+    v3 normal = hit_normal(s, h);
+    v3 ffn = orc_faceforward(normal, vneg(d), normal);
+    v3 w_i = vneg(d);
+    R ax = M->roughness[0], ay = M->roughness[1];
+    int dist = M->normal_distribution, model = M->microfacet_model;
+    R recip = (R)1.0f / P->relative_ior;
+    R reflect_xi = orc_rnd_tea(t);
+    R z1 = orc_rnd_tea(t), z2 = orc_rnd_tea(t);
+    v3 mn;
+    if (model == 0) {
+        mn = orc_sample_normal_walter(ffn, ax, z1, z2, dist); /* the reference's argument order (:96) */
+    } else {
+        v3 l = orc_to_local(w_i, ffn);
+        mn = orc_to_world(dist == 1 ? orc_ggx_sample_vndf(l, ax, ay, z1, z2) : orc_beckmann_sample_vndf(l, ax, ay, z1, z2),
+                          ffn);
+    }
+    R cos_i = vdot(w_i, mn);
+    int inside = vdot(w_i, normal) < 0;
+    /* NEE over the reference's lights (the environment entry is not one of them) */
+    int nreal = s->nlights - (s->env_is ? 1 : 0);
+    uint32_t li = (uint32_t)((R)nreal * orc_rnd_tea(t));
+    v3 result = zero;
+    int refl_emit = emit_light;
+    if (nreal > 0) {
+        if (li >= (uint32_t)nreal) li = (uint32_t)nreal - 1;
+        R light_pdf = (R)1 / (R)nreal;
+        v3 wl, rad;
+        R ldist, penv;
+        direct_light(s, &s->lights[li], p, t, &wl, &rad, &ldist, &penv);
+        rad = V3(rad.x / light_pdf, rad.y / light_pdf, rad.z / light_pdf);
+        R cl = vdot(ffn, wl);
+        if (!inside && cl > 0) {
+            v3 c;
+            if (model == 2) {
+                R f = orc_ggx_ms_dielectric_eval(orc_to_local(w_i, ffn), orc_to_local(wl, ffn), recip, ax, ay, t,
+                                                 MS_WALK_CAP);
+                c = vscale(rad, f);
+            } else {
+                v3 rn = orc_normalize(vadd(w_i, wl));
+                R cr = vdot(rn, wl);
+                R Rr = 1;
+                R sr2 = recip * recip * ((R)1.0f - cr * cr);
+                if (sr2 < (R)1.0f) Rr = orc_fresnel_R3(cr, sqrt((R)1.0f - sr2), recip);
+                R gi = masking_xy(w_i, rn, ffn, ax, ay, dist);
+                R go = masking_xy(wl, rn, ffn, ax, ay, dist);
+                R D = orc_eval_visible_normal(w_i, rn, ffn, ax, ay, dist);
+                R wr = gi * go * D / ((R)4.0f * fabs(vdot(w_i, ffn) * vdot(wl, ffn)));
+                c = vscale(vscale(rad, Rr), wr);
+            }
+            ohit sh;
+            st->shd++;
+            if (!trace(s, p, wl, s->eps, ldist, 1, &sh)) {
+                result = c;
+                if (model != 2) refl_emit = 1; /* :165 */
+            }
+        }
+    }
+    v3 beam = V3(1, 1, 1);
+    v3 nd;
+    R weight = 1;
+    int refract;
+    if (model == 2) {
+        v3 lo;
+        int order;
+        if (!orc_ggx_ms_dielectric_sample(orc_to_local(w_i, normal), (R)1.0f / P->relative_ior, ax, ay, t, MS_WALK_CAP,
+                                          &lo, &order))
+            return result;
+        nd = orc_to_world(lo, normal);
+        refract = vdot(nd, ffn) < 0;
+    }
+    if (inside) {
+        v3 ext = P->dipole_model == 1 ? P->deltaEddExtinction : P->extinction;
+        beam = V3(F_EXP((-h->t) * ext.x), F_EXP((-h->t) * ext.y), F_EXP((-h->t) * ext.z));
+        R prob = (beam.x + beam.y + beam.z) / (R)3.0f;
+        if (orc_rnd_tea(t) >= prob) return result;
+        beam = V3(beam.x / prob, beam.y / prob, beam.z / prob);
+        recip = P->relative_ior;
+    }
+    v3 no = inside ? vneg(normal) : normal;
+    if (model != 2) {
+        R st2 = recip * recip * ((R)1.0f - cos_i * cos_i);
+        R cos_t = 1, Rf = 1;
+        if (st2 < (R)1.0f) {
+            cos_t = sqrt((R)1.0f - st2);
+            Rf = orc_fresnel_R3(cos_i, cos_t, recip);
+        }
+        refract = reflect_xi >= Rf;
+        nd = refract ? vsub(vscale(vsub(vscale(mn, cos_i), w_i), recip), vscale(mn, cos_t))
+                     : vsub(vscale(mn, (R)2.0f * cos_i), w_i);
+        if (model == 0) {
+            R gi = orc_masking_iso(w_i, mn, ffn, ax, dist);
+            R go = orc_masking_iso(nd, mn, ffn, ax, dist);
+            weight = fabs(vdot(w_i, mn)) * gi * go / (fabs(vdot(w_i, ffn)) * fabs(vdot(ffn, mn)));
+        } else {
+            weight = masking_xy(nd, mn, ffn, ax, ay, dist);
+        }
+    }
+    v3 acc = zero;
+    int child_emit = 1;
+    if (refract) {
+        if (!inside && mi == s->probe_mat) acc = vscale(sss_gather(s, P, p, no, t), weight);
+    } else if (model != 2) {
+        child_emit = refl_emit;
+    }
+    v3 child = radiance(s, p, nd, depth + 1, child_emit, t, (R)-1, st);
+    acc = vadd(acc, vscale(child, weight));
+    return vadd(result, vmul(acc, beam));
+}
+
 static v3 radiance(const oscene* s, v3 o, v3 d, int depth, int emit_light, uint32_t* seed, R prev_pdf, ostats* st)
 {
     ohit h;
@@ -1127,6 +1286,8 @@ static v3 shade_hit(const oscene* s, v3 o, v3 d, const ohit* hp, int depth, int 
     if (m->kind == ORC_DIELECTRIC) return shade_dielectric(s, m, o, d, &h, depth, seed, st);
     if (m->kind == ORC_INTERFACE) return shade_interface(s, m, o, d, &h, depth, emit_light, seed, st);
     if (m->kind == ORC_TRANSLUCENT) return shade_translucent(s, s->tri_mat[h.prim], o, d, &h, depth, seed, st);
+    if (m->kind == ORC_ROUGH_TRANSLUCENT)
+        return shade_rough_translucent(s, s->tri_mat[h.prim], o, d, &h, depth, emit_light, seed, st);
     return V3(0, 0, 0);
 }
 
@@ -1248,6 +1409,7 @@ void orc_scene_destroy(void* p)
     free(s->probe_L);
     free(s->probe_w12);
     free(s->probe_T12);
+    free(s->probe_sn);
     free(s);
 }
 
@@ -1439,6 +1601,7 @@ void orc_probes_set(void* p, int n, const float* pos, const float* nrm, float ce
     free(s->probe_L);
     free(s->probe_w12);
     free(s->probe_T12);
+    free(s->probe_sn);
     s->nprobes = n;
     s->probe_cell = (R)cell;
     s->probe_samples = samples;
@@ -1448,9 +1611,11 @@ void orc_probes_set(void* p, int n, const float* pos, const float* nrm, float ce
     s->probe_L = (v3*)calloc((size_t)n + 1, sizeof(v3));
     s->probe_w12 = (v3*)calloc((size_t)n + 1, sizeof(v3));
     s->probe_T12 = (R*)calloc((size_t)n + 1, sizeof(R));
+    s->probe_sn = (v3*)calloc((size_t)n + 1, sizeof(v3));
     for (int j = 0; j < n; j++) {
         s->probe_pos[j] = V3(pos[4 * j], pos[4 * j + 1], pos[4 * j + 2]);
         s->probe_nrm[j] = V3(nrm[4 * j], nrm[4 * j + 1], nrm[4 * j + 2]);
+        s->probe_sn[j] = s->probe_nrm[j];
     }
 }
 
@@ -1478,13 +1643,30 @@ static v3 probe_indirect(const oscene* s, v3 o, v3 d, uint32_t* t, ostats* st)
     return shade_hit(s, o, d, &h, 0, 0, t, (R)-1, st);
 }
 
-/* CU/sample_camera.cu:109-212 for one probe (TEA stream tea16(j, frame)). */
-static void probe_sample(const oscene* s, const osss* P, int j, uint32_t frame, v3* L_out, v3* w12_out, R* T12_out)
+/* Microfacet normal of a rough boundary for a probe lit from wi
+ * (CU/sample_camera.cu:130-151, :173-191; the Walter branch keeps the
+ * reference's (a_x, z1, z2) argument order). */
+static v3 probe_facet(const orc_material* M, v3 wi, v3 n, uint32_t* t)
+{
+    R z1 = orc_rnd_tea(t);
+    R z2 = orc_rnd_tea(t);
+    R ax = M->roughness[0], ay = M->roughness[1];
+    int dist = M->normal_distribution;
+    if (M->microfacet_model == 0) return orc_sample_normal_walter(n, ax, z1, z2, dist);
+    v3 l = orc_to_local(wi, n);
+    v3 m = dist == 1 ? orc_ggx_sample_vndf(l, ax, ay, z1, z2) : orc_beckmann_sample_vndf(l, ax, ay, z1, z2);
+    return orc_to_world(m, n);
+}
+
+/* CU/sample_camera.cu:109-268 for one probe (TEA stream tea16(j, frame)). */
+static void probe_sample(const oscene* s, const osss* P, const orc_material* M, int j, uint32_t frame, v3* L_out,
+                         v3* w12_out, R* T12_out, v3* sn_out)
 {
     v3 pos = s->probe_pos[j], n = s->probe_nrm[j];
+    int rough = M->kind == ORC_ROUGH_TRANSLUCENT;
     uint32_t t = orc_tea16((uint32_t)j, frame);
     R recip = (R)1.0f / P->relative_ior;
-    v3 Le = V3(0, 0, 0), wi = n;
+    v3 Le = V3(0, 0, 0), wi = n, sn = n;
     ostats st = {0, 0};
     if (orc_rnd_tea(&t) < (R)0.5f) {
         uint32_t li = (uint32_t)((R)s->nlights * orc_rnd_tea(&t));
@@ -1495,6 +1677,7 @@ static void probe_sample(const oscene* s, const osss* P, int j, uint32_t frame, 
             R dist, penv;
             direct_light(s, &s->lights[li], pos, &t, &wl, &rad, &dist, &penv);
             wi = wl;
+            if (rough) sn = probe_facet(M, wi, n, &t);
             R cs = fmax(vdot(wl, n), (R)0);
             if (cs > 0) {
                 ohit sh;
@@ -1503,35 +1686,62 @@ static void probe_sample(const oscene* s, const osss* P, int j, uint32_t frame, 
                     Le = V3(a.x / light_pdf, a.y / light_pdf, a.z / light_pdf);
                 }
             }
+        } else if (rough) {
+            sn = probe_facet(M, wi, n, &t);
         }
     } else {
         wi = orc_sample_cosine_weighted(n, &t);
+        if (rough) sn = probe_facet(M, wi, n, &t);
         Le = vscale(probe_indirect(s, pos, wi, &t, &st), PI_R);
     }
     Le = V3(Le.x / (R)0.5f, Le.y / (R)0.5f, Le.z / (R)0.5f);
-    R ci = fmax(vdot(wi, n), (R)0);
+    R ci = fmax(vdot(wi, sn), (R)0);
     R st2 = recip * recip * ((R)1.0f - ci * ci);
     R ct = sqrt((R)1.0f - st2);
-    *w12_out = vsub(vscale(vsub(vscale(n, ci), wi), recip), vscale(n, ct));
-    *T12_out = (R)1.0f - orc_fresnel_R3(ci, ct, recip);
+    v3 w12 = vsub(vscale(vsub(vscale(sn, ci), wi), recip), vscale(sn, ct));
+    R T12 = (R)1.0f - orc_fresnel_R3(ci, ct, recip);
+    R weight = T12;
+    if (rough && M->microfacet_model != 2) {
+        /* CU/sample_camera.cu:240-254: the boundary's transmission weight */
+        R ax = M->roughness[0], ay = M->roughness[1];
+        int dist = M->normal_distribution;
+        R go = masking_xy(w12, sn, n, ax, ay, dist);
+        R w = go;
+        if (M->microfacet_model == 0) {
+            R gi = masking_xy(wi, sn, n, ax, ay, dist);
+            w = fabs(vdot(wi, sn)) * gi / (fabs(vdot(wi, n)) * fabs(vdot(n, sn))) * go;
+        }
+        weight = w * T12;
+    }
+    *w12_out = w12;
+    *T12_out = weight;
+    *sn_out = sn;
     *L_out = Le;
 }
 
-void orc_probe_pass(void* p, uint32_t frame, float* L, float* w12, int threads)
+void orc_probe_pass(void* p, uint32_t frame, float* L, float* w12, float* sn, int threads)
 {
     oscene* s = (oscene*)p;
     if (s->nprobes <= 0 || s->probe_mat < 0) return;
     oscene off = *s;
     off.probe_mat = -1; /* no gathers while the probes are being written */
     const osss* P = &s->sss[s->probe_mat];
+    const orc_material* M = &s->mats[s->probe_mat];
 #pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
     for (int j = 0; j < s->nprobes; j++) {
-        v3 Lj, wj;
+        v3 Lj, wj, nj;
         R Tj;
-        probe_sample(&off, P, j, frame, &Lj, &wj, &Tj);
+        probe_sample(&off, P, M, j, frame, &Lj, &wj, &Tj, &nj);
         s->probe_L[j] = Lj;
         s->probe_w12[j] = wj;
         s->probe_T12[j] = Tj;
+        s->probe_sn[j] = nj;
+        if (sn) {
+            sn[4 * j] = (float)nj.x;
+            sn[4 * j + 1] = (float)nj.y;
+            sn[4 * j + 2] = (float)nj.z;
+            sn[4 * j + 3] = 0.0f;
+        }
         if (L) {
             L[4 * j] = (float)Lj.x;
             L[4 * j + 1] = (float)Lj.y;

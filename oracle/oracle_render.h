@@ -13,7 +13,15 @@
 
 #include "orf_oracle.h"
 
-enum { ORC_LAMBERT = 0, ORC_CONDUCTOR = 1, ORC_EMITTER = 2, ORC_DIELECTRIC = 3, ORC_INTERFACE = 4, ORC_TRANSLUCENT = 5 };
+enum {
+    ORC_LAMBERT = 0,
+    ORC_CONDUCTOR = 1,
+    ORC_EMITTER = 2,
+    ORC_DIELECTRIC = 3,
+    ORC_INTERFACE = 4,
+    ORC_TRANSLUCENT = 5,
+    ORC_ROUGH_TRANSLUCENT = 6
+};
 enum { ORC_POINT = 0, ORC_DIRECTIONAL = 1, ORC_TRIANGLES = 2, ORC_ENV = 5 };
 
 typedef struct {
@@ -89,7 +97,7 @@ void orc_closest_hits(void* s, int n, const float* o, const float* d, float tmin
  * per entering hit; probe pass for sample index `frame` (outputs may be NULL);
  * probe values may also be loaded directly. */
 void orc_probes_set(void* s, int n, const float* pos, const float* nrm, float cell, int samples, int material);
-void orc_probe_pass(void* s, uint32_t frame, float* L, float* w12, int threads);
+void orc_probe_pass(void* s, uint32_t frame, float* L, float* w12, float* sn, int threads);
 void orc_probes_values(void* s, const float* L, const float* w12);
 /* Derived coefficients of material m (64 floats, the product's field order). */
 void orc_sss_coefficients(void* s, int m, float* out);

@@ -103,6 +103,18 @@ R orc_height_inv_cdf(R z);
 R orc_ggx_masking_height(v3 wi, R h, R ax, R ay);
 R orc_ggx_sample_height(v3 wr, R hr, R z, R ax, R ay);
 int orc_ggx_ms_dielectric_sample(v3 local_wi, R eta, R ax, R ay, uint32_t* seed, int cap, v3* w_out, int* order);
+R orc_fresnel_R_dir(v3 w_i, v3 w_m, R eta);
+R orc_ggx_eval_dielectric_phase(v3 w_i, v3 w_o, int wi_out, int wo_out, R m_eta, R ax, R ay);
+R orc_ggx_ms_dielectric_eval(v3 w_i, v3 w_o, R eta, R ax, R ay, uint32_t* seed, int cap);
+/* Walter et al. 2007 model and the visible-normal density (ORF/Microfacet.h). */
+R orc_masking_iso(v3 v, v3 m, v3 n, R a, int dist);
+v3 orc_sample_normal_walter(v3 n, R z1, R z2, R a, int dist);
+R orc_eval_visible_normal(v3 wi, v3 wm, v3 n, R ax, R ay, int dist);
+void orc_ggx_ms_eval_batch(int n, const float* wi, const float* wo, float eta, float alpha, uint32_t seed, int cap,
+                           float* out);
+void orc_walter_sample_batch(int n, const float* z1, const float* z2, float alpha, int dist, float* m);
+void orc_reflect_weight_batch(int n, const float* wi, const float* z1, const float* z2, float alpha, int dist,
+                              int model, float* out);
 
 /* One query of the rough-conductor operator (CU/metallic_shader.cu:84-158,
  * visible-normal branch, GGX, surface normal (0,0,1)).  Returns 1 when the

@@ -49,6 +49,11 @@ class Oracle:
         L.orc_erfinv_batch.argtypes = [ctypes.c_int, c_vp, c_vp]
         L.orc_ggx_ms_sample_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
                                               ctypes.c_int, c_vp, c_vp, c_vp]
+        L.orc_ggx_ms_eval_batch.argtypes = [ctypes.c_int, c_vp, c_vp, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
+                                            ctypes.c_int, c_vp]
+        L.orc_walter_sample_batch.argtypes = [ctypes.c_int, c_vp, c_vp, ctypes.c_float, ctypes.c_int, c_vp]
+        L.orc_reflect_weight_batch.argtypes = [ctypes.c_int, c_vp, c_vp, c_vp, ctypes.c_float, ctypes.c_int,
+                                               ctypes.c_int, c_vp]
 
     # ---- RNG
     def tea(self, v0, v1, rounds=16, key=None):
@@ -192,3 +197,26 @@ class Oracle:
         ok = np.zeros(n, np.int32)
         self.lib.orc_ggx_ms_sample_batch(n, _p(wi), eta, alpha, seed, cap, _p(wo), _p(order), _p(ok))
         return wo, order, ok
+
+    def ggx_ms_eval(self, wi, wo, eta, alpha, seed=1, cap=10):
+        """One multiple-scattering BSDF x cos estimate per row of wo (local frame)."""
+        wi = np.ascontiguousarray(wi, np.float32)
+        wo = np.ascontiguousarray(wo, np.float32)
+        out = np.zeros(len(wo), np.float32)
+        self.lib.orc_ggx_ms_eval_batch(len(wo), _p(wi), _p(wo), eta, alpha, seed, cap, _p(out))
+        return out
+
+    def walter_sample(self, z1, z2, alpha, dist):
+        z1 = np.ascontiguousarray(z1, np.float32)
+        z2 = np.ascontiguousarray(z2, np.float32)
+        m = np.zeros((len(z1), 3), np.float32)
+        self.lib.orc_walter_sample_batch(len(z1), _p(z1), _p(z2), alpha, dist, _p(m))
+        return m
+
+    def reflect_weight(self, wi, z1, z2, alpha, dist, model):
+        wi = np.ascontiguousarray(wi, np.float32)
+        z1 = np.ascontiguousarray(z1, np.float32)
+        z2 = np.ascontiguousarray(z2, np.float32)
+        out = np.zeros(len(z1), np.float32)
+        self.lib.orc_reflect_weight_batch(len(z1), _p(wi), _p(z1), _p(z2), alpha, dist, model, _p(out))
+        return out

@@ -39,7 +39,8 @@ class OrcSceneDesc(ctypes.Structure):
 
 
 _KIND = {"diffuse_shader.cu": 0, "metallic_shader.cu": 1, "arealight_shader.cu": 2, "rough_transparent_shader.cu": 3,
-         "lambertian_interface_shader.cu": 4, "subsurface_scattering_shader.cu": 5}
+         "lambertian_interface_shader.cu": 4, "subsurface_scattering_shader.cu": 5,
+         "rough_subsurface_scattering_shader.cu": 6}
 # Wholemilk (ORF/RoughTranslucentMaterial.cpp:258-263) and the default scale
 SSS_DEFAULTS = {"absorption": (0.0011, 0.0024, 0.014), "scattering": (2.55, 3.21, 3.77),
                 "meancosine": (0.0, 0.0, 0.0), "scale": (100.0,)}
@@ -67,7 +68,7 @@ class OracleScene:
         L.orc_env_bsearch.argtypes = [c_vp, ctypes.c_int, ctypes.c_double]
         L.orc_closest_hits.argtypes = [c_vp, ctypes.c_int, c_vp, c_vp, ctypes.c_float, c_vp, c_vp, ctypes.c_int]
         L.orc_probes_set.argtypes = [c_vp, ctypes.c_int, c_vp, c_vp, ctypes.c_float, ctypes.c_int, ctypes.c_int]
-        L.orc_probe_pass.argtypes = [c_vp, ctypes.c_uint32, c_vp, c_vp, ctypes.c_int]
+        L.orc_probe_pass.argtypes = [c_vp, ctypes.c_uint32, c_vp, c_vp, c_vp, ctypes.c_int]
         L.orc_probes_values.argtypes = [c_vp, c_vp, c_vp]
         L.orc_sss_coefficients.argtypes = [c_vp, ctypes.c_int, c_vp]
         L.orc_dipole_eval.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, c_vp, c_vp]
@@ -87,7 +88,8 @@ class OracleScene:
             mm = mats[i]
             mm.kind = _KIND[m.program]
             mm.microfacet_model = m.uints.get("microfacet_model", (1,))[0]
-            mm.normal_distribution = m.uints.get("normal_distribution", (1,))[0]
+            # RoughTranslucentMaterial defaults to Beckmann (ORF/RoughTranslucentMaterial.cpp:17)
+            mm.normal_distribution = m.uints.get("normal_distribution", (0,) if mm.kind == 6 else (1,))[0]
             for k, v in enumerate(m.floats.get("diffuse_color", (0.8, 0.8, 0.8))):
                 mm.diffuse[k] = v
             for k, v in enumerate(m.floats.get("radiance", (1.0, 1.0, 1.0))):
@@ -97,7 +99,7 @@ class OracleScene:
             r = m.floats.get("roughness", (0.1, 0.1))
             mm.roughness[0] = r[0]
             mm.roughness[1] = r[1] if len(r) > 1 else r[0]
-            mm.ior = m.floats.get("ior", (1.3,) if mm.kind == 5 else (1.5,))[0]
+            mm.ior = m.floats.get("ior", (1.3,) if mm.kind in (5, 6) else (1.5,))[0]
             for key, field in (("absorption", "sss_absorption"), ("scattering", "sss_scattering"),
                                ("meancosine", "sss_meancosine")):
                 getattr(mm, field)[:] = [float(x) for x in m.floats.get(key, SSS_DEFAULTS[key])]
@@ -240,12 +242,14 @@ class OracleScene:
         pos, nrm = self._probe_keep
         self.lib.orc_probes_set(self.handle, len(pos), _p(pos), _p(nrm), probes["cell"], samples, material)
 
-    def probe_pass(self, frame, threads=8):
+    def probe_pass(self, frame, threads=8, normals=False):
+        """(L, w12) of the probe pass for sample index `frame` (+ the entry normals sn)."""
         n = len(self._probe_keep[0])
         L = np.zeros((n, 4), np.float32)
         w = np.zeros((n, 4), np.float32)
-        self.lib.orc_probe_pass(self.handle, frame, _p(L), _p(w), threads)
-        return L, w
+        sn = np.zeros((n, 4), np.float32)
+        self.lib.orc_probe_pass(self.handle, frame, _p(L), _p(w), _p(sn), threads)
+        return (L, w, sn) if normals else (L, w)
 
     def probes_values(self, L, w12):
         self._probe_vals = [np.ascontiguousarray(L, np.float32), np.ascontiguousarray(w12, np.float32)]

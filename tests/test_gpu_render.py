@@ -92,6 +92,20 @@ def _material_variant(kind):
         d.meshes[1].material = len(d.materials) - 1
     elif kind == "lambert_sphere":
         d.meshes[1].material = 0
+    elif kind.startswith("walter_"):
+        # Walter et al. 2007 model: walter_{conductor,glass}_{ggx,beckmann}
+        _, what, dname = kind.split("_")
+        dist = 1 if dname == "ggx" else 0
+        if what == "conductor":
+            m = scenes.Material("metallic_shader.cu", floats={"roughness": (0.3, 0.3)},
+                                uints={"microfacet_model": (0,), "normal_distribution": (dist,)})
+            m.ior_complex = ((0.27, 2.78),) * 3
+        else:
+            m = scenes.Material("rough_transparent_shader.cu",
+                                floats={"roughness": (0.3,), "ior": (1.5,), "glass_absorption": (0.2, 0.1, 0.05)},
+                                uints={"microfacet_model": (0,), "normal_distribution": (dist,)})
+        d.materials.append(m)
+        d.meshes[1].material = len(d.materials) - 1
     return d
 
 
@@ -99,7 +113,9 @@ def _material_variant(kind):
 # the GPU where the reference (and the oracle) use FP64, and its Newton-bisection
 # loop amplifies that difference on a fraction of samples
 @pytest.mark.parametrize("kind,bar", [("interface_sphere", 0.999), ("interface_ground", 0.999),
-                                      ("lambert_sphere", 0.999), ("beckmann_glass", 0.99)])
+                                      ("lambert_sphere", 0.999), ("beckmann_glass", 0.99),
+                                      ("walter_conductor_ggx", 0.999), ("walter_conductor_beckmann", 0.99),
+                                      ("walter_glass_ggx", 0.999), ("walter_glass_beckmann", 0.99)])
 def test_per_sample_parity_materials(kind, bar):
     from tests.oracle_scene import OracleScene
 

@@ -27,9 +27,9 @@ def _say(*a):
     sys.stdout.flush()
 
 
-def _scene(dipole_model=0, probes=8192):
+def _scene(dipole_model=0, probes=8192, rough=None):
     return scenes.subsurface_scene(width=W, height=W, level=5, glass_level=3, env_size=(128, 64), probes=probes,
-                                   probe_samples=4, dipole_model=dipole_model)
+                                   probe_samples=4, dipole_model=dipole_model, rough_boundary=rough)
 
 
 def _ok(got, want):
@@ -85,11 +85,18 @@ def test_probe_set_is_a_poisson_disk_on_the_surface():
     assert p["area"] == pytest.approx(area, rel=1e-4)
 
 
-@pytest.mark.parametrize("model", [0, 1])
-def test_probe_pass_and_render_per_sample(model):
+# rough boundaries: (microfacet_model, normal_distribution, roughness), dipole
+ROUGH = [((1, 0, 0.3), 0), ((1, 1, 0.3), 1), ((0, 1, 0.3), 0), ((0, 0, 0.4), 1), ((2, 1, 0.5), 0)]
+
+
+@pytest.mark.parametrize("model,rough", [(0, None), (1, None)] + [(dm, rb) for rb, dm in ROUGH])
+def test_probe_pass_and_render_per_sample(model, rough):
+    """Smooth boundary for both dipoles, then rough boundaries (visible normals,
+    Walter, multiple scattering).  Beckmann boundaries run the FP32 erfinv where
+    the oracle uses FP64: their bar is 0.99 (probes) and 0.99 (samples)."""
     from tests.oracle_scene import OracleScene
 
-    d = _scene(model)
+    d = _scene(model, rough=rough)
     ctx = scenes.to_context(d)
     ctx.accel_build()
     p = ctx.probes()
@@ -97,15 +104,15 @@ def test_probe_pass_and_render_per_sample(model):
     orc.probes_set(p, 4, MILK)
     ys, xs = np.mgrid[0:W, 0:W]
     for frame in (0, 9):
-        _say("dipole %d frame %d" % (model, frame))
+        _say("dipole %d rough %s frame %d" % (model, rough, frame))
         ctx.film_clear()
         ctx.launch(W, W, frame, 1)  # runs the probe pass for `frame` first
         gp = ctx.probes()
-        oL, ow = orc.probe_pass(frame)
-        okL, okw = _ok(gp["L"], oL), _ok(gp["w12"], ow)
+        oL, ow, osn = orc.probe_pass(frame, normals=True)
+        okL, okw, oks = _ok(gp["L"], oL), _ok(gp["w12"], ow), _ok(gp["sn"], osn)
         lit = (oL[:, :3].sum(1) > 0).mean()
-        _say("  probe L ok %.5f w12 ok %.5f lit %.3f" % (okL.mean(), okw.mean(), lit))
-        assert okL.mean() >= 0.99 and okw.mean() >= 0.99 and lit > 0.2
+        _say("  probe L ok %.5f w12 ok %.5f sn ok %.5f lit %.3f" % (okL.mean(), okw.mean(), oks.mean(), lit))
+        assert okL.mean() >= 0.99 and okw.mean() >= 0.99 and oks.mean() >= 0.99 and lit > 0.2
         got = ctx.film_sums(W, W)[..., :3]
         want = orc.samples(W, W, xs.ravel(), ys.ravel(), frame).reshape(W, W, 3)
         ok = _ok(got, want)
@@ -134,3 +141,36 @@ def test_converged_image_vs_fp64_oracle():
     _say("rel rmse vs fp64 %.3g counters %s / %s" % (rel, c, rc))
     assert c["nonfinite"] == 0
     assert rel < 3e-2
+
+
+def test_rough_boundary_converged_vs_fp64_oracle():
+    from tests.oracle_scene import OracleScene
+
+    d = _scene(0, rough=(1, 1, 0.3))
+    ctx = scenes.to_context(d)
+    ctx.accel_build()
+    spp = 16
+    ctx.film_clear()
+    ctx.launch(W, W, 0, spp)
+    img = ctx.film(W, W)[..., :3]
+    orc = OracleScene(d, fp64=True)
+    orc.probes_set(ctx.probes(), 4, MILK)
+    orc.probe_pass(0)
+    ref, rc = orc.render(W, W, 0, spp)
+    ref = ref[..., :3] / ref[..., 3:4]
+    rel = np.sqrt(((img - ref) ** 2).mean()) / ref.mean()
+    _say("rough: rel rmse vs fp64 %.3g" % rel)
+    assert np.isfinite(img).all() and rel < 3e-2
+
+
+@pytest.mark.parametrize("program,model,dist", [("rough_transparent_shader.cu", 2, 0),
+                                                ("rough_subsurface_scattering_shader.cu", 2, 0),
+                                                ("metallic_shader.cu", 2, 1)])
+def test_unbuilt_walks_are_refused(program, model, dist):
+    from open_rendering_framework_amd import rt
+
+    d = _scene(0, probes=0)
+    d.materials[1] = scenes.Material(program, uints={"microfacet_model": (model,), "normal_distribution": (dist,)})
+    ctx = scenes.to_context(d)
+    with pytest.raises(rt.OrfError, match="not built"):
+        ctx.accel_build()

@@ -18,7 +18,10 @@ the dipoles of ORF/dipoles/, and the importance-sampled gather):
   slab's the probe rays cross the slab obliquely, so a dropped |n_i . n_o|
   factor moves the mean by sqrt(2), far beyond the Monte-Carlo error;
 - the directional dipole is non-negative and rotationally symmetric about the
-  normal at normal incidence (its values are otherwise parity unpinned).
+  normal at normal incidence (its values are otherwise parity unpinned);
+- a rough boundary (rough_subsurface_scattering_shader.cu, visible normals)
+  with a vanishing roughness renders the smooth boundary's image (the NEE
+  lobe and the masking weights go to a delta and 1).
 """
 import math
 
@@ -163,3 +166,29 @@ def test_directional_dipole_symmetry_and_sign():
     assert np.all(S >= 0) and np.all(np.isfinite(S))
     np.testing.assert_allclose(S, np.repeat(S[:, :1], 4, axis=1), rtol=1e-4)
     assert np.all(S[:-1].sum(-1) > S[1:].sum(-1))  # decreasing with distance
+
+
+def test_rough_boundary_tends_to_the_smooth_one():
+    W = 24
+    base = dict(width=W, height=W, level=4, glass_level=2, env_size=(64, 32), probes=2048, probe_samples=4)
+    imgs = []
+    for rb in (None, (1, 1, 1e-4)):
+        d = scenes.subsurface_scene(rough_boundary=rb, **base)
+        d.ctx_uints = {}
+        orc = OracleScene(d)
+        # probes on the milk sphere from its own vertices (the generator is the product's)
+        milk = d.meshes[1]
+        pos = np.zeros((len(milk.verts), 4), np.float32)
+        pos[:, :3] = milk.verts
+        nrm = np.zeros_like(pos)
+        nrm[:, :3] = milk.normals
+        orc.probes_set({"pos": pos, "nrm": nrm, "cell": 0.2}, 4, 1)
+        acc = np.zeros((W, W, 3))
+        for f in range(16):
+            orc.probe_pass(f)
+            img, _ = orc.render(W, W, f, 1)
+            acc += img[..., :3] / np.maximum(img[..., 3:4], 1)
+        imgs.append(acc / 16)
+    a, b = imgs
+    rel = abs(a.mean() - b.mean()) / a.mean()
+    assert rel < 0.015, (a.mean(), b.mean())  # roughness 0.5 differs by 4.7%
