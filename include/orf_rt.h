@@ -286,6 +286,42 @@ orf_status orf_gen_icosphere(int level, uint32_t noise_seed, float amplitude, fl
 orf_status orf_gen_sky_envmap(int width, int height, const float sun_dir[3], float sun_sigma_deg,
                               float sun_peak, float* rgba);
 
+/* ------------------------------------------------------------------------
+ * Scene files.
+ *
+ * orf_scene_load_json replaces OptixSceneLoader::readJSON
+ * (ORF/OptixSceneLoader.cpp:105-177): Integrator (PathTracer), Background
+ * (EnvMapBackground with a Radiance .hdr, ConstantBackground), Camera
+ * (PinholeCamera), Geometries (obj with a material {mtl_name, mtl_parameters})
+ * and Lights (PointLight, DirectionalLight, TrianglesAreaLight) become the
+ * calls above.  Relative paths resolve against the JSON file's directory.  The
+ * reference does not importance-sample its environment; flag
+ * ORF_LOAD_ENV_IMPORTANCE turns that on.  Classes without a program here are
+ * refused with ORF_ERROR_INVALID_VALUE naming them.
+ * ------------------------------------------------------------------------ */
+enum { ORF_LOAD_ENV_IMPORTANCE = 1 };
+
+typedef struct {
+    int32_t width, height;   /* PinholeCamera width / height (0: no camera) */
+    int32_t frame_count;     /* "frame_count" of the file (0 if absent) */
+    int32_t meshes, materials, lights;
+    char buffer_path[512];   /* "buffer_path", resolved; "" if absent */
+} orf_scene_info;
+
+orf_status orf_scene_load_json(orf_context ctx, const char* path, uint32_t flags, orf_scene_info* info);
+
+/* HDRLoader + loadHDRTexture (ORF/HDRLoader.cpp:63-181, 226-289): Radiance RGBE,
+ * -Y +X only; RGBA FP32 with row 0 = the file's bottom row.  rgba may be NULL
+ * (size query). */
+orf_status orf_hdr_read(const char* path, int32_t* width, int32_t* height, float* rgba);
+
+/* glmReadOBJ + ObjLoader::loadVertexData / createGeometryInstances
+ * (ORF/glm.cpp, ORF/ObjLoader.cpp:198-330): v, vn and f (polygons as fans,
+ * negative indices relative); 0-based indices.  has_nindex = 0 when some face
+ * lacks normals.  Array pointers may be NULL (size query). */
+orf_status orf_obj_read(const char* path, int64_t* nverts, int64_t* ntris, int64_t* nnormals, int32_t* has_nindex,
+                        float* verts, int32_t* vindex, float* normals, int32_t* nindex);
+
 #ifdef __cplusplus
 }
 #endif

@@ -79,6 +79,10 @@ _SIGS = {
     "orf_probes_read": [c_vp, ctypes.POINTER(c_i64), c_dblp, c_f32p, c_f32p, c_vp, c_vp, c_vp, c_vp, c_vp],
     "orf_probe_pass": [c_vp, c_u32, c_vp],
     "orf_diff_adjoint": [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp],
+    "orf_scene_load_json": [c_vp, ctypes.c_char_p, c_u32, c_vp],
+    "orf_hdr_read": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), c_vp],
+    "orf_obj_read": [ctypes.c_char_p, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64), ctypes.POINTER(c_i64),
+                     ctypes.POINTER(ctypes.c_int32), c_vp, c_vp, c_vp, c_vp],
 }
 for _name, _args in _SIGS.items():
     _f = getattr(lib, _name)
@@ -174,6 +178,43 @@ def gen_sky_envmap(width, height, sun_dir, sigma_deg, peak):
     return out
 
 
+def hdr_read(path):
+    """Radiance RGBE image as (h, w, 4) FP32, row 0 = the file's bottom row."""
+    import numpy as np
+
+    w, h = ctypes.c_int32(), ctypes.c_int32()
+    p = str(path).encode()
+    call("orf_hdr_read", p, ctypes.byref(w), ctypes.byref(h), None)
+    out = np.zeros((h.value, w.value, 4), np.float32)
+    call("orf_hdr_read", p, ctypes.byref(w), ctypes.byref(h), _np_ptr(out))
+    return out
+
+
+def obj_read(path):
+    """Wavefront OBJ: dict verts (n,3), tris (m,3), normals (k,3), nindex (m,3) or None."""
+    import numpy as np
+
+    nv, nt, nn, hn = c_i64(), c_i64(), c_i64(), ctypes.c_int32()
+    p = str(path).encode()
+    call("orf_obj_read", p, ctypes.byref(nv), ctypes.byref(nt), ctypes.byref(nn), ctypes.byref(hn),
+         None, None, None, None)
+    v = np.zeros((nv.value, 3), np.float32)
+    t = np.zeros((nt.value, 3), np.int32)
+    n = np.zeros((nn.value, 3), np.float32)
+    ni = np.zeros((nt.value, 3), np.int32) if hn.value else None
+    call("orf_obj_read", p, ctypes.byref(nv), ctypes.byref(nt), ctypes.byref(nn), ctypes.byref(hn),
+         _np_ptr(v), _np_ptr(t), _np_ptr(n), _np_ptr(ni))
+    return {"verts": v, "tris": t, "normals": n, "nindex": ni}
+
+
+class SceneInfo(ctypes.Structure):
+    """orf_scene_info."""
+    _fields_ = [("width", ctypes.c_int32), ("height", ctypes.c_int32), ("frame_count", ctypes.c_int32),
+                ("meshes", ctypes.c_int32), ("materials", ctypes.c_int32), ("lights", ctypes.c_int32),
+                ("buffer_path", ctypes.c_char * 512)]
+
+
+LOAD_ENV_IMPORTANCE = 1
 SCOPE_CONTEXT, SCOPE_MATERIAL = 0, 1
 BATCH_WAVEFRONT, BATCH_MEGAKERNEL, BATCH_PERSISTENT = 0, 1, 2
 
@@ -442,6 +483,14 @@ class Context:
         call("orf_probes_read", self.handle, None, None, None, None, *[_np_ptr(a) for a in arrs])
         return {"count": n.value, "area": area.value, "cell": cell.value, "rmin": rmin.value,
                 "pos": arrs[0], "nrm": arrs[1], "L": arrs[2], "w12": arrs[3], "sn": arrs[4]}
+
+    def load_scene(self, path, env_importance=False):
+        """Populate the context from a JSON scene file (orf_scene_load_json); returns its info dict."""
+        info = SceneInfo()
+        call("orf_scene_load_json", self.handle, str(path).encode(), LOAD_ENV_IMPORTANCE if env_importance else 0,
+             ctypes.byref(info))
+        return {"width": info.width, "height": info.height, "frame_count": info.frame_count, "meshes": info.meshes,
+                "materials": info.materials, "lights": info.lights, "buffer_path": info.buffer_path.decode()}
 
     def probe_pass(self, frame, stream=None):
         call("orf_probe_pass", self.handle, frame, stream_ptr(stream))
