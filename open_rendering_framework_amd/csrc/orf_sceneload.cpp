@@ -325,8 +325,10 @@ void scene_load_json(orf_context ctx, const std::string& path, uint32_t flags, o
             if (ok) {
                 try {
                     img = hdr_read(resolve(base, hp));
-                } catch (const std::exception&) {
+                } catch (const Error&) {
                     ok = false;  // loadHDRTexture falls back to the background color (ORF/HDRLoader.cpp:243-259)
+                } catch (const std::exception&) {
+                    ok = false;
                 }
             }
             if (ok)
