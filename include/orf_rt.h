@@ -132,6 +132,20 @@ orf_status orf_light_add_directional(orf_context ctx, const float direction[3], 
  * world space; tri_normals: 9 floats per triangle or NULL. */
 orf_status orf_light_add_triangles(orf_context ctx, const float* tri_verts, const float* tri_normals,
                                    int64_t ntris, const float radiance[3]);
+/* DiskAreaLight (ORF/Light.cpp:367-505): a one-sided disk emitter of the given
+ * radius whose normal is (sin theta sin phi, cos theta, sin theta cos phi),
+ * theta and phi in degrees; visible to rays as the disk intersection of
+ * CU/disk_mesh.cu:41-63 with an arealight material, sampled for NEE by
+ * evaluate_disk_area_light (ORF/LightSampler.h:73-101).  The emitter's
+ * material handle is returned in *emitter_material (may be NULL); batch
+ * remaps must cover it like any mesh material. */
+orf_status orf_light_add_disk(orf_context ctx, const float position[3], float radius, float theta, float phi,
+                              const float radiance[3], int* emitter_material);
+/* SphericalLight (ORF/Light.cpp:507-611): a sphere emitter (CU/sphere_mesh.cu:42-91)
+ * sampled by evaluate_spherical_area_light (ORF/LightSampler.h:103-141). */
+orf_status orf_light_add_sphere(orf_context ctx, const float position[3], float radius, const float radiance[3],
+                                int* emitter_material);
+/* Removes every light, the disk and sphere emitters included. */
 orf_status orf_lights_clear(orf_context ctx);
 
 /* EnvMapBackground (ORF/Background.cpp:57-105): FP32 RGBA lat-long texture,
@@ -216,6 +230,7 @@ orf_status orf_wavefront_autotune(orf_context ctx, const orf_render_item* items,
 /* ------------------------------------------------------------------------
  * Differentiable renders (no counterpart in the reference).
  *
+ * Scenes with disk or sphere lights are refused.
  * A Lambert material's albedo and a GGX conductor's roughness can come from
  * textures (nearest texel of a planar x-z or a spherical mapping of the hit
  * point).  orf_diff_render writes per-view mean images; orf_diff_adjoint runs

@@ -56,6 +56,8 @@ static long long stage_batch(orf_context_t* c, const orf_render_item* items, int
     // geometry material ids must be valid indices into every item's table
     for (const MeshRec& m : c->meshes)
         ORF_REQUIRE(m.material < stride, "render batch: remap_stride smaller than a mesh material id");
+    for (int m : c->analytic_mat)
+        ORF_REQUIRE(m < stride, "render batch: remap_stride smaller than a disk/sphere emitter material id");
     c->batch_renders.clear();
     long long paths = 0, pixels = 0;
     for (int i = 0; i < n; ++i) {

@@ -15,6 +15,7 @@
 
 #include "orf_common.h"
 #include "orf_context.h"
+#include "orf_math.h"
 
 namespace orf {
 
@@ -212,6 +213,9 @@ static void build_geometry(orf_context_t* c, hipStream_t)
             shade.push_back(s);
         }
     }
+    // disk / sphere emitters: shade records after the triangles'
+    c->analytic_prim = (int)shade.size();
+    for (int m : c->analytic_mat) shade.push_back(make_int4(-1, -1, -1, m));
     BvhResult bvh = build_bvh(V.data(), (int64_t)(V.size() / 3), I.data(), (int64_t)(I.size() / 3));
     c->bvh_stats = bvh.stats;
     if (bvh.tris.empty()) bvh.tris.assign(3, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
@@ -222,6 +226,9 @@ static void build_geometry(orf_context_t* c, hipStream_t)
     c->d_tri.upload(bvh.tris);
     c->d_shade.upload(shade);
     c->d_normals.upload(normals);
+    std::vector<float4> an = c->analytic;
+    if (an.empty()) an.assign(2, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+    c->d_analytic.upload(an);
     // probes on the first mesh whose material is translucent
     c->probe_sss = -1;
     c->probe_count = 0;
@@ -333,6 +340,9 @@ SceneGPU scene_view(orf_context_t* c)
     s.lights = c->d_lights.as<LightGPU>();
     s.nlights = c->nlights_dev;
     s.tri_lights = c->d_trilights.as<TriLightGPU>();
+    s.analytic = c->d_analytic.as<float4>();
+    s.nanalytic = (int)(c->analytic.size() / 2);
+    s.analytic_prim = c->analytic_prim;
     s.has_env = c->env_w > 0;
     s.env.tex = c->d_env.as<float4>();
     s.env.w = c->env_w;
@@ -605,12 +615,74 @@ extern "C" orf_status orf_light_add_triangles(orf_context ctx, const float* tv, 
     ORF_API_END
 }
 
+// Emitter material of a disk or sphere light (the arealight material each
+// DiskAreaLight / SphericalLight instance carries, ORF/Light.cpp:476-505, 583-611).
+static int emitter_material(orf_context_t* c, const float radiance[3])
+{
+    c->mats.push_back(orf::default_material(orf::MAT_EMITTER));
+    c->mats.back().radiance = orf::mk3(radiance[0], radiance[1], radiance[2]);
+    c->mat_sss.push_back(orf::SssParamsHost());
+    c->dirty_mat = true;
+    return (int)c->mats.size() - 1;
+}
+
+// A disk or sphere light: the light record, its analytic emitter and the
+// emitter's material.
+static void add_analytic_light(orf_context_t* c, unsigned type, const float position[3], float radius, orf::f3 normal,
+                               const float radiance[3], int* material)
+{
+    const int m = emitter_material(c, radiance);
+    const int idx = (int)(c->analytic.size() / 2);
+    c->analytic.push_back(make_float4(position[0], position[1], position[2], radius));
+    c->analytic.push_back(make_float4(normal.x, normal.y, normal.z, type == orf::DISK_LIGHT ? 0.0f : 1.0f));
+    c->analytic_mat.push_back(m);
+    orf::LightGPU l;
+    std::memset(&l, 0, sizeof(l));
+    l.type = type;
+    l.radiance = orf::mk3(radiance[0], radiance[1], radiance[2]);
+    l.params = orf::mk3(position[0], position[1], position[2]);
+    l.pad = orf::mk3(radius, (float)idx, 0.0f);
+    c->lights.push_back(l);
+    c->dirty_light = true;
+    c->dirty_geo = true;
+    if (material) *material = m;
+}
+
+extern "C" orf_status orf_light_add_disk(orf_context ctx, const float position[3], float radius, float theta,
+                                         float phi, const float radiance[3], int* emitter_material)
+{
+    ORF_API_BEGIN
+    orf_context_t* c = ctx_check(ctx);
+    ORF_REQUIRE(position && radiance, "orf_light_add_disk: null arguments");
+    ORF_REQUIRE(radius > 0.0f && std::isfinite(radius), "orf_light_add_disk: radius must be positive and finite");
+    // DiskAreaLight::computeNormal (ORF/Light.cpp:469-474): theta from +y, phi about y, in degrees
+    const float t = theta * orf::kPi / 180.0f, p = phi * orf::kPi / 180.0f;
+    const orf::f3 n = orf::mk3(orf_sinf(t) * orf_sinf(p), orf_cosf(t), orf_sinf(t) * orf_cosf(p));
+    add_analytic_light(c, orf::DISK_LIGHT, position, radius, n, radiance, emitter_material);
+    ORF_API_END
+}
+
+extern "C" orf_status orf_light_add_sphere(orf_context ctx, const float position[3], float radius,
+                                           const float radiance[3], int* emitter_material)
+{
+    ORF_API_BEGIN
+    orf_context_t* c = ctx_check(ctx);
+    ORF_REQUIRE(position && radiance, "orf_light_add_sphere: null arguments");
+    ORF_REQUIRE(radius > 0.0f && std::isfinite(radius), "orf_light_add_sphere: radius must be positive and finite");
+    add_analytic_light(c, orf::SPHERICAL_LIGHT, position, radius, orf::mk3(0.0f, 0.0f, 0.0f), radiance,
+                       emitter_material);
+    ORF_API_END
+}
+
 extern "C" orf_status orf_lights_clear(orf_context ctx)
 {
     ORF_API_BEGIN
     orf_context_t* c = ctx_check(ctx);
     c->lights.clear();
     c->tri_lights.clear();
+    if (!c->analytic.empty()) c->dirty_geo = true;  // the emitters go with their lights
+    c->analytic.clear();
+    c->analytic_mat.clear();
     c->dirty_light = true;
     ORF_API_END
 }

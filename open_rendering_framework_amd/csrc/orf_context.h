@@ -37,13 +37,16 @@ struct orf_context_t {
     std::vector<orf::MeshRec> meshes;
     std::vector<orf::LightGPU> lights;
     std::vector<orf::TriLightGPU> tri_lights;
+    std::vector<float4> analytic;        // disk / sphere emitters: 2 float4 each
+    std::vector<int> analytic_mat;       // their emitter material handles
+    int analytic_prim = 0;               // shade index of the first emitter
     std::vector<float4> env_tex;
     int env_w = 0, env_h = 0;
     bool env_is = false;
     bool dirty_geo = true, dirty_mat = true, dirty_light = true, dirty_env = true;
     orf::BvhStats bvh_stats;
     // device copies
-    orf::DevBuf d_bvh, d_tri, d_shade, d_normals, d_mats, d_lights, d_trilights;
+    orf::DevBuf d_bvh, d_tri, d_shade, d_normals, d_mats, d_lights, d_trilights, d_analytic;
     orf::DevBuf d_env, d_mpdf, d_mcdf, d_cpdf, d_ccdf, d_envscratch;
     orf::DevBuf d_film, d_counters;
     // translucent material: coefficients and surface probes

@@ -110,6 +110,12 @@ ORF_HD bool diff_vertex(const SceneGPU& sc, const DiffParams& dp, PathState& ps,
     return more;
 }
 
+// The differentiable form takes scenes without disk and sphere emitters (the
+// C-ABI refuses them), so its traversals leave the emitter test out.  With the
+// test compiled in (and never taken), hipcc of ROCm 7.2 produced an adjoint
+// kernel whose roughness gradients were wrong on gfx950 while a host build of
+// the same code was right.
+
 // Pass 1: the sample's radiance (the plain estimator with textured parameters).
 ORF_HD f3 diff_path(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32_t rng, unsigned* stk, int stride)
 {
@@ -117,7 +123,7 @@ ORF_HD f3 diff_path(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32
     path_begin(ps, o, d, rng);
     for (;;) {
         Hit h;
-        if (!trace<false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk, stride)) {
+        if (!trace<false, false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk, stride)) {
             shade_miss(sc, ps);
             break;
         }
@@ -126,7 +132,7 @@ ORF_HD f3 diff_path(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32
         bool more = diff_vertex(sc, dp, ps, h, sr, dv);
         if (sr.valid) {
             Hit hs;
-            if (!trace<true>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) shadow_unoccluded(ps, sr);
+            if (!trace<true, false>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) shadow_unoccluded(ps, sr);
         }
         if (!more) break;
     }
@@ -144,14 +150,14 @@ ORF_HD void diff_replay(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, ui
     path_begin(ps, o, d, rng);
     for (;;) {
         Hit h;
-        if (!trace<false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk, stride)) break;
+        if (!trace<false, false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk, stride)) break;
         f3 rem_before = L_total - ps.L;  // radiance from this vertex on
         ShadowRay sr;
         DiffVertex dv;
         bool more = diff_vertex(sc, dp, ps, h, sr, dv);
         if (sr.valid) {
             Hit hs;
-            if (!trace<true>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) shadow_unoccluded(ps, sr);
+            if (!trace<true, false>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) shadow_unoccluded(ps, sr);
         }
         if (dv.kind == 1) {
             if (dv.rho.x != 0.0f) sink.albedo(dv.texel, 0, delta.x * rem_before.x / dv.rho.x);

@@ -11,6 +11,7 @@ namespace orf {
 static DiffParams diff_params(orf_context_t* c, const orf_diff_desc* d, const float* albedo, const float* rough)
 {
     ORF_REQUIRE(d, "diff: null description");
+    ORF_REQUIRE(c->analytic.empty(), "diff: scenes with disk or sphere lights are not supported");
     ORF_REQUIRE(d->tex_w > 0 && d->tex_h > 0 && d->tex_w <= 8192 && d->tex_h <= 8192, "diff: bad texture size");
     int nm = (int)c->mats.size();
     ORF_REQUIRE(d->albedo_material < nm && d->roughness_material < nm, "diff: material handle out of range");

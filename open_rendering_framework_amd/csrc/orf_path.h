@@ -103,6 +103,13 @@ ORF_HD HitGeom hit_geometry(const SceneGPU& sc, const PathState& ps, const Hit& 
 {
     HitGeom g;
     g.p = ps.o + h.t * ps.d;
+    if (h.leaf < 0) {  // disk or sphere emitter
+        const int i = -1 - h.leaf;
+        g.sh = sc.shade[h.prim];
+        g.normal = analytic_normal(sc.analytic[2 * i], sc.analytic[2 * i + 1], ps.o, ps.d, h.t);
+        g.ffn = faceforward(g.normal, -ps.d, g.normal);
+        return g;
+    }
     const float4* tg = sc.tri + 3 * (size_t)h.leaf;
     float4 q1 = tg[1], q2 = tg[2];
     g.normal = shading_normal(sc, h.prim, h.b, h.g, mk3(q1.x, q1.y, q1.z), mk3(q2.x, q2.y, q2.z), g.sh);

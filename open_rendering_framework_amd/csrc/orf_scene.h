@@ -127,6 +127,13 @@ struct SceneGPU {
     int nlights;              // including the environment entry when sampled
     int root_is_leaf;         // tiny scenes: the root is a single leaf
     const TriLightGPU* tri_lights;
+    // Analytic emitters of the disk and sphere lights (CU/disk_mesh.cu,
+    // CU/sphere_mesh.cu): 2 float4 each, (centre, radius), (disk normal, kind
+    // 0 disk / 1 sphere); tested after the BVH by every traversal.  A hit on
+    // emitter i reports prim = analytic_prim + i (its shade record) and leaf = -1 - i.
+    const float4* analytic;
+    int nanalytic;
+    int analytic_prim;
     EnvGPU env;
     int has_env;
     int max_depth;

@@ -10,7 +10,8 @@
 // parameter names of DiffuseMaterial, MetallicMaterial, LambertianInterface-,
 // RoughTransparent-, Translucent- and RoughTranslucentMaterial::loadFromJSON)
 // and the lights (ORF/Light.cpp: PointLight :22-40, DirectionalLight :75-93,
-// TriangleAreaLight :149-290 with applyTransformationMatrix).  Every scene
+// TriangleAreaLight :149-290 with applyTransformationMatrix, DiskAreaLight
+// :367-505, SphericalLight :507-611).  Every scene
 // element becomes the same C-ABI calls a hand-written host would make.
 //
 // Differences: relative paths resolve against the JSON file's directory (the
@@ -18,8 +19,7 @@
 // take this library's material defaults (the reference's JSON constructors
 // leave them unset); classes with no radiance program here (NormalMaterial,
 // TransparentMaterial, FlatMaterial, RoughDiffuseMaterial, Anisotropic-,
-// DepthTracer, DiskAreaLight, SphericalLight) are refused with an error naming
-// them.
+// DepthTracer) are refused with an error naming them.
 #include <cctype>
 #include <cmath>
 #include <cstdio>
@@ -414,6 +414,22 @@ void scene_load_json(orf_context ctx, const std::string& path, uint32_t flags, o
                 vec(p, "direction", v, 3);
                 float fd[3] = {(float)v[0], (float)v[1], (float)v[2]};
                 check(orf_light_add_directional(ctx, fd, fr));
+            } else if (iequals(type, "DiskAreaLight") || iequals(type, "SphericalLight")) {
+                // DiskAreaLight / SphericalLight::readJSON (ORF/Light.cpp:389-423, 525-548):
+                // defaults radius 1 (disk) / 0.05 (sphere), theta = phi = 0
+                const bool disk = iequals(type, "DiskAreaLight");
+                double radius = disk ? 1.0 : 0.05, theta = 0.0, phi = 0.0;
+                vec(p, "position", v, 3);
+                num(p, "radius", radius);
+                num(p, "theta", theta);
+                num(p, "phi", phi);
+                float fp[3] = {(float)v[0], (float)v[1], (float)v[2]};
+                int m = -1;
+                if (disk)
+                    check(orf_light_add_disk(ctx, fp, (float)radius, (float)theta, (float)phi, fr, &m));
+                else
+                    check(orf_light_add_sphere(ctx, fp, (float)radius, fr, &m));
+                out.materials++;
             } else if (iequals(type, "TrianglesAreaLight")) {
                 ORF_REQUIRE(p && !str(*p, "path").empty(), "scene: a triangle light needs a path");
                 ObjMesh o = obj_read(resolve(base, str(*p, "path")));

@@ -146,6 +146,57 @@ ORF_HD bool sample_light(const SceneGPU& sc, const LightGPU& lt, f3 pos, uint32_
         L = lt.radiance * (tl.area * cp / d2 * (float)ntri);
         return true;
     }
+    if (lt.type == DISK_LIGHT) {
+        // evaluate_disk_area_light (ORF/LightSampler.h:73-101): a uniform point of
+        // the disk (r = sqrt(u1) R, angle 2 pi u2 in the create_onb frame of its
+        // normal), L = Le cos' / d^2 / (1 / area).  params: centre; pad: (radius,
+        // emitter record), whose second float4 holds the normal.
+        const float R = lt.pad.x;
+        const float4 nb = sc.analytic[2 * (int)lt.pad.y + 1];
+        const f3 n = mk3(nb.x, nb.y, nb.z);
+        const float pdf = 1.0f / (kPi * R * R);
+        f3 U, V;
+        make_frame(n, U, V);
+        float u1 = rnd_tea(rng);
+        float u2 = rnd_tea(rng);
+        float r = sqrtf(u1) * R;
+        float t = 2.0f * kPi * u2;
+        f3 dp = mk3(r * orf_cosf(t), r * orf_sinf(t), 0.0f);
+        dp = dp.x * U + dp.y * V + dp.z * n;
+        dir = (lt.params + dp) - pos;
+        float d2 = dot(dir, dir);
+        dist = sqrtf(d2);
+        dir = normalize(dir);
+        float cp = fmaxf(dot(n, -dir), 0.0f);
+        L = lt.radiance * (cp / d2) / pdf;
+        return true;
+    }
+    if (lt.type == SPHERICAL_LIGHT) {
+        // evaluate_spherical_area_light (ORF/LightSampler.h:103-141): a uniform
+        // point of the hemisphere facing pos (z = u1 along the create_onb frame
+        // of normalize(pos - centre)) weighted by the whole sphere's area density
+        // 1 / (4 pi R^2), as the reference does (twice the hemisphere's).
+        const float R = lt.pad.x;
+        const f3 c = lt.params;
+        const f3 vn = normalize(pos - c);
+        f3 U, V;
+        make_frame(vn, U, V);
+        const float pdf = 1.0f / (4.0f * kPi * R * R);
+        float u1 = rnd_tea(rng);
+        float u2 = rnd_tea(rng);
+        float z = u1;
+        float s = sqrtf(1.0f - z * z);
+        f3 sp = mk3(orf_cosf(2.0f * kPi * u2) * s, orf_sinf(2.0f * kPi * u2) * s, z) * R;
+        sp = sp.x * U + sp.y * V + sp.z * vn + c;
+        f3 n = normalize(sp - c);
+        dir = sp - pos;
+        float d2 = dot(dir, dir);
+        dist = sqrtf(d2);
+        dir = normalize(dir);
+        float cp = fmaxf(dot(n, -dir), 0.0f);
+        L = lt.radiance * (cp / d2) / pdf;
+        return true;
+    }
     if (lt.type == ENV_LIGHT) {
         env_sample(sc.env, rng, dir, L, pdf_env);
         dist = kRayMax;

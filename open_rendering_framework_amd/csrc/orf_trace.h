@@ -11,6 +11,8 @@
 // e1 x e2.  A hit is accepted when tmin < t < t_best (strict).
 // Box test: slabs with precomputed 1/d and o/d (explicit fma), far distance
 // inflated by 1 + 2^-21 so rounding never culls a box containing a hit.
+// Disk and sphere emitters are tested one by one after the BVH (a scene has a
+// handful; none in most scenes).
 // Traversal stack: the top kLdsStack entries of a lane live in a column of the
 // workgroup's LDS array (stride = block size), deeper ones (up to kStackDepth)
 // in a private array (the builder bounds the tree depth below kStackDepth).
@@ -45,6 +47,77 @@ ORF_HD bool intersect_tri(f3 o, f3 d, f3 p0, f3 e1, f3 e2, float tmin, float tma
     b = u;
     g = v;
     return true;
+}
+
+// Disk emitter (CU/disk_mesh.cu:41-63): the plane is hit when |n.d| >
+// scene_epsilon, at t = -(o - c).n / (n.d), inside when |p - c|^2 <= r^2.
+ORF_HD bool intersect_disk(f3 o, f3 d, f3 c, f3 n, float r, float eps, float tmin, float tmax, float& t)
+{
+    f3 O = o - c;
+    float den = dot(n, d);
+    if (!(fabsf(den) > eps)) return false;
+    float tt = -dot(O, n) / den;
+    f3 v = (o + tt * d) - c;
+    if (!(dot(v, v) <= r * r)) return false;
+    if (!(tt > tmin && tt < tmax)) return false;
+    t = tt;
+    return true;
+}
+
+// Sphere emitter (CU/sphere_mesh.cu:42-91): near root -b - sqrt(b^2 - c) of the
+// unit-direction quadratic, refined once from the point it gives when it lies
+// beyond 10 radii; the far root is tried when the near one is out of range.
+// As in the reference, a refinement whose discriminant is not positive keeps
+// the refined b with the first sqrt(disc).
+ORF_HD bool intersect_sphere(f3 o, f3 d, f3 c, float r, float tmin, float tmax, float& t)
+{
+    f3 O = o - c;
+    float b = dot(O, d);
+    float cc = dot(O, O) - r * r;
+    float disc = b * b - cc;
+    if (!(disc > 0.0f)) return false;
+    float sdisc = sqrtf(disc);
+    float root1 = -b - sdisc;
+    const bool refine = fabsf(root1) > 10.0f * r;
+    float root11 = 0.0f;
+    if (refine) {
+        f3 O1 = O + root1 * d;
+        b = dot(O1, d);
+        cc = dot(O1, O1) - r * r;
+        disc = b * b - cc;
+        if (disc > 0.0f) {
+            sdisc = sqrtf(disc);
+            root11 = -b - sdisc;
+        }
+    }
+    const float near = root1 + root11;
+    if (near > tmin && near < tmax) {
+        t = near;
+        return true;
+    }
+    const float root2 = (-b + sdisc) + (refine ? root1 : 0.0f);
+    if (root2 > tmin && root2 < tmax) {
+        t = root2;
+        return true;
+    }
+    return false;
+}
+
+// Analytic emitter record (centre, radius), (disk normal, kind).
+ORF_HD bool intersect_analytic(float4 a, float4 b, f3 o, f3 d, float eps, float tmin, float tmax, float& t)
+{
+    const f3 c = mk3(a.x, a.y, a.z);
+    return b.w == 0.0f ? intersect_disk(o, d, c, mk3(b.x, b.y, b.z), a.w, eps, tmin, tmax, t)
+                       : intersect_sphere(o, d, c, a.w, tmin, tmax, t);
+}
+
+// Unit normal of an analytic emitter at o + t d: the disk normal, or the
+// sphere's (o - c + t d) / r (CU/sphere_mesh.cu:73,81), normalised as the
+// emitter's closest hit does.
+ORF_HD f3 analytic_normal(float4 a, float4 b, f3 o, f3 d, float t)
+{
+    if (b.w == 0.0f) return normalize(mk3(b.x, b.y, b.z));
+    return normalize(((o - mk3(a.x, a.y, a.z)) + t * d) * (1.0f / a.w));
 }
 
 struct RayPre {
@@ -98,8 +171,10 @@ ORF_HD unsigned stack_pop(unsigned* stk, int stride, const unsigned* ovf, int& s
 
 // Closest hit (any-hit off) or first hit (any-hit on) in (tmin, tmax).  kMode 0:
 // closest, 1: any, 2: chosen per lane by `any` (one traversal loop serves
-// extension and shadow rays of different lanes at once).
-template <int kMode>
+// extension and shadow rays of different lanes at once).  kEmit: the disk and
+// sphere emitters are tested (kernels for scenes that cannot hold them leave
+// them out).
+template <int kMode, bool kEmit = true>
 ORF_HD bool trace_impl(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, bool any, Hit& h, unsigned* stk,
                        int stride)
 {
@@ -162,6 +237,15 @@ ORF_HD bool trace_impl(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, b
         if (sp == 0) break;
         node = pop();
     }
+    for (int i = 0; kEmit && i < sc.nanalytic; ++i) {
+        float t;
+        if (intersect_analytic(sc.analytic[2 * i], sc.analytic[2 * i + 1], o, d, sc.scene_epsilon, tmin, tbest, t)) {
+            tbest = t;
+            prim = sc.analytic_prim + i;
+            leaf = -1 - i;
+            if (ANY) break;
+        }
+    }
     h.t = tbest;
     h.b = hb;
     h.g = hg;
@@ -170,10 +254,10 @@ ORF_HD bool trace_impl(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, b
     return prim >= 0;
 }
 
-template <bool ANY>
+template <bool ANY, bool kEmit = true>
 ORF_HD bool trace(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, Hit& h, unsigned* stk, int stride)
 {
-    return trace_impl<ANY ? 1 : 0>(sc, o, d, tmin, tmax, ANY, h, stk, stride);
+    return trace_impl<ANY ? 1 : 0, kEmit>(sc, o, d, tmin, tmax, ANY, h, stk, stride);
 }
 
 ORF_HD bool trace_dyn(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, bool any, Hit& h, unsigned* stk,

@@ -54,6 +54,8 @@ _SIGS = {
     "orf_light_add_point": [c_vp, c_vp, c_vp],
     "orf_light_add_directional": [c_vp, c_vp, c_vp],
     "orf_light_add_triangles": [c_vp, c_vp, c_vp, c_i64, c_vp],
+    "orf_light_add_disk": [c_vp, c_vp, ctypes.c_float, ctypes.c_float, ctypes.c_float, c_vp, c_vp],
+    "orf_light_add_sphere": [c_vp, c_vp, ctypes.c_float, c_vp, c_vp],
     "orf_lights_clear": [c_vp],
     "orf_envmap_set": [c_vp, c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int],
     "orf_envmap_tables": [c_vp, c_vp, c_vp, c_vp, c_vp],
@@ -345,6 +347,20 @@ class Context:
         tn = None if tri_normals is None else np.ascontiguousarray(tri_normals, np.float32).reshape(-1, 9)
         call("orf_light_add_triangles", self.handle, _np_ptr(tv), _np_ptr(tn), len(tv),
              (ctypes.c_float * 3)(*radiance))
+
+    def disk_light(self, position, radius, theta, phi, radiance):
+        """DiskAreaLight (normal from theta, phi in degrees); returns the emitter's material handle."""
+        m = ctypes.c_int()
+        call("orf_light_add_disk", self.handle, (ctypes.c_float * 3)(*position), radius, theta, phi,
+             (ctypes.c_float * 3)(*radiance), ctypes.byref(m))
+        return m.value
+
+    def sphere_light(self, position, radius, radiance):
+        """SphericalLight; returns the emitter's material handle."""
+        m = ctypes.c_int()
+        call("orf_light_add_sphere", self.handle, (ctypes.c_float * 3)(*position), radius,
+             (ctypes.c_float * 3)(*radiance), ctypes.byref(m))
+        return m.value
 
     def envmap(self, rgba, importance_sample=True):
         import numpy as np

@@ -45,12 +45,15 @@ class Mesh:
 
 @dataclass
 class Light:
-    kind: str                         # "point" | "directional" | "triangles"
+    kind: str                         # "point" | "directional" | "triangles" | "disk" | "sphere"
     radiance: tuple
-    position: tuple = None
+    position: tuple = None            # point light position; disk / sphere centre
     direction: tuple = None
     tri_verts: np.ndarray = None      # (t, 9) float32
     tri_normals: np.ndarray = None    # (t, 9) float32
+    radius: float = None              # disk / sphere
+    theta: float = 0.0                # disk normal (sin t sin p, cos t, sin t cos p), degrees
+    phi: float = 0.0
 
 
 @dataclass
@@ -322,6 +325,10 @@ def to_context(desc, ctx=None):
             ctx.point_light(l.position, l.radiance)
         elif l.kind == "directional":
             ctx.directional_light(l.direction, l.radiance)
+        elif l.kind == "disk":
+            ctx.disk_light(l.position, l.radius, l.theta, l.phi, l.radiance)
+        elif l.kind == "sphere":
+            ctx.sphere_light(l.position, l.radius, l.radiance)
         else:
             ctx.triangle_light(l.tri_verts, l.tri_normals, l.radiance)
     if desc.env is not None:

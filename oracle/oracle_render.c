@@ -91,6 +91,13 @@ typedef struct {
     v3* probe_w12;
     R* probe_T12;
     v3* probe_sn; /* normal the probe's light entered through (microfacet normal of a rough boundary) */
+    /* disk and sphere emitters (one per such light, in light order); a hit on
+     * emitter i reports prim ntris + i */
+    int32_t nan;
+    v3* an_c;
+    v3* an_n; /* disk normal */
+    R* an_r;
+    int32_t* an_sphere;
 } oscene;
 
 typedef struct {
@@ -267,13 +274,86 @@ static int isect(v3 o, v3 d, v3 p0, v3 e1, v3 e2, R tmin, R tmax, R* t, R* b, R*
 typedef struct {
     R t, b, g;
     int32_t prim;
+    v3 n; /* unit normal of a disk or sphere emitter hit */
 } ohit;
+
+/* Disk emitter (CU/disk_mesh.cu:41-63): plane hit when |n.d| > scene_epsilon, at
+ * t = -(o - c).n / (n.d), kept when |p - c|^2 <= r^2. */
+static int isect_disk(v3 o, v3 d, v3 c, v3 n, R r, R eps, R tmin, R tmax, R* t)
+{
+    v3 O = vsub(o, c);
+    R den = vdot(n, d);
+    if (!(fabs(den) > eps)) return 0;
+    R tt = -vdot(O, n) / den;
+    v3 v = vsub(vadd(o, vscale(d, tt)), c);
+    if (!(vdot(v, v) <= r * r)) return 0;
+    if (!(tt > tmin && tt < tmax)) return 0;
+    *t = tt;
+    return 1;
+}
+
+/* Sphere emitter (CU/sphere_mesh.cu:42-91): near root -b - sqrt(b^2 - c), refined
+ * once from its point when beyond 10 radii (a refinement with disc <= 0 keeps
+ * the refined b and the first sqrt); the far root when the near one is out of
+ * (tmin, tmax). */
+static int isect_sphere(v3 o, v3 d, v3 c, R r, R tmin, R tmax, R* t)
+{
+    v3 O = vsub(o, c);
+    R b = vdot(O, d);
+    R cc = vdot(O, O) - r * r;
+    R disc = b * b - cc;
+    if (!(disc > 0)) return 0;
+    R sdisc = sqrt(disc);
+    R root1 = -b - sdisc;
+    int refine = fabs(root1) > (R)10.0f * r;
+    R root11 = 0;
+    if (refine) {
+        v3 O1 = vadd(O, vscale(d, root1));
+        b = vdot(O1, d);
+        cc = vdot(O1, O1) - r * r;
+        disc = b * b - cc;
+        if (disc > 0) {
+            sdisc = sqrt(disc);
+            root11 = -b - sdisc;
+        }
+    }
+    R nr = root1 + root11;
+    if (nr > tmin && nr < tmax) {
+        *t = nr;
+        return 1;
+    }
+    R root2 = (-b + sdisc) + (refine ? root1 : (R)0);
+    if (root2 > tmin && root2 < tmax) {
+        *t = root2;
+        return 1;
+    }
+    return 0;
+}
+
+/* Emitters after the triangles: closest (or any) hit among them in (tmin, h->t). */
+static int trace_emitters(const oscene* s, v3 o, v3 d, R tmin, int any, ohit* h)
+{
+    for (int32_t i = 0; i < s->nan; i++) {
+        R t;
+        int hit = s->an_sphere[i] ? isect_sphere(o, d, s->an_c[i], s->an_r[i], tmin, h->t, &t)
+                                  : isect_disk(o, d, s->an_c[i], s->an_n[i], s->an_r[i], s->eps, tmin, h->t, &t);
+        if (!hit) continue;
+        h->t = t;
+        h->b = h->g = 0;
+        h->prim = (int32_t)s->ntris + i;
+        /* normalize(shading_normal): the disk normal, or (o - c + t d) / r */
+        h->n = s->an_sphere[i] ? orc_normalize(vscale(vadd(vsub(o, s->an_c[i]), vscale(d, t)), (R)1.0f / s->an_r[i]))
+                               : orc_normalize(s->an_n[i]);
+        if (any) return 1;
+    }
+    return h->prim >= 0;
+}
 
 static int trace(const oscene* s, v3 o, v3 d, R tmin, R tmax, int any, ohit* h)
 {
     h->prim = -1;
     h->t = tmax;
-    if (s->nnodes == 0) return 0;
+    if (s->nnodes == 0) return trace_emitters(s, o, d, tmin, any, h);
     double od[3] = {o.x, o.y, o.z};
     double inv[3] = {1.0 / (double)d.x, 1.0 / (double)d.y, 1.0 / (double)d.z};
     int32_t stack[128];
@@ -300,7 +380,7 @@ static int trace(const oscene* s, v3 o, v3 d, R tmin, R tmax, int any, ohit* h)
         stack[sp++] = b->right;
         stack[sp++] = b->left;
     }
-    return h->prim >= 0;
+    return trace_emitters(s, o, d, tmin, any, h);
 }
 
 static int trace_brute(const oscene* s, v3 o, v3 d, R tmin, R tmax, ohit* h)
@@ -318,7 +398,7 @@ static int trace_brute(const oscene* s, v3 o, v3 d, R tmin, R tmax, ohit* h)
             h->prim = (int32_t)id;
         }
     }
-    return h->prim >= 0;
+    return trace_emitters(s, o, d, tmin, 0, h);
 }
 
 /* ---------------------------------------------------------------- environment */
@@ -517,6 +597,58 @@ static void direct_light(const oscene* s, const orc_light* lt, v3 pos, uint32_t*
         R cp = vdot(n, vneg(*dir));
         if (!(cp >= 0)) cp = 0;
         *L = vscale(rad, (R)tl->area * cp / d2 * (R)ntri);
+    } else if (lt->type == ORC_DISK) {
+        /* evaluate_disk_area_light (ORF/LightSampler.h:73-101) */
+        R radius = lt->shape[0];
+        R th = (R)lt->shape[1] * PI_R / (R)180.0f, ph = (R)lt->shape[2] * PI_R / (R)180.0f;
+        R area = PI_R * radius * radius;
+        R pdf = (R)1.0f / area;
+        R sth = F_SIN(th), cth = F_COS(th), sph = F_SIN(ph), cph = F_COS(ph);
+        v3 n = V3(sth * sph, cth, sth * cph);
+        v3 U, Vv;
+        orc_create_onb(n, &U, &Vv);
+        R u1 = orc_rnd_tea(seed);
+        R u2 = orc_rnd_tea(seed);
+        R su = sqrt(u1);
+        R r = su * radius;
+        R t = (R)2.0f * PI_R * u2;
+        R dx = r * F_COS(t), dy = r * F_SIN(t), dz = 0;
+        v3 dp = vadd(vadd(vscale(U, dx), vscale(Vv, dy)), vscale(n, dz));
+        *dir = vsub(vadd(prm, dp), pos);
+        R d2 = vdot(*dir, *dir);
+        *dist = sqrt(d2);
+        *dir = orc_normalize(*dir);
+        R cp = vdot(n, vneg(*dir));
+        if (!(cp >= 0)) cp = 0;
+        R k = cp / d2;
+        *L = V3(rad.x * k / pdf, rad.y * k / pdf, rad.z * k / pdf);
+    } else if (lt->type == ORC_SPHERE) {
+        /* evaluate_spherical_area_light (ORF/LightSampler.h:103-141): the hemisphere
+         * facing pos, uniform by area (z = u1), divided by the whole sphere's
+         * density 1 / (4 pi R^2) */
+        R radius = lt->shape[0];
+        v3 vn = orc_normalize(vsub(pos, prm));
+        v3 U, Vv;
+        orc_create_onb(vn, &U, &Vv);
+        R area = (R)4.0f * PI_R * radius * radius;
+        R pdf = (R)1.0f / area;
+        R u1 = orc_rnd_tea(seed);
+        R u2 = orc_rnd_tea(seed);
+        R z = u1;
+        R sz = sqrt((R)1.0f - z * z);
+        R a = (R)2.0f * PI_R * u2;
+        R ca = F_COS(a), sa = F_SIN(a);
+        v3 sp = V3(ca * sz * radius, sa * sz * radius, z * radius);
+        sp = vadd(vadd(vadd(vscale(U, sp.x), vscale(Vv, sp.y)), vscale(vn, sp.z)), prm);
+        v3 n = orc_normalize(vsub(sp, prm));
+        *dir = vsub(sp, pos);
+        R d2 = vdot(*dir, *dir);
+        *dist = sqrt(d2);
+        *dir = orc_normalize(*dir);
+        R cp = vdot(n, vneg(*dir));
+        if (!(cp >= 0)) cp = 0;
+        R k = cp / d2;
+        *L = V3(rad.x * k / pdf, rad.y * k / pdf, rad.z * k / pdf);
     } else if (lt->type == ORC_ENV) {
         env_sample(s, seed, dir, L, pdf_env);
         *dist = RAY_MAX;
@@ -544,6 +676,7 @@ static v3 radiance(const oscene* s, v3 o, v3 d, int depth, int emit_light, uint3
 static v3 hit_normal(const oscene* s, const ohit* h)
 {
     int32_t t = h->prim;
+    if (t >= s->ntris) return h->n; /* disk or sphere emitter */
     v3 n;
     if (!s->nidx || s->nidx[3 * t] < 0 || s->nidx[3 * t + 1] < 0 || s->nidx[3 * t + 2] < 0) {
         n = orc_normalize(vcross(s->e1[t], s->e2[t]));
@@ -1335,8 +1468,30 @@ void* orc_scene_create(const orc_scene_desc* d)
         for (int64_t k = 0; k < d->nnormals; k++)
             s->normals[k] = V3(d->normals[3 * k], d->normals[3 * k + 1], d->normals[3 * k + 2]);
     }
-    s->tri_mat = (int32_t*)malloc(sizeof(int32_t) * (size_t)(d->ntris + 1));
+    /* disk and sphere emitters, in light order; their materials follow the
+     * triangles' in tri_mat */
+    int32_t nan = 0;
+    for (int i = 0; i < d->nlights; i++) nan += d->lights[i].type == ORC_DISK || d->lights[i].type == ORC_SPHERE;
+    s->nan = nan;
+    s->an_c = (v3*)malloc(sizeof(v3) * (size_t)(nan + 1));
+    s->an_n = (v3*)malloc(sizeof(v3) * (size_t)(nan + 1));
+    s->an_r = (R*)malloc(sizeof(R) * (size_t)(nan + 1));
+    s->an_sphere = (int32_t*)malloc(sizeof(int32_t) * (size_t)(nan + 1));
+    s->tri_mat = (int32_t*)malloc(sizeof(int32_t) * (size_t)(d->ntris + nan + 1));
     memcpy(s->tri_mat, d->tri_mat, sizeof(int32_t) * (size_t)d->ntris);
+    for (int i = 0, k = 0; i < d->nlights; i++) {
+        const orc_light* l = &d->lights[i];
+        if (l->type != ORC_DISK && l->type != ORC_SPHERE) continue;
+        s->an_c[k] = V3(l->params[0], l->params[1], l->params[2]);
+        s->an_r[k] = l->shape[0];
+        s->an_sphere[k] = l->type == ORC_SPHERE;
+        /* DiskAreaLight::computeNormal (ORF/Light.cpp:469-474) */
+        R th = (R)l->shape[1] * PI_R / (R)180.0f, ph = (R)l->shape[2] * PI_R / (R)180.0f;
+        R sth = F_SIN(th);
+        s->an_n[k] = V3(sth * F_SIN(ph), F_COS(th), sth * F_COS(ph));
+        s->tri_mat[d->ntris + k] = l->material;
+        k++;
+    }
     s->nmats = d->nmats;
     s->mats = (orc_material*)malloc(sizeof(orc_material) * (size_t)(d->nmats + 1));
     memcpy(s->mats, d->mats, sizeof(orc_material) * (size_t)d->nmats);
@@ -1388,6 +1543,10 @@ void orc_scene_destroy(void* p)
     free(s->nidx);
     free(s->normals);
     free(s->tri_mat);
+    free(s->an_c);
+    free(s->an_n);
+    free(s->an_r);
+    free(s->an_sphere);
     free(s->nodes);
     free(s->order);
     free(s->mats);

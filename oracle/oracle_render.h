@@ -22,7 +22,7 @@ enum {
     ORC_TRANSLUCENT = 5,
     ORC_ROUGH_TRANSLUCENT = 6
 };
-enum { ORC_POINT = 0, ORC_DIRECTIONAL = 1, ORC_TRIANGLES = 2, ORC_ENV = 5 };
+enum { ORC_POINT = 0, ORC_DIRECTIONAL = 1, ORC_TRIANGLES = 2, ORC_DISK = 3, ORC_SPHERE = 4, ORC_ENV = 5 };
 
 typedef struct {
     int32_t kind;
@@ -45,7 +45,10 @@ typedef struct {
 typedef struct {
     uint32_t type;
     float radiance[3];
-    float params[3];  /* point: position; directional: direction; triangles: count, start, end */
+    float params[3];  /* point: position; directional: direction; triangles: count, start, end;
+                         disk, sphere: centre */
+    float shape[3];   /* disk: radius, theta, phi (degrees); sphere: radius */
+    int32_t material; /* disk, sphere: material of the emitter the rays see */
 } orc_light;
 
 typedef struct {

@@ -20,7 +20,8 @@ class OrcMaterial(ctypes.Structure):
 
 
 class OrcLight(ctypes.Structure):
-    _fields_ = [("type", ctypes.c_uint32), ("radiance", ctypes.c_float * 3), ("params", ctypes.c_float * 3)]
+    _fields_ = [("type", ctypes.c_uint32), ("radiance", ctypes.c_float * 3), ("params", ctypes.c_float * 3),
+                ("shape", ctypes.c_float * 3), ("material", ctypes.c_int32)]
 
 
 class OrcTriLight(ctypes.Structure):
@@ -83,7 +84,13 @@ class OracleScene:
         V, T, N, NI, M = desc.flat()
         self.arrays = [np.ascontiguousarray(V), np.ascontiguousarray(T), np.ascontiguousarray(N),
                        np.ascontiguousarray(NI), np.ascontiguousarray(M)]
-        mats = (OrcMaterial * max(1, len(desc.materials)))()
+        # disk and sphere lights carry an emitter material each, after the scene's
+        emitters = [l for l in desc.lights if l.kind in ("disk", "sphere")]
+        nm = len(desc.materials)
+        mats = (OrcMaterial * max(1, nm + len(emitters)))()
+        for k, l in enumerate(emitters):
+            mats[nm + k].kind = 2
+            mats[nm + k].radiance[:] = [float(x) for x in l.radiance]
         for i, m in enumerate(desc.materials):
             mm = mats[i]
             mm.kind = _KIND[m.program]
@@ -110,6 +117,7 @@ class OracleScene:
                 mm.ior_re[k] = ior[k][0]
                 mm.ior_im[k] = ior[k][1]
         lights, tls = [], []
+        n_em = 0
         for l in desc.lights:
             ol = OrcLight()
             ol.radiance[:] = list(l.radiance)
@@ -121,6 +129,12 @@ class OracleScene:
                 d = np.asarray(l.direction, np.float64)
                 d = (d / np.linalg.norm(d)).astype(np.float32)
                 ol.params[:] = [float(x) for x in d]
+            elif l.kind in ("disk", "sphere"):
+                ol.type = 3 if l.kind == "disk" else 4
+                ol.params[:] = list(l.position)
+                ol.shape[:] = [l.radius, l.theta, l.phi]
+                ol.material = nm + n_em
+                n_em += 1
             else:
                 ol.type = 2
                 n = len(l.tri_verts)
@@ -146,7 +160,7 @@ class OracleScene:
         if len(N) == 0:
             d.normals = None
             d.nidx = None
-        d.nmats = len(desc.materials)
+        d.nmats = nm + len(emitters)
         d.mats = ctypes.cast(self.mats, c_vp)
         d.nlights = len(lights)
         d.lights = ctypes.cast(self.lights, c_vp)

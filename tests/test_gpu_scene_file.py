@@ -210,7 +210,7 @@ def test_loaded_translucent_scene(tmp_path):
 
 @pytest.mark.parametrize("patch,match", [
     (lambda d: d["Geometries"][0]["parameters"]["material"].update(mtl_name="normal_shader"), "normal_shader"),
-    (lambda d: d["Lights"].append({"type": "SphericalLight", "parameters": {}}), "SphericalLight"),
+    (lambda d: d["Lights"].append({"type": "SpotLight", "parameters": {}}), "SpotLight"),
     (lambda d: d["Camera"].update(type="EnvCamera"), "EnvCamera"),
     (lambda d: d["Geometries"][1]["parameters"].update(path="missing.obj"), "missing.obj"),
 ])
