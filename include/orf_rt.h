@@ -308,8 +308,8 @@ orf_status orf_gen_sky_envmap(int width, int height, const float sun_dir[3], flo
  * (ORF/OptixSceneLoader.cpp:105-177): Integrator (PathTracer), Background
  * (EnvMapBackground with a Radiance .hdr, ConstantBackground), Camera
  * (PinholeCamera), Geometries (obj with a material {mtl_name, mtl_parameters})
- * and Lights (PointLight, DirectionalLight, TrianglesAreaLight) become the
- * calls above.  Relative paths resolve against the JSON file's directory.  The
+ * and Lights (PointLight, DirectionalLight, TrianglesAreaLight, DiskAreaLight,
+ * SphericalLight) become the calls above.  Relative paths resolve against the JSON file's directory.  The
  * reference does not importance-sample its environment; flag
  * ORF_LOAD_ENV_IMPORTANCE turns that on.  Classes without a program here are
  * refused with ORF_ERROR_INVALID_VALUE naming them.
@@ -324,6 +324,36 @@ typedef struct {
 } orf_scene_info;
 
 orf_status orf_scene_load_json(orf_context ctx, const char* path, uint32_t flags, orf_scene_info* info);
+
+/* ------------------------------------------------------------------------
+ * Progressive frames and film files.
+ *
+ * The film accumulates FP64 sums and counts; its mean equals the reference's
+ * FP32 running-mean output_buffer (CU/path_tracer.cu:64-68).  A .raw film is
+ * width x height FP32 RGBA, row 0 = launch row 0 (the bottom of the image),
+ * alpha 1, no header (OptixScene::saveScene, ORF/OptixScene.cpp:202-236).
+ * ------------------------------------------------------------------------ */
+/* OptixScene::renderScene for frames [frame_begin, frame_end)
+ * (ORF/OptixScene.cpp:139-168): per frame, the translucent probe pass and one
+ * sample per pixel with that frame's seeds (= orf_launch(frame, 1) per frame). */
+orf_status orf_render_frames(orf_context ctx, int width, int height, uint32_t frame_begin, uint32_t frame_end,
+                             void* stream);
+/* The film's mean as a .raw file. */
+orf_status orf_film_save_raw(orf_context ctx, const char* path);
+/* OptixScene::loadBuffer (ORF/OptixScene.cpp:251-299): a width x height .raw
+ * becomes the film after `frames` frames (sums = mean * frames, counts =
+ * frames), so frames rendered next continue its running mean.  A file of
+ * another size is refused with ORF_ERROR_FILE_FORMAT. */
+orf_status orf_film_load_raw(orf_context ctx, const char* path, int width, int height, uint32_t frames);
+/* bufferToImage + QImage::save (ORF/ImageLoader.cpp:110-133): 8-bit RGBA PNG of
+ * the film's mean, top row first, each channel (int)(v * 255) clamped. */
+orf_status orf_film_save_png(orf_context ctx, const char* path);
+/* OptixScene::saveScene + OptixSceneLoader::saveJSONScene
+ * (ORF/OptixSceneLoader.cpp:48-70): for path "<base>.<ext>" writes <base>.json
+ * (the document last loaded by orf_scene_load_json with frame_count and
+ * buffer_path = "<base name>.raw", relative to the document), <base>.png and
+ * <base>.raw. */
+orf_status orf_scene_save(orf_context ctx, const char* path, uint32_t frame_count);
 
 /* HDRLoader + loadHDRTexture (ORF/HDRLoader.cpp:63-181, 226-289): Radiance RGBE,
  * -Y +X only; RGBA FP32 with row 0 = the file's bottom row.  rgba may be NULL

@@ -229,6 +229,81 @@ JsonValue json_parse(const std::string& text)
     return v;
 }
 
+namespace {
+
+void json_string(std::string& out, const std::string& s)
+{
+    out += '"';
+    for (unsigned char ch : s) {
+        switch (ch) {
+        case '"': out += "\\\""; break;
+        case '\\': out += "\\\\"; break;
+        case '\b': out += "\\b"; break;
+        case '\f': out += "\\f"; break;
+        case '\n': out += "\\n"; break;
+        case '\r': out += "\\r"; break;
+        case '\t': out += "\\t"; break;
+        default:
+            if (ch < 0x20) {
+                char buf[8];
+                std::snprintf(buf, sizeof(buf), "\\u%04x", ch);
+                out += buf;
+            } else {
+                out += (char)ch;
+            }
+        }
+    }
+    out += '"';
+}
+
+void json_emit(std::string& out, const JsonValue& v, int depth)
+{
+    const std::string pad(2 * (size_t)(depth + 1), ' '), end(2 * (size_t)depth, ' ');
+    switch (v.kind) {
+    case JsonValue::NUL: out += "null"; break;
+    case JsonValue::BOOL: out += v.b ? "true" : "false"; break;
+    case JsonValue::NUMBER: {
+        char buf[32];
+        std::snprintf(buf, sizeof(buf), "%.17g", v.num);  // round-trips every double
+        out += buf;
+        break;
+    }
+    case JsonValue::STRING: json_string(out, v.str); break;
+    case JsonValue::ARRAY: {
+        out += '[';
+        for (size_t i = 0; i < v.arr.size(); ++i) {
+            if (i) out += ", ";
+            json_emit(out, v.arr[i], depth + 1);
+        }
+        out += ']';
+        break;
+    }
+    case JsonValue::OBJECT: {
+        out += "{\n";
+        size_t i = 0;
+        for (const auto& kv : v.obj) {
+            out += pad;
+            json_string(out, kv.first);
+            out += ": ";
+            json_emit(out, kv.second, depth + 1);
+            out += ++i < v.obj.size() ? ",\n" : "\n";
+        }
+        out += end + "}";
+        break;
+    }
+    }
+}
+
+}  // namespace
+
+std::string json_write(const JsonValue& v)
+{
+    std::string out;
+    json_emit(out, v, 0);
+    out += '\n';
+    return out;
+}
+
 // ------------------------------------------------------------------ OBJ
 
 namespace {

@@ -32,6 +32,10 @@ struct JsonValue {
 // malformed input.
 JsonValue json_parse(const std::string& text);
 
+// JSON text of v (objects with sorted keys, two-space indentation, numbers as
+// %.17g so every double reads back exactly).
+std::string json_write(const JsonValue& v);
+
 // ------------------------------------------------------------------ OBJ
 
 // Triangles of every face of the file (polygons as fans v0, vi, vi+1), indices

@@ -384,13 +384,13 @@ void run_probe_pass(orf_context_t* c, unsigned frame, hipStream_t stream)
     launch_probe_pass(s, c->d_sss.as<ScatteringGPU>(), sss, frame, stream);
 }
 
-static void film_clear(orf_context_t* c)
+void film_clear(orf_context_t* c)
 {
     if (c->d_film.p) ORF_HIP_CHECK(hipMemset(c->d_film.p, 0, c->d_film.bytes));
     if (c->d_counters.p) ORF_HIP_CHECK(hipMemset(c->d_counters.p, 0, c->d_counters.bytes));
 }
 
-static void film_ensure(orf_context_t* c, int w, int h)
+void film_ensure(orf_context_t* c, int w, int h)
 {
     if (w == c->film_w && h == c->film_h && c->d_film.p) return;
     c->d_film.release();

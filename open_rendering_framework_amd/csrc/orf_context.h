@@ -70,6 +70,8 @@ struct orf_context_t {
     // differentiable renders
     orf::DevBuf d_diff_views, d_diff_stats;
     long long last_batch_iterations = 0;
+    // the scene document last read by orf_scene_load_json (saved back by orf_scene_save_json)
+    std::string scene_doc;
 };
 
 namespace orf {
@@ -77,5 +79,7 @@ orf_context_t* ctx_check(orf_context ctx);
 void prepare(orf_context_t* c, hipStream_t stream);
 SceneGPU scene_view(orf_context_t* c);
 void run_probe_pass(orf_context_t* c, unsigned frame, hipStream_t stream);
+void film_ensure(orf_context_t* c, int w, int h);
+void film_clear(orf_context_t* c);
 }  // namespace orf
 

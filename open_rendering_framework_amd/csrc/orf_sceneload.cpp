@@ -30,6 +30,7 @@
 
 #include "orf_assets.h"
 #include "orf_common.h"
+#include "orf_context.h"
 
 namespace orf {
 namespace {
@@ -290,6 +291,7 @@ void scene_load_json(orf_context ctx, const std::string& path, uint32_t flags, o
         std::fclose(f);
     }
     const JsonValue doc = json_parse(text);
+    ctx->scene_doc = text;
     bool translucent_seen = false;
     ORF_REQUIRE(doc.is_object(), "scene: the document must be a JSON object");
     const size_t slash = path.find_last_of('/');

@@ -82,6 +82,11 @@ _SIGS = {
     "orf_probe_pass": [c_vp, c_u32, c_vp],
     "orf_diff_adjoint": [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp],
     "orf_scene_load_json": [c_vp, ctypes.c_char_p, c_u32, c_vp],
+    "orf_render_frames": [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, c_vp],
+    "orf_film_save_raw": [c_vp, ctypes.c_char_p],
+    "orf_film_load_raw": [c_vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32],
+    "orf_film_save_png": [c_vp, ctypes.c_char_p],
+    "orf_scene_save": [c_vp, ctypes.c_char_p, ctypes.c_uint32],
     "orf_hdr_read": [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32), c_vp],
     "orf_obj_read": [ctypes.c_char_p, ctypes.POINTER(c_i64), ctypes.POINTER(c_i64), ctypes.POINTER(c_i64),
                      ctypes.POINTER(ctypes.c_int32), c_vp, c_vp, c_vp, c_vp],
@@ -507,6 +512,22 @@ class Context:
              ctypes.byref(info))
         return {"width": info.width, "height": info.height, "frame_count": info.frame_count, "meshes": info.meshes,
                 "materials": info.materials, "lights": info.lights, "buffer_path": info.buffer_path.decode()}
+
+    def render_frames(self, width, height, frame_begin, frame_end, stream=None):
+        """One launch per frame in [frame_begin, frame_end) (orf_render_frames)."""
+        call("orf_render_frames", self.handle, width, height, frame_begin, frame_end, stream_ptr(stream))
+
+    def save_raw(self, path):
+        call("orf_film_save_raw", self.handle, str(path).encode())
+
+    def load_raw(self, path, width, height, frames):
+        call("orf_film_load_raw", self.handle, str(path).encode(), width, height, frames)
+
+    def save_png(self, path):
+        call("orf_film_save_png", self.handle, str(path).encode())
+
+    def save_scene(self, path, frame_count):
+        call("orf_scene_save", self.handle, str(path).encode(), frame_count)
 
     def probe_pass(self, frame, stream=None):
         call("orf_probe_pass", self.handle, frame, stream_ptr(stream))
