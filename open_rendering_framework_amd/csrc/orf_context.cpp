@@ -57,6 +57,8 @@ static int program_kind(const std::string& prog)
     if (p == "lambertian_interface_shader") return MAT_INTERFACE;
     if (p == "subsurface_scattering_shader" || p == "translucent_shader") return MAT_TRANSLUCENT;
     if (p == "rough_subsurface_scattering_shader") return MAT_ROUGH_TRANSLUCENT;
+    if (p == "transparent_shader") return MAT_TRANSPARENT;
+    if (p == "normal_shader") return MAT_NORMAL;
     return -1;
 }
 
@@ -218,6 +220,17 @@ static void build_geometry(orf_context_t* c, hipStream_t)
     for (int m : c->analytic_mat) shade.push_back(make_int4(-1, -1, -1, m));
     BvhResult bvh = build_bvh(V.data(), (int64_t)(V.size() / 3), I.data(), (int64_t)(I.size() / 3));
     c->bvh_stats = bvh.stats;
+    // triangles whose material lets shadow rays through
+    for (size_t k = 0; k < bvh.tris.size(); k += 3) {
+        uint32_t id;
+        std::memcpy(&id, &bvh.tris[k].w, 4);
+        const int m = shade[id].w;
+        if (m >= 0 && m < (int)c->mats.size() && c->mats[m].kind == MAT_NORMAL) {
+            ORF_REQUIRE(id <= kTriIdMask, "too many triangles");
+            id |= kTriNoShadow;
+            std::memcpy(&bvh.tris[k].w, &id, 4);
+        }
+    }
     if (bvh.tris.empty()) bvh.tris.assign(3, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     if (shade.empty()) shade.push_back(make_int4(-1, -1, -1, 0));
     std::vector<float4> normals(N.size() / 3 + 1);
@@ -356,6 +369,8 @@ SceneGPU scene_view(orf_context_t* c)
     s.scene_epsilon = c->scene_epsilon;
     s.features = 0u;
     for (const MaterialGPU& m : c->mats) s.features |= material_features(m.kind);
+    // disk / sphere emitters are traced only by the all-feature megakernel variant
+    if (!c->analytic.empty()) s.features = kFeatAll;
     s.sss = c->d_sss.as<ScatteringGPU>();
     s.probes.sss = -1;
     if (c->probe_count > 0) {
@@ -499,7 +514,7 @@ extern "C" orf_status orf_material_create(orf_context ctx, const char* program, 
     if (kind < 0) ORF_THROW(ORF_ERROR_NOT_FOUND, std::string("no radiance program for '") + program + "'");
     c->mats.push_back(orf::default_material(kind));
     c->mat_sss.push_back(orf::SssParamsHost());
-    if (orf::is_translucent(kind)) c->mats.back().ior = 1.3f;
+    if (orf::is_translucent(kind) || kind == orf::MAT_TRANSPARENT) c->mats.back().ior = 1.3f;
     if (kind == orf::MAT_ROUGH_TRANSLUCENT) {  // ORF/RoughTranslucentMaterial.cpp:14-18
         c->mats.back().normal_distribution = orf::BECKMANN_DISTRIBUTION;
         c->mats.back().microfacet_model = orf::VISIBLE_NORMALS_MODEL;

@@ -211,6 +211,45 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         ps.prev_bsdf_pdf = -1.0f;
         return true;
     }
+    if (kDiel && mat.kind == MAT_TRANSPARENT) {
+        // CU/transparent_shader.cu:51-114 (smooth glass, no diffuse part): depth
+        // test depth > max_depth, Beer absorption with Russian roulette inside,
+        // Fresnel choice between the mirror and the refracted direction.
+        if (ps.depth > sc.max_depth) return false;
+        f3 beam = mk3(1.0f);
+        float eta = 1.0f / mat.ior;
+        f3 n = normal;
+        float c = dot(-d, n);
+        if (c < 0.0f) {
+            f3 e = (-h.t) * mat.glass_absorption;
+            beam = mk3(orf_expf(e.x), orf_expf(e.y), orf_expf(e.z));
+            float prob = (beam.x + beam.y + beam.z) / 3.0f;
+            if (rnd_tea(ps.rng) >= prob) return false;
+            beam = beam / prob;
+            eta = mat.ior;
+            n = -n;
+            c = -c;
+        }
+        float R = 1.0f, ct = 0.0f;
+        float st2 = eta * eta * (1.0f - c * c);
+        if (st2 < 1.0f) {
+            ct = sqrtf(1.0f - st2);
+            R = fresnel_R3(c, ct, eta);
+        }
+        float xi = rnd_tea(ps.rng);
+        ps.d = xi < R ? d - (n * 2.0f) * dot(n, d) : eta * d + n * (eta * c - ct);
+        ps.thr = ps.thr * beam;
+        ps.o = p;
+        ps.depth++;
+        ps.emit = 1;
+        ps.prev_bsdf_pdf = -1.0f;
+        return true;
+    }
+    if (kDiel && mat.kind == MAT_NORMAL) {
+        // CU/normal_shader.cu:25-29: the shading normal mapped to [0, 1]^3; the path ends
+        ps.L += ps.thr * (normal * 0.5f + 0.5f);
+        return false;
+    }
     if (kDiel && mat.kind == MAT_INTERFACE) {
         // Lambertian interface (CU/lambertian_interface_shader.cu:51-173): a Lambert
         // base under a smooth dielectric of index ior.  NEE through both Fresnel

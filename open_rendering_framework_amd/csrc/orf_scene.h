@@ -29,6 +29,8 @@ enum MaterialKind : int {
     MAT_INTERFACE = 4,      // lambertian_interface_shader.cu
     MAT_TRANSLUCENT = 5,    // subsurface_scattering_shader.cu (smooth boundary)
     MAT_ROUGH_TRANSLUCENT = 6,  // rough_subsurface_scattering_shader.cu (microfacet boundary)
+    MAT_TRANSPARENT = 7,    // transparent_shader.cu (smooth glass)
+    MAT_NORMAL = 8,         // normal_shader.cu (shading-normal display)
     MAT_COUNT
 };
 
@@ -145,7 +147,8 @@ struct SceneGPU {
 
 // Material code a megakernel variant is compiled with: Lambert, conductor and
 // emitter always; kFeatDielectric adds the rough dielectric (with the
-// microsurface walk) and the Lambertian interface, kFeatTranslucent the
+// microsurface walk), the Lambertian interface, smooth glass and the normal
+// display, kFeatTranslucent the
 // translucent material.  Launches pick the smallest variant covering the scene,
 // so simple scenes do not pay the register pressure of unused materials.
 constexpr unsigned kFeatDielectric = 1u, kFeatTranslucent = 2u, kFeatAll = 3u;
@@ -154,7 +157,7 @@ inline bool is_translucent(int kind) { return kind == MAT_TRANSLUCENT || kind ==
 
 inline unsigned material_features(int kind)
 {
-    return kind == MAT_DIELECTRIC || kind == MAT_INTERFACE
+    return kind == MAT_DIELECTRIC || kind == MAT_INTERFACE || kind == MAT_TRANSPARENT || kind == MAT_NORMAL
                ? kFeatDielectric
                : (kind == MAT_TRANSLUCENT || kind == MAT_ROUGH_TRANSLUCENT ? kFeatTranslucent : 0u);
 }
@@ -170,6 +173,11 @@ constexpr int kStackDepth = 48;   // builder caps BVH depth below this
 constexpr int kLdsStack = ORF_LDS_STACK;
 static_assert(kLdsStack >= 1 && kLdsStack <= kStackDepth && (kLdsStack & (kLdsStack - 1)) == 0,
               "LDS stack size must be a power of two no larger than kStackDepth");
+
+// Triangle record word 3 of the first float4: the original triangle id, with
+// kTriNoShadow set for triangles of a material whose shadow any-hit does not
+// block (normal_shader.cu:18-22 terminates without clearing the attenuation).
+constexpr unsigned kTriNoShadow = 0x40000000u, kTriIdMask = 0x3FFFFFFFu;
 
 // BVH child link: internal node index, or leaf = 0x80000000 | (count-1) << 27 | first
 ORF_HD bool link_is_leaf(unsigned c) { return (c & 0x80000000u) != 0; }

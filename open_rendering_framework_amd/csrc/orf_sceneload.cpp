@@ -203,6 +203,8 @@ int load_material(orf_context ctx, const JsonValue& mj)
     else if (iequals(name, "rough_transparent_shader")) program = "rough_transparent_shader.cu";
     else if (iequals(name, "translucent_shader")) program = "subsurface_scattering_shader.cu";
     else if (iequals(name, "rough_translucent_shader")) program = "rough_subsurface_scattering_shader.cu";
+    else if (iequals(name, "transparent_shader")) program = "transparent_shader.cu";
+    else if (iequals(name, "normal_shader")) program = "normal_shader.cu";
     else throw std::runtime_error("scene: material '" + name + "' has no radiance program here");
     int m = -1;
     check(orf_material_create(ctx, program, &m));

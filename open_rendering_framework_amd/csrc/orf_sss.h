@@ -130,7 +130,7 @@ ORF_HD int probe_ray_hits(const SceneGPU& sc, int sss, f3 o, f3 dir, float tmax,
             if (!intersect_tri(o, dir, mk3(q0.x, q0.y, q0.z), mk3(q1.x, q1.y, q1.z), mk3(q2.x, q2.y, q2.z), 0.0f,
                                tcull, c.t, c.b, c.g))
                 continue;
-            c.prim = (int)f2u(q0.w);
+            c.prim = (int)(f2u(q0.w) & kTriIdMask);
             const MaterialGPU& m = sc.materials[sc.shade[c.prim].w];
             if (m.sss != sss) continue;  // the probes' own translucent material only
             c.leaf = (int)(first + k);

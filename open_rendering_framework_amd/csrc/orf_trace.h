@@ -219,10 +219,12 @@ ORF_HD bool trace_impl(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, b
             float t, bb, gg;
             if (intersect_tri(o, d, mk3(q0.x, q0.y, q0.z), mk3(q1.x, q1.y, q1.z), mk3(q2.x, q2.y, q2.z), tmin,
                               tbest, t, bb, gg)) {
+                const unsigned id = f2u(q0.w);
+                if (ANY && (id & kTriNoShadow)) continue;  // shadow rays pass through it
                 tbest = t;
                 hb = bb;
                 hg = gg;
-                prim = (int)f2u(q0.w);
+                prim = (int)(id & kTriIdMask);
                 leaf = (int)(first + k);
                 if (ANY) {
                     h.t = t;

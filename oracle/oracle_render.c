@@ -368,6 +368,9 @@ static int trace(const oscene* s, v3 o, v3 d, R tmin, R tmax, int any, ohit* h)
                 int32_t id = s->order[b->first + k];
                 R t, bb, gg;
                 if (isect(o, d, s->p0[id], s->e1[id], s->e2[id], tmin, h->t, &t, &bb, &gg)) {
+                    /* normal_shader's any-hit terminates without clearing the attenuation
+                     * (CU/normal_shader.cu:18-22): shadow rays pass through it */
+                    if (any && s->mats[s->tri_mat[id]].kind == ORC_NORMAL) continue;
                     h->t = t;
                     h->b = bb;
                     h->g = gg;
@@ -903,6 +906,42 @@ static v3 shade_dielectric(const oscene* s, const orc_material* m, v3 o, v3 d, c
 
 /* Fresnel reflectance at incidence cosine c through relative index eta; 1 under
  * total internal reflection (CU/lambertian_interface_shader.cu:68-75). */
+/* CU/transparent_shader.cu:51-114 (DIFFUSE_PART off): depth > max_depth ends the
+ * path; inside, Beer absorption with Russian roulette on its mean and the
+ * normal and index flipped; the Fresnel reflectance picks mirror or refraction. */
+static v3 shade_transparent(const oscene* s, const orc_material* m, v3 o, v3 d, const ohit* h, int depth,
+                            uint32_t* t, ostats* st)
+{
+    v3 zero = V3(0, 0, 0);
+    if (depth > s->max_depth) return zero;
+    v3 hit_point = vadd(o, vscale(d, h->t));
+    v3 normal = hit_normal(s, h);
+    v3 beam = V3(1, 1, 1);
+    R n12 = (R)1.0f / (R)m->ior;
+    R c = vdot(vneg(d), normal);
+    if (c < 0) {
+        beam = V3(F_EXP(-h->t * (R)m->absorption[0]), F_EXP(-h->t * (R)m->absorption[1]),
+                  F_EXP(-h->t * (R)m->absorption[2]));
+        R prob = (beam.x + beam.y + beam.z) / (R)3.0f;
+        if (orc_rnd_tea(t) >= prob) return zero;
+        beam = V3(beam.x / prob, beam.y / prob, beam.z / prob);
+        n12 = (R)m->ior;
+        normal = vneg(normal);
+        c = -c;
+    }
+    R Rf = 1, ct = 0;
+    R st2 = n12 * n12 * ((R)1.0f - c * c);
+    if (st2 < (R)1.0f) {
+        ct = sqrt((R)1.0f - st2);
+        Rf = orc_fresnel_R3(c, ct, n12);
+    }
+    R xi = orc_rnd_tea(t);
+    v3 dir = xi < Rf ? vsub(d, vscale(vscale(normal, (R)2.0f), vdot(normal, d)))
+                     : vadd(vscale(d, n12), vscale(normal, n12 * c - ct));
+    v3 child = radiance(s, hit_point, dir, depth + 1, 1, t, (R)-1, st);
+    return vmul(child, beam);
+}
+
 static R fresnel_R_cos(R c, R eta)
 {
     R st2 = eta * eta * ((R)1.0f - c * c);
@@ -1417,6 +1456,12 @@ static v3 shade_hit(const oscene* s, v3 o, v3 d, const ohit* hp, int depth, int 
     if (m->kind == ORC_LAMBERT) return shade_lambert(s, m, o, d, &h, depth, &emit_light, seed, st);
     if (m->kind == ORC_CONDUCTOR) return shade_conductor(s, m, &s->eta[s->tri_mat[h.prim]], o, d, &h, depth, seed, st);
     if (m->kind == ORC_DIELECTRIC) return shade_dielectric(s, m, o, d, &h, depth, seed, st);
+    if (m->kind == ORC_TRANSPARENT) return shade_transparent(s, m, o, d, &h, depth, seed, st);
+    if (m->kind == ORC_NORMAL) {
+        /* CU/normal_shader.cu:25-29: normal * 0.5 + 0.5 */
+        v3 n = hit_normal(s, &h);
+        return V3(n.x * (R)0.5f + (R)0.5f, n.y * (R)0.5f + (R)0.5f, n.z * (R)0.5f + (R)0.5f);
+    }
     if (m->kind == ORC_INTERFACE) return shade_interface(s, m, o, d, &h, depth, emit_light, seed, st);
     if (m->kind == ORC_TRANSLUCENT) return shade_translucent(s, s->tri_mat[h.prim], o, d, &h, depth, seed, st);
     if (m->kind == ORC_ROUGH_TRANSLUCENT)

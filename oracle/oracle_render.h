@@ -20,7 +20,9 @@ enum {
     ORC_DIELECTRIC = 3,
     ORC_INTERFACE = 4,
     ORC_TRANSLUCENT = 5,
-    ORC_ROUGH_TRANSLUCENT = 6
+    ORC_ROUGH_TRANSLUCENT = 6,
+    ORC_TRANSPARENT = 7, /* transparent_shader.cu */
+    ORC_NORMAL = 8       /* normal_shader.cu */
 };
 enum { ORC_POINT = 0, ORC_DIRECTIONAL = 1, ORC_TRIANGLES = 2, ORC_DISK = 3, ORC_SPHERE = 4, ORC_ENV = 5 };
 

@@ -41,7 +41,7 @@ class OrcSceneDesc(ctypes.Structure):
 
 _KIND = {"diffuse_shader.cu": 0, "metallic_shader.cu": 1, "arealight_shader.cu": 2, "rough_transparent_shader.cu": 3,
          "lambertian_interface_shader.cu": 4, "subsurface_scattering_shader.cu": 5,
-         "rough_subsurface_scattering_shader.cu": 6}
+         "rough_subsurface_scattering_shader.cu": 6, "transparent_shader.cu": 7, "normal_shader.cu": 8}
 # Wholemilk (ORF/RoughTranslucentMaterial.cpp:258-263) and the default scale
 SSS_DEFAULTS = {"absorption": (0.0011, 0.0024, 0.014), "scattering": (2.55, 3.21, 3.77),
                 "meancosine": (0.0, 0.0, 0.0), "scale": (100.0,)}
@@ -106,7 +106,7 @@ class OracleScene:
             r = m.floats.get("roughness", (0.1, 0.1))
             mm.roughness[0] = r[0]
             mm.roughness[1] = r[1] if len(r) > 1 else r[0]
-            mm.ior = m.floats.get("ior", (1.3,) if mm.kind in (5, 6) else (1.5,))[0]
+            mm.ior = m.floats.get("ior", (1.3,) if mm.kind in (5, 6, 7) else (1.5,))[0]
             for key, field in (("absorption", "sss_absorption"), ("scattering", "sss_scattering"),
                                ("meancosine", "sss_meancosine")):
                 getattr(mm, field)[:] = [float(x) for x in m.floats.get(key, SSS_DEFAULTS[key])]

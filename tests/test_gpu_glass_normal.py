@@ -1,0 +1,127 @@
+"""GPU parity of the smooth glass (CU/transparent_shader.cu) and normal display
+(CU/normal_shader.cu, shadows pass through it) materials through the C-ABI
+against the CPU oracle.
+
+Tolerances: per sample |gpu - oracle| <= 1e-5 |oracle| + 1e-6 on >= 99% of
+samples; converged image vs the FP64 oracle: relative RMSE of 8 x 8 block means < 3e-2
+(glass caustics make single pixels noisy once FP32 and FP64 paths part); wavefront
+and persistent batch forms vs the megakernel: relative 1e-9, equal ray counts.
+"""
+import json
+import sys
+
+import numpy as np
+import pytest
+
+from open_rendering_framework_amd import rt, scenes
+
+pytestmark = pytest.mark.gpu
+
+RTOL, ATOL = 1e-5, 1e-6
+W = 48
+
+
+def _say(*a):
+    print(*a)
+    sys.stdout.flush()
+
+
+def _scene():
+    """The path-tracer scene at test scale with a glass sphere (absorbing) and a
+    normal-display quad between the area light and the ground."""
+    d = scenes.path_scene(width=W, height=W, level=4, env_size=(128, 64), max_depth=8)
+    d.materials[1] = scenes.Material("transparent_shader.cu", floats={"ior": (1.5,), "glass_absorption": (0.3, 0.1, 0.6)})
+    d.materials.append(scenes.Material("normal_shader.cu"))
+    qv, qt = scenes.quad(2.0, 0.6, facing_down=True)
+    qv = qv + np.array([0.8, 0.0, 0.3], np.float32)
+    d.meshes.append(scenes.Mesh(qv, qt, 3, normals=np.tile([[0.3, -0.9, 0.3]], (4, 1)).astype(np.float32)))
+    return d
+
+
+def _ok(got, want):
+    return (np.abs(got - want) <= RTOL * np.abs(want) + ATOL).all(axis=-1)
+
+
+def test_per_sample_parity():
+    from tests.oracle_scene import OracleScene
+
+    d = _scene()
+    ctx = scenes.to_context(d)
+    ctx.accel_build()
+    orc = OracleScene(d)
+    ys, xs = np.mgrid[0:W, 0:W]
+    for frame in (0, 4):
+        ctx.film_clear()
+        ctx.launch(W, W, frame, 1)
+        got = ctx.film_sums(W, W)[..., :3]
+        want = orc.samples(W, W, xs.ravel(), ys.ravel(), frame).reshape(W, W, 3)
+        ok = _ok(got, want)
+        _say("frame %d samples ok %.5f mean %.4g / %.4g" % (frame, ok.mean(), got.mean(), want.mean()))
+        assert ok.mean() >= 0.99
+
+
+def test_converged_vs_fp64_oracle():
+    from tests.oracle_scene import OracleScene
+
+    d = _scene()
+    ctx = scenes.to_context(d)
+    ctx.accel_build()
+    spp = 32
+    ctx.film_clear()
+    ctx.launch(W, W, 0, spp)
+    img = ctx.film(W, W)[..., :3]
+    ref, _ = OracleScene(d, fp64=True).render(W, W, 0, spp)
+    ref = ref[..., :3] / ref[..., 3:4]
+    rel = np.sqrt(((img - ref) ** 2).mean()) / ref.mean()
+    # glass caustics make single pixels noisy once FP32 and FP64 paths part ways:
+    # compare 8 x 8 pixel block means
+    blk = lambda a: a.reshape(W // 8, 8, W // 8, 8, 3).mean(axis=(1, 3))  # noqa: E731
+    rel_blk = np.sqrt(((blk(img) - blk(ref)) ** 2).mean()) / ref.mean()
+    _say("rel rmse vs fp64 %.3g (pixels), %.3g (8x8 blocks); counters %s" % (rel, rel_blk, ctx.counters()))
+    assert ctx.counters()["nonfinite"] == 0
+    assert rel_blk < 3e-2
+
+
+def test_batch_forms_match_megakernel():
+    d = _scene()
+    ctx = scenes.to_context(d)
+    items = [{"width": W, "height": W, "spp": 4, "frame0": f, "eye": d.eye, "U": d.U, "V": d.V, "W": d.W}
+             for f in (0, 9)]
+    ctx.render_batch(items, None, mode=rt.BATCH_MEGAKERNEL)
+    ref = [ctx.batch_film(i, W, W, sums=True) for i in range(len(items))]
+    ref_counts = ctx.batch_counters()
+    for mode in (rt.BATCH_WAVEFRONT, rt.BATCH_PERSISTENT):
+        ctx.render_batch(items, None, mode=mode)
+        counts = ctx.batch_counters()
+        _say("mode %d counters %s (megakernel %s)" % (mode, counts, ref_counts))
+        for i in range(len(items)):
+            np.testing.assert_allclose(ctx.batch_film(i, W, W, sums=True)[..., :3], ref[i][..., :3], rtol=1e-9,
+                                       atol=1e-12)
+        assert counts["ext_rays"] == ref_counts["ext_rays"]
+        assert counts["shadow_rays"] == ref_counts["shadow_rays"]
+
+
+def test_scene_file_with_glass_and_normal_display(tmp_path):
+    from tests.oracle_scene import OracleScene
+    from tests.test_gpu_scene_file import H as SH, W as SW, _scene_files
+
+    path, desc = _scene_files(tmp_path)
+    doc = json.loads(path.read_text())
+    doc["Geometries"][1]["parameters"]["material"] = {"mtl_name": "normal_shader", "mtl_parameters": {}}
+    doc["Geometries"][2]["parameters"]["material"] = {
+        "mtl_name": "transparent_shader", "mtl_parameters": {"ior": 1.45, "glass_absorption": [0.2, 0.3, 0.4]}}
+    path.write_text(json.dumps(doc))
+    desc.materials[1] = scenes.Material("normal_shader.cu")
+    desc.materials[2] = scenes.Material("transparent_shader.cu", floats={"ior": (1.45,),
+                                                                         "glass_absorption": (0.2, 0.3, 0.4)})
+    ctx = rt.Context()
+    ctx.load_scene(path, env_importance=True)
+    ctx.accel_build()
+    ctx.film_clear()
+    ctx.launch(SW, SH, 2, 1)
+    got = ctx.film_sums(SW, SH)[..., :3]
+    ys, xs = np.mgrid[0:SH, 0:SW]
+    want = OracleScene(desc).samples(SW, SH, xs.ravel(), ys.ravel(), 2).reshape(SH, SW, 3)
+    ok = _ok(got, want)
+    _say("scene file: samples ok %.5f" % ok.mean())
+    assert ok.mean() >= 0.99
