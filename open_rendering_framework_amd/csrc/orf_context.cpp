@@ -59,6 +59,7 @@ static int program_kind(const std::string& prog)
     if (p == "rough_subsurface_scattering_shader") return MAT_ROUGH_TRANSLUCENT;
     if (p == "transparent_shader") return MAT_TRANSPARENT;
     if (p == "normal_shader") return MAT_NORMAL;
+    if (p == "flat_shader") return MAT_FLAT;
     return -1;
 }
 
@@ -119,12 +120,17 @@ static MaterialGPU& material_ref(orf_context_t* c, int handle)
 static void set_material_float(orf_context_t* c, int handle, const std::string& name, const float* v, int n)
 {
     MaterialGPU& m = material_ref(c, handle);
-    if (name == "diffuse_color" || name == "radiance" || name == "glass_absorption") {
+    if (name == "diffuse_color" || name == "radiance" || name == "glass_absorption" || name == "highlight_color") {
         ORF_REQUIRE(n == 3, "material variable '" + name + "' takes 3 floats");
         f3 x = mk3(v[0], v[1], v[2]);
         if (name == "diffuse_color") m.diffuse_color = x;
-        else if (name == "radiance") m.radiance = x;
+        else if (name == "radiance" || name == "highlight_color") m.radiance = x;
         else m.glass_absorption = x;
+    } else if (name == "highlight_threshold") {
+        ORF_REQUIRE(n == 1, "highlight_threshold takes 1 float");
+        m.threshold = v[0];
+    } else if (name == "shininess") {
+        ORF_REQUIRE(n == 1, "shininess takes 1 float");  // declared by FlatMaterial, unused by flat_shader.cu
     } else if (name == "roughness") {
         ORF_REQUIRE(n == 1 || n == 2, "roughness takes 1 or 2 floats");
         m.roughness = mk2(v[0], n == 2 ? v[1] : v[0]);
@@ -161,6 +167,9 @@ static void set_material_int(orf_context_t* c, int handle, const std::string& na
     } else if (name == "dipole_model") {
         ORF_REQUIRE(v[0] == 0 || v[0] == 1, "dipole_model must be 0 (standard) or 1 (directional)");
         c->mat_sss[handle].dipole_model = (int)v[0];
+    } else if (name == "flat_shadow" || name == "highlight") {
+        const int bit = name == "flat_shadow" ? kFlatShadow : kFlatHighlight;
+        m.flags = v[0] == 1 ? (m.flags | bit) : (m.flags & ~bit);  // the shader tests == 1
     } else {
         ORF_THROW(ORF_ERROR_INVALID_VALUE, "unknown integer material variable '" + name + "'");
     }
@@ -515,6 +524,14 @@ extern "C" orf_status orf_material_create(orf_context ctx, const char* program, 
     c->mats.push_back(orf::default_material(kind));
     c->mat_sss.push_back(orf::SssParamsHost());
     if (orf::is_translucent(kind) || kind == orf::MAT_TRANSPARENT) c->mats.back().ior = 1.3f;
+    if (kind == orf::MAT_FLAT) {  // FlatMaterial::FlatMaterial (ORF/FlatMaterial.cpp:5-21)
+        orf::MaterialGPU& m = c->mats.back();
+        m.diffuse_color = orf::mk3(1.0f);
+        m.radiance = orf::mk3(1.0f);
+        m.roughness = orf::mk2(0.999f, 0.999f);
+        m.normal_distribution = orf::BECKMANN_DISTRIBUTION;
+        m.ior = 1.3f;
+    }
     if (kind == orf::MAT_ROUGH_TRANSLUCENT) {  // ORF/RoughTranslucentMaterial.cpp:14-18
         c->mats.back().normal_distribution = orf::BECKMANN_DISTRIBUTION;
         c->mats.back().microfacet_model = orf::VISIBLE_NORMALS_MODEL;

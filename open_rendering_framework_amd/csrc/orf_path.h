@@ -250,6 +250,56 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         ps.L += ps.thr * (normal * 0.5f + 0.5f);
         return false;
     }
+    if (kDiel && mat.kind == MAT_FLAT) {
+        // CU/flat_shader.cu:40-127: the flat colour; with flat_shadow it is
+        // multiplied by the light's visibility V, and with highlight a visible
+        // light whose microfacet reflection (luminance of Le G1(i) G1(o) D_wi R /
+        // (4 |i.n l.n|)) exceeds the threshold shows the highlight colour.  The
+        // colour for V = 1 is A, for V = 0 it is B; B is added now and A - B when
+        // the shadow segment resolves unoccluded.
+        if (ps.depth > sc.max_depth) return false;
+        const f3 w_i = -d;
+        rnd_tea(ps.rng);  // z1, z2 of a microfacet normal the shader samples and never uses
+        rnd_tea(ps.rng);
+        const f3 B = (mat.flags & kFlatShadow) ? mk3(0.0f) : mat.diffuse_color;
+        unsigned li = (unsigned)((float)sc.nlights * rnd_tea(ps.rng));
+        if (sc.nlights > 0) {
+            if (li >= (unsigned)sc.nlights) li = sc.nlights - 1;
+            const float light_pdf = 1.0f / (float)sc.nlights;
+            f3 wl = mk3(0.0f), rad = mk3(0.0f);
+            float dist = 0.0f, penv;
+            sample_light(sc, sc.lights[li], p, ps.rng, wl, rad, dist, penv);
+            rad = rad / light_pdf;
+            if (dot(ffn, wl) > 0.0f) {
+                f3 A = mat.diffuse_color;
+                if (mat.flags & kFlatHighlight) {
+                    const f3 rn = normalize(w_i + wl);
+                    const float cr = dot(rn, wl), eta = 1.0f / mat.ior;
+                    float R = 1.0f;
+                    const float st2 = eta * eta * (1.0f - cr * cr);
+                    if (st2 < 1.0f) R = fresnel_R3(cr, sqrtf(1.0f - st2), eta);
+                    const float ax = mat.roughness.x, ay = mat.roughness.y;
+                    const int nd = mat.normal_distribution;
+                    const float gi = masking_g1(w_i, rn, ffn, ax, ay, nd);
+                    const float go = masking_g1(wl, rn, ffn, ax, ay, nd);
+                    const float D = eval_visible_normal(w_i, rn, ffn, ax, ay, nd);
+                    const float wr = gi * go * D * R / (4.0f * fabsf(dot(w_i, ffn) * dot(wl, ffn)));
+                    const f3 spec = rad * wr;
+                    const float lum = 0.2126f * spec.x + 0.7152f * spec.y + 0.0722f * spec.z;
+                    if (lum > mat.threshold) A = mat.radiance;
+                }
+                ps.L += ps.thr * B;
+                sr.o = p;
+                sr.d = wl;
+                sr.tmax = dist - eps;
+                sr.contrib = ps.thr * (A - B);
+                sr.valid = true;
+                return false;
+            }
+        }
+        ps.L += ps.thr * B;
+        return false;
+    }
     if (kDiel && mat.kind == MAT_INTERFACE) {
         // Lambertian interface (CU/lambertian_interface_shader.cu:51-173): a Lambert
         // base under a smooth dielectric of index ior.  NEE through both Fresnel

@@ -31,6 +31,7 @@ enum MaterialKind : int {
     MAT_ROUGH_TRANSLUCENT = 6,  // rough_subsurface_scattering_shader.cu (microfacet boundary)
     MAT_TRANSPARENT = 7,    // transparent_shader.cu (smooth glass)
     MAT_NORMAL = 8,         // normal_shader.cu (shading-normal display)
+    MAT_FLAT = 9,           // flat_shader.cu (flat colour, optional shadow and highlight)
     MAT_COUNT
 };
 
@@ -44,13 +45,16 @@ struct MaterialGPU {
     int normal_distribution;
     int sss;            // translucent: index into SceneGPU::sss (-1: none)
     f3 diffuse_color;   // Lambert albedo / interface albedo
-    f3 radiance;        // emitter radiance
+    f3 radiance;        // emitter radiance / flat highlight colour
     f3 glass_absorption;
     f2 roughness;       // (ax, ay)
     float ior;          // dielectric ior (inside / outside)
-    float pad1;
+    int flags;          // flat: kFlatShadow | kFlatHighlight
     ConductorEta eta;   // conductor: n1 / ior per channel
+    float threshold;    // flat: highlight luminance threshold
 };
+
+constexpr int kFlatShadow = 1, kFlatHighlight = 2;
 
 // Light types (ORF/structs.h:146-153) + the environment as a sampled light.
 enum LightType : unsigned { POINT_LIGHT = 0, DIRECTIONAL_LIGHT = 1, TRIANGLES_AREA_LIGHT = 2,
@@ -147,8 +151,8 @@ struct SceneGPU {
 
 // Material code a megakernel variant is compiled with: Lambert, conductor and
 // emitter always; kFeatDielectric adds the rough dielectric (with the
-// microsurface walk), the Lambertian interface, smooth glass and the normal
-// display, kFeatTranslucent the
+// microsurface walk), the Lambertian interface, smooth glass, the normal
+// display and the flat material, kFeatTranslucent the
 // translucent material.  Launches pick the smallest variant covering the scene,
 // so simple scenes do not pay the register pressure of unused materials.
 constexpr unsigned kFeatDielectric = 1u, kFeatTranslucent = 2u, kFeatAll = 3u;
@@ -157,7 +161,8 @@ inline bool is_translucent(int kind) { return kind == MAT_TRANSLUCENT || kind ==
 
 inline unsigned material_features(int kind)
 {
-    return kind == MAT_DIELECTRIC || kind == MAT_INTERFACE || kind == MAT_TRANSPARENT || kind == MAT_NORMAL
+    return kind == MAT_DIELECTRIC || kind == MAT_INTERFACE || kind == MAT_TRANSPARENT || kind == MAT_NORMAL ||
+                   kind == MAT_FLAT
                ? kFeatDielectric
                : (kind == MAT_TRANSLUCENT || kind == MAT_ROUGH_TRANSLUCENT ? kFeatTranslucent : 0u);
 }

@@ -205,6 +205,7 @@ int load_material(orf_context ctx, const JsonValue& mj)
     else if (iequals(name, "rough_translucent_shader")) program = "rough_subsurface_scattering_shader.cu";
     else if (iequals(name, "transparent_shader")) program = "transparent_shader.cu";
     else if (iequals(name, "normal_shader")) program = "normal_shader.cu";
+    else if (iequals(name, "flat_shader")) program = "flat_shader.cu";
     else throw std::runtime_error("scene: material '" + name + "' has no radiance program here");
     int m = -1;
     check(orf_material_create(ctx, program, &m));
@@ -236,6 +237,12 @@ int load_material(orf_context ctx, const JsonValue& mj)
         if (vec(p, k, v, 3)) set_f(ctx, S, m, k, v, 3);
     if (num(p, "scale", x)) set_f(ctx, S, m, "scale", &x, 1);
     if (num(p, "dipole_model", x)) set_u(ctx, S, m, "dipole_model", x);
+    // FlatMaterial::readJSON (ORF/FlatMaterial.cpp:205-245)
+    if (vec(p, "highlight_color", v, 3)) set_f(ctx, S, m, "highlight_color", v, 3);
+    if (num(p, "flat_shadow", x)) set_u(ctx, S, m, "flat_shadow", x);
+    if (num(p, "highlight", x)) set_u(ctx, S, m, "highlight", x);
+    if (num(p, "highlight_threshold", x)) set_f(ctx, S, m, "highlight_threshold", &x, 1);
+    if (num(p, "shininess", x)) set_f(ctx, S, m, "shininess", &x, 1);
     return m;
 }
 

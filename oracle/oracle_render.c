@@ -906,6 +906,60 @@ static v3 shade_dielectric(const oscene* s, const orc_material* m, v3 o, v3 d, c
 
 /* Fresnel reflectance at incidence cosine c through relative index eta; 1 under
  * total internal reflection (CU/lambertian_interface_shader.cu:68-75). */
+/* CU/flat_shader.cu:40-127: the flat colour, times the picked light's
+ * visibility with flat_shadow; with highlight, a visible light whose microfacet
+ * reflection Le G1(i) G1(o) D_wi R / (4 |i.n l.n|) has luminance above the
+ * threshold shows the highlight colour.  The microfacet normal the shader
+ * samples is never used: only its two random numbers are drawn. */
+static v3 shade_flat(const oscene* s, const orc_material* m, v3 o, v3 d, const ohit* h, int depth, uint32_t* t,
+                     ostats* st)
+{
+    if (depth > s->max_depth) return V3(0, 0, 0);
+    v3 hit_pos = vadd(o, vscale(d, h->t));
+    v3 normal = hit_normal(s, h);
+    v3 ffn = orc_faceforward(normal, vneg(d), normal);
+    v3 w_i = vneg(d);
+    v3 result = V3(m->diffuse[0], m->diffuse[1], m->diffuse[2]);
+    orc_rnd_tea(t);
+    orc_rnd_tea(t);
+    R V = 0;
+    uint32_t light_idx = (uint32_t)((R)s->nlights * orc_rnd_tea(t));
+    v3 w_l = V3(0, 0, 0), rad = V3(0, 0, 0);
+    if (s->nlights > 0) {
+        if (light_idx >= (uint32_t)s->nlights) light_idx = (uint32_t)s->nlights - 1;
+        R light_pdf = (R)1 / (R)s->nlights;
+        R dist, penv;
+        direct_light(s, &s->lights[light_idx], hit_pos, t, &w_l, &rad, &dist, &penv);
+        rad = V3(rad.x / light_pdf, rad.y / light_pdf, rad.z / light_pdf);
+        if (vdot(ffn, w_l) > 0) {
+            ohit sh;
+            st->shd++;
+            V = trace(s, hit_pos, w_l, s->eps, dist - s->eps, 1, &sh) ? (R)0 : (R)1;
+        }
+    }
+    if (m->flat_flags & 1) result = vscale(result, V);
+    if ((m->flat_flags & 2) && V > 0) {
+        v3 rn = orc_normalize(vadd(w_i, w_l));
+        R cr = vdot(rn, w_l), n12 = (R)1.0f / (R)m->ior;
+        R Rf = 1;
+        R st2 = n12 * n12 * ((R)1.0f - cr * cr);
+        if (st2 < (R)1.0f) {
+            R ct = sqrt((R)1.0f - st2);
+            Rf = orc_fresnel_R3(cr, ct, n12);
+        }
+        R ax = m->roughness[0], ay = m->roughness[1];
+        int nd = m->normal_distribution;
+        R gi = masking_xy(w_i, rn, ffn, ax, ay, nd);
+        R go = masking_xy(w_l, rn, ffn, ax, ay, nd);
+        R D = orc_eval_visible_normal(w_i, rn, ffn, ax, ay, nd);
+        R wr = gi * go * D * Rf / ((R)4.0f * fabs(vdot(w_i, ffn) * vdot(w_l, ffn)));
+        v3 spec = vscale(rad, wr);
+        R lum = (R)0.2126f * spec.x + (R)0.7152f * spec.y + (R)0.0722f * spec.z;
+        if (lum > (R)m->flat_threshold) result = vscale(V3(m->radiance[0], m->radiance[1], m->radiance[2]), V);
+    }
+    return result;
+}
+
 /* CU/transparent_shader.cu:51-114 (DIFFUSE_PART off): depth > max_depth ends the
  * path; inside, Beer absorption with Russian roulette on its mean and the
  * normal and index flipped; the Fresnel reflectance picks mirror or refraction. */
@@ -1457,6 +1511,7 @@ static v3 shade_hit(const oscene* s, v3 o, v3 d, const ohit* hp, int depth, int 
     if (m->kind == ORC_CONDUCTOR) return shade_conductor(s, m, &s->eta[s->tri_mat[h.prim]], o, d, &h, depth, seed, st);
     if (m->kind == ORC_DIELECTRIC) return shade_dielectric(s, m, o, d, &h, depth, seed, st);
     if (m->kind == ORC_TRANSPARENT) return shade_transparent(s, m, o, d, &h, depth, seed, st);
+    if (m->kind == ORC_FLAT) return shade_flat(s, m, o, d, &h, depth, seed, st);
     if (m->kind == ORC_NORMAL) {
         /* CU/normal_shader.cu:25-29: normal * 0.5 + 0.5 */
         v3 n = hit_normal(s, &h);

@@ -22,7 +22,8 @@ enum {
     ORC_TRANSLUCENT = 5,
     ORC_ROUGH_TRANSLUCENT = 6,
     ORC_TRANSPARENT = 7, /* transparent_shader.cu */
-    ORC_NORMAL = 8       /* normal_shader.cu */
+    ORC_NORMAL = 8,      /* normal_shader.cu */
+    ORC_FLAT = 9         /* flat_shader.cu */
 };
 enum { ORC_POINT = 0, ORC_DIRECTIONAL = 1, ORC_TRIANGLES = 2, ORC_DISK = 3, ORC_SPHERE = 4, ORC_ENV = 5 };
 
@@ -42,6 +43,10 @@ typedef struct {
     float sss_scale;
     int32_t dipole_model;
     int32_t pad2;
+    /* flat material: bit 0 flat_shadow, bit 1 highlight; the highlight colour
+     * is `radiance` */
+    int32_t flat_flags;
+    float flat_threshold;
 } orc_material;
 
 typedef struct {

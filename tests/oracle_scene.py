@@ -16,7 +16,7 @@ class OrcMaterial(ctypes.Structure):
                 ("ior_re", ctypes.c_double * 3), ("ior_im", ctypes.c_double * 3),
                 ("sss_absorption", ctypes.c_float * 3), ("sss_scattering", ctypes.c_float * 3),
                 ("sss_meancosine", ctypes.c_float * 3), ("sss_scale", ctypes.c_float), ("dipole_model", ctypes.c_int32),
-                ("pad2", ctypes.c_int32)]
+                ("pad2", ctypes.c_int32), ("flat_flags", ctypes.c_int32), ("flat_threshold", ctypes.c_float)]
 
 
 class OrcLight(ctypes.Structure):
@@ -41,7 +41,8 @@ class OrcSceneDesc(ctypes.Structure):
 
 _KIND = {"diffuse_shader.cu": 0, "metallic_shader.cu": 1, "arealight_shader.cu": 2, "rough_transparent_shader.cu": 3,
          "lambertian_interface_shader.cu": 4, "subsurface_scattering_shader.cu": 5,
-         "rough_subsurface_scattering_shader.cu": 6, "transparent_shader.cu": 7, "normal_shader.cu": 8}
+         "rough_subsurface_scattering_shader.cu": 6, "transparent_shader.cu": 7, "normal_shader.cu": 8,
+         "flat_shader.cu": 9}
 # Wholemilk (ORF/RoughTranslucentMaterial.cpp:258-263) and the default scale
 SSS_DEFAULTS = {"absorption": (0.0011, 0.0024, 0.014), "scattering": (2.55, 3.21, 3.77),
                 "meancosine": (0.0, 0.0, 0.0), "scale": (100.0,)}
@@ -112,6 +113,16 @@ class OracleScene:
                 getattr(mm, field)[:] = [float(x) for x in m.floats.get(key, SSS_DEFAULTS[key])]
             mm.sss_scale = m.floats.get("scale", SSS_DEFAULTS["scale"])[0]
             mm.dipole_model = m.uints.get("dipole_model", (0,))[0]
+            if mm.kind == 9:  # FlatMaterial defaults (ORF/FlatMaterial.cpp:5-21)
+                mm.diffuse[:] = [float(x) for x in m.floats.get("diffuse_color", (1.0, 1.0, 1.0))]
+                mm.radiance[:] = [float(x) for x in m.floats.get("highlight_color", (1.0, 1.0, 1.0))]
+                r = m.floats.get("roughness", (0.999, 0.999))
+                mm.roughness[0], mm.roughness[1] = r[0], r[-1]
+                mm.normal_distribution = m.uints.get("normal_distribution", (0,))[0]
+                mm.ior = m.floats.get("ior", (1.3,))[0]
+                mm.flat_flags = (1 if m.uints.get("flat_shadow", (0,))[0] == 1 else 0) | \
+                    (2 if m.uints.get("highlight", (0,))[0] == 1 else 0)
+                mm.flat_threshold = m.floats.get("highlight_threshold", (0.0,))[0]
             ior = m.ior_complex or ((1.5, 0.0),) * 3
             for k in range(3):
                 mm.ior_re[k] = ior[k][0]

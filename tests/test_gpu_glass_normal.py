@@ -125,3 +125,38 @@ def test_scene_file_with_glass_and_normal_display(tmp_path):
     ok = _ok(got, want)
     _say("scene file: samples ok %.5f" % ok.mean())
     assert ok.mean() >= 0.99
+
+
+def _flat_scene():
+    """The path-tracer scene with a flat ground (shadowed, highlighted above a
+    luminance threshold that splits the lit region) and a conductor sphere."""
+    d = scenes.path_scene(width=W, height=W, level=4, env_size=(128, 64), max_depth=6)
+    d.materials[0] = scenes.Material("flat_shader.cu",
+                                     floats={"diffuse_color": (0.3, 0.5, 0.2), "highlight_color": (1.0, 0.9, 0.6),
+                                             "highlight_threshold": (0.05,), "roughness": (0.4, 0.4)},
+                                     uints={"flat_shadow": (1,), "highlight": (1,), "normal_distribution": (1,)})
+    return d
+
+
+def test_flat_material_per_sample_and_batch():
+    from tests.oracle_scene import OracleScene
+
+    d = _flat_scene()
+    ctx = scenes.to_context(d)
+    ctx.accel_build()
+    orc = OracleScene(d)
+    ys, xs = np.mgrid[0:W, 0:W]
+    for frame in (0, 3):
+        ctx.film_clear()
+        ctx.launch(W, W, frame, 1)
+        got = ctx.film_sums(W, W)[..., :3]
+        want = orc.samples(W, W, xs.ravel(), ys.ravel(), frame).reshape(W, W, 3)
+        ok = _ok(got, want)
+        hl = np.all(np.abs(want - [1.0, 0.9, 0.6]) < 1e-6, axis=-1).mean()
+        _say("flat frame %d samples ok %.5f (highlighted %.3f)" % (frame, ok.mean(), hl))
+        assert ok.mean() >= 0.99
+    items = [{"width": W, "height": W, "spp": 2, "frame0": 0, "eye": d.eye, "U": d.U, "V": d.V, "W": d.W}]
+    ctx.render_batch(items, None, mode=rt.BATCH_MEGAKERNEL)
+    ref = ctx.batch_film(0, W, W, sums=True)
+    ctx.render_batch(items, None, mode=rt.BATCH_WAVEFRONT)
+    np.testing.assert_allclose(ctx.batch_film(0, W, W, sums=True)[..., :3], ref[..., :3], rtol=1e-9, atol=1e-12)

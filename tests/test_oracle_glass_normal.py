@@ -121,3 +121,36 @@ def test_shadow_rays_pass_through_normal_display(blocker, expect):
         np.testing.assert_array_equal(a[..., 1], b[..., 1])
     else:
         assert a[..., :3].max() == 0
+
+
+def _flat_scene(floats=None, uints=None, blocker=False):
+    """A flat ground under a point light straight above; optionally a diffuse
+    blocker between them.  The camera looks at the ground below the light."""
+    ground = _plane(-1.0, 10.0, 0)
+    meshes = [ground]
+    mats = [scenes.Material("flat_shader.cu", floats=floats or {}, uints=uints or {}),
+            scenes.Material("diffuse_shader.cu")]
+    if blocker:
+        bv, bt = scenes.quad(1.0, 0.5, facing_down=True)
+        meshes.append(scenes.Mesh(bv, bt, 1))
+    light = [scenes.Light("point", (10.0, 10.0, 10.0), position=(0.0, 3.0, 0.0))]
+    cam = scenes.pinhole((1.0, 0.0, 0.0), (0.0, -1.0, 0.0), (0.0, 1.0, 0.0), 0.5, 4, 4)
+    return scenes.SceneDesc(meshes, mats, light, None, False, *cam, 4, 4, 0)
+
+
+@pytest.mark.parametrize("shadow,blocker,want", [(0, True, (0.2, 0.4, 0.6)), (1, False, (0.2, 0.4, 0.6)),
+                                                 (1, True, (0.0, 0.0, 0.0))])
+def test_flat_colour_and_shadow(shadow, blocker, want):
+    d = _flat_scene({"diffuse_color": (0.2, 0.4, 0.6)}, {"flat_shadow": (shadow,)}, blocker)
+    img, _ = OracleScene(d).render(4, 4, 0, 2)
+    np.testing.assert_allclose(img[..., :3] / img[..., 3:4], np.broadcast_to(want, (4, 4, 3)), rtol=1e-6)
+
+
+@pytest.mark.parametrize("threshold,want", [(0.0, (1.0, 0.5, 0.0)), (1e30, (0.2, 0.4, 0.6))])
+def test_flat_highlight_threshold(threshold, want):
+    """A visible light with a positive specular term shows the highlight colour
+    at threshold 0 and never at an unreachable threshold."""
+    d = _flat_scene({"diffuse_color": (0.2, 0.4, 0.6), "highlight_color": (1.0, 0.5, 0.0),
+                     "highlight_threshold": (threshold,), "roughness": (0.5, 0.5)}, {"highlight": (1,)})
+    img, _ = OracleScene(d).render(4, 4, 0, 2)
+    np.testing.assert_allclose(img[..., :3] / img[..., 3:4], np.broadcast_to(want, (4, 4, 3)), rtol=1e-6)
