@@ -168,6 +168,14 @@ orf_status orf_accel_build(orf_context ctx, int64_t* stats);
  * FP64 sums of finite samples and their count; it is cleared when the size
  * changes or by orf_film_clear. */
 orf_status orf_launch(orf_context ctx, int width, int height, uint32_t frame0, uint32_t spp, void* stream);
+/* setRayGenerationProgram of the Integrator classes (ORF/Integrator.cpp:26-50
+ * PathTracer, :100-118 DepthTracer): which program orf_launch runs.  The depth
+ * tracer writes (1 + n) / 2 of the face-forwarded shading normal as RGB and
+ * the hit distance as alpha (CU/depth_tracer.cu:33-58, CU/depth_shader.cu:13-19);
+ * orf_film_read returns that alpha.  Batch and differentiable renders always
+ * path trace. */
+enum { ORF_INTEGRATOR_PATH_TRACER = 0, ORF_INTEGRATOR_DEPTH = 1 };
+orf_status orf_integrator_set(orf_context ctx, int integrator);
 orf_status orf_film_clear(orf_context ctx);
 /* Film mean as RGBA (A = 1) in FP64 (rgba64) and/or FP32 (rgba32); host arrays
  * of 4*width*height, row 0 = bottom row (launch index y = 0). */
@@ -305,7 +313,7 @@ orf_status orf_gen_sky_envmap(int width, int height, const float sun_dir[3], flo
  * Scene files.
  *
  * orf_scene_load_json replaces OptixSceneLoader::readJSON
- * (ORF/OptixSceneLoader.cpp:105-177): Integrator (PathTracer), Background
+ * (ORF/OptixSceneLoader.cpp:105-177): Integrator (PathTracer, DepthTracer), Background
  * (EnvMapBackground with a Radiance .hdr, ConstantBackground), Camera
  * (PinholeCamera), Geometries (obj with a material {mtl_name, mtl_parameters})
  * and Lights (PointLight, DirectionalLight, TrianglesAreaLight, DiskAreaLight,

@@ -411,6 +411,7 @@ void run_probe_pass(orf_context_t* c, unsigned frame, hipStream_t stream)
 void film_clear(orf_context_t* c)
 {
     if (c->d_film.p) ORF_HIP_CHECK(hipMemset(c->d_film.p, 0, c->d_film.bytes));
+    if (c->d_depth.p) ORF_HIP_CHECK(hipMemset(c->d_depth.p, 0, c->d_depth.bytes));
     if (c->d_counters.p) ORF_HIP_CHECK(hipMemset(c->d_counters.p, 0, c->d_counters.bytes));
 }
 
@@ -419,6 +420,8 @@ void film_ensure(orf_context_t* c, int w, int h)
     if (w == c->film_w && h == c->film_h && c->d_film.p) return;
     c->d_film.release();
     c->d_film.ensure((size_t)w * h * 4 * sizeof(double));
+    c->d_depth.release();
+    c->d_depth.ensure((size_t)w * h * sizeof(double));
     c->d_counters.ensure(16 * sizeof(unsigned long long));  // [0..3] counters, [8..15] queue counters
     c->film_w = w;
     c->film_h = h;
@@ -785,10 +788,24 @@ extern "C" orf_status orf_launch(orf_context ctx, int width, int height, uint32_
         cam.patch_dims[0] = (unsigned)width;
         cam.patch_dims[1] = (unsigned)height;
     }
-    orf::run_probe_pass(c, frame0, st);
     orf::FilmGPU film{c->d_film.as<double>(), c->d_counters.as<unsigned long long>(), width, height,
                       c->d_counters.as<unsigned long long>() + 8};
+    if (c->integrator == 1) {  // DepthTracer (ORF/Integrator.cpp:100-118)
+        orf::launch_depth(orf::scene_view(c), cam, film, c->d_depth.as<double>(), frame0, spp, st);
+        return ORF_SUCCESS;
+    }
+    orf::run_probe_pass(c, frame0, st);
     orf::launch_megakernel(orf::scene_view(c), cam, film, nullptr, frame0, spp, st);
+    ORF_API_END
+}
+
+extern "C" orf_status orf_integrator_set(orf_context ctx, int integrator)
+{
+    ORF_API_BEGIN
+    orf_context_t* c = ctx_check(ctx);
+    ORF_REQUIRE(integrator == ORF_INTEGRATOR_PATH_TRACER || integrator == ORF_INTEGRATOR_DEPTH,
+                "orf_integrator_set: unknown integrator");
+    c->integrator = integrator;
     ORF_API_END
 }
 
@@ -821,6 +838,11 @@ extern "C" orf_status orf_film_read(orf_context ctx, double* rgba64, float* rgba
     std::vector<double> s(4 * n);
     ORF_HIP_CHECK(hipDeviceSynchronize());
     ORF_HIP_CHECK(hipMemcpy(s.data(), c->d_film.p, s.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> dep;
+    if (c->integrator == 1) {
+        dep.resize(n);
+        ORF_HIP_CHECK(hipMemcpy(dep.data(), c->d_depth.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    }
     for (size_t i = 0; i < n; ++i) {
         double cnt = s[4 * i + 3];
         for (int k = 0; k < 3; ++k) {
@@ -828,8 +850,11 @@ extern "C" orf_status orf_film_read(orf_context ctx, double* rgba64, float* rgba
             if (rgba64) rgba64[4 * i + k] = v;
             if (rgba32) rgba32[4 * i + k] = (float)v;
         }
-        if (rgba64) rgba64[4 * i + 3] = 1.0;
-        if (rgba32) rgba32[4 * i + 3] = 1.0f;
+        // alpha: 1 for the path tracer (CU/path_tracer.cu:67), the mean hit
+        // distance for the depth tracer (CU/depth_tracer.cu:52)
+        double a = dep.empty() ? 1.0 : (cnt > 0 ? dep[i] / cnt : 0.0);
+        if (rgba64) rgba64[4 * i + 3] = a;
+        if (rgba32) rgba32[4 * i + 3] = (float)a;
     }
     ORF_API_END
 }

@@ -70,7 +70,10 @@ struct orf_context_t {
     // differentiable renders
     orf::DevBuf d_diff_views, d_diff_stats;
     long long last_batch_iterations = 0;
-    // the scene document last read by orf_scene_load_json (saved back by orf_scene_save_json)
+    // ray generation program: 0 path tracer, 1 depth tracer; the depth film
+    int integrator = 0;
+    orf::DevBuf d_depth;
+    // the scene document last read by orf_scene_load_json (saved back by orf_scene_save)
     std::string scene_doc;
 };
 

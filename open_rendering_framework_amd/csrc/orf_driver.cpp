@@ -32,11 +32,16 @@ std::vector<float> film_mean(orf_context_t* c)
     std::vector<double> s(4 * n);
     ORF_HIP_CHECK(hipDeviceSynchronize());
     ORF_HIP_CHECK(hipMemcpy(s.data(), c->d_film.p, s.size() * sizeof(double), hipMemcpyDeviceToHost));
+    std::vector<double> dep;
+    if (c->integrator == 1) {
+        dep.resize(n);
+        ORF_HIP_CHECK(hipMemcpy(dep.data(), c->d_depth.p, n * sizeof(double), hipMemcpyDeviceToHost));
+    }
     std::vector<float> m(4 * n);
     for (size_t i = 0; i < n; ++i) {
         const double cnt = s[4 * i + 3];
         for (int k = 0; k < 3; ++k) m[4 * i + k] = (float)(cnt > 0 ? s[4 * i + k] / cnt : 0.0);
-        m[4 * i + 3] = 1.0f;
+        m[4 * i + 3] = dep.empty() ? 1.0f : (float)(cnt > 0 ? dep[i] / cnt : 0.0);  // depth tracer: hit distance
     }
     return m;
 }
@@ -180,14 +185,17 @@ extern "C" orf_status orf_film_load_raw(orf_context ctx, const char* path, int w
     if (got != m.size() || extra)
         ORF_THROW(ORF_ERROR_FILE_FORMAT, std::string("'") + path + "' is not a " + std::to_string(width) + " x " +
                                              std::to_string(height) + " FP32 RGBA film");
-    std::vector<double> s(4 * n);
+    std::vector<double> s(4 * n), dep(n);
     for (size_t i = 0; i < n; ++i) {
         for (int k = 0; k < 3; ++k) s[4 * i + k] = (double)m[4 * i + k] * (double)frames;
         s[4 * i + 3] = (double)frames;
+        dep[i] = (double)m[4 * i + 3] * (double)frames;  // the depth tracer's alpha
     }
     film_ensure(c, width, height);
     ORF_HIP_CHECK(hipDeviceSynchronize());
     ORF_HIP_CHECK(hipMemcpy(c->d_film.p, s.data(), s.size() * sizeof(double), hipMemcpyHostToDevice));
+    if (c->integrator == 1)
+        ORF_HIP_CHECK(hipMemcpy(c->d_depth.p, dep.data(), dep.size() * sizeof(double), hipMemcpyHostToDevice));
     ORF_API_END
 }
 

@@ -60,6 +60,11 @@ void zero_queue_counters(unsigned long long* qctr, hipStream_t stream);
 void launch_megakernel(const SceneGPU& sc, const CameraGPU& cam, const FilmGPU& film, const int* remap,
                        unsigned frame0, unsigned spp, hipStream_t stream);
 
+// Depth integrator (DepthTracer): frames frame0 .. frame0+spp-1; normal colour
+// into film.sums, hit distance sums into depth (one double per pixel).
+void launch_depth(const SceneGPU& sc, const CameraGPU& cam, const FilmGPU& film, double* depth, unsigned frame0,
+                  unsigned spp, hipStream_t stream);
+
 // Persistent batched megakernel: one launch for every item of a batch.  Work
 // items are (render, 8x8 tile, sample chunk) in render order, split into 8
 // contiguous partitions, one per XCD; a wave takes items from its own XCD's

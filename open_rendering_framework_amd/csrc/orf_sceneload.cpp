@@ -314,7 +314,9 @@ void scene_load_json(orf_context ctx, const std::string& path, uint32_t flags, o
 
     if (const JsonValue* ij = doc.get("Integrator")) {
         const std::string type = str(*ij, "type");
-        ORF_REQUIRE(iequals(type, "PathTracer"), "scene: integrator '" + type + "' has no program here");
+        const bool depth = iequals(type, "DepthTracer");
+        ORF_REQUIRE(iequals(type, "PathTracer") || depth, "scene: integrator '" + type + "' has no program here");
+        check(orf_integrator_set(ctx, depth ? ORF_INTEGRATOR_DEPTH : ORF_INTEGRATOR_PATH_TRACER));
         const JsonValue* p = params(*ij);
         if (num(p, "max_depth", x)) {
             int32_t d = (int32_t)x;

@@ -82,6 +82,7 @@ _SIGS = {
     "orf_probe_pass": [c_vp, c_u32, c_vp],
     "orf_diff_adjoint": [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp],
     "orf_scene_load_json": [c_vp, ctypes.c_char_p, c_u32, c_vp],
+    "orf_integrator_set": [c_vp, ctypes.c_int],
     "orf_render_frames": [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, c_vp],
     "orf_film_save_raw": [c_vp, ctypes.c_char_p],
     "orf_film_load_raw": [c_vp, ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_uint32],
@@ -222,6 +223,7 @@ class SceneInfo(ctypes.Structure):
 
 
 LOAD_ENV_IMPORTANCE = 1
+INTEGRATOR_PATH_TRACER, INTEGRATOR_DEPTH = 0, 1
 SCOPE_CONTEXT, SCOPE_MATERIAL = 0, 1
 BATCH_WAVEFRONT, BATCH_MEGAKERNEL, BATCH_PERSISTENT = 0, 1, 2
 
@@ -512,6 +514,10 @@ class Context:
              ctypes.byref(info))
         return {"width": info.width, "height": info.height, "frame_count": info.frame_count, "meshes": info.meshes,
                 "materials": info.materials, "lights": info.lights, "buffer_path": info.buffer_path.decode()}
+
+    def integrator(self, kind):
+        """INTEGRATOR_PATH_TRACER or INTEGRATOR_DEPTH (orf_integrator_set)."""
+        call("orf_integrator_set", self.handle, kind)
 
     def render_frames(self, width, height, frame_begin, frame_end, stream=None):
         """One launch per frame in [frame_begin, frame_end) (orf_render_frames)."""

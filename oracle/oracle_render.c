@@ -1706,6 +1706,37 @@ void orc_render(void* p, int W, int H, int x0, int y0, int x1, int y1, uint32_t 
     }
 }
 
+/* DepthTracer (CU/depth_tracer.cu:33-58 + CU/depth_shader.cu:13-19): per pixel
+ * of `frame`, (1 + n) / 2 of the face-forwarded shading normal and the hit
+ * distance; 0.5 grey and 0 where the ray leaves the scene or meets a disk or
+ * sphere emitter (no depth program on their materials).  out: 4 per pixel. */
+void orc_depth_samples(void* p, int W, int H, const int32_t* px, const int32_t* py, int n, uint32_t frame,
+                       double* out)
+{
+    const oscene* s = (const oscene*)p;
+    for (int i = 0; i < n; i++) {
+        int x = px[i], y = py[i];
+        uint32_t seed = orc_tea16((uint32_t)W * (uint32_t)y + (uint32_t)x, frame);
+        R jx = orc_rnd_tea(&seed);
+        R jy = orc_rnd_tea(&seed);
+        R ipx = ((R)x + jx) / (R)W * (R)2.0f - (R)1.0f;
+        R ipy = ((R)y + jy) / (R)H * (R)2.0f - (R)1.0f;
+        v3 dir = orc_normalize(vadd(vadd(vscale(s->U, ipx), vscale(s->V, ipy)), s->W));
+        ohit h;
+        v3 nrm = V3(0, 0, 0);
+        R t = 0;
+        if (trace(s, s->eye, dir, s->eps, RAY_MAX, 0, &h) && h.prim < s->ntris) {
+            v3 nn = hit_normal(s, &h);
+            nrm = orc_faceforward(nn, vneg(dir), nn);
+            t = h.t;
+        }
+        out[4 * i] = (double)(((R)1.0f + nrm.x) * (R)0.5f);
+        out[4 * i + 1] = (double)(((R)1.0f + nrm.y) * (R)0.5f);
+        out[4 * i + 2] = (double)(((R)1.0f + nrm.z) * (R)0.5f);
+        out[4 * i + 3] = (double)t;
+    }
+}
+
 void orc_render_samples(void* p, int W, int H, const int32_t* px, const int32_t* py, int n, uint32_t frame,
                         double* out, int threads)
 {

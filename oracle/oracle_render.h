@@ -95,6 +95,9 @@ void orc_render(void* s, int W, int H, int x0, int y0, int x1, int y1, uint32_t 
 /* Per-sample radiance of one frame for a pixel list: out has 3 doubles per pixel. */
 void orc_render_samples(void* s, int W, int H, const int32_t* px, const int32_t* py, int n, uint32_t frame,
                         double* out, int threads);
+/* The depth integrator's per-pixel (R, G, B, hit distance) of one frame. */
+void orc_depth_samples(void* s, int W, int H, const int32_t* px, const int32_t* py, int n, uint32_t frame,
+                       double* out);
 /* The environment tables the oracle built (FP64 -> out arrays as double). */
 void orc_env_tables(void* s, double* marg_pdf, double* marg_cdf, double* cond_pdf, double* cond_cdf);
 void orc_env_sample_batch(void* s, uint32_t seed, int n, double* dir, double* pdf_sample, double* pdf_eval);

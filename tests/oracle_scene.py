@@ -64,6 +64,8 @@ class OracleScene:
                                                                ctypes.c_int]
         L.orc_render_samples.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, c_vp, c_vp, ctypes.c_int,
                                          ctypes.c_uint32, c_vp, ctypes.c_int]
+        L.orc_depth_samples.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, c_vp, c_vp, ctypes.c_int, ctypes.c_uint32,
+                                        c_vp]
         L.orc_env_tables.argtypes = [c_vp] * 5
         L.orc_env_sample_batch.argtypes = [c_vp, ctypes.c_uint32, ctypes.c_int, c_vp, c_vp, c_vp]
         L.orc_env_bsearch.restype = ctypes.c_int
@@ -208,6 +210,14 @@ class OracleScene:
         py = np.ascontiguousarray(py, np.int32)
         out = np.zeros((len(px), 3), np.float64)
         self.lib.orc_render_samples(self.handle, width, height, _p(px), _p(py), len(px), frame, _p(out), threads)
+        return out
+
+    def depth_samples(self, width, height, px, py, frame):
+        """Depth integrator: (n, 4) normal colour and hit distance for one frame."""
+        px = np.ascontiguousarray(px, np.int32)
+        py = np.ascontiguousarray(py, np.int32)
+        out = np.zeros((len(px), 4))
+        self.lib.orc_depth_samples(self.handle, width, height, _p(px), _p(py), len(px), frame, _p(out))
         return out
 
     def env_tables(self):
