@@ -60,6 +60,7 @@ static int program_kind(const std::string& prog)
     if (p == "transparent_shader") return MAT_TRANSPARENT;
     if (p == "normal_shader") return MAT_NORMAL;
     if (p == "flat_shader") return MAT_FLAT;
+    if (p == "rough_diffuse_shader") return MAT_ROUGH_DIFFUSE;
     return -1;
 }
 
@@ -334,6 +335,8 @@ void prepare(orf_context_t* c, hipStream_t stream)
             sss.push_back(scattering_coefficients(c->mat_sss[i]));
             m[i].sss = is_translucent(m[i].kind) ? (int)i : -1;
             const bool ms = m[i].microfacet_model == MULTISCATTERING_MODEL;
+            ORF_REQUIRE(!(ms && m[i].kind == MAT_ROUGH_DIFFUSE),
+                        "material " + std::to_string(i) + ": the rough diffuse multiple-scattering walk is not built");
             ORF_REQUIRE(!(ms && m[i].kind == MAT_CONDUCTOR),
                         "material " + std::to_string(i) + ": the conductor multiple-scattering walk is not built");
             ORF_REQUIRE(!(ms && m[i].normal_distribution == BECKMANN_DISTRIBUTION &&
@@ -527,6 +530,13 @@ extern "C" orf_status orf_material_create(orf_context ctx, const char* program, 
     c->mats.push_back(orf::default_material(kind));
     c->mat_sss.push_back(orf::SssParamsHost());
     if (orf::is_translucent(kind) || kind == orf::MAT_TRANSPARENT) c->mats.back().ior = 1.3f;
+    if (kind == orf::MAT_ROUGH_DIFFUSE) {  // RoughDiffuseMaterial (ORF/RoughDiffuseMaterial.cpp:7-17)
+        orf::MaterialGPU& m = c->mats.back();
+        m.diffuse_color = orf::mk3(1.0f);
+        m.ior = 1.3f;
+        m.normal_distribution = orf::BECKMANN_DISTRIBUTION;
+        m.microfacet_model = orf::WALTER_MODEL;
+    }
     if (kind == orf::MAT_FLAT) {  // FlatMaterial::FlatMaterial (ORF/FlatMaterial.cpp:5-21)
         orf::MaterialGPU& m = c->mats.back();
         m.diffuse_color = orf::mk3(1.0f);

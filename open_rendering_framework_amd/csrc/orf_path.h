@@ -250,6 +250,91 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         ps.L += ps.thr * (normal * 0.5f + 0.5f);
         return false;
     }
+    if (kDiel && mat.kind == MAT_ROUGH_DIFFUSE) {
+        // CU/rough_diffuse_shader.cu:40-290 (visible-normal and Walter models
+        // alike; the multiple-scattering walk is refused at build): a Lambert
+        // base under a rough dielectric interface.  NEE adds the transmitted
+        // diffuse and the microfacet-reflected terms (shadow segment to the full
+        // light distance, emission off once a light is seen); the continuation
+        // is the diffuse bounce about the sampled microfacet normal with
+        // probability 1 - R_i, else the mirror about it, weighted as the
+        // reference weights them (not divided by the choice probability).
+        if (ps.depth > sc.max_depth) return false;
+        const f3 w_i = -d;
+        const float n12 = 1.0f / mat.ior;
+        const float r10 = two_C1(mat.ior);
+        const float ax = mat.roughness.x, ay = mat.roughness.y;
+        const int nd = mat.normal_distribution;
+        const f3 rho = mat.diffuse_color;
+        const f3 denom = mk3(1.0f - rho.x * r10, 1.0f - rho.y * r10, 1.0f - rho.z * r10);
+        const float z1 = rnd_tea(ps.rng);
+        const float z2 = rnd_tea(ps.rng);
+        const f3 mn = sample_visible_normal(w_i, ffn, ax, ay, z1, z2, nd);
+        const float ci = dot(w_i, mn);
+        float Ri = 1.0f;
+        float st2 = n12 * n12 * (1.0f - ci * ci);
+        if (st2 < 1.0f) Ri = fresnel_R3(ci, sqrtf(1.0f - st2), n12);
+        const float Ti = 1.0f - Ri;
+        unsigned li = (unsigned)((float)sc.nlights * rnd_tea(ps.rng));
+        if (sc.nlights > 0) {
+            if (li >= (unsigned)sc.nlights) li = sc.nlights - 1;
+            const float light_pdf = 1.0f / (float)sc.nlights;
+            f3 wl = mk3(0.0f), rad = mk3(0.0f);
+            float dist = 0.0f, penv;
+            sample_light(sc, sc.lights[li], p, ps.rng, wl, rad, dist, penv);
+            rad = rad / light_pdf;
+            const float cl = dot(ffn, wl);
+            // the environment reaches this material through escaping rays only
+            if (cl > 0.0f && penv < 0.0f) {
+                const f3 rn = normalize(w_i + wl);
+                const float cr = dot(rn, wl);
+                float Rr = 1.0f;
+                st2 = n12 * n12 * (1.0f - cr * cr);
+                if (st2 < 1.0f) Rr = fresnel_R3(cr, sqrtf(1.0f - st2), n12);
+                const float cd = dot(mn, wl);
+                float Rd = 1.0f;
+                st2 = n12 * n12 * (1.0f - cd * cd);
+                if (st2 < 1.0f) Rd = fresnel_R3(cd, sqrtf(1.0f - st2), n12);
+                const float Td = 1.0f - Rd;
+                const float gi = masking_g1(w_i, rn, ffn, ax, ay, nd);
+                const float go = masking_g1(wl, rn, ffn, ax, ay, nd);
+                const float god = masking_g1(wl, mn, ffn, ax, ay, nd);
+                const float D = eval_visible_normal(w_i, rn, ffn, ax, ay, nd);
+                const float wr = gi * go * D / (4.0f * fabsf(dot(w_i, ffn) * dot(wl, ffn)));
+                const f3 diff = rad * kInvPi * cl * rho * n12 * n12 * Ti * Td / denom * god;
+                const f3 spec = rad * Rr * wr;
+                sr.o = p;
+                sr.d = wl;
+                sr.tmax = dist;  // the reference's shadow ray runs to the light itself (:96)
+                sr.contrib = ps.thr * (diff + spec);
+                sr.emit = 0;
+                sr.valid = true;
+            }
+        }
+        const float xi = rnd_tea(ps.rng);
+        f3 nd_dir, w;
+        if (xi > Ri) {
+            nd_dir = sample_cosine(mn, ps.rng);
+            const float cr = dot(nd_dir, mn);
+            const float gom = masking_g1(nd_dir, mn, ffn, ax, ay, nd);
+            if (gom < 0.0f) return false;
+            float Rr = 1.0f;
+            st2 = n12 * n12 * (1.0f - cr * cr);
+            if (st2 < 1.0f) Rr = fresnel_R3(cr, sqrtf(1.0f - st2), n12);
+            w = rho * n12 * n12 * (1.0f - Rr) / denom * gom;
+        } else {
+            nd_dir = d - (mn * 2.0f) * dot(mn, d);
+            const float gom = masking_g1(nd_dir, mn, ffn, ax, ay, nd);
+            if (gom < 0.0f) return false;
+            w = mk3(gom);
+        }
+        ps.thr = ps.thr * w;
+        ps.o = p;
+        ps.d = nd_dir;
+        ps.depth++;
+        ps.prev_bsdf_pdf = -1.0f;
+        return true;
+    }
     if (kDiel && mat.kind == MAT_FLAT) {
         // CU/flat_shader.cu:40-127: the flat colour; with flat_shadow it is
         // multiplied by the light's visibility V, and with highlight a visible

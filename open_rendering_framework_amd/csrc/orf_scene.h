@@ -32,6 +32,7 @@ enum MaterialKind : int {
     MAT_TRANSPARENT = 7,    // transparent_shader.cu (smooth glass)
     MAT_NORMAL = 8,         // normal_shader.cu (shading-normal display)
     MAT_FLAT = 9,           // flat_shader.cu (flat colour, optional shadow and highlight)
+    MAT_ROUGH_DIFFUSE = 10, // rough_diffuse_shader.cu (Lambert base under a rough dielectric)
     MAT_COUNT
 };
 
@@ -152,7 +153,7 @@ struct SceneGPU {
 // Material code a megakernel variant is compiled with: Lambert, conductor and
 // emitter always; kFeatDielectric adds the rough dielectric (with the
 // microsurface walk), the Lambertian interface, smooth glass, the normal
-// display and the flat material, kFeatTranslucent the
+// display, the flat and the rough diffuse materials, kFeatTranslucent the
 // translucent material.  Launches pick the smallest variant covering the scene,
 // so simple scenes do not pay the register pressure of unused materials.
 constexpr unsigned kFeatDielectric = 1u, kFeatTranslucent = 2u, kFeatAll = 3u;
@@ -162,7 +163,7 @@ inline bool is_translucent(int kind) { return kind == MAT_TRANSLUCENT || kind ==
 inline unsigned material_features(int kind)
 {
     return kind == MAT_DIELECTRIC || kind == MAT_INTERFACE || kind == MAT_TRANSPARENT || kind == MAT_NORMAL ||
-                   kind == MAT_FLAT
+                   kind == MAT_FLAT || kind == MAT_ROUGH_DIFFUSE
                ? kFeatDielectric
                : (kind == MAT_TRANSLUCENT || kind == MAT_ROUGH_TRANSLUCENT ? kFeatTranslucent : 0u);
 }

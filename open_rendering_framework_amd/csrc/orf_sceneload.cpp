@@ -206,6 +206,7 @@ int load_material(orf_context ctx, const JsonValue& mj)
     else if (iequals(name, "transparent_shader")) program = "transparent_shader.cu";
     else if (iequals(name, "normal_shader")) program = "normal_shader.cu";
     else if (iequals(name, "flat_shader")) program = "flat_shader.cu";
+    else if (iequals(name, "rough_diffuse_shader")) program = "rough_diffuse_shader.cu";
     else throw std::runtime_error("scene: material '" + name + "' has no radiance program here");
     int m = -1;
     check(orf_material_create(ctx, program, &m));

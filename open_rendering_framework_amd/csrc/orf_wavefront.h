@@ -186,7 +186,7 @@ ORF_HD int hit_kind(const WfView& v, int slot, const Hit& h)
     if (k == MAT_LAMBERT || k == MAT_INTERFACE) return Q_LAMBERT;
     if (k == MAT_CONDUCTOR) return Q_COND;
     if (k == MAT_DIELECTRIC || k == MAT_TRANSLUCENT || k == MAT_ROUGH_TRANSLUCENT || k == MAT_TRANSPARENT ||
-        k == MAT_NORMAL || k == MAT_FLAT)
+        k == MAT_NORMAL || k == MAT_FLAT || k == MAT_ROUGH_DIFFUSE)
         return Q_DIEL;
     return Q_EMIT;  // kinds without a program end their path like an emitter that does not emit
 }

@@ -906,6 +906,115 @@ static v3 shade_dielectric(const oscene* s, const orc_material* m, v3 o, v3 d, c
 
 /* Fresnel reflectance at incidence cosine c through relative index eta; 1 under
  * total internal reflection (CU/lambertian_interface_shader.cu:68-75). */
+static R two_C1(R n);
+
+/* CU/rough_diffuse_shader.cu:40-290 without the multiple-scattering walk: a
+ * Lambert base under a rough dielectric.  NEE (a light other than the
+ * environment, shadow ray to the full light distance) adds
+ * Li/pi cos_l rho n12^2 T_i T_d / (1 - rho r10) G1(l, m) and
+ * Li R_r G1(i, h) G1(l, h) D_wi(h) / (4 |i.n l.n|), and turns emission off once
+ * a light is seen; the continuation is the cosine bounce about the sampled
+ * microfacet normal m (weight rho n12^2 T_r / (1 - rho r10) G1(o, m)) when
+ * xi > R_i, else the mirror about m (weight G1(o, m)). */
+static v3 shade_rough_diffuse(const oscene* s, const orc_material* m, v3 o, v3 d, const ohit* h, int depth,
+                              int emit_light, uint32_t* t, ostats* st)
+{
+    v3 result = V3(0, 0, 0);
+    if (depth > s->max_depth) return result;
+    v3 hit_pos = vadd(o, vscale(d, h->t));
+    v3 normal = hit_normal(s, h);
+    v3 ffn = orc_faceforward(normal, vneg(d), normal);
+    v3 w_i = vneg(d);
+    R n12 = (R)1.0f / (R)m->ior;
+    R r10 = two_C1((R)m->ior);
+    R ax = m->roughness[0], ay = m->roughness[1];
+    int nd = m->normal_distribution;
+    v3 rho = V3(m->diffuse[0], m->diffuse[1], m->diffuse[2]);
+    v3 denom = V3((R)1.0f - rho.x * r10, (R)1.0f - rho.y * r10, (R)1.0f - rho.z * r10);
+    R z1 = orc_rnd_tea(t);
+    R z2 = orc_rnd_tea(t);
+    v3 l = orc_to_local(w_i, ffn);
+    v3 mn = orc_to_world(nd == 1 ? orc_ggx_sample_vndf(l, ax, ay, z1, z2) : orc_beckmann_sample_vndf(l, ax, ay, z1, z2),
+                         ffn);
+    R ci = vdot(w_i, mn);
+    R Ri = 1;
+    R st2 = n12 * n12 * ((R)1.0f - ci * ci);
+    if (st2 < (R)1.0f) {
+        R ct = sqrt((R)1.0f - st2);
+        Ri = orc_fresnel_R3(ci, ct, n12);
+    }
+    R Ti = (R)1.0f - Ri;
+    uint32_t light_idx = (uint32_t)((R)s->nlights * orc_rnd_tea(t));
+    if (s->nlights > 0) {
+        if (light_idx >= (uint32_t)s->nlights) light_idx = (uint32_t)s->nlights - 1;
+        R light_pdf = (R)1 / (R)s->nlights;
+        v3 w_l, rad;
+        R dist, penv;
+        direct_light(s, &s->lights[light_idx], hit_pos, t, &w_l, &rad, &dist, &penv);
+        rad = V3(rad.x / light_pdf, rad.y / light_pdf, rad.z / light_pdf);
+        R cl = vdot(ffn, w_l);
+        if (cl > 0 && penv < 0) {
+            ohit sh;
+            st->shd++;
+            if (!trace(s, hit_pos, w_l, s->eps, dist, 1, &sh)) {
+                v3 rn = orc_normalize(vadd(w_i, w_l));
+                R cr = vdot(rn, w_l);
+                R Rr = 1;
+                st2 = n12 * n12 * ((R)1.0f - cr * cr);
+                if (st2 < (R)1.0f) {
+                    R ct = sqrt((R)1.0f - st2);
+                    Rr = orc_fresnel_R3(cr, ct, n12);
+                }
+                R cd = vdot(mn, w_l);
+                R Rd = 1;
+                st2 = n12 * n12 * ((R)1.0f - cd * cd);
+                if (st2 < (R)1.0f) {
+                    R ct = sqrt((R)1.0f - st2);
+                    Rd = orc_fresnel_R3(cd, ct, n12);
+                }
+                R Td = (R)1.0f - Rd;
+                R gi = masking_xy(w_i, rn, ffn, ax, ay, nd);
+                R go = masking_xy(w_l, rn, ffn, ax, ay, nd);
+                R god = masking_xy(w_l, mn, ffn, ax, ay, nd);
+                R D = orc_eval_visible_normal(w_i, rn, ffn, ax, ay, nd);
+                R wr = gi * go * D / ((R)4.0f * fabs(vdot(w_i, ffn) * vdot(w_l, ffn)));
+                v3 a = vscale(vscale(rad, INV_PI_R), cl);
+                a = vmul(a, rho);
+                a = vscale(vscale(vscale(vscale(a, n12), n12), Ti), Td);
+                a = V3(a.x / denom.x, a.y / denom.y, a.z / denom.z);
+                a = vscale(a, god);
+                v3 b = vscale(vscale(rad, Rr), wr);
+                result = vadd(result, vadd(a, b));
+                emit_light = 0;
+            }
+        }
+    }
+    R xi = orc_rnd_tea(t);
+    if (xi > Ri) {
+        v3 dd = orc_sample_cosine_weighted(mn, t);
+        R cr = vdot(dd, mn);
+        R gom = masking_xy(dd, mn, ffn, ax, ay, nd);
+        if (gom < 0) return result;
+        R Rr = 1;
+        st2 = n12 * n12 * ((R)1.0f - cr * cr);
+        if (st2 < (R)1.0f) {
+            R ct = sqrt((R)1.0f - st2);
+            Rr = orc_fresnel_R3(cr, ct, n12);
+        }
+        v3 child = radiance(s, hit_pos, dd, depth + 1, emit_light, t, (R)-1, st);
+        v3 c = vscale(vscale(vscale(vmul(child, rho), n12), n12), (R)1.0f - Rr);
+        c = V3(c.x / denom.x, c.y / denom.y, c.z / denom.z);
+        result = vadd(result, vscale(c, gom));
+    } else {
+        v3 rd = vsub(d, vscale(vscale(mn, (R)2.0f), vdot(mn, d)));
+        R gom = masking_xy(rd, mn, ffn, ax, ay, nd);
+        if (gom < 0) return result;
+        v3 child = radiance(s, hit_pos, rd, depth + 1, emit_light, t, (R)-1, st);
+        result = vadd(result, vscale(child, gom));
+    }
+    return result;
+}
+
 /* CU/flat_shader.cu:40-127: the flat colour, times the picked light's
  * visibility with flat_shadow; with highlight, a visible light whose microfacet
  * reflection Le G1(i) G1(o) D_wi R / (4 |i.n l.n|) has luminance above the
@@ -1512,6 +1621,7 @@ static v3 shade_hit(const oscene* s, v3 o, v3 d, const ohit* hp, int depth, int 
     if (m->kind == ORC_DIELECTRIC) return shade_dielectric(s, m, o, d, &h, depth, seed, st);
     if (m->kind == ORC_TRANSPARENT) return shade_transparent(s, m, o, d, &h, depth, seed, st);
     if (m->kind == ORC_FLAT) return shade_flat(s, m, o, d, &h, depth, seed, st);
+    if (m->kind == ORC_ROUGH_DIFFUSE) return shade_rough_diffuse(s, m, o, d, &h, depth, emit_light, seed, st);
     if (m->kind == ORC_NORMAL) {
         /* CU/normal_shader.cu:25-29: normal * 0.5 + 0.5 */
         v3 n = hit_normal(s, &h);

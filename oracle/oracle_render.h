@@ -23,7 +23,8 @@ enum {
     ORC_ROUGH_TRANSLUCENT = 6,
     ORC_TRANSPARENT = 7, /* transparent_shader.cu */
     ORC_NORMAL = 8,      /* normal_shader.cu */
-    ORC_FLAT = 9         /* flat_shader.cu */
+    ORC_FLAT = 9,        /* flat_shader.cu */
+    ORC_ROUGH_DIFFUSE = 10 /* rough_diffuse_shader.cu */
 };
 enum { ORC_POINT = 0, ORC_DIRECTIONAL = 1, ORC_TRIANGLES = 2, ORC_DISK = 3, ORC_SPHERE = 4, ORC_ENV = 5 };
 

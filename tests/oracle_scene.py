@@ -42,7 +42,7 @@ class OrcSceneDesc(ctypes.Structure):
 _KIND = {"diffuse_shader.cu": 0, "metallic_shader.cu": 1, "arealight_shader.cu": 2, "rough_transparent_shader.cu": 3,
          "lambertian_interface_shader.cu": 4, "subsurface_scattering_shader.cu": 5,
          "rough_subsurface_scattering_shader.cu": 6, "transparent_shader.cu": 7, "normal_shader.cu": 8,
-         "flat_shader.cu": 9}
+         "flat_shader.cu": 9, "rough_diffuse_shader.cu": 10}
 # Wholemilk (ORF/RoughTranslucentMaterial.cpp:258-263) and the default scale
 SSS_DEFAULTS = {"absorption": (0.0011, 0.0024, 0.014), "scattering": (2.55, 3.21, 3.77),
                 "meancosine": (0.0, 0.0, 0.0), "scale": (100.0,)}
@@ -115,6 +115,11 @@ class OracleScene:
                 getattr(mm, field)[:] = [float(x) for x in m.floats.get(key, SSS_DEFAULTS[key])]
             mm.sss_scale = m.floats.get("scale", SSS_DEFAULTS["scale"])[0]
             mm.dipole_model = m.uints.get("dipole_model", (0,))[0]
+            if mm.kind == 10:  # RoughDiffuseMaterial defaults (ORF/RoughDiffuseMaterial.cpp:7-17)
+                mm.diffuse[:] = [float(x) for x in m.floats.get("diffuse_color", (1.0, 1.0, 1.0))]
+                mm.ior = m.floats.get("ior", (1.3,))[0]
+                mm.normal_distribution = m.uints.get("normal_distribution", (0,))[0]
+                mm.microfacet_model = m.uints.get("microfacet_model", (0,))[0]
             if mm.kind == 9:  # FlatMaterial defaults (ORF/FlatMaterial.cpp:5-21)
                 mm.diffuse[:] = [float(x) for x in m.floats.get("diffuse_color", (1.0, 1.0, 1.0))]
                 mm.radiance[:] = [float(x) for x in m.floats.get("highlight_color", (1.0, 1.0, 1.0))]

@@ -209,7 +209,8 @@ def test_loaded_translucent_scene(tmp_path):
 
 
 @pytest.mark.parametrize("patch,match", [
-    (lambda d: d["Geometries"][0]["parameters"]["material"].update(mtl_name="rough_diffuse_shader"), "rough_diffuse_shader"),
+    (lambda d: d["Geometries"][0]["parameters"]["material"].update(mtl_name="anisotropic_structure_shader"),
+     "anisotropic_structure_shader"),
     (lambda d: d["Lights"].append({"type": "SpotLight", "parameters": {}}), "SpotLight"),
     (lambda d: d["Camera"].update(type="EnvCamera"), "EnvCamera"),
     (lambda d: d["Geometries"][1]["parameters"].update(path="missing.obj"), "missing.obj"),

@@ -1,0 +1,55 @@
+"""Pins the oracle's rough diffuse material (CU/rough_diffuse_shader.cu:40-290,
+visible-normal sampling; the multiple-scattering walk is not built), CPU only.
+
+As the roughness goes to 0 the microfacet normal is the shading normal, the
+specular NEE term vanishes for a point light and both continuations keep weight
+1, so the material becomes the smooth Lambertian interface
+(CU/lambertian_interface_shader.cu, pinned by its white furnace): the FP64
+images of the same point-lit scene with either material agree (image mean to
+1%, 4 x 4 block means to 4%).  At roughness 0.5 the block means part by more
+than 6% (a material that ignored its roughness would not).
+"""
+import numpy as np
+import pytest
+
+from open_rendering_framework_amd import scenes
+from tests.oracle_scene import OracleScene
+
+W = 32
+
+
+def _scene(ground):
+    d = scenes.path_scene(width=W, height=W, level=3, env=False, max_depth=4)
+    d.materials[0] = ground
+    d.materials[1] = scenes.Material("diffuse_shader.cu", floats={"diffuse_color": (0.7, 0.4, 0.3)})
+    d.lights = [scenes.Light("point", (30.0, 30.0, 30.0), position=(1.0, 3.0, 1.5))]
+    d.meshes = d.meshes[:2]
+    return d
+
+
+def _render(mat):
+    img, _ = OracleScene(_scene(mat), fp64=True).render(W, W, 0, 256)
+    return img[..., :3] / img[..., 3:4]
+
+
+@pytest.mark.parametrize("dist", [0, 1])
+def test_smooth_limit_is_the_lambertian_interface(dist):
+    rho, ior = (0.6, 0.5, 0.4), 1.5
+    rough = _render(scenes.Material("rough_diffuse_shader.cu",
+                                    floats={"diffuse_color": rho, "ior": (ior,), "roughness": (1e-3, 1e-3)},
+                                    uints={"normal_distribution": (dist,), "microfacet_model": (1,)}))
+    smooth = _render(scenes.Material("lambertian_interface_shader.cu", floats={"diffuse_color": rho, "ior": (ior,)}))
+    blk = lambda a: a.reshape(4, W // 4, 4, W // 4, 3).mean(axis=(1, 3))  # noqa: E731
+    rel_mean = abs(rough.mean() - smooth.mean()) / smooth.mean()
+    rel_blk = np.abs(blk(rough) - blk(smooth)).max() / blk(smooth).mean()
+    assert rel_mean < 0.01 and rel_blk < 0.04, (rel_mean, rel_blk)
+
+
+def test_roughness_matters():
+    rho, ior = (0.6, 0.5, 0.4), 1.5
+    rough = _render(scenes.Material("rough_diffuse_shader.cu",
+                                    floats={"diffuse_color": rho, "ior": (ior,), "roughness": (0.5, 0.5)},
+                                    uints={"normal_distribution": (1,)}))
+    smooth = _render(scenes.Material("lambertian_interface_shader.cu", floats={"diffuse_color": rho, "ior": (ior,)}))
+    blk = lambda a: a.reshape(4, W // 4, 4, W // 4, 3).mean(axis=(1, 3))  # noqa: E731
+    assert np.abs(blk(rough) - blk(smooth)).max() / blk(smooth).mean() > 0.06
