@@ -67,7 +67,8 @@ __device__ __forceinline__ void mega_pixel(const SceneGPU& sc, const CameraGPU& 
         Hit hit;
         st.ext++;
         bool more = false;
-        if (!trace<false, kFeat == kFeatAll>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, hit, stk, kMegaBlock)) {
+        if (!trace<false, kFeat == kFeatAll, (kFeat != 0u)>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, hit, stk,
+                                                            kMegaBlock)) {
             shade_miss(sc, ps);
         } else {
             ShadowRay shadow;
@@ -75,8 +76,8 @@ __device__ __forceinline__ void mega_pixel(const SceneGPU& sc, const CameraGPU& 
             if (shadow.valid) {
                 Hit hs;
                 st.shd++;
-                if (!trace<true, kFeat == kFeatAll>(sc, shadow.o, shadow.d, sc.scene_epsilon, shadow.tmax, hs, stk,
-                                                      kMegaBlock))
+                if (!trace<true, kFeat == kFeatAll, (kFeat != 0u)>(sc, shadow.o, shadow.d, sc.scene_epsilon, shadow.tmax,
+                                                                     hs, stk, kMegaBlock))
                     shadow_unoccluded(ps, shadow);
             }
         }

@@ -99,11 +99,12 @@ struct HitGeom {
     int4 sh;
 };
 
+template <bool kEmit = true>
 ORF_HD HitGeom hit_geometry(const SceneGPU& sc, const PathState& ps, const Hit& h)
 {
     HitGeom g;
     g.p = ps.o + h.t * ps.d;
-    if (h.leaf < 0) {  // disk or sphere emitter
+    if (kEmit && h.leaf < 0) {  // disk or sphere emitter
         const int i = -1 - h.leaf;
         g.sh = sc.shade[h.prim];
         g.normal = analytic_normal(sc.analytic[2 * i], sc.analytic[2 * i + 1], ps.o, ps.d, h.t);
@@ -130,7 +131,7 @@ template <unsigned kFeat = kFeatAll>
 ORF_HD bool shade_vertex(const SceneGPU& sc, const int* remap, PathState& ps, const Hit& h, ShadowRay& sr,
                          unsigned* stk = nullptr, int stride = 1)
 {
-    HitGeom g = hit_geometry(sc, ps, h);
+    HitGeom g = hit_geometry<kFeat == kFeatAll>(sc, ps, h);  // emitters only in scenes of the all-feature variant
     return shade_with<kFeat>(sc, hit_material(sc, remap, g.sh.w), ps, h, g, sr, stk, stride);
 }
 
@@ -159,7 +160,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
             LightGPU lt = sc.lights[li];
             f3 wl = mk3(0.0f), rad = mk3(0.0f);
             float dist = 0.0f, penv;
-            sample_light(sc, lt, p, ps.rng, wl, rad, dist, penv);
+            sample_light<kFeat == kFeatAll>(sc, lt, p, ps.rng, wl, rad, dist, penv);
             rad = rad / light_pdf;
             float cs = dot(ffn, wl);
             if (cs > 0.0f) {
@@ -281,7 +282,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
             const float light_pdf = 1.0f / (float)sc.nlights;
             f3 wl = mk3(0.0f), rad = mk3(0.0f);
             float dist = 0.0f, penv;
-            sample_light(sc, sc.lights[li], p, ps.rng, wl, rad, dist, penv);
+            sample_light<kFeat == kFeatAll>(sc, sc.lights[li], p, ps.rng, wl, rad, dist, penv);
             rad = rad / light_pdf;
             const float cl = dot(ffn, wl);
             // the environment reaches this material through escaping rays only
@@ -353,7 +354,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
             const float light_pdf = 1.0f / (float)sc.nlights;
             f3 wl = mk3(0.0f), rad = mk3(0.0f);
             float dist = 0.0f, penv;
-            sample_light(sc, sc.lights[li], p, ps.rng, wl, rad, dist, penv);
+            sample_light<kFeat == kFeatAll>(sc, sc.lights[li], p, ps.rng, wl, rad, dist, penv);
             rad = rad / light_pdf;
             if (dot(ffn, wl) > 0.0f) {
                 f3 A = mat.diffuse_color;
@@ -405,7 +406,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
             LightGPU lt = sc.lights[li];
             f3 wl = mk3(0.0f), rad = mk3(0.0f);
             float dist = 0.0f, penv;
-            sample_light(sc, lt, p, ps.rng, wl, rad, dist, penv);
+            sample_light<kFeat == kFeatAll>(sc, lt, p, ps.rng, wl, rad, dist, penv);
             rad = rad / light_pdf;
             float cl = dot(ffn, wl);
             if (cl > 0.0f) {
@@ -515,7 +516,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
             const float light_pdf = 1.0f / (float)nreal;
             f3 wl = mk3(0.0f), rad = mk3(0.0f);
             float ldist = 0.0f, penv;
-            sample_light(sc, sc.lights[li], p, ps.rng, wl, rad, ldist, penv);
+            sample_light<kFeat == kFeatAll>(sc, sc.lights[li], p, ps.rng, wl, rad, ldist, penv);
             rad = rad / light_pdf;
             const float cl = dot(ffn, wl);
             if (!inside && cl > 0.0f) {
@@ -613,7 +614,8 @@ ORF_HD f3 path_radiance(const SceneGPU& sc, const int* remap, f3 o, f3 d, uint32
     for (;;) {
         Hit h;
         st.ext++;
-        if (!trace<false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk, stride)) {
+        if (!trace<false, kFeat == kFeatAll, (kFeat != 0u)>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk,
+                                                            stride)) {
             shade_miss(sc, ps);
             break;
         }
@@ -622,7 +624,9 @@ ORF_HD f3 path_radiance(const SceneGPU& sc, const int* remap, f3 o, f3 d, uint32
         if (sr.valid) {
             Hit hs;
             st.shd++;
-            if (!trace<true>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) shadow_unoccluded(ps, sr);
+            if (!trace<true, kFeat == kFeatAll, (kFeat != 0u)>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk,
+                                                               stride))
+                shadow_unoccluded(ps, sr);
         }
         if (!more) break;
     }

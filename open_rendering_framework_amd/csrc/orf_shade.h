@@ -110,7 +110,10 @@ ORF_HD void env_sample(const EnvGPU& env, uint32_t& rng, f3& dir, f3& L, float& 
 }
 
 // evaluate_direct_illumination (ORF/LightSampler.h:143-166) for one picked light.
-// Returns false for light types without a sampler here.
+// Returns false for light types without a sampler here.  kAnalytic: disk and
+// sphere lights are sampled (kernel variants for scenes without them leave
+// that code out).
+template <bool kAnalytic = true>
 ORF_HD bool sample_light(const SceneGPU& sc, const LightGPU& lt, f3 pos, uint32_t& rng, f3& dir, f3& L,
                          float& dist, float& pdf_env)
 {
@@ -146,7 +149,7 @@ ORF_HD bool sample_light(const SceneGPU& sc, const LightGPU& lt, f3 pos, uint32_
         L = lt.radiance * (tl.area * cp / d2 * (float)ntri);
         return true;
     }
-    if (lt.type == DISK_LIGHT) {
+    if (kAnalytic && lt.type == DISK_LIGHT) {
         // evaluate_disk_area_light (ORF/LightSampler.h:73-101): a uniform point of
         // the disk (r = sqrt(u1) R, angle 2 pi u2 in the create_onb frame of its
         // normal), L = Le cos' / d^2 / (1 / area).  params: centre; pad: (radius,
@@ -171,7 +174,7 @@ ORF_HD bool sample_light(const SceneGPU& sc, const LightGPU& lt, f3 pos, uint32_
         L = lt.radiance * (cp / d2) / pdf;
         return true;
     }
-    if (lt.type == SPHERICAL_LIGHT) {
+    if (kAnalytic && lt.type == SPHERICAL_LIGHT) {
         // evaluate_spherical_area_light (ORF/LightSampler.h:103-141): a uniform
         // point of the hemisphere facing pos (z = u1 along the create_onb frame
         // of normalize(pos - centre)) weighted by the whole sphere's area density

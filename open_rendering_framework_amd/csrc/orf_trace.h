@@ -172,9 +172,9 @@ ORF_HD unsigned stack_pop(unsigned* stk, int stride, const unsigned* ovf, int& s
 // Closest hit (any-hit off) or first hit (any-hit on) in (tmin, tmax).  kMode 0:
 // closest, 1: any, 2: chosen per lane by `any` (one traversal loop serves
 // extension and shadow rays of different lanes at once).  kEmit: the disk and
-// sphere emitters are tested (kernels for scenes that cannot hold them leave
-// them out).
-template <int kMode, bool kEmit = true>
+// sphere emitters are tested; kPass: shadow rays pass through triangles marked
+// kTriNoShadow (kernels for scenes that cannot hold either leave it out).
+template <int kMode, bool kEmit = true, bool kPass = true>
 ORF_HD bool trace_impl(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, bool any, Hit& h, unsigned* stk,
                        int stride)
 {
@@ -220,7 +220,7 @@ ORF_HD bool trace_impl(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, b
             if (intersect_tri(o, d, mk3(q0.x, q0.y, q0.z), mk3(q1.x, q1.y, q1.z), mk3(q2.x, q2.y, q2.z), tmin,
                               tbest, t, bb, gg)) {
                 const unsigned id = f2u(q0.w);
-                if (ANY && (id & kTriNoShadow)) continue;  // shadow rays pass through it
+                if (ANY && kPass && (id & kTriNoShadow)) continue;  // shadow rays pass through it
                 tbest = t;
                 hb = bb;
                 hg = gg;
@@ -256,10 +256,10 @@ ORF_HD bool trace_impl(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, b
     return prim >= 0;
 }
 
-template <bool ANY, bool kEmit = true>
+template <bool ANY, bool kEmit = true, bool kPass = true>
 ORF_HD bool trace(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, Hit& h, unsigned* stk, int stride)
 {
-    return trace_impl<ANY ? 1 : 0, kEmit>(sc, o, d, tmin, tmax, ANY, h, stk, stride);
+    return trace_impl<ANY ? 1 : 0, kEmit, kPass>(sc, o, d, tmin, tmax, ANY, h, stk, stride);
 }
 
 ORF_HD bool trace_dyn(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, bool any, Hit& h, unsigned* stk,
