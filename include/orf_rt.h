@@ -95,7 +95,9 @@ enum { ORF_SCOPE_CONTEXT = 0, ORF_SCOPE_MATERIAL = 1 };
  * max_depth, scene_epsilon, eye, U, V, W, frame, patch_origin, patch_dims,
  * exception_color (ORF/Integrator.cpp:6-24, ORF/Camera.cpp:83-103).
  * Material variables: diffuse_color, radiance, roughness, ior,
- * microfacet_model, normal_distribution, glass_absorption. */
+ * microfacet_model, normal_distribution, glass_absorption; translucent:
+ * absorption, scattering, meancosine, scale, dipole_model; flat: flat_shadow,
+ * highlight, highlight_color, highlight_threshold, shininess. */
 orf_status orf_variable_set_float(orf_context ctx, int scope, int handle, const char* name,
                                   const float* v, int n);
 orf_status orf_variable_set_int(orf_context ctx, int scope, int handle, const char* name,
@@ -109,7 +111,10 @@ orf_status orf_variable_set_user_data(orf_context ctx, int scope, int handle, co
  * "closest_hit")) (e.g. ORF/MetallicMaterial.cpp:108-118).  `program` is the
  * reference program file: "diffuse_shader.cu", "metallic_shader.cu",
  * "arealight_shader.cu", "rough_transparent_shader.cu",
- * "lambertian_interface_shader.cu".  Returns the material handle. */
+ * "lambertian_interface_shader.cu", "subsurface_scattering_shader.cu"
+ * (also "translucent_shader"), "rough_subsurface_scattering_shader.cu",
+ * "transparent_shader.cu", "normal_shader.cu", "flat_shader.cu",
+ * "rough_diffuse_shader.cu".  Returns the material handle. */
 orf_status orf_material_create(orf_context ctx, const char* program, int* material);
 
 /* createGeometry + vertex/index/normal buffers + createGeometryInstance
