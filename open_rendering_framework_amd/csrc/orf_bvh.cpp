@@ -327,6 +327,87 @@ BvhResult build_bvh(const float* verts, int64_t nverts, const int32_t* idx, int6
         res.stats.nodes = (int)order.size();
     }
     res.stats.sah_cost = sah;
+
+    // ---- BVH4: each emitted node gathers up to 4 children by repeatedly
+    // opening its internal child of largest surface area; nodes in DFS order.
+    {
+        Box none;
+        for (int k = 0; k < 3; ++k) none.lo[k] = none.hi[k] = 3.0e38f;
+        auto gather = [&](int i, int kids[4]) {
+            int n = 0;
+            if (tn[i].left < 0) {  // the root is a leaf
+                kids[n++] = i;
+            } else {
+                kids[n++] = tn[i].left;
+                kids[n++] = tn[i].right;
+            }
+            while (n < 4) {
+                int best = -1;
+                double ba = -1.0;
+                for (int k = 0; k < n; ++k)
+                    if (tn[kids[k]].left >= 0 && tn[kids[k]].box.area() > ba) {
+                        ba = tn[kids[k]].box.area();
+                        best = k;
+                    }
+                if (best < 0) break;
+                const int c = kids[best];
+                kids[best] = tn[c].left;
+                kids[n++] = tn[c].right;
+            }
+            for (int k = n; k < 4; ++k) kids[k] = -1;
+        };
+        std::vector<int> flat4(tn.size(), -1), order4;
+        std::vector<std::pair<int, int>> st{{0, 0}};
+        while (!st.empty()) {
+            const int i = st.back().first, d = st.back().second;
+            st.pop_back();
+            flat4[i] = (int)order4.size();
+            order4.push_back(i);
+            res.depth4 = std::max(res.depth4, d);
+            int kids[4];
+            gather(i, kids);
+            for (int k = 3; k >= 0; --k)
+                if (kids[k] >= 0 && tn[kids[k]].left >= 0 && kids[k] != i) st.push_back({kids[k], d + 1});
+        }
+        // stack bound: a visit pushes (live children - 1) entries at most
+        std::vector<int> need(tn.size(), 0);
+        for (size_t o = order4.size(); o-- > 0;) {
+            const int i = order4[o];
+            int kids[4], live = 0, deeper = 0;
+            gather(i, kids);
+            for (int k = 0; k < 4; ++k) {
+                const int c = kids[k];
+                if (c < 0 || (tn[c].left < 0 && tn[c].count == 0)) continue;
+                live++;
+                if (tn[c].left >= 0 && c != i) deeper = std::max(deeper, need[c]);
+            }
+            need[i] = std::max(0, live - 1) + deeper;
+        }
+        res.stack4 = order4.empty() ? 0 : need[order4[0]];
+        res.nodes4.reserve(order4.size() * 8);
+        for (int i : order4) {
+            int kids[4];
+            gather(i, kids);
+            const Box* b[4];
+            uint32_t l[4];
+            for (int k = 0; k < 4; ++k) {
+                const int c = kids[k];
+                if (c < 0 || (tn[c].left < 0 && tn[c].count == 0)) {
+                    b[k] = &none;
+                    l[k] = 0x80000000u;
+                } else {
+                    b[k] = &tn[c].box;
+                    l[k] = tn[c].left < 0 ? leaf_link[c] : (uint32_t)flat4[c];
+                }
+            }
+            for (int a = 0; a < 3; ++a) {
+                res.nodes4.push_back(make_float4(b[0]->lo[a], b[1]->lo[a], b[2]->lo[a], b[3]->lo[a]));
+                res.nodes4.push_back(make_float4(b[0]->hi[a], b[1]->hi[a], b[2]->hi[a], b[3]->hi[a]));
+            }
+            res.nodes4.push_back(make_float4(bitsf(l[0]), bitsf(l[1]), bitsf(l[2]), bitsf(l[3])));
+            res.nodes4.push_back(make_float4(0.0f, 0.0f, 0.0f, 0.0f));
+        }
+    }
     res.root_is_leaf = false;
     res.root_link = 0;
     (void)fbits;

@@ -25,6 +25,12 @@ struct BvhStats {
 
 struct BvhResult {
     std::vector<float4> nodes;  // 4 float4 per node
+    // The same tree collapsed to 4 children per node (each BVH2 node absorbs
+    // the children of its largest internal children): 8 float4 per node,
+    // lo.x, hi.x, lo.y, hi.y, lo.z, hi.z of the 4 children, their links, pad.
+    std::vector<float4> nodes4;
+    int depth4 = 0;             // deepest BVH4 node level
+    int stack4 = 0;             // most traversal-stack entries a BVH4 ray can hold
     std::vector<float4> tris;   // 3 float4 per leaf-ordered triangle (p0|id, e1, e2)
     unsigned root_link = 0;     // leaf link when the whole scene is one leaf
     bool root_is_leaf = false;

@@ -245,7 +245,9 @@ static void build_geometry(orf_context_t* c, hipStream_t)
     if (shade.empty()) shade.push_back(make_int4(-1, -1, -1, 0));
     std::vector<float4> normals(N.size() / 3 + 1);
     for (size_t k = 0; k + 2 < N.size(); k += 3) normals[k / 3] = make_float4(N[k], N[k + 1], N[k + 2], 0.0f);
+    ORF_REQUIRE(bvh.stack4 <= kStackDepth, "scene: the BVH needs a deeper traversal stack than this build has");
     c->d_bvh.upload(bvh.nodes);
+    c->d_bvh4.upload(bvh.nodes4);
     c->d_tri.upload(bvh.tris);
     c->d_shade.upload(shade);
     c->d_normals.upload(normals);
@@ -358,6 +360,7 @@ SceneGPU scene_view(orf_context_t* c)
     SceneGPU s;
     std::memset(&s, 0, sizeof(s));
     s.bvh = c->d_bvh.as<float4>();
+    s.bvh4 = c->d_bvh4.as<float4>();
     s.tri = c->d_tri.as<float4>();
     s.shade = c->d_shade.as<int4>();
     s.normals = c->d_normals.as<float4>();

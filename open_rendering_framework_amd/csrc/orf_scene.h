@@ -125,7 +125,8 @@ struct ProbesGPU {
 };
 
 struct SceneGPU {
-    const float4* bvh;        // 4 float4 per node
+    const float4* bvh;        // 4 float4 per node (BVH2; the translucent crossing search)
+    const float4* bvh4;       // 8 float4 per node (the same tree, 4-wide; every other traversal)
     const float4* tri;        // 3 float4 per leaf-ordered triangle
     const int4* shade;        // per original triangle: nidx.xyz, material id
     const float4* normals;    // per normal
