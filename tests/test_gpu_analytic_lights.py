@@ -133,3 +133,36 @@ def test_differentiable_form_refuses_emitters():
     r = torch.full((4, 4), 0.3, device="cuda")
     with pytest.raises(rt.OrfError, match="disk or sphere"):
         ctx.diff_render(desc, views, a, r)
+
+
+def test_scene_file_disk_and_sphere_lights(tmp_path):
+    """DiskAreaLight / SphericalLight entries of a scene file (ORF/Light.cpp:389-423,
+    525-548) render per sample as the oracle on the hand-written equivalent."""
+    import json
+
+    from tests.oracle_scene import OracleScene
+    from tests.test_gpu_scene_file import H as SH, W as SW, _scene_files
+
+    path, desc = _scene_files(tmp_path)
+    doc = json.loads(path.read_text())
+    doc["Lights"] += [
+        {"type": "DiskAreaLight", "parameters": {"position": [0.8, 1.2, 1.0], "radius": 0.3, "theta": 120.0,
+                                                 "phi": 20.0, "radiance": [4.0, 3.0, 2.0]}},
+        {"type": "SphericalLight", "parameters": {"position": [-0.9, 0.9, 0.6], "radius": 0.2,
+                                                  "radiance": [2.0, 3.0, 5.0]}},
+    ]
+    path.write_text(json.dumps(doc))
+    desc.lights += [scenes.Light("disk", (4.0, 3.0, 2.0), position=(0.8, 1.2, 1.0), radius=0.3, theta=120.0, phi=20.0),
+                    scenes.Light("sphere", (2.0, 3.0, 5.0), position=(-0.9, 0.9, 0.6), radius=0.2)]
+    ctx = rt.Context()
+    info = ctx.load_scene(path, env_importance=True)
+    assert info["lights"] == 5
+    ctx.accel_build()
+    ctx.film_clear()
+    ctx.launch(SW, SH, 1, 1)
+    got = ctx.film_sums(SW, SH)[..., :3]
+    ys, xs = np.mgrid[0:SH, 0:SW]
+    want = OracleScene(desc).samples(SW, SH, xs.ravel(), ys.ravel(), 1).reshape(SH, SW, 3)
+    ok = _ok(got, want)
+    _say("scene-file disk/sphere lights: samples ok %.5f" % ok.mean())
+    assert ok.mean() >= 0.99
