@@ -339,8 +339,8 @@ void prepare(orf_context_t* c, hipStream_t stream)
             const bool ms = m[i].microfacet_model == MULTISCATTERING_MODEL;
             ORF_REQUIRE(!(ms && m[i].kind == MAT_ROUGH_DIFFUSE),
                         "material " + std::to_string(i) + ": the rough diffuse multiple-scattering walk is not built");
-            ORF_REQUIRE(!(ms && m[i].kind == MAT_CONDUCTOR),
-                        "material " + std::to_string(i) + ": the conductor multiple-scattering walk is not built");
+            ORF_REQUIRE(!(ms && m[i].kind == MAT_CONDUCTOR && m[i].normal_distribution == BECKMANN_DISTRIBUTION),
+                        "material " + std::to_string(i) + ": the Beckmann conductor multiple-scattering walk is not built");
             ORF_REQUIRE(!(ms && m[i].normal_distribution == BECKMANN_DISTRIBUTION &&
                           (m[i].kind == MAT_DIELECTRIC || m[i].kind == MAT_ROUGH_TRANSLUCENT)),
                         "material " + std::to_string(i) + ": the Beckmann multiple-scattering walk is not built");
@@ -383,7 +383,11 @@ SceneGPU scene_view(orf_context_t* c)
     s.max_depth = c->max_depth;
     s.scene_epsilon = c->scene_epsilon;
     s.features = 0u;
-    for (const MaterialGPU& m : c->mats) s.features |= material_features(m.kind);
+    for (const MaterialGPU& m : c->mats) {
+        s.features |= material_features(m.kind);
+        // the conductor's microsurface walk is compiled with the dielectric code
+        if (m.kind == MAT_CONDUCTOR && m.microfacet_model == MULTISCATTERING_MODEL) s.features |= kFeatDielectric;
+    }
     // disk / sphere emitters are traced only by the all-feature megakernel variant
     if (!c->analytic.empty()) s.features = kFeatAll;
     s.sss = c->d_sss.as<ScatteringGPU>();

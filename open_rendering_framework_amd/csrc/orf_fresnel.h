@@ -22,6 +22,8 @@ struct ConductorEta {
     cf eta[3];
     cf eta2[3];
     int gray;  // all three channels equal: evaluate once
+    cf ieta[3];   // 1 / eta (= ior): a microsurface walk event met from below
+    cf ieta2[3];
 };
 
 ORF_HD cf csqrt_ref(float zr, float zi)
@@ -66,6 +68,17 @@ ORF_HD f3 fresnel_conductor(f3 w_i, f3 w_m, const ConductorEta& e)
     if (e.gray) return mk3(r);
     return mk3(r, fresnel_conductor_channel(c, s, e.eta[1], e.eta2[1]),
                fresnel_conductor_channel(c, s, e.eta[2], e.eta2[2]));
+}
+
+// fresnel_conductor with 1 / eta (the walk's events met from below).
+ORF_HD f3 fresnel_conductor_inv(f3 w_i, f3 w_m, const ConductorEta& e)
+{
+    float c = dot(w_i, w_m);
+    float s = sqrtf(1.0f - c * c);
+    float r = fresnel_conductor_channel(c, s, e.ieta[0], e.ieta2[0]);
+    if (e.gray) return mk3(r);
+    return mk3(r, fresnel_conductor_channel(c, s, e.ieta[1], e.ieta2[1]),
+               fresnel_conductor_channel(c, s, e.ieta[2], e.ieta2[2]));
 }
 
 // Unpolarised dielectric reflectance for relative IOR ior1/ior2.

@@ -187,11 +187,21 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
     }
     if (mat.kind == MAT_CONDUCTOR) {
         if (ps.depth >= sc.max_depth) return false;
-        float z1 = rnd_tea(ps.rng);
-        float z2 = rnd_tea(ps.rng);
         f3 wo, w;
-        conductor_step(-d, ffn, mat, z1, z2, wo, w);
-        if (w.x == 0.0f && w.y == 0.0f && w.z == 0.0f) return false;
+        if (kDiel && mat.microfacet_model == MULTISCATTERING_MODEL) {
+            // CU/metallic_shader.cu:66-82: the GGX microsurface walk in the frame of
+            // the unflipped normal; its weight multiplies the continuation
+            f3 lo;
+            if (!ggx_ms_conductor_sample(to_local(-d, normal), mat.eta, mat.roughness.x, mat.roughness.y, ps.rng, lo,
+                                         w))
+                return false;
+            wo = to_world(lo, normal);
+        } else {
+            float z1 = rnd_tea(ps.rng);
+            float z2 = rnd_tea(ps.rng);
+            conductor_step(-d, ffn, mat, z1, z2, wo, w);
+            if (w.x == 0.0f && w.y == 0.0f && w.z == 0.0f) return false;
+        }
         ps.thr = ps.thr * w;
         ps.o = p;
         ps.d = wo;

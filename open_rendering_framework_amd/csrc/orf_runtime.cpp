@@ -20,6 +20,8 @@ void conductor_eta_host(const double ior_re[3], const double ior_im[3], Conducto
         double ei = -ior_im[c] * (1.0 / den);
         e.eta[c] = cf{(float)er, (float)ei};
         e.eta2[c] = cf{(float)(er * er - ei * ei), (float)(2.0 * er * ei)};
+        e.ieta[c] = cf{(float)ior_re[c], (float)ior_im[c]};
+        e.ieta2[c] = cf{(float)(ior_re[c] * ior_re[c] - ior_im[c] * ior_im[c]), (float)(2.0 * ior_re[c] * ior_im[c])};
     }
     e.gray = (ior_re[0] == ior_re[1] && ior_re[1] == ior_re[2] && ior_im[0] == ior_im[1] &&
               ior_im[1] == ior_im[2]);

@@ -403,6 +403,39 @@ ORF_HD bool ggx_ms_dielectric_sample(f3 local_wi, float eta, float ax, float ay,
     return true;
 }
 
+// ggx_multiscattering_conductor_BSDF_sample (ORF/MicrofacetGGX.h:614-637), local
+// frame: the random walk on the GGX microsurface of a conductor.  Each event
+// reflects about a visible normal of the arriving direction and multiplies the
+// weight by the Fresnel reflectance with eta, or 1 / eta when that direction
+// points below the mean plane.  False when the walk exceeds kMsWalkCap events
+// (the path ends there; the reference's loop has no bound).
+ORF_HD bool ggx_ms_conductor_sample(f3 local_wi, const ConductorEta& eta, float ax, float ay, uint32_t& rng,
+                                    f3& w_out, f3& weight)
+{
+    float hr = 1.0f + height_inv_cdf(0.9999f);
+    f3 w_o = -local_wi;
+    weight = mk3(1.0f);
+    int order = 0;
+    for (;;) {
+        const float u = rnd_tea(rng);
+        hr = ggx_sample_height(w_o, hr, u, ax, ay);
+        if (hr == kRayMax) break;
+        if (++order > kMsWalkCap) return false;
+        const f3 w_t = -w_o;
+        const float z1 = rnd_tea(rng);
+        const float z2 = rnd_tea(rng);
+        const f3 wm = ggx_sample_visible(w_t, ax, ay, z1, z2);
+        w_o = -w_t + (wm * 2.0f) * dot(w_t, wm);
+        weight = weight * (w_t.z > 0.0f ? fresnel_conductor(w_t, wm, eta) : fresnel_conductor_inv(w_t, wm, eta));
+        if (hr != hr || w_o.z != w_o.z) {
+            w_o = mk3(0.0f, 0.0f, 1.0f);
+            break;
+        }
+    }
+    w_out = w_o;
+    return true;
+}
+
 // ggx_multiscattering_dielectric_BSDF_eval (ORF/MicrofacetGGX.h:444-478): one
 // random-walk estimate of the multiple-scattering BSDF times cos for (wi, wo),
 // local frame: the sum over the walk's events of phase(wo) x masking of wo at

@@ -229,6 +229,35 @@ void orc_ggx_ms_sample_batch(int n, const float* wi, float eta, float alpha, uin
     }
 }
 
+/* Conductor microsurface walks from the fixed local wi (stream tea16(i, seed)),
+ * complex ior (re, im) per channel given as 6 doubles: exit direction, weight
+ * (3 per walk), order and ok. */
+void orc_ggx_ms_conductor_batch(int n, const float* wi, const double* ior, float alpha, uint32_t seed, int cap,
+                                float* wo, float* weight, int32_t* order, int32_t* ok)
+{
+    cplx3 eta, ieta;
+    ieta.x = (cplx){ior[0], ior[1]};
+    ieta.y = (cplx){ior[2], ior[3]};
+    ieta.z = (cplx){ior[4], ior[5]};
+    eta.x = orc_cdiv_fc((R)1.0f, ieta.x);
+    eta.y = orc_cdiv_fc((R)1.0f, ieta.y);
+    eta.z = orc_cdiv_fc((R)1.0f, ieta.z);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        uint32_t t = orc_tea16((uint32_t)i, seed);
+        v3 out = V3(0, 0, 0), w = V3(0, 0, 0);
+        int o = 0;
+        ok[i] = orc_ggx_ms_conductor_sample(V3(wi[0], wi[1], wi[2]), &eta, &ieta, alpha, alpha, &t, cap, &out, &w, &o);
+        order[i] = o;
+        wo[3 * i] = (float)out.x;
+        wo[3 * i + 1] = (float)out.y;
+        wo[3 * i + 2] = (float)out.z;
+        weight[3 * i] = (float)w.x;
+        weight[3 * i + 1] = (float)w.y;
+        weight[3 * i + 2] = (float)w.z;
+    }
+}
+
 /* One multiple-scattering BSDF x cos estimate per direction pair wo[i] for the
  * fixed local wi (stream tea16(i, seed)), at most `cap` events. */
 void orc_ggx_ms_eval_batch(int n, const float* wi, const float* wo, float eta, float alpha, uint32_t seed, int cap,

@@ -716,6 +716,39 @@ static v3 ggx_sample_dielectric_phase(v3 w_i, R m_eta, uint32_t* seed, int* outs
 /* ggx_multiscattering_dielectric_BSDF_sample (ORF/MicrofacetGGX.h:412-438) with
  * at most `cap` scattering events (the reference's loop is unbounded); returns
  * 0 when the walk exceeds the cap.  *order receives the event count. */
+/* ggx_multiscattering_conductor_BSDF_sample (ORF/MicrofacetGGX.h:614-637): the
+ * random walk on the GGX microsurface of a conductor, local frame.  Each event
+ * reflects about a visible normal of the arriving direction w_t and multiplies
+ * the weight by the complex Fresnel reflectance with eta (w_t.z > 0) or 1/eta.
+ * Returns 0 when the walk exceeds `cap` events (new: the reference's loop has no
+ * bound). */
+int orc_ggx_ms_conductor_sample(v3 local_wi, const cplx3* eta, const cplx3* ieta, R ax, R ay, uint32_t* seed, int cap,
+                                v3* w_out, v3* weight, int* order)
+{
+    R hr = (R)1.0f + orc_height_inv_cdf((R)0.9999f);
+    v3 w_o = vneg(local_wi);
+    *weight = V3(1, 1, 1);
+    *order = 0;
+    for (;;) {
+        R u = orc_rnd_tea(seed);
+        hr = orc_ggx_sample_height(w_o, hr, u, ax, ay);
+        if (hr == RAY_MAX_R) break;
+        if (++*order > cap) return 0;
+        v3 w_t = vneg(w_o);
+        R z1 = orc_rnd_tea(seed);
+        R z2 = orc_rnd_tea(seed);
+        v3 wm = orc_ggx_sample_vndf(w_t, ax, ay, z1, z2);
+        w_o = vadd(vneg(w_t), vscale(vscale(wm, (R)2.0f), vdot(w_t, wm)));
+        *weight = vmul(*weight, orc_fresnel_conductor(w_t, wm, w_t.z > 0 ? eta : ieta));
+        if (hr != hr || w_o.z != w_o.z) {
+            w_o = V3(0, 0, 1);
+            break;
+        }
+    }
+    *w_out = w_o;
+    return 1;
+}
+
 int orc_ggx_ms_dielectric_sample(v3 local_wi, R eta, R ax, R ay, uint32_t* seed, int cap, v3* w_out, int* order)
 {
     R hr = (R)1.0f + orc_height_inv_cdf((R)0.9999f);

@@ -59,6 +59,7 @@ typedef struct {
     int32_t nmats;
     orc_material* mats;
     cplx3* eta;
+    cplx3* ieta; /* 1 / eta = the complex ior (conductor walk events met from below) */
     int32_t nlights;
     orc_light* lights;
     orc_trilight* trilights;
@@ -777,6 +778,18 @@ static v3 shade_conductor(const oscene* s, const orc_material* m, const cplx3* e
         int k = diff_texel(s->rough_uv, s->rough_uvp, hit_point, s->tex_w, s->tex_h);
         ax = ay = s->rough_frozen[k];
         ae = s->rough_eval[k];
+    }
+    if (m->microfacet_model == 2) {
+        /* multiple-scattering model (CU/metallic_shader.cu:66-82): the GGX walk in
+         * the frame of the unflipped normal, child radiance times its weight */
+        v3 lo, wt;
+        int order;
+        const cplx3* ie = s->ieta + (eta - s->eta);
+        if (!orc_ggx_ms_conductor_sample(orc_to_local(w_i, normal), eta, ie, ax, ay, seed, MS_WALK_CAP, &lo, &wt,
+                                         &order))
+            return V3(0, 0, 0);
+        v3 child = radiance(s, hit_point, orc_to_world(lo, normal), depth + 1, 1, seed, (R)-1, st);
+        return vmul(child, wt);
     }
     R z1 = orc_rnd_tea(seed);
     R z2 = orc_rnd_tea(seed);
@@ -1714,6 +1727,12 @@ void* orc_scene_create(const orc_scene_desc* d)
         s->eta[i].y = orc_cdiv_fc((R)1.0f, c1);
         s->eta[i].z = orc_cdiv_fc((R)1.0f, c2);
     }
+    s->ieta = (cplx3*)malloc(sizeof(cplx3) * (size_t)(d->nmats + 1));
+    for (int i = 0; i < d->nmats; i++) {
+        s->ieta[i].x = (cplx){d->mats[i].ior_re[0], d->mats[i].ior_im[0]};
+        s->ieta[i].y = (cplx){d->mats[i].ior_re[1], d->mats[i].ior_im[1]};
+        s->ieta[i].z = (cplx){d->mats[i].ior_re[2], d->mats[i].ior_im[2]};
+    }
     s->sss = (osss*)calloc((size_t)(d->nmats + 1), sizeof(osss));
     for (int i = 0; i < d->nmats; i++) s->sss[i] = sss_coefficients(&d->mats[i]);
     s->probe_mat = -1;
@@ -1753,6 +1772,7 @@ void orc_scene_destroy(void* p)
     free(s->nidx);
     free(s->normals);
     free(s->tri_mat);
+    free(s->ieta);
     free(s->an_c);
     free(s->an_n);
     free(s->an_r);

@@ -195,3 +195,25 @@ def test_rough_diffuse_per_sample_and_batch(dist):
     ms.set_uint("microfacet_model", 2, material=0)
     with pytest.raises(rt.OrfError, match="not built"):
         ms.accel_build()
+
+
+def test_multiscattering_conductor_per_sample():
+    """CU/metallic_shader.cu:66-82 with the GGX microsurface walk
+    (ORF/MicrofacetGGX.h:614-637) on the sphere of the path-tracer scene."""
+    from tests.oracle_scene import OracleScene
+
+    d = scenes.path_scene(width=W, height=W, level=4, env_size=(128, 64), max_depth=6, alpha=0.6)
+    d.materials[1].uints = {"microfacet_model": (2,), "normal_distribution": (1,)}
+    d.materials[1].ior_complex = ((0.18, 3.42), (0.42, 2.35), (1.37, 1.77))
+    ctx = scenes.to_context(d)
+    ctx.accel_build()
+    orc = OracleScene(d)
+    ys, xs = np.mgrid[0:W, 0:W]
+    for frame in (0, 2):
+        ctx.film_clear()
+        ctx.launch(W, W, frame, 1)
+        got = ctx.film_sums(W, W)[..., :3]
+        want = orc.samples(W, W, xs.ravel(), ys.ravel(), frame).reshape(W, W, 3)
+        ok = _ok(got, want)
+        _say("ms conductor frame %d samples ok %.5f mean %.4g / %.4g" % (frame, ok.mean(), got.mean(), want.mean()))
+        assert ok.mean() >= 0.99
