@@ -339,11 +339,9 @@ void prepare(orf_context_t* c, hipStream_t stream)
             const bool ms = m[i].microfacet_model == MULTISCATTERING_MODEL;
             ORF_REQUIRE(!(ms && m[i].kind == MAT_ROUGH_DIFFUSE),
                         "material " + std::to_string(i) + ": the rough diffuse multiple-scattering walk is not built");
-            ORF_REQUIRE(!(ms && m[i].kind == MAT_CONDUCTOR && m[i].normal_distribution == BECKMANN_DISTRIBUTION),
-                        "material " + std::to_string(i) + ": the Beckmann conductor multiple-scattering walk is not built");
-            ORF_REQUIRE(!(ms && m[i].normal_distribution == BECKMANN_DISTRIBUTION &&
-                          (m[i].kind == MAT_DIELECTRIC || m[i].kind == MAT_ROUGH_TRANSLUCENT)),
-                        "material " + std::to_string(i) + ": the Beckmann multiple-scattering walk is not built");
+            ORF_REQUIRE(!(ms && m[i].normal_distribution == BECKMANN_DISTRIBUTION && m[i].kind == MAT_ROUGH_TRANSLUCENT),
+                        "material " + std::to_string(i) +
+                            ": the Beckmann multiple-scattering walk of the rough translucent boundary is not built");
         }
         if (m.empty()) m.push_back(default_material(MAT_LAMBERT));
         if (sss.empty()) sss.push_back(scattering_coefficients(SssParamsHost()));

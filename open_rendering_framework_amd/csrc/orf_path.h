@@ -189,12 +189,15 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         if (ps.depth >= sc.max_depth) return false;
         f3 wo, w;
         if (kDiel && mat.microfacet_model == MULTISCATTERING_MODEL) {
-            // CU/metallic_shader.cu:66-82: the GGX microsurface walk in the frame of
-            // the unflipped normal; its weight multiplies the continuation
+            // CU/metallic_shader.cu:66-82: the microsurface walk (GGX or Beckmann) in
+            // the frame of the unflipped normal; its weight multiplies the continuation
             f3 lo;
-            if (!ggx_ms_conductor_sample(to_local(-d, normal), mat.eta, mat.roughness.x, mat.roughness.y, ps.rng, lo,
-                                         w))
-                return false;
+            const bool ok = mat.normal_distribution == GGX_DISTRIBUTION
+                                ? ggx_ms_conductor_sample(to_local(-d, normal), mat.eta, mat.roughness.x,
+                                                          mat.roughness.y, ps.rng, lo, w)
+                                : beckmann_ms_conductor_sample(to_local(-d, normal), mat.eta, mat.roughness.x,
+                                                               mat.roughness.y, ps.rng, lo, w);
+            if (!ok) return false;
             wo = to_world(lo, normal);
         } else {
             float z1 = rnd_tea(ps.rng);

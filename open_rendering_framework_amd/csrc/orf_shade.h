@@ -436,6 +436,99 @@ ORF_HD bool ggx_ms_conductor_sample(f3 local_wi, const ConductorEta& eta, float 
     return true;
 }
 
+// beckmann_smith_lambda (ORF/MicrofacetBeckmann.h:24-38).
+ORF_HD float beckmann_smith_lambda(f3 wi, float ax, float ay)
+{
+    if (wi.z > 0.9999f) return 0.0f;
+    if (wi.z < -0.9999f) return -1.0f;
+    const float t = orf_tanf(orf_acosf(wi.z));
+    if (__builtin_isinf(t)) return 0.0f;
+    const float a = 1.0f / (t * ggx_alpha_proj(wi, ax, ay));  // beckmann_alpha_i is the same expression
+    return 0.5f * (orf_erff(a) - 1.0f) + 0.25f * 1.12837916709551257390f / a * orf_expf(-a * a);
+}
+
+// beckmann_sample_height (ORF/MicrofacetBeckmann.h:284-306): the GGX form with
+// 0.999 direction limits and the height-averaged masking 1 / (1 + Lambda)
+// (beckmann_G(wi, a_x, a_y), :130-141) as the escape probability.
+ORF_HD float beckmann_sample_height(f3 wr, float hr, float z, float ax, float ay)
+{
+    if (wr.z > 0.999f) return kRayMax;
+    if (wr.z < -0.999f) return height_inv_cdf(z * height_cdf(hr));
+    if (fabsf(wr.z) < 0.00001f) return hr;
+    const float g1 = (wr.z > 0.9999f) ? 1.0f : (wr.z <= 0.0f ? 0.0f : 1.0f / (1.0f + beckmann_smith_lambda(wr, ax, ay)));
+    if (z > 1.0f - g1) return kRayMax;
+    return height_inv_cdf(height_cdf(hr) / orf_powf(1.0f - z, 1.0f / beckmann_smith_lambda(wr, ax, ay)));
+}
+
+// beckmann_sample_dielectric_phase (ORF/MicrofacetBeckmann.h:310-340): visible
+// normal from the side `outside` names, reflection when the selector is <= F,
+// results normalised.
+ORF_HD f3 beckmann_sample_dielectric_phase(f3 w_i, float m_eta, uint32_t& rng, bool& outside, float ax, float ay)
+{
+    const float z1 = rnd_tea(rng);
+    const float z2 = rnd_tea(rng);
+    const bool wi_out = outside;
+    const float eta = wi_out ? m_eta : 1.0f / m_eta;
+    const f3 wm = wi_out ? beckmann_sample_visible(w_i, ax, ay, z1, z2) : -beckmann_sample_visible(-w_i, ax, ay, z1, z2);
+    const float c = fabsf(dot(wm, w_i));
+    const float st2 = eta * eta * (1.0f - c * c);
+    const float ct = sqrtf(1.0f - st2);
+    const float F = st2 < 1.0f ? fresnel_R3(c, ct, eta) : 1.0f;
+    if (rnd_tea(rng) <= F) return normalize(-w_i + (wm * 2.0f) * dot(w_i, wm));
+    outside = !wi_out;
+    return normalize((-eta) * w_i + wm * (eta * c - ct));
+}
+
+// beckmann_multiscattering_dielectric_BSDF_sample (ORF/MicrofacetBeckmann.h:445-475).
+ORF_HD bool beckmann_ms_dielectric_sample(f3 local_wi, float eta, float ax, float ay, uint32_t& rng, f3& w_out)
+{
+    float hr = 1.0f + height_inv_cdf(0.9999f);
+    f3 w_o = -local_wi;
+    bool outside = (-w_o.z) > 0.0f;
+    int order = 0;
+    for (;;) {
+        const float z = rnd_tea(rng);
+        hr = outside ? beckmann_sample_height(w_o, hr, z, ax, ay) : -beckmann_sample_height(-w_o, -hr, z, ax, ay);
+        if (hr == kRayMax || hr == -kRayMax) break;
+        if (++order > kMsWalkCap) return false;
+        w_o = beckmann_sample_dielectric_phase(-w_o, eta, rng, outside, ax, ay);
+        if (hr != hr || w_o.z != w_o.z) {
+            w_o = mk3(0.0f, 0.0f, 1.0f);
+            break;
+        }
+    }
+    w_out = w_o;
+    return true;
+}
+
+// beckmann_multiscattering_conductor_BSDF_sample (ORF/MicrofacetBeckmann.h:711-736).
+ORF_HD bool beckmann_ms_conductor_sample(f3 local_wi, const ConductorEta& eta, float ax, float ay, uint32_t& rng,
+                                         f3& w_out, f3& weight)
+{
+    float hr = 1.0f + height_inv_cdf(0.9999f);
+    f3 w_o = -local_wi;
+    weight = mk3(1.0f);
+    int order = 0;
+    for (;;) {
+        const float u = rnd_tea(rng);
+        hr = beckmann_sample_height(w_o, hr, u, ax, ay);
+        if (hr == kRayMax) break;
+        if (++order > kMsWalkCap) return false;
+        const f3 w_t = -w_o;
+        const float z1 = rnd_tea(rng);
+        const float z2 = rnd_tea(rng);
+        const f3 wm = beckmann_sample_visible(w_t, ax, ay, z1, z2);
+        w_o = -w_t + (wm * 2.0f) * dot(w_t, wm);
+        weight = weight * (w_t.z > 0.0f ? fresnel_conductor(w_t, wm, eta) : fresnel_conductor_inv(w_t, wm, eta));
+        if (hr != hr || w_o.z != w_o.z) {
+            w_o = mk3(0.0f, 0.0f, 1.0f);
+            break;
+        }
+    }
+    w_out = w_o;
+    return true;
+}
+
 // ggx_multiscattering_dielectric_BSDF_eval (ORF/MicrofacetGGX.h:444-478): one
 // random-walk estimate of the multiple-scattering BSDF times cos for (wi, wo),
 // local frame: the sum over the walk's events of phase(wo) x masking of wo at
@@ -488,11 +581,14 @@ ORF_HD bool dielectric_step(f3 d, float t_hit, f3 normal, f3 ffn, const Material
     float z2 = rnd_tea(rng);
     float a = m.roughness.x;
     int dist = m.normal_distribution;
-    if (m.microfacet_model == MULTISCATTERING_MODEL && dist == GGX_DISTRIBUTION) {
+    if (m.microfacet_model == MULTISCATTERING_MODEL) {
         // CU/rough_transparent_shader.cu:98-116: the walk starts from the
         // un-flipped normal's frame (z1, z2 above are drawn and unused, as there)
         f3 lo;
-        if (!ggx_ms_dielectric_sample(to_local(w_i, normal), 1.0f / m.ior, a, a, rng, lo)) return false;
+        const bool ok = dist == GGX_DISTRIBUTION
+                            ? ggx_ms_dielectric_sample(to_local(w_i, normal), 1.0f / m.ior, a, a, rng, lo)
+                            : beckmann_ms_dielectric_sample(to_local(w_i, normal), 1.0f / m.ior, a, a, rng, lo);
+        if (!ok) return false;
         new_dir = to_world(lo, normal);
         factor = beam;
         return true;

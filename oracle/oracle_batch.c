@@ -229,11 +229,11 @@ void orc_ggx_ms_sample_batch(int n, const float* wi, float eta, float alpha, uin
     }
 }
 
-/* Conductor microsurface walks from the fixed local wi (stream tea16(i, seed)),
+/* Conductor microsurface walks (dist 1 GGX, 0 Beckmann) from the fixed local wi (stream tea16(i, seed)),
  * complex ior (re, im) per channel given as 6 doubles: exit direction, weight
  * (3 per walk), order and ok. */
-void orc_ggx_ms_conductor_batch(int n, const float* wi, const double* ior, float alpha, uint32_t seed, int cap,
-                                float* wo, float* weight, int32_t* order, int32_t* ok)
+void orc_ggx_ms_conductor_batch(int n, const float* wi, const double* ior, float alpha, int dist, uint32_t seed,
+                                int cap, float* wo, float* weight, int32_t* order, int32_t* ok)
 {
     cplx3 eta, ieta;
     ieta.x = (cplx){ior[0], ior[1]};
@@ -247,7 +247,10 @@ void orc_ggx_ms_conductor_batch(int n, const float* wi, const double* ior, float
         uint32_t t = orc_tea16((uint32_t)i, seed);
         v3 out = V3(0, 0, 0), w = V3(0, 0, 0);
         int o = 0;
-        ok[i] = orc_ggx_ms_conductor_sample(V3(wi[0], wi[1], wi[2]), &eta, &ieta, alpha, alpha, &t, cap, &out, &w, &o);
+        ok[i] = dist == 1 ? orc_ggx_ms_conductor_sample(V3(wi[0], wi[1], wi[2]), &eta, &ieta, alpha, alpha, &t, cap,
+                                                        &out, &w, &o)
+                          : orc_beckmann_ms_conductor_sample(V3(wi[0], wi[1], wi[2]), &eta, &ieta, alpha, alpha, &t,
+                                                             cap, &out, &w, &o);
         order[i] = o;
         wo[3 * i] = (float)out.x;
         wo[3 * i + 1] = (float)out.y;
@@ -255,6 +258,24 @@ void orc_ggx_ms_conductor_batch(int n, const float* wi, const double* ior, float
         weight[3 * i] = (float)w.x;
         weight[3 * i + 1] = (float)w.y;
         weight[3 * i + 2] = (float)w.z;
+    }
+}
+
+/* Beckmann walks from the fixed local wi (stream tea16(i, seed)): dielectric
+ * (relative ior eta) exit directions, orders and flags. */
+void orc_beckmann_ms_sample_batch(int n, const float* wi, float eta, float alpha, uint32_t seed, int cap, float* wo,
+                                  int32_t* order, int32_t* ok)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        uint32_t t = orc_tea16((uint32_t)i, seed);
+        v3 out = V3(0, 0, 0);
+        int o = 0;
+        ok[i] = orc_beckmann_ms_dielectric_sample(V3(wi[0], wi[1], wi[2]), eta, alpha, alpha, &t, cap, &out, &o);
+        order[i] = o;
+        wo[3 * i] = (float)out.x;
+        wo[3 * i + 1] = (float)out.y;
+        wo[3 * i + 2] = (float)out.z;
     }
 }
 

@@ -716,6 +716,99 @@ static v3 ggx_sample_dielectric_phase(v3 w_i, R m_eta, uint32_t* seed, int* outs
 /* ggx_multiscattering_dielectric_BSDF_sample (ORF/MicrofacetGGX.h:412-438) with
  * at most `cap` scattering events (the reference's loop is unbounded); returns
  * 0 when the walk exceeds the cap.  *order receives the event count. */
+/* beckmann_smith_lambda (ORF/MicrofacetBeckmann.h:24-38). */
+R orc_beckmann_smith_lambda(v3 wi, R ax, R ay)
+{
+    if (wi.z > (R)0.9999f) return 0;
+    if (wi.z < (R)-0.9999f) return -1;
+    R th = F_ACOS(wi.z);
+    R t = F_TAN(th);
+    if (isinf(t)) return 0;
+    R a = (R)1.0f / (t * orc_ggx_alpha_i(wi, ax, ay));
+    return (R)0.5f * ((R)F_ERF(a) - (R)1.0f) + (R)0.25f * M_2_SQRTPI_R / a * F_EXP(-a * a);
+}
+
+/* beckmann_sample_height (ORF/MicrofacetBeckmann.h:284-306): 0.999 direction
+ * limits; escape probability 1 / (1 + Lambda) (beckmann_G(wi, a_x, a_y), :130-141). */
+R orc_beckmann_sample_height(v3 wr, R hr, R z, R ax, R ay)
+{
+    if (wr.z > (R)0.999f) return RAY_MAX_R;
+    if (wr.z < (R)-0.999f) return orc_height_inv_cdf(z * orc_height_cdf(hr));
+    if (fabs(wr.z) < (R)0.00001f) return hr;
+    R g1 = wr.z > (R)0.9999f ? (R)1 : (wr.z <= 0 ? (R)0 : (R)1.0f / ((R)1.0f + orc_beckmann_smith_lambda(wr, ax, ay)));
+    if (z > (R)1.0f - g1) return RAY_MAX_R;
+    R p = F_POW((R)1.0f - z, (R)1.0f / orc_beckmann_smith_lambda(wr, ax, ay));
+    return orc_height_inv_cdf(orc_height_cdf(hr) / p);
+}
+
+/* beckmann_sample_dielectric_phase (ORF/MicrofacetBeckmann.h:310-340). */
+static v3 beckmann_sample_dielectric_phase(v3 w_i, R m_eta, uint32_t* seed, int* outside, R ax, R ay)
+{
+    R z1 = orc_rnd_tea(seed);
+    R z2 = orc_rnd_tea(seed);
+    int wi_out = *outside;
+    R eta = wi_out ? m_eta : (R)1.0f / m_eta;
+    v3 wm = wi_out ? orc_beckmann_sample_vndf(w_i, ax, ay, z1, z2) : vneg(orc_beckmann_sample_vndf(vneg(w_i), ax, ay, z1, z2));
+    R c = fabs(vdot(wm, w_i));
+    R st2 = eta * eta * ((R)1.0f - c * c);
+    R ct = sqrt((R)1.0f - st2);
+    R F = st2 < (R)1.0f ? orc_fresnel_R3(c, ct, eta) : (R)1.0f;
+    if (orc_rnd_tea(seed) <= F) return orc_normalize(vadd(vneg(w_i), vscale(vscale(wm, (R)2.0f), vdot(w_i, wm))));
+    *outside = !wi_out;
+    return orc_normalize(vadd(vscale(w_i, -eta), vscale(wm, eta * c - ct)));
+}
+
+/* beckmann_multiscattering_dielectric_BSDF_sample (ORF/MicrofacetBeckmann.h:445-475). */
+int orc_beckmann_ms_dielectric_sample(v3 local_wi, R eta, R ax, R ay, uint32_t* seed, int cap, v3* w_out, int* order)
+{
+    R hr = (R)1.0f + orc_height_inv_cdf((R)0.9999f);
+    v3 w_o = vneg(local_wi);
+    int outside = (-w_o.z) > 0;
+    *order = 0;
+    for (;;) {
+        R z = orc_rnd_tea(seed);
+        hr = outside ? orc_beckmann_sample_height(w_o, hr, z, ax, ay)
+                     : -orc_beckmann_sample_height(vneg(w_o), -hr, z, ax, ay);
+        if (hr == RAY_MAX_R || hr == -RAY_MAX_R) break;
+        if (++*order > cap) return 0;
+        w_o = beckmann_sample_dielectric_phase(vneg(w_o), eta, seed, &outside, ax, ay);
+        if (hr != hr || w_o.z != w_o.z) {
+            w_o = V3(0, 0, 1);
+            break;
+        }
+    }
+    *w_out = w_o;
+    return 1;
+}
+
+/* beckmann_multiscattering_conductor_BSDF_sample (ORF/MicrofacetBeckmann.h:711-736). */
+int orc_beckmann_ms_conductor_sample(v3 local_wi, const cplx3* eta, const cplx3* ieta, R ax, R ay, uint32_t* seed,
+                                     int cap, v3* w_out, v3* weight, int* order)
+{
+    R hr = (R)1.0f + orc_height_inv_cdf((R)0.9999f);
+    v3 w_o = vneg(local_wi);
+    *weight = V3(1, 1, 1);
+    *order = 0;
+    for (;;) {
+        R u = orc_rnd_tea(seed);
+        hr = orc_beckmann_sample_height(w_o, hr, u, ax, ay);
+        if (hr == RAY_MAX_R) break;
+        if (++*order > cap) return 0;
+        v3 w_t = vneg(w_o);
+        R z1 = orc_rnd_tea(seed);
+        R z2 = orc_rnd_tea(seed);
+        v3 wm = orc_beckmann_sample_vndf(w_t, ax, ay, z1, z2);
+        w_o = vadd(vneg(w_t), vscale(vscale(wm, (R)2.0f), vdot(w_t, wm)));
+        *weight = vmul(*weight, orc_fresnel_conductor(w_t, wm, w_t.z > 0 ? eta : ieta));
+        if (hr != hr || w_o.z != w_o.z) {
+            w_o = V3(0, 0, 1);
+            break;
+        }
+    }
+    *w_out = w_o;
+    return 1;
+}
+
 /* ggx_multiscattering_conductor_BSDF_sample (ORF/MicrofacetGGX.h:614-637): the
  * random walk on the GGX microsurface of a conductor, local frame.  Each event
  * reflects about a visible normal of the arriving direction w_t and multiplies

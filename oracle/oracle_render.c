@@ -785,9 +785,12 @@ static v3 shade_conductor(const oscene* s, const orc_material* m, const cplx3* e
         v3 lo, wt;
         int order;
         const cplx3* ie = s->ieta + (eta - s->eta);
-        if (!orc_ggx_ms_conductor_sample(orc_to_local(w_i, normal), eta, ie, ax, ay, seed, MS_WALK_CAP, &lo, &wt,
-                                         &order))
-            return V3(0, 0, 0);
+        int ok = m->normal_distribution == 1
+                     ? orc_ggx_ms_conductor_sample(orc_to_local(w_i, normal), eta, ie, ax, ay, seed, MS_WALK_CAP, &lo,
+                                                   &wt, &order)
+                     : orc_beckmann_ms_conductor_sample(orc_to_local(w_i, normal), eta, ie, ax, ay, seed, MS_WALK_CAP,
+                                                        &lo, &wt, &order);
+        if (!ok) return V3(0, 0, 0);
         v3 child = radiance(s, hit_point, orc_to_world(lo, normal), depth + 1, 1, seed, (R)-1, st);
         return vmul(child, wt);
     }
@@ -883,14 +886,16 @@ static v3 shade_dielectric(const oscene* s, const orc_material* m, v3 o, v3 d, c
     R z2 = orc_rnd_tea(t);
     R a = m->roughness[0];
     int dist = m->normal_distribution;
-    if (m->microfacet_model == 2 && dist == 1) {
+    if (m->microfacet_model == 2) {
         /* multiple-scattering model (CU/rough_transparent_shader.cu:98-116): z1, z2
          * above are drawn and unused; the walk runs in the frame of `normal` */
         v3 lo;
         int order;
-        if (!orc_ggx_ms_dielectric_sample(orc_to_local(w_i, normal), (R)1.0f / (R)m->ior, a, a, t, MS_WALK_CAP, &lo,
-                                          &order))
-            return zero;
+        int ok = dist == 1 ? orc_ggx_ms_dielectric_sample(orc_to_local(w_i, normal), (R)1.0f / (R)m->ior, a, a, t,
+                                                          MS_WALK_CAP, &lo, &order)
+                           : orc_beckmann_ms_dielectric_sample(orc_to_local(w_i, normal), (R)1.0f / (R)m->ior, a, a, t,
+                                                               MS_WALK_CAP, &lo, &order);
+        if (!ok) return zero;
         v3 dir = orc_to_world(lo, normal);
         v3 child = radiance(s, hit_point, dir, depth + 1, 1, t, (R)-1, st);
         return vmul(child, beam);

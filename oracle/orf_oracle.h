@@ -103,6 +103,9 @@ R orc_height_inv_cdf(R z);
 R orc_ggx_masking_height(v3 wi, R h, R ax, R ay);
 R orc_ggx_sample_height(v3 wr, R hr, R z, R ax, R ay);
 int orc_ggx_ms_dielectric_sample(v3 local_wi, R eta, R ax, R ay, uint32_t* seed, int cap, v3* w_out, int* order);
+int orc_beckmann_ms_dielectric_sample(v3 local_wi, R eta, R ax, R ay, uint32_t* seed, int cap, v3* w_out, int* order);
+int orc_beckmann_ms_conductor_sample(v3 local_wi, const cplx3* eta, const cplx3* ieta, R ax, R ay, uint32_t* seed,
+                                     int cap, v3* w_out, v3* weight, int* order);
 int orc_ggx_ms_conductor_sample(v3 local_wi, const cplx3* eta, const cplx3* ieta, R ax, R ay, uint32_t* seed, int cap,
                                 v3* w_out, v3* weight, int* order);
 R orc_fresnel_R_dir(v3 w_i, v3 w_m, R eta);

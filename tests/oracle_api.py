@@ -47,8 +47,10 @@ class Oracle:
         L.orc_beckmann_sample_vndf_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_float, ctypes.c_float, c_vp, c_vp,
                                                      c_vp]
         L.orc_erfinv_batch.argtypes = [ctypes.c_int, c_vp, c_vp]
-        L.orc_ggx_ms_conductor_batch.argtypes = [ctypes.c_int, c_vp, c_vp, ctypes.c_float, ctypes.c_uint32,
-                                                 ctypes.c_int, c_vp, c_vp, c_vp, c_vp]
+        L.orc_ggx_ms_conductor_batch.argtypes = [ctypes.c_int, c_vp, c_vp, ctypes.c_float, ctypes.c_int,
+                                                 ctypes.c_uint32, ctypes.c_int, c_vp, c_vp, c_vp, c_vp]
+        L.orc_beckmann_ms_sample_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_float, ctypes.c_float,
+                                                   ctypes.c_uint32, ctypes.c_int, c_vp, c_vp, c_vp]
         L.orc_ggx_ms_sample_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
                                               ctypes.c_int, c_vp, c_vp, c_vp]
         L.orc_ggx_ms_eval_batch.argtypes = [ctypes.c_int, c_vp, c_vp, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
@@ -200,7 +202,16 @@ class Oracle:
         self.lib.orc_ggx_ms_sample_batch(n, _p(wi), eta, alpha, seed, cap, _p(wo), _p(order), _p(ok))
         return wo, order, ok
 
-    def ggx_ms_conductor(self, wi, ior, alpha, n, seed=1, cap=10):
+    def beckmann_ms_sample(self, wi, eta, alpha, n, seed=1, cap=10):
+        """Beckmann dielectric walks: (exit directions (n, 3), orders, completed flags)."""
+        wi = np.ascontiguousarray(wi, np.float32)
+        wo = np.zeros((n, 3), np.float32)
+        order = np.zeros(n, np.int32)
+        ok = np.zeros(n, np.int32)
+        self.lib.orc_beckmann_ms_sample_batch(n, _p(wi), eta, alpha, seed, cap, _p(wo), _p(order), _p(ok))
+        return wo, order, ok
+
+    def ggx_ms_conductor(self, wi, ior, alpha, n, seed=1, cap=10, dist=1):
         """Conductor walks: (exit directions (n, 3), weights (n, 3), orders, completed flags).
         ior: ((re, im),) * 3."""
         wi = np.ascontiguousarray(wi, np.float32)
@@ -209,7 +220,8 @@ class Oracle:
         w = np.zeros((n, 3), np.float32)
         order = np.zeros(n, np.int32)
         ok = np.zeros(n, np.int32)
-        self.lib.orc_ggx_ms_conductor_batch(n, _p(wi), _p(c), alpha, seed, cap, _p(wo), _p(w), _p(order), _p(ok))
+        self.lib.orc_ggx_ms_conductor_batch(n, _p(wi), _p(c), alpha, dist, seed, cap, _p(wo), _p(w), _p(order),
+                                            _p(ok))
         return wo, w, order, ok
 
     def ggx_ms_eval(self, wi, wo, eta, alpha, seed=1, cap=10):

@@ -197,13 +197,14 @@ def test_rough_diffuse_per_sample_and_batch(dist):
         ms.accel_build()
 
 
-def test_multiscattering_conductor_per_sample():
-    """CU/metallic_shader.cu:66-82 with the GGX microsurface walk
-    (ORF/MicrofacetGGX.h:614-637) on the sphere of the path-tracer scene."""
+@pytest.mark.parametrize("dist", [1, 0])
+def test_multiscattering_conductor_per_sample(dist):
+    """CU/metallic_shader.cu:66-82 with the GGX / Beckmann microsurface walks
+    (ORF/MicrofacetGGX.h:614-637, ORF/MicrofacetBeckmann.h:711-736) on the sphere."""
     from tests.oracle_scene import OracleScene
 
     d = scenes.path_scene(width=W, height=W, level=4, env_size=(128, 64), max_depth=6, alpha=0.6)
-    d.materials[1].uints = {"microfacet_model": (2,), "normal_distribution": (1,)}
+    d.materials[1].uints = {"microfacet_model": (2,), "normal_distribution": (dist,)}
     d.materials[1].ior_complex = ((0.18, 3.42), (0.42, 2.35), (1.37, 1.77))
     ctx = scenes.to_context(d)
     ctx.accel_build()
@@ -216,4 +217,28 @@ def test_multiscattering_conductor_per_sample():
         want = orc.samples(W, W, xs.ravel(), ys.ravel(), frame).reshape(W, W, 3)
         ok = _ok(got, want)
         _say("ms conductor frame %d samples ok %.5f mean %.4g / %.4g" % (frame, ok.mean(), got.mean(), want.mean()))
+        assert ok.mean() >= 0.99
+
+
+def test_beckmann_multiscattering_glass_per_sample():
+    """CU/rough_transparent_shader.cu:98-116 with the Beckmann microsurface walk
+    (ORF/MicrofacetBeckmann.h:445-475).  Beckmann runs the FP32 erfinv where the
+    oracle uses FP64: bar 0.99."""
+    from tests.oracle_scene import OracleScene
+
+    d = scenes.path_scene(width=W, height=W, level=4, env_size=(128, 64), max_depth=6)
+    d.materials[1] = scenes.Material("rough_transparent_shader.cu",
+                                     floats={"roughness": (0.5,), "ior": (1.5,), "glass_absorption": (0.1, 0.2, 0.3)},
+                                     uints={"microfacet_model": (2,), "normal_distribution": (0,)})
+    ctx = scenes.to_context(d)
+    ctx.accel_build()
+    orc = OracleScene(d)
+    ys, xs = np.mgrid[0:W, 0:W]
+    for frame in (0, 3):
+        ctx.film_clear()
+        ctx.launch(W, W, frame, 1)
+        got = ctx.film_sums(W, W)[..., :3]
+        want = orc.samples(W, W, xs.ravel(), ys.ravel(), frame).reshape(W, W, 3)
+        ok = _ok(got, want)
+        _say("beckmann ms glass frame %d samples ok %.5f" % (frame, ok.mean()))
         assert ok.mean() >= 0.99

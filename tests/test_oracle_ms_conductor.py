@@ -6,10 +6,15 @@
   weight 1, and the walk completes within 10 events almost always.
 - At low roughness the walk is single scattering: 99% of walks end after one
   event at alpha 0.05.
+- Both the GGX and the Beckmann walks (ORF/MicrofacetBeckmann.h:711-736).
+- The Beckmann dielectric walk at roughness 0.02 reflects the Fresnel share.
 - Energy lost by a real conductor grows with the number of bounces: the mean
   weight of order-k walks is below that of order-(k-1) walks for gold-like ior.
 """
+import math
+
 import numpy as np
+import pytest
 
 from tests.oracle_api import Oracle
 
@@ -22,10 +27,11 @@ def _wi(theta_deg):
     return np.array([np.sin(t), 0.0, np.cos(t)], np.float32)
 
 
-def test_mirror_walk_conserves_energy():
+@pytest.mark.parametrize("dist", [1, 0])
+def test_mirror_walk_conserves_energy(dist):
     o = Oracle()
     for theta, alpha in ((10, 0.3), (60, 0.6), (80, 0.9)):
-        wo, w, order, ok = o.ggx_ms_conductor(_wi(theta), MIRROR, alpha, 20000)
+        wo, w, order, ok = o.ggx_ms_conductor(_wi(theta), MIRROR, alpha, 20000, dist=dist)
         done = ok == 1
         assert done.mean() > 0.999, (theta, alpha, done.mean())
         assert np.all(wo[done, 2] > 0.0)
@@ -50,3 +56,20 @@ def test_gold_loses_energy_per_bounce():
     m3 = w[done & (order == 3)].mean(axis=0)
     assert np.all(m1 > m2) and np.all(m2 > m3), (m1, m2, m3)
     assert np.all((w[done] >= 0) & (w[done] <= 1))
+
+
+def test_beckmann_dielectric_walk_smooth_limit_is_fresnel():
+    """Beckmann dielectric walk (ORF/MicrofacetBeckmann.h:445-475) at roughness
+    0.02: the share of walks leaving upwards is the unpolarised Fresnel
+    reflectance at the incidence angle (45 degrees, relative index 1/1.5)."""
+    o = Oracle()
+    wo, order, ok = o.beckmann_ms_sample(_wi(45), 1.0 / 1.5, 0.02, 40000)
+    up = (wo[ok == 1, 2] > 0).mean()
+    ci = math.cos(math.radians(45))
+    st = math.sin(math.radians(45)) / 1.5
+    ct = math.sqrt(1 - st * st)
+    rs = (ci - 1.5 * ct) / (ci + 1.5 * ct)
+    rp = (1.5 * ci - ct) / (1.5 * ci + ct)
+    want = 0.5 * (rs * rs + rp * rp)
+    assert abs(up - want) < 0.05 * want, (up, want)
+    assert (ok == 1).mean() > 0.999
