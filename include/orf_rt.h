@@ -190,6 +190,21 @@ orf_status orf_film_read_sums(orf_context ctx, double* sums);
 /* Counters since the last clear: [0] non-finite samples dropped,
  * [1] extension (closest-hit) rays, [2] shadow rays, [3] paths. */
 orf_status orf_counters(orf_context ctx, uint64_t out[4]);
+/* No counterpart in the reference: orf_launch for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  Same kernels, seeds and film arithmetic as orf_launch
+ * (path tracer integrator only), but the film is the caller's: `film` is a
+ * device array of 4*width*height doubles (sum R, G, B, count per pixel, row 0 =
+ * bottom row) that the launch adds to.  `counters` (device, 4 uint64, may be
+ * NULL) receives the orf_counters statistics the same way.  The scene must
+ * already be on the device (orf_accel_build, or a previous launch, after the
+ * last scene change), else ORF_ERROR_INVALID_VALUE.  The launch's own working
+ * memory is a fixed 128-byte per-device static array (8 work-queue counters and
+ * 4 fallback statistics counters), cleared by a kernel at the start of each
+ * call: the call allocates nothing and needs no size limit beyond orf_launch's,
+ * but calls that overlap on one device race on that array. */
+orf_status orf_launch_noalloc(orf_context ctx, int width, int height, uint32_t frame0, uint32_t spp, double* film,
+                              uint64_t* counters, void* stream);
 
 /* ------------------------------------------------------------------------
  * Batched renders: many uneven renders of the context's scene in one call.

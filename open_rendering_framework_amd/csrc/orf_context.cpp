@@ -353,6 +353,11 @@ void prepare(orf_context_t* c, hipStream_t stream)
     if (c->dirty_light) upload_lights(c);
 }
 
+bool scene_ready(const orf_context_t* c)
+{
+    return !c->dirty_geo && !c->dirty_mat && !c->dirty_env && !c->dirty_light && c->d_bvh4.p != nullptr;
+}
+
 SceneGPU scene_view(orf_context_t* c)
 {
     SceneGPU s;

@@ -80,6 +80,8 @@ struct orf_context_t {
 namespace orf {
 orf_context_t* ctx_check(orf_context ctx);
 void prepare(orf_context_t* c, hipStream_t stream);
+// True when every scene change is on the device (prepare() would do nothing).
+bool scene_ready(const orf_context_t* c);
 SceneGPU scene_view(orf_context_t* c);
 void run_probe_pass(orf_context_t* c, unsigned frame, hipStream_t stream);
 void film_ensure(orf_context_t* c, int w, int h);

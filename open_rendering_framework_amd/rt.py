@@ -62,6 +62,7 @@ _SIGS = {
     "orf_accel_mark_dirty": [c_vp],
     "orf_accel_build": [c_vp, c_vp],
     "orf_launch": [c_vp, ctypes.c_int, ctypes.c_int, c_u32, c_u32, c_vp],
+    "orf_launch_noalloc": [c_vp, ctypes.c_int, ctypes.c_int, c_u32, c_u32, c_vp, c_vp, c_vp],
     "orf_film_clear": [c_vp],
     "orf_film_read": [c_vp, c_vp, c_vp],
     "orf_film_read_sums": [c_vp, c_vp],
@@ -394,6 +395,17 @@ class Context:
     # ---- launches and film
     def launch(self, width, height, frame0=0, spp=1, stream=None):
         call("orf_launch", self.handle, width, height, frame0, spp, stream_ptr(stream))
+
+    def launch_noalloc(self, width, height, frame0, spp, film, counters=None, stream=None):
+        """orf_launch into a caller-owned device film (float64 tensor of
+        height*width*4 sums) and optional device counters (uint64/int64 tensor
+        of 4); the call allocates no device memory."""
+        assert film.is_cuda and film.dtype == torch.float64 and film.numel() == 4 * width * height
+        assert film.is_contiguous()
+        if counters is not None:
+            assert counters.is_cuda and counters.element_size() == 8 and counters.numel() == 4
+        call("orf_launch_noalloc", self.handle, width, height, frame0, spp, ptr(film), ptr(counters),
+             stream_ptr(stream))
 
     def film_clear(self):
         call("orf_film_clear", self.handle)
