@@ -71,6 +71,14 @@ orf_status orf_gen_bsdf_queries(int64_t n, uint64_t seed, uint64_t seq,
                                 float* wo_x, float* wo_y, float* wo_z, float* alpha,
                                 float* u1, float* u2);
 
+/* Inspection of the translucent programs' 64-bit random stream, evaluated on
+ * the host by the same code the kernels run: n draws of rnd_accurate
+ * (ORF/random.h:134-142, MD5 of ORF/md5.h) from the state (x, y), with the
+ * state after each draw (2 words per draw); and the value step alone,
+ * (float)((double)l / 2^63) for l < 2^63.  No counterpart in the reference. */
+orf_status orf_rng_accurate_stream(uint32_t x, uint32_t y, int n, uint32_t* states, float* values);
+orf_status orf_rng_unit63(const uint64_t* l, int n, float* out);
+
 /* ------------------------------------------------------------------------
  * Render context: scene objects, named variables, launches, film.
  *

@@ -31,7 +31,7 @@ __global__ __launch_bounds__(kDepthBlock) void k_depth(SceneGPU sc, CameraGPU ca
         f3 dir;
         uint32_t rng = camera_ray(cam, w, h, x, y, frame0 + s, dir);
         PathState ps;
-        path_begin(ps, cam.eye, dir, rng);
+        path_begin(ps, cam.eye, dir, rng, seed64(0u));
         Hit hit;
         f3 n = mk3(0.0f);
         float t = 0.0f;

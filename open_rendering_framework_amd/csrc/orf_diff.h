@@ -117,10 +117,11 @@ ORF_HD bool diff_vertex(const SceneGPU& sc, const DiffParams& dp, PathState& ps,
 // the same code was right.
 
 // Pass 1: the sample's radiance (the plain estimator with textured parameters).
-ORF_HD f3 diff_path(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32_t rng, unsigned* stk, int stride)
+ORF_HD f3 diff_path(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32_t rng, Seed64 s64, unsigned* stk,
+                    int stride)
 {
     PathState ps;
-    path_begin(ps, o, d, rng);
+    path_begin(ps, o, d, rng, s64);
     for (;;) {
         Hit h;
         if (!trace<false, false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk, stride)) {
@@ -143,11 +144,11 @@ ORF_HD f3 diff_path(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32
 // gradient terms to `sink` (albedo(texel, channel, value), rough(texel, value));
 // delta = dLoss/dL of this sample, L_total its pass-1 radiance.
 template <class Sink>
-ORF_HD void diff_replay(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32_t rng, f3 L_total, f3 delta,
-                        Sink& sink, unsigned* stk, int stride)
+ORF_HD void diff_replay(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32_t rng, Seed64 s64, f3 L_total,
+                        f3 delta, Sink& sink, unsigned* stk, int stride)
 {
     PathState ps;
-    path_begin(ps, o, d, rng);
+    path_begin(ps, o, d, rng, s64);
     for (;;) {
         Hit h;
         if (!trace<false, false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk, stride)) break;

@@ -52,8 +52,9 @@ __global__ __launch_bounds__(kDiffBlock) void k_diff(SceneGPU sc, DiffParams dp,
         unsigned cnt = 0;
         for (unsigned s = 0; s < R.spp; ++s) {
             f3 dir;
-            uint32_t rng = camera_ray(R.cam, R.width, R.height, x, y, R.frame0 + s, dir);
-            f3 L = diff_path(sc, dp, R.cam.eye, dir, rng, stk, kDiffBlock);
+            uint32_t s0;
+            uint32_t rng = camera_ray(R.cam, R.width, R.height, x, y, R.frame0 + s, dir, &s0);
+            f3 L = diff_path(sc, dp, R.cam.eye, dir, rng, seed64(s0), stk, kDiffBlock);
             if (finite3(L)) {
                 acc += L;
                 cnt++;
@@ -72,13 +73,14 @@ __global__ __launch_bounds__(kDiffBlock) void k_diff(SceneGPU sc, DiffParams dp,
         if (delta.x == 0.0f && delta.y == 0.0f && delta.z == 0.0f) return;
         for (unsigned s = 0; s < R.spp; ++s) {
             f3 dir;
-            uint32_t rng = camera_ray(R.cam, R.width, R.height, x, y, R.frame0 + frame_shift + s, dir);
-            f3 L = diff_path(sc, dp, R.cam.eye, dir, rng, stk, kDiffBlock);
+            uint32_t s0;
+            uint32_t rng = camera_ray(R.cam, R.width, R.height, x, y, R.frame0 + frame_shift + s, dir, &s0);
+            f3 L = diff_path(sc, dp, R.cam.eye, dir, rng, seed64(s0), stk, kDiffBlock);
             if (!finite3(L)) {
                 bad++;
                 continue;
             }
-            diff_replay(sc, dp, R.cam.eye, dir, rng, L, delta, sink, stk, kDiffBlock);
+            diff_replay(sc, dp, R.cam.eye, dir, rng, seed64(s0), L, delta, sink, stk, kDiffBlock);
         }
     }
     if (bad) atomicAdd(counters, (unsigned long long)bad);

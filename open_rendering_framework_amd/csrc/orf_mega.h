@@ -60,8 +60,9 @@ __device__ __forceinline__ void mega_pixel(const SceneGPU& sc, const CameraGPU& 
         if (!live) {
             if (k >= k1) break;
             f3 dir;
-            uint32_t rng = camera_ray(cam, w, h, x, y, frame0 + k, dir);
-            path_begin(ps, cam.eye, dir, rng);
+            uint32_t s0;
+            uint32_t rng = camera_ray(cam, w, h, x, y, frame0 + k, dir, &s0);
+            path_begin(ps, cam.eye, dir, rng, seed64(s0));
             live = true;
         }
         Hit hit;

@@ -31,6 +31,7 @@ struct PathState {
     f3 thr;              // throughput
     f3 L;                // radiance gathered so far
     uint32_t rng;        // TEA state (prd.seed)
+    Seed64 s64;          // MD5 state of the translucent programs (prd.seed64)
     int depth;           // prd.depth
     int emit;            // prd.emit_light
     float prev_bsdf_pdf; // > 0: the ray left a Lambert vertex that sampled lights (env MIS)
@@ -52,10 +53,14 @@ ORF_HD void shadow_unoccluded(PathState& ps, const ShadowRay& sr)
 }
 
 // Camera ray of pixel (x, y) of a w x h film for sample `frame`
-// (CU/path_tracer.cu:45-55); returns the seeded TEA state.
-ORF_HD uint32_t camera_ray(const CameraGPU& cam, int w, int h, int x, int y, unsigned frame, f3& dir)
+// (CU/path_tracer.cu:45-55); returns the seeded TEA state.  `seed0`, when
+// given, receives the pixel seed before the jitter draws (both words of the
+// path's 64-bit state, :49).
+ORF_HD uint32_t camera_ray(const CameraGPU& cam, int w, int h, int x, int y, unsigned frame, f3& dir,
+                           uint32_t* seed0 = nullptr)
 {
     uint32_t rng = tea<16>((unsigned)w * (unsigned)y + (unsigned)x, frame);
+    if (seed0) *seed0 = rng;
     float jx = rnd_tea(rng);
     float jy = rnd_tea(rng);
     float ipx = ((float)x + jx) / (float)w * 2.0f - 1.0f;
@@ -64,13 +69,14 @@ ORF_HD uint32_t camera_ray(const CameraGPU& cam, int w, int h, int x, int y, uns
     return rng;
 }
 
-ORF_HD void path_begin(PathState& ps, f3 eye, f3 dir, uint32_t rng)
+ORF_HD void path_begin(PathState& ps, f3 eye, f3 dir, uint32_t rng, Seed64 s64)
 {
     ps.o = eye;
     ps.d = dir;
     ps.thr = mk3(1.0f);
     ps.L = mk3(0.0f);
     ps.rng = rng;
+    ps.s64 = s64;
     ps.depth = 0;
     ps.emit = 1;
     ps.prev_bsdf_pdf = -1.0f;
@@ -459,10 +465,11 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         // the diffusion term (sss_gather, before the path continues); inside, Beer
         // extinction with Russian roulette first.
         if (ps.depth > sc.max_depth) return false;
+        // Every draw here is from the 64-bit stream (RND_64, :8).
         const ScatteringGPU& P = sc.sss[mat.sss];
         f3 no = normal, wo = -d;
         float recip = 1.0f / P.relative_ior;
-        float reflect_xi = rnd_tea(ps.rng);
+        float reflect_xi = rnd_accurate(ps.s64);
         float co = dot(wo, no);
         bool inside = co < 0.0f;
         f3 beam = mk3(1.0f);
@@ -470,7 +477,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
             f3 ext = P.dipole_model == 1 ? P.deltaEddExtinction : P.extinction;
             beam = exp3((-h.t) * ext);
             float prob = (beam.x + beam.y + beam.z) / 3.0f;
-            if (rnd_tea(ps.rng) >= prob) return false;
+            if (rnd_accurate(ps.s64) >= prob) return false;
             beam = beam / prob;
             recip = P.relative_ior;
             no = -no;
@@ -486,7 +493,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         f3 nd;
         if (reflect_xi >= R) {
             if (!inside && mat.sss == sc.probes.sss)
-                ps.L += ps.thr * sss_gather(sc, P, mat.sss, p, no, ps.rng, stk, stride);
+                ps.L += ps.thr * sss_gather(sc, P, mat.sss, p, no, ps.s64, stk, stride);
             nd = recip * (co * no - wo) - no * ct;
         } else {
             nd = 2.0f * co * no - wo;
@@ -511,9 +518,13 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         const f3 w_i = -d;
         const float ax = mat.roughness.x, ay = mat.roughness.y;
         const int dist = mat.normal_distribution, model = mat.microfacet_model;
+        // reflect_xi, the facet's z1 and z2, the light pick, the extinction
+        // roulette and the gather draw from the 64-bit stream (RND_64, :8,
+        // :81-84, :108-109, :186-187); the light sample and the multiple-
+        // scattering walk and evaluation from the TEA stream
         float recip = 1.0f / P.relative_ior;
-        const float reflect_xi = rnd_tea(ps.rng);
-        const float z1 = rnd_tea(ps.rng), z2 = rnd_tea(ps.rng);
+        const float reflect_xi = rnd_accurate(ps.s64);
+        const float z1 = rnd_accurate(ps.s64), z2 = rnd_accurate(ps.s64);
         // the reference passes (a_x, z1, z2) to microfacet_sample_normal's
         // (z1, z2, alpha) parameters (:96); kept as written
         f3 mn = model == WALTER_MODEL ? sample_normal_walter(ffn, ax, z1, z2, dist)
@@ -522,7 +533,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         const bool inside = dot(w_i, normal) < 0.0f;
         // NEE over the reference's lights (the env, a light here, is not one there)
         const int nreal = sc.nlights - (sc.env.sampled ? 1 : 0);
-        unsigned li = (unsigned)((float)nreal * rnd_tea(ps.rng));
+        unsigned li = (unsigned)((float)nreal * rnd_accurate(ps.s64));
         bool nee_reflect = false;
         if (nreal > 0) {
             if (li >= (unsigned)nreal) li = nreal - 1;
@@ -573,7 +584,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
             f3 ext = P.dipole_model == 1 ? P.deltaEddExtinction : P.extinction;
             beam = exp3((-h.t) * ext);
             float prob = (beam.x + beam.y + beam.z) / 3.0f;
-            if (rnd_tea(ps.rng) >= prob) return false;
+            if (rnd_accurate(ps.s64) >= prob) return false;
             beam = beam / prob;
             recip = P.relative_ior;
         }
@@ -600,7 +611,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         ps.thr = ps.thr * beam;
         if (refract) {
             if (!inside && mat.sss == sc.probes.sss)
-                ps.L += ps.thr * weight * sss_gather(sc, P, mat.sss, p, no, ps.rng, stk, stride);
+                ps.L += ps.thr * weight * sss_gather(sc, P, mat.sss, p, no, ps.s64, stk, stride);
             ps.emit = 1;
         } else if (model == MULTISCATTERING_MODEL) {
             ps.emit = 1;
@@ -619,11 +630,11 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
 
 // The whole path of one camera sample (megakernel form).
 template <unsigned kFeat = kFeatAll>
-ORF_HD f3 path_radiance(const SceneGPU& sc, const int* remap, f3 o, f3 d, uint32_t rng, unsigned* stk, int stride,
-                        PathStats& st)
+ORF_HD f3 path_radiance(const SceneGPU& sc, const int* remap, f3 o, f3 d, uint32_t rng, Seed64 s64, unsigned* stk,
+                        int stride, PathStats& st)
 {
     PathState ps;
-    path_begin(ps, o, d, rng);
+    path_begin(ps, o, d, rng, s64);
     for (;;) {
         Hit h;
         st.ext++;

@@ -7,8 +7,14 @@
 // bounced (direct light is the other strategy's); Le is divided by 1/2.  The
 // probe keeps Le, the refracted incident direction w12 and T12 = 1 - R (times
 // the microfacet boundary's transmission weight for a rough translucent
-// material, whose probe normal is then a sampled microfacet normal).  TEA
-// streams tea<16>(probe, frame) replace the reference's RND_64 (MD5) streams.
+// material, whose probe normal is then a sampled microfacet normal).  Random
+// streams as the reference's: t = tea<16>(probe, frame) and the 64-bit state
+// (t, t) (:54-57); the three 64-bit draws that place the reference's probe
+// (triangle pick and barycentrics, :58,77-78) are drawn and unused, the probe
+// positions being fixed here, so the later 64-bit draws sit where the
+// reference's do; the strategy choice is the LCG draw rnd(t) (:116), the light
+// pick and the facet draws are 64-bit (:122, :135-136, :178-179), the light
+// sample and the cosine direction TEA (:124, :161).
 #pragma once
 
 #include "orf_path.h"
@@ -18,10 +24,10 @@ namespace orf {
 // Microfacet normal of a rough boundary for a probe lit from wi (as
 // CU/sample_camera.cu:130-151 and :173-191 draw it; the Walter branch keeps the
 // reference's (a_x, z1, z2) argument order).
-ORF_HD f3 probe_facet(const MaterialGPU& M, f3 wi, f3 n, uint32_t& t)
+ORF_HD f3 probe_facet(const MaterialGPU& M, f3 wi, f3 n, Seed64& t64)
 {
-    float z1 = rnd_tea(t);
-    float z2 = rnd_tea(t);
+    float z1 = rnd_accurate(t64);
+    float z2 = rnd_accurate(t64);
     if (M.microfacet_model == WALTER_MODEL) return sample_normal_walter(n, M.roughness.x, z1, z2, M.normal_distribution);
     return sample_visible_normal(wi, n, M.roughness.x, M.roughness.y, z1, z2, M.normal_distribution);
 }
@@ -36,10 +42,12 @@ ORF_HD void probe_sample(const SceneGPU& sc, const ScatteringGPU& P, const Mater
     float4 pp = pr.pos[j], nn = pr.nrm[j];
     f3 pos = mk3(pp.x, pp.y, pp.z), n = mk3(nn.x, nn.y, nn.z);
     uint32_t t = tea<16>((unsigned)j, frame);
+    Seed64 t64 = seed64(t);
+    for (int k = 0; k < 3; ++k) rnd_accurate(t64);  // the reference's probe placement
     float recip = 1.0f / P.relative_ior;
     f3 Le = mk3(0.0f), wi = n, sn = n;
-    if (rnd_tea(t) < 0.5f) {
-        unsigned li = (unsigned)((float)sc.nlights * rnd_tea(t));
+    if (rnd_lcg(t) < 0.5f) {
+        unsigned li = (unsigned)((float)sc.nlights * rnd_accurate(t64));
         if (sc.nlights > 0) {
             if (li >= (unsigned)sc.nlights) li = sc.nlights - 1;
             float light_pdf = 1.0f / (float)sc.nlights;
@@ -47,20 +55,20 @@ ORF_HD void probe_sample(const SceneGPU& sc, const ScatteringGPU& P, const Mater
             float dist = 0.0f, penv;
             sample_light(sc, sc.lights[li], pos, t, wl, rad, dist, penv);
             wi = wl;
-            if (rough) sn = probe_facet(M, wi, n, t);
+            if (rough) sn = probe_facet(M, wi, n, t64);
             float cs = fmaxf(dot(wl, n), 0.0f);
             if (cs > 0.0f) {
                 Hit hs;
                 if (!trace<true>(sc, pos, wl, sc.scene_epsilon, dist, hs, stk, stride)) Le = rad * cs / light_pdf;
             }
         } else if (rough) {
-            sn = probe_facet(M, wi, n, t);
+            sn = probe_facet(M, wi, n, t64);
         }
     } else {
         wi = sample_cosine(n, t);
-        if (rough) sn = probe_facet(M, wi, n, t);
+        if (rough) sn = probe_facet(M, wi, n, t64);
         PathState ps;
-        path_begin(ps, pos, wi, t);
+        path_begin(ps, pos, wi, t, t64);
         ps.emit = 0;  // an emitter seen directly is the light-source strategy's
         for (bool first = true;; first = false) {
             Hit h;

@@ -5,6 +5,7 @@
 
 #include "orf_common.h"
 #include "orf_render.h"
+#include "orf_rng.h"
 
 namespace orf {
 
@@ -80,6 +81,30 @@ extern "C" orf_status orf_gen_bsdf_queries(int64_t n, uint64_t seed, uint64_t se
         alpha[i] = 0.05f + 0.95f * u[2];
         u1[i] = u[3];
         u2[i] = u[4];
+    }
+    ORF_API_END
+}
+
+extern "C" orf_status orf_rng_accurate_stream(uint32_t x, uint32_t y, int n, uint32_t* states, float* values)
+{
+    ORF_API_BEGIN
+    ORF_REQUIRE(n >= 0 && (n == 0 || (states && values)), "orf_rng_accurate_stream: bad arguments");
+    orf::Seed64 s{x, y};
+    for (int i = 0; i < n; ++i) {
+        values[i] = orf::rnd_accurate(s);
+        states[2 * i] = s.x;
+        states[2 * i + 1] = s.y;
+    }
+    ORF_API_END
+}
+
+extern "C" orf_status orf_rng_unit63(const uint64_t* l, int n, float* out)
+{
+    ORF_API_BEGIN
+    ORF_REQUIRE(n >= 0 && (n == 0 || (l && out)), "orf_rng_unit63: bad arguments");
+    for (int i = 0; i < n; ++i) {
+        ORF_REQUIRE(l[i] < (1ull << 63), "orf_rng_unit63: value of 64 bits");
+        out[i] = orf::unit63(l[i]);
     }
     ORF_API_END
 }

@@ -143,10 +143,12 @@ ORF_HD int probe_ray_hits(const SceneGPU& sc, int sss, f3 o, f3 dir, float tmax,
     return n;
 }
 
-// Diffusion term entering at xo (normal no facing the incident side).  The probe
-// rays use the caller's traversal stack (stk, stride) -- free between traces --
-// or a private one when stk is null.
-ORF_HD f3 sss_gather(const SceneGPU& sc, const ScatteringGPU& P, int sss, f3 xo, f3 no, uint32_t& rng,
+// Diffusion term entering at xo (normal no facing the incident side).  Its random
+// numbers come from the path's 64-bit stream, as the reference's gather loop
+// draws its roulette numbers (CU/subsurface_scattering_shader.cu:152-156).  The
+// probe rays use the caller's traversal stack (stk, stride) -- free between
+// traces -- or a private one when stk is null.
+ORF_HD f3 sss_gather(const SceneGPU& sc, const ScatteringGPU& P, int sss, f3 xo, f3 no, Seed64& rng,
                      unsigned* stk = nullptr, int stride = 1)
 {
     const ProbesGPU& pr = sc.probes;
@@ -162,7 +164,7 @@ ORF_HD f3 sss_gather(const SceneGPU& sc, const ScatteringGPU& P, int sss, f3 xo,
     float smin = fminf(P.transport.x, fminf(P.transport.y, P.transport.z));
     float r_max = kSssRadiusSigmas / smin;
     for (int k = 0; k < pr.samples; ++k) {
-        float u = rnd_tea(rng), v = rnd_tea(rng), w = rnd_tea(rng), s = rnd_tea(rng);
+        const float u = rnd_accurate(rng), v = rnd_accurate(rng), w = rnd_accurate(rng), s = rnd_accurate(rng);
         int c = min(2, (int)(3.0f * u));
         float sig = c == 0 ? P.transport.x : (c == 1 ? P.transport.y : P.transport.z);
         float r = -orf_logf(1.0f - v) / sig;

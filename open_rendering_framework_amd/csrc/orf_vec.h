@@ -64,6 +64,7 @@ ORF_HD uint32_t f2u(float f)
     c.f = f;
     return c.u;
 }
+ORF_HD float u2f(uint32_t u) { return orf_u2f(u); }
 
 constexpr float kPi = 3.14159265358979323846f;
 constexpr float kInvPi = 0.318309886183790671538f;

@@ -45,9 +45,9 @@ struct WfView {
     int sort_policy;        // 0: one shade queue, 1: per-material queues
     float regen_threshold;  // refill when continuing paths < threshold * capacity
     // path pool (SoA, `capacity` entries)
-    float4* o;        // origin xyz
+    float4* o;        // origin xyz, w = bits of the 64-bit random state's first word
     float4* d;        // direction xyz, w = prev_bsdf_pdf
-    float4* thr;      // throughput xyz
+    float4* thr;      // throughput xyz, w = bits of the 64-bit random state's second word
     float4* L;        // radiance xyz
     uint4* st;        // rng, depth | emit << 31, render, pixel
     float4* hit;      // t, b, g, prim bits
@@ -96,15 +96,16 @@ ORF_HD void load_path(const WfView& v, int slot, PathState& ps)
     ps.thr = mk3(t.x, t.y, t.z);
     ps.L = mk3(l.x, l.y, l.z);
     ps.rng = s.x;
+    ps.s64 = Seed64{f2u(o.w), f2u(t.w)};
     ps.depth = (int)(s.y & 0x7FFFFFFFu);
     ps.emit = (int)(s.y >> 31);
 }
 
 ORF_HD void store_path(const WfView& v, int slot, const PathState& ps)
 {
-    v.o[slot] = make_float4(ps.o.x, ps.o.y, ps.o.z, 0.0f);
+    v.o[slot] = make_float4(ps.o.x, ps.o.y, ps.o.z, u2f(ps.s64.x));
     v.d[slot] = make_float4(ps.d.x, ps.d.y, ps.d.z, ps.prev_bsdf_pdf);
-    v.thr[slot] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, 0.0f);
+    v.thr[slot] = make_float4(ps.thr.x, ps.thr.y, ps.thr.z, u2f(ps.s64.y));
     v.L[slot] = make_float4(ps.L.x, ps.L.y, ps.L.z, 0.0f);
     uint4 s = v.st[slot];
     s.x = ps.rng;
@@ -162,9 +163,10 @@ ORF_HD void start_path(const WfView& v, int slot, long long id)
     int pix = (int)(local - (long long)s * npix);
     int x = pix % R.width, y = pix / R.width;
     f3 dir;
-    uint32_t rng = camera_ray(R.cam, R.width, R.height, x, y, R.frame0 + s, dir);
+    uint32_t s0;
+    uint32_t rng = camera_ray(R.cam, R.width, R.height, x, y, R.frame0 + s, dir, &s0);
     PathState ps;
-    path_begin(ps, R.cam.eye, dir, rng);
+    path_begin(ps, R.cam.eye, dir, rng, seed64(s0));
     uint4 st;
     st.x = 0;
     st.y = 0;

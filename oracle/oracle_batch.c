@@ -20,6 +20,18 @@ void orc_tea_stream(uint32_t seed, int n, uint32_t *out_state, float *out_u)
     }
 }
 
+/* n draws of rnd_accurate from the state (x, y): states after each draw (2 words) and values. */
+void orc_accurate_stream(uint32_t x, uint32_t y, int n, uint32_t *out_state, float *out_u)
+{
+    uint32_t st[2] = {x, y};
+    for (int i = 0; i < n; i++) {
+        R u = orc_rnd_accurate(st);
+        out_state[2 * i] = st[0];
+        out_state[2 * i + 1] = st[1];
+        out_u[i] = (float)u;
+    }
+}
+
 void orc_tea_batch(const uint32_t *v0, const uint32_t *v1, int n, uint32_t rounds,
                    const uint32_t *key, uint32_t *out)
 {

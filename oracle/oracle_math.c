@@ -56,6 +56,73 @@ uint32_t orc_lcg(uint32_t *prev)
 
 R orc_rnd(uint32_t *prev) { return (R)((float)orc_lcg(prev) / (float)0x01000000); }
 
+/* __md5Hash (ORF/md5.h:58-225): MD5 (RFC 1321) of a message of at most 55
+ * bytes, so one padded 512-bit block: the message bytes packed little-endian
+ * into words, 0x80 after them, the bit length in word 14.  The additive
+ * constants are formed from their definition, floor(|sin(i + 1)| 2^32), and the
+ * four rounds use F, G, H, I with the RFC's shifts and message-word orders. */
+void orc_md5(const uint8_t* msg, uint32_t len, uint32_t out[4])
+{
+    static uint32_t K[64];
+    static int have_k = 0;
+    if (!have_k) {
+        for (int i = 0; i < 64; i++) K[i] = (uint32_t)floor(fabs(sin((double)(i + 1))) * 4294967296.0);
+        have_k = 1;
+    }
+    static const int SH[4][4] = {{7, 12, 17, 22}, {5, 9, 14, 20}, {4, 11, 16, 23}, {6, 10, 15, 21}};
+    uint32_t w[16];
+    memset(w, 0, sizeof(w));
+    uint32_t i;
+    for (i = 0; i < len && i < 55; i++) w[i / 4] |= (uint32_t)msg[i] << ((i % 4) * 8);
+    w[i / 4] |= 0x80u << ((i % 4) * 8);
+    w[14] = len * 8u;
+    const uint32_t init[4] = {0x67452301u, 0xEFCDAB89u, 0x98BADCFEu, 0x10325476u};
+    uint32_t a = init[0], b = init[1], c = init[2], d = init[3];
+    for (int r = 0; r < 4; r++) {
+        for (int k = 0; k < 16; k++) {
+            int step = 16 * r + k, idx;
+            uint32_t f;
+            switch (r) {
+            case 0: f = (b & c) | (~b & d); idx = k; break;
+            case 1: f = (b & d) | (c & ~d); idx = (1 + 5 * k) % 16; break;
+            case 2: f = b ^ c ^ d; idx = (5 + 3 * k) % 16; break;
+            default: f = c ^ (b | ~d); idx = (7 * k) % 16; break;
+            }
+            uint32_t v = a + f + w[idx] + K[step];
+            int sh = SH[r][k % 4];
+            v = (v << sh) | (v >> (32 - sh));
+            a = d;
+            d = c;
+            c = b;
+            b = b + v;
+        }
+    }
+    out[0] = a + init[0];
+    out[1] = b + init[1];
+    out[2] = c + init[2];
+    out[3] = d + init[3];
+}
+
+/* rnd_accurate (ORF/random.h:128-142): the state (two words, Seed64's uint2)
+ * hashed as 8 bytes (rand_md5 with scrambler 0, ORF/md5.h:242-249), the state
+ * replaced by the first two digest words, the value (low 63 bits of the 64-bit
+ * state) / 2^63 in double, then rounded to float. */
+R orc_rnd_accurate(uint32_t s[2])
+{
+    uint8_t m[8];
+    for (int k = 0; k < 4; k++) {
+        m[k] = (uint8_t)(s[0] >> (8 * k));
+        m[4 + k] = (uint8_t)(s[1] >> (8 * k));
+    }
+    uint32_t h[4];
+    orc_md5(m, 8, h);
+    s[0] = h[0];
+    s[1] = h[1];
+    uint64_t l = ((uint64_t)h[1] << 32 | (uint64_t)h[0]) & 0x7FFFFFFFFFFFFFFFull;
+    double val = (double)l / 9223372036854775808.0;
+    return (R)(float)val;
+}
+
 /* PCG32, pcg-c-basic (O'Neill 2014): published algorithm, used only to
  * generate the synthetic operator queries. */
 uint32_t orc_pcg32_next(orc_pcg32 *r)

@@ -1393,7 +1393,14 @@ static int probe_collect(const oscene* s, v3 o, v3 d, R tmax, int m, ohit* out)
     return n;
 }
 
-static v3 sss_gather(const oscene* s, const osss* P, v3 xo, v3 no, uint32_t* t)
+/* The 64-bit random state of the path whose TEA state `t` points to: every
+ * path's state is an orc_rng and the shaders receive a pointer to its first
+ * member (a pointer to a struct's first member converts back to the struct). */
+static uint32_t* path_s64(uint32_t* t) { return ((orc_rng*)t)->s64; }
+
+/* The gather's draws come from the 64-bit stream, as the reference's gather
+ * loop draws its roulette numbers (CU/subsurface_scattering_shader.cu:152-156). */
+static v3 sss_gather(const oscene* s, const osss* P, v3 xo, v3 no, uint32_t* s64)
 {
     v3 acc = V3(0, 0, 0);
     if (s->nprobes <= 0 || s->probe_samples <= 0) return acc;
@@ -1402,7 +1409,7 @@ static v3 sss_gather(const oscene* s, const osss* P, v3 xo, v3 no, uint32_t* t)
     R smin = fmin(P->transport.x, fmin(P->transport.y, P->transport.z));
     R r_max = SSS_RADIUS_SIGMAS / smin;
     for (int k = 0; k < s->probe_samples; k++) {
-        R u = orc_rnd_tea(t), v = orc_rnd_tea(t), w = orc_rnd_tea(t), sx = orc_rnd_tea(t);
+        R u = orc_rnd_accurate(s64), v = orc_rnd_accurate(s64), w = orc_rnd_accurate(s64), sx = orc_rnd_accurate(s64);
         int c = (int)((R)3.0f * u);
         if (c > 2) c = 2;
         R sig = c == 0 ? P->transport.x : (c == 1 ? P->transport.y : P->transport.z);
@@ -1455,7 +1462,8 @@ static v3 shade_translucent(const oscene* s, int mi, v3 o, v3 d, const ohit* h, 
     v3 no = hit_normal(s, h);
     v3 wo = vneg(d);
     R recip = (R)1.0f / P->relative_ior;
-    R reflect_xi = orc_rnd_tea(t);
+    uint32_t* t64 = path_s64(t); /* every draw of this program is 64-bit (RND_64, :8) */
+    R reflect_xi = orc_rnd_accurate(t64);
     R co = vdot(wo, no);
     int inside = co < 0;
     v3 beam = V3(1, 1, 1);
@@ -1463,7 +1471,7 @@ static v3 shade_translucent(const oscene* s, int mi, v3 o, v3 d, const ohit* h, 
         v3 ext = P->dipole_model == 1 ? P->deltaEddExtinction : P->extinction;
         beam = V3(F_EXP((-h->t) * ext.x), F_EXP((-h->t) * ext.y), F_EXP((-h->t) * ext.z));
         R prob = (beam.x + beam.y + beam.z) / (R)3.0f;
-        if (orc_rnd_tea(t) >= prob) return zero;
+        if (orc_rnd_accurate(t64) >= prob) return zero;
         beam = V3(beam.x / prob, beam.y / prob, beam.z / prob);
         recip = P->relative_ior;
         no = vneg(no);
@@ -1477,7 +1485,7 @@ static v3 shade_translucent(const oscene* s, int mi, v3 o, v3 d, const ohit* h, 
     }
     v3 result = zero;
     if (reflect_xi >= Rf) {
-        if (!inside && mi == s->probe_mat) result = sss_gather(s, P, xo, no, t);
+        if (!inside && mi == s->probe_mat) result = sss_gather(s, P, xo, no, t64);
         v3 wt = vsub(vscale(vsub(vscale(no, co), wo), recip), vscale(no, ct));
         result = vadd(result, radiance(s, xo, wt, depth + 1, 1, t, (R)-1, st));
     } else {
@@ -1506,8 +1514,12 @@ static v3 shade_rough_translucent(const oscene* s, int mi, v3 o, v3 d, const ohi
     R ax = M->roughness[0], ay = M->roughness[1];
     int dist = M->normal_distribution, model = M->microfacet_model;
     R recip = (R)1.0f / P->relative_ior;
-    R reflect_xi = orc_rnd_tea(t);
-    R z1 = orc_rnd_tea(t), z2 = orc_rnd_tea(t);
+    /* 64-bit draws (RND_64): reflect_xi, z1, z2 (:81-84), the light pick
+     * (:108-109), the extinction roulette (:186-187) and the gather; TEA: the
+     * light sample and the multiple-scattering walk and evaluation */
+    uint32_t* t64 = path_s64(t);
+    R reflect_xi = orc_rnd_accurate(t64);
+    R z1 = orc_rnd_accurate(t64), z2 = orc_rnd_accurate(t64);
     v3 mn;
     if (model == 0) {
         mn = orc_sample_normal_walter(ffn, ax, z1, z2, dist); /* the reference's argument order (:96) */
@@ -1520,7 +1532,7 @@ static v3 shade_rough_translucent(const oscene* s, int mi, v3 o, v3 d, const ohi
     int inside = vdot(w_i, normal) < 0;
     /* NEE over the reference's lights (the environment entry is not one of them) */
     int nreal = s->nlights - (s->env_is ? 1 : 0);
-    uint32_t li = (uint32_t)((R)nreal * orc_rnd_tea(t));
+    uint32_t li = (uint32_t)((R)nreal * orc_rnd_accurate(t64));
     v3 result = zero;
     int refl_emit = emit_light;
     if (nreal > 0) {
@@ -1574,7 +1586,7 @@ static v3 shade_rough_translucent(const oscene* s, int mi, v3 o, v3 d, const ohi
         v3 ext = P->dipole_model == 1 ? P->deltaEddExtinction : P->extinction;
         beam = V3(F_EXP((-h->t) * ext.x), F_EXP((-h->t) * ext.y), F_EXP((-h->t) * ext.z));
         R prob = (beam.x + beam.y + beam.z) / (R)3.0f;
-        if (orc_rnd_tea(t) >= prob) return result;
+        if (orc_rnd_accurate(t64) >= prob) return result;
         beam = V3(beam.x / prob, beam.y / prob, beam.z / prob);
         recip = P->relative_ior;
     }
@@ -1600,7 +1612,7 @@ static v3 shade_rough_translucent(const oscene* s, int mi, v3 o, v3 d, const ohi
     v3 acc = zero;
     int child_emit = 1;
     if (refract) {
-        if (!inside && mi == s->probe_mat) acc = vscale(sss_gather(s, P, p, no, t), weight);
+        if (!inside && mi == s->probe_mat) acc = vscale(sss_gather(s, P, p, no, t64), weight);
     } else if (model != 2) {
         child_emit = refl_emit;
     }
@@ -1655,13 +1667,16 @@ static v3 shade_hit(const oscene* s, v3 o, v3 d, const ohit* hp, int depth, int 
 /* path_tracer (CU/path_tracer.cu:32-74) for one (pixel, frame). */
 static v3 pixel_sample_cam(const oscene* s, const ocam* c, int W, int H, int x, int y, uint32_t frame, ostats* st)
 {
-    uint32_t seed = orc_tea16((uint32_t)W * (uint32_t)y + (uint32_t)x, frame);
-    R jx = orc_rnd_tea(&seed);
-    R jy = orc_rnd_tea(&seed);
+    /* prd.seed and prd.seed64 (CU/path_tracer.cu:48-49) */
+    orc_rng r;
+    r.tea = orc_tea16((uint32_t)W * (uint32_t)y + (uint32_t)x, frame);
+    r.s64[0] = r.s64[1] = r.tea;
+    R jx = orc_rnd_tea(&r.tea);
+    R jy = orc_rnd_tea(&r.tea);
     R ipx = ((R)x + jx) / (R)W * (R)2.0f - (R)1.0f;
     R ipy = ((R)y + jy) / (R)H * (R)2.0f - (R)1.0f;
     v3 dir = orc_normalize(vadd(vadd(vscale(c->U, ipx), vscale(c->V, ipy)), c->W));
-    return radiance(s, c->eye, dir, 0, 1, &seed, (R)-1, st);
+    return radiance(s, c->eye, dir, 0, 1, &r.tea, (R)-1, st);
 }
 
 static v3 pixel_sample(const oscene* s, int W, int H, int x, int y, uint32_t frame, ostats* st)
@@ -2071,10 +2086,10 @@ static v3 probe_indirect(const oscene* s, v3 o, v3 d, uint32_t* t, ostats* st)
 /* Microfacet normal of a rough boundary for a probe lit from wi
  * (CU/sample_camera.cu:130-151, :173-191; the Walter branch keeps the
  * reference's (a_x, z1, z2) argument order). */
-static v3 probe_facet(const orc_material* M, v3 wi, v3 n, uint32_t* t)
+static v3 probe_facet(const orc_material* M, v3 wi, v3 n, uint32_t* t64)
 {
-    R z1 = orc_rnd_tea(t);
-    R z2 = orc_rnd_tea(t);
+    R z1 = orc_rnd_accurate(t64);
+    R z2 = orc_rnd_accurate(t64);
     R ax = M->roughness[0], ay = M->roughness[1];
     int dist = M->normal_distribution;
     if (M->microfacet_model == 0) return orc_sample_normal_walter(n, ax, z1, z2, dist);
@@ -2083,26 +2098,33 @@ static v3 probe_facet(const orc_material* M, v3 wi, v3 n, uint32_t* t)
     return orc_to_world(m, n);
 }
 
-/* CU/sample_camera.cu:109-268 for one probe (TEA stream tea16(j, frame)). */
+/* CU/sample_camera.cu:109-268 for one probe. */
 static void probe_sample(const oscene* s, const osss* P, const orc_material* M, int j, uint32_t frame, v3* L_out,
                          v3* w12_out, R* T12_out, v3* sn_out)
 {
     v3 pos = s->probe_pos[j], n = s->probe_nrm[j];
     int rough = M->kind == ORC_ROUGH_TRANSLUCENT;
-    uint32_t t = orc_tea16((uint32_t)j, frame);
+    /* t and t64 as CU/sample_camera.cu:54-57 seeds them; the 64-bit draws
+     * that place the reference's probe (:58, :77-78) are drawn and unused */
+    orc_rng r;
+    r.tea = orc_tea16((uint32_t)j, frame);
+    r.s64[0] = r.s64[1] = r.tea;
+    uint32_t* t = &r.tea;
+    uint32_t* t64 = r.s64;
+    for (int k = 0; k < 3; k++) orc_rnd_accurate(t64);
     R recip = (R)1.0f / P->relative_ior;
     v3 Le = V3(0, 0, 0), wi = n, sn = n;
     ostats st = {0, 0};
-    if (orc_rnd_tea(&t) < (R)0.5f) {
-        uint32_t li = (uint32_t)((R)s->nlights * orc_rnd_tea(&t));
+    if (orc_rnd(t) < (R)0.5f) { /* rnd(t), the LCG draw (:116) */
+        uint32_t li = (uint32_t)((R)s->nlights * orc_rnd_accurate(t64));
         if (s->nlights > 0) {
             if (li >= (uint32_t)s->nlights) li = (uint32_t)s->nlights - 1;
             R light_pdf = (R)1 / (R)s->nlights;
             v3 wl, rad;
             R dist, penv;
-            direct_light(s, &s->lights[li], pos, &t, &wl, &rad, &dist, &penv);
+            direct_light(s, &s->lights[li], pos, t, &wl, &rad, &dist, &penv);
             wi = wl;
-            if (rough) sn = probe_facet(M, wi, n, &t);
+            if (rough) sn = probe_facet(M, wi, n, t64);
             R cs = fmax(vdot(wl, n), (R)0);
             if (cs > 0) {
                 ohit sh;
@@ -2112,12 +2134,12 @@ static void probe_sample(const oscene* s, const osss* P, const orc_material* M, 
                 }
             }
         } else if (rough) {
-            sn = probe_facet(M, wi, n, &t);
+            sn = probe_facet(M, wi, n, t64);
         }
     } else {
-        wi = orc_sample_cosine_weighted(n, &t);
-        if (rough) sn = probe_facet(M, wi, n, &t);
-        Le = vscale(probe_indirect(s, pos, wi, &t, &st), PI_R);
+        wi = orc_sample_cosine_weighted(n, t);
+        if (rough) sn = probe_facet(M, wi, n, t64);
+        Le = vscale(probe_indirect(s, pos, wi, t, &st), PI_R);
     }
     Le = V3(Le.x / (R)0.5f, Le.y / (R)0.5f, Le.z / (R)0.5f);
     R ci = fmax(vdot(wi, sn), (R)0);
@@ -2232,8 +2254,9 @@ void orc_sss_gather_eval(void* p, int m, const float* xo, const float* no, int r
     const oscene* s = (const oscene*)p;
     double acc[3] = {0, 0, 0};
     for (int i = 0; i < reps; i++) {
-        uint32_t t = orc_tea16((uint32_t)i, seed);
-        v3 g = sss_gather(s, &s->sss[m], V3(xo[0], xo[1], xo[2]), V3(no[0], no[1], no[2]), &t);
+        uint32_t t64[2];
+        t64[0] = t64[1] = orc_tea16((uint32_t)i, seed);
+        v3 g = sss_gather(s, &s->sss[m], V3(xo[0], xo[1], xo[2]), V3(no[0], no[1], no[2]), t64);
         acc[0] += (double)g.x;
         acc[1] += (double)g.y;
         acc[2] += (double)g.z;

@@ -44,6 +44,17 @@ uint32_t orc_tea16(uint32_t v0, uint32_t v1);
 R orc_rnd_tea(uint32_t *prev);
 uint32_t orc_lcg(uint32_t *prev);
 R orc_rnd(uint32_t *prev);
+/* MD5 (ORF/md5.h) of a message of at most 55 bytes; rnd_accurate (ORF/random.h:134-142)
+ * on a two-word state. */
+void orc_md5(const uint8_t* msg, uint32_t len, uint32_t out[4]);
+R orc_rnd_accurate(uint32_t s[2]);
+/* A path's random state: the TEA state every program draws from (first
+ * member: the shaders receive a pointer to it) and the 64-bit state of the
+ * translucent programs (PerRayData_radiance seed / seed64, ORF/structs.h:20-28). */
+typedef struct {
+    uint32_t tea;
+    uint32_t s64[2];
+} orc_rng;
 /* PCG32 (O'Neill 2014, pcg32_srandom_r / pcg32_random_r), used by the
  * synthetic BSDF-query generator; not in the reference. */
 typedef struct { uint64_t state, inc; } orc_pcg32;

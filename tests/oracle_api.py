@@ -28,6 +28,8 @@ class Oracle:
         L.orc_pcg32_stream.argtypes = [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, c_vp]
         L.orc_tea_stream.argtypes = [ctypes.c_uint32, ctypes.c_int, c_vp, c_vp]
         L.orc_lcg_stream.argtypes = [ctypes.c_uint32, ctypes.c_int, c_vp, c_vp]
+        L.orc_md5.argtypes = [ctypes.c_char_p, ctypes.c_uint32, c_vp]
+        L.orc_accurate_stream.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int, c_vp, c_vp]
         L.orc_gen_bsdf_queries.argtypes = [ctypes.c_int64, ctypes.c_uint64, ctypes.c_uint64] + [c_vp] * 6
         L.orc_bsdf_ggx_conductor_batch.argtypes = (
             [ctypes.c_int64] + [c_vp] * 6 + [c_vp, c_vp] + [c_vp] * 7 + [ctypes.c_int]
@@ -75,6 +77,17 @@ class Oracle:
         st = np.zeros(n, np.uint32)
         u = np.zeros(n, np.float32)
         self.lib.orc_tea_stream(seed, n, _p(st), _p(u))
+        return st, u
+
+    def md5(self, msg):
+        out = np.zeros(4, np.uint32)
+        self.lib.orc_md5(bytes(msg), len(msg), _p(out))
+        return out
+
+    def accurate_stream(self, x, y, n):
+        st = np.zeros((n, 2), np.uint32)
+        u = np.zeros(n, np.float32)
+        self.lib.orc_accurate_stream(x, y, n, _p(st), _p(u))
         return st, u
 
     def lcg_stream(self, seed, n):
