@@ -2235,6 +2235,21 @@ void orc_dipole_eval(void* p, int m, int n, const float* r, float* out)
     }
 }
 
+/* S_infinite of material m for rows of 4 doubles (r^2, x.w12, no.w12, x.no),
+ * the same value for the three channels' inputs; out: 3 doubles per row. */
+void orc_dirpole_S_infinite_eval(void* p, int m, int n, const double* rows, double* out)
+{
+    const oscene* s = (const oscene*)p;
+    for (int i = 0; i < n; i++) {
+        const double* q = rows + 4 * i;
+        R r2 = (R)q[0], xw = (R)q[1], xn = (R)q[3];
+        v3 v = S_infinite(V3(r2, r2, r2), V3(xw, xw, xw), (R)q[2], V3(xn, xn, xn), &s->sss[m]);
+        out[3 * i] = (double)v.x;
+        out[3 * i + 1] = (double)v.y;
+        out[3 * i + 2] = (double)v.z;
+    }
+}
+
 void orc_dirpole_eval(void* p, int m, int n, const float* rows, float* out)
 {
     const oscene* s = (const oscene*)p;

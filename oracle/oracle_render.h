@@ -119,6 +119,9 @@ void orc_sss_coefficients(void* s, int m, float* out);
  * directional at (xi, ni, w12, xo, no) per row of 15 floats.  out: 3 per entry. */
 void orc_dipole_eval(void* s, int m, int n, const float* r, float* out);
 void orc_dirpole_eval(void* s, int m, int n, const float* rows, float* out);
+/* The directional dipole's infinite-medium term S_infinite alone: rows of
+ * (r^2, x.w12, no.w12, x.no); out: 3 per row. */
+void orc_dirpole_S_infinite_eval(void* s, int m, int n, const double* rows, double* out);
 void orc_sss_gather_eval(void* s, int m, const float* xo, const float* no, int reps, uint32_t seed, double* out);
 
 /* Differentiable form (new): textured albedo / roughness with frozen sampling,

@@ -77,6 +77,7 @@ class OracleScene:
         L.orc_sss_coefficients.argtypes = [c_vp, ctypes.c_int, c_vp]
         L.orc_dipole_eval.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, c_vp, c_vp]
         L.orc_dirpole_eval.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, c_vp, c_vp]
+        L.orc_dirpole_S_infinite_eval.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, c_vp, c_vp]
         L.orc_sss_gather_eval.argtypes = [c_vp, ctypes.c_int, c_vp, c_vp, ctypes.c_int, ctypes.c_uint32, c_vp]
         L.orc_diff_set.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_vp, ctypes.c_int,
                                    ctypes.c_int, c_vp, c_vp, c_vp, c_vp, c_vp]
@@ -310,6 +311,12 @@ class OracleScene:
         rows = np.ascontiguousarray(rows, np.float32)
         out = np.zeros((len(rows), 3), np.float32)
         self.lib.orc_dirpole_eval(self.handle, material, len(rows), _p(rows), _p(out))
+        return out
+
+    def dirpole_S_infinite(self, material, rows):
+        rows = np.ascontiguousarray(rows, np.float64)
+        out = np.zeros((len(rows), 3), np.float64)
+        self.lib.orc_dirpole_S_infinite_eval(self.handle, material, len(rows), _p(rows), _p(out))
         return out
 
     def sss_gather(self, material, xo, no, reps, seed=1):

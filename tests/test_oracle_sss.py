@@ -17,8 +17,17 @@ the dipoles of ORF/dipoles/, and the importance-sampled gather):
   one probe-ray crossing); with the entry normal tilted 45 degrees from the
   slab's the probe rays cross the slab obliquely, so a dropped |n_i . n_o|
   factor moves the mean by sqrt(2), far beyond the Monte-Carlo error;
-- the directional dipole is non-negative and rotationally symmetric about the
-  normal at normal incidence (its values are otherwise parity unpinned);
+- the directional dipole's infinite-medium term S_infinite (ORF/dipoles/
+  directional_dipole.h:8-20) against diffusion theory of a ray source (Frisvad
+  et al. 2014): with the point-source fluence phi(r) = exp(-s_tr r) / (D r) and
+  the ray source's fluence Phi = phi - 3D w.grad(phi), S_infinite = C_phi Phi
+  + C_E (-D grad(Phi) . n_o), the derivatives taken numerically in FP64; a
+  dropped term, a flipped sign or a wrong factor in any of its five terms
+  moves the value far beyond the finite-difference error.  Averaged over w and
+  -w it is the isotropic solution 3 C_phi s'_t exp(-s_tr r) / r where x . n_o = 0;
+- the whole directional dipole: non-negative, rotationally symmetric about the
+  normal at normal incidence, and decaying as exp(-s_tr r) in the far field
+  (log-slope within 3/r of -s_tr per channel);
 - a rough boundary (rough_subsurface_scattering_shader.cu, visible normals)
   with a vanishing roughness renders the smooth boundary's image (the NEE
   lobe and the masking weights go to a delta and 1).
@@ -166,6 +175,73 @@ def test_directional_dipole_symmetry_and_sign():
     assert np.all(S >= 0) and np.all(np.isfinite(S))
     np.testing.assert_allclose(S, np.repeat(S[:, :1], 4, axis=1), rtol=1e-4)
     assert np.all(S[:-1].sum(-1) > S[1:].sum(-1))  # decreasing with distance
+
+
+def _ray_source_S(c, x, w, n):
+    """C_phi Phi + C_E (-D grad Phi . n) per channel, numerical derivatives in FP64."""
+    out = np.zeros(3)
+    for ch in range(3):
+        D, s = c["D"][ch], c["transport"][ch]
+
+        def phi(y):
+            r = np.linalg.norm(y)
+            return math.exp(-s * r) / (D * r)
+
+        def grad(f, y, h):
+            return np.array([(f(y + h * e) - f(y - h * e)) / (2 * h) for e in np.eye(3)])
+
+        r = np.linalg.norm(x)
+        h1, h2 = 1e-4 * r, 1e-3 * r
+
+        def Phi(y):
+            return phi(y) - 3.0 * D * np.dot(w, grad(phi, y, h1))
+
+        out[ch] = c["C_phi"][ch] * Phi(x) + c["C_E"][ch] * (-D * np.dot(grad(Phi, x, h2), n))
+    return out
+
+
+def test_directional_dipole_S_infinite_is_the_ray_source_diffusion_solution():
+    orc = OracleScene(_slab_scene(1), fp64=True)
+    c = _fields(orc.sss_coefficients(0))
+    np.testing.assert_allclose(c["D"] * 3.0 * c["reducedExtinction"], 1.0, rtol=1e-6)
+    rng = np.random.default_rng(4)
+    rows, want = [], []
+    for _ in range(24):
+        r = 10 ** rng.uniform(-3.0, -0.5)  # 1e-3 .. 0.3 scene units (0.1 .. 30 mm)
+        x = rng.normal(size=3)
+        x *= r / np.linalg.norm(x)
+        w = rng.normal(size=3)
+        w /= np.linalg.norm(w)
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        rows.append([r * r, x @ w, n @ w, x @ n])
+        want.append(_ray_source_S(c, x, w, n))
+    got = orc.dirpole_S_infinite(0, np.array(rows))
+    want = np.array(want)
+    np.testing.assert_allclose(got, want, rtol=2e-4, atol=1e-9 * np.abs(want).max())
+    # averaged over w and -w, with x . n = 0: the isotropic solution
+    for r in (0.001, 0.01, 0.1):
+        pair = np.array([[r * r, 0.3 * r, 0.2, 0.0], [r * r, -0.3 * r, -0.2, 0.0]])
+        avg = orc.dirpole_S_infinite(0, pair).mean(axis=0)
+        iso = 3.0 * c["C_phi"] * c["reducedExtinction"] * np.exp(-c["transport"] * r) / r
+        np.testing.assert_allclose(avg, iso, rtol=1e-6)
+
+
+def test_directional_dipole_far_field_decay():
+    orc = OracleScene(_slab_scene(1), fp64=True)
+    c = _fields(orc.sss_coefficients(0))
+    n = np.array([0.0, 1.0, 0.0])
+    w12 = np.array([0.3, -0.9, 0.1])
+    w12 /= np.linalg.norm(w12)
+    for ch in range(3):
+        s_tr = c["transport"][ch]
+        for k in (12.0, 24.0):  # r = k / s_tr: deep in the far field, above FP32 underflow
+            r = k / s_tr
+            rows = [np.concatenate([[rr, 0.0, 0.0], n, w12, np.zeros(3), n]) for rr in (r, r * 1.01)]
+            S = orc.dirpole(0, np.array(rows, np.float32)).astype(np.float64)[:, ch]
+            assert np.all(S > 0)
+            slope = (math.log(S[1]) - math.log(S[0])) / (0.01 * r)
+            assert abs(slope + s_tr) < 3.0 / r, (ch, k, slope, s_tr)
 
 
 def test_rough_boundary_tends_to_the_smooth_one():
