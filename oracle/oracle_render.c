@@ -1495,6 +1495,36 @@ static v3 shade_translucent(const oscene* s, int mi, v3 o, v3 d, const ohit* h, 
     return vmul(result, beam);
 }
 
+/* The NEE reflection factor of the rough translucent boundary's Walter and
+ * visible-normal models (CU/rough_subsurface_scattering_shader.cu:146-165):
+ * returns G1(i, h) G1(l, h) D_wi(h) / (4 |i.n| |l.n|) with h the half vector of
+ * i and l, and the Fresnel reflectance at l.h in *Rr. */
+static R rough_nee_reflect(v3 w_i, v3 wl, v3 ffn, R ax, R ay, int dist, R recip, R* Rr)
+{
+    v3 rn = orc_normalize(vadd(w_i, wl));
+    R cr = vdot(rn, wl);
+    *Rr = 1;
+    R sr2 = recip * recip * ((R)1.0f - cr * cr);
+    if (sr2 < (R)1.0f) *Rr = orc_fresnel_R3(cr, sqrt((R)1.0f - sr2), recip);
+    R gi = masking_xy(w_i, rn, ffn, ax, ay, dist);
+    R go = masking_xy(wl, rn, ffn, ax, ay, dist);
+    R D = orc_eval_visible_normal(w_i, rn, ffn, ax, ay, dist);
+    return gi * go * D / ((R)4.0f * fabs(vdot(w_i, ffn) * vdot(wl, ffn)));
+}
+
+/* Batch of rough_nee_reflect in the frame n = +z: 3 floats per direction;
+ * out = Rr * factor per pair. */
+void orc_rough_nee_reflect_batch(int n, const float* wi, const float* wl, float ax, float ay, int dist, float recip,
+                                 double* out)
+{
+    for (int k = 0; k < n; k++) {
+        R Rr;
+        R f = rough_nee_reflect(V3(wi[3 * k], wi[3 * k + 1], wi[3 * k + 2]), V3(wl[3 * k], wl[3 * k + 1], wl[3 * k + 2]),
+                                V3(0, 0, 1), ax, ay, dist, recip, &Rr);
+        out[k] = (double)(Rr * f);
+    }
+}
+
 /* CU/rough_subsurface_scattering_shader.cu:64-455: microfacet boundary (Walter,
  * visible-normal or multiple-scattering model) over the diffusion term.  The
  * NEE BSDF factor is evaluated before the shadow test (the product traces that
@@ -1550,15 +1580,8 @@ static v3 shade_rough_translucent(const oscene* s, int mi, v3 o, v3 d, const ohi
                                                  MS_WALK_CAP);
                 c = vscale(rad, f);
             } else {
-                v3 rn = orc_normalize(vadd(w_i, wl));
-                R cr = vdot(rn, wl);
-                R Rr = 1;
-                R sr2 = recip * recip * ((R)1.0f - cr * cr);
-                if (sr2 < (R)1.0f) Rr = orc_fresnel_R3(cr, sqrt((R)1.0f - sr2), recip);
-                R gi = masking_xy(w_i, rn, ffn, ax, ay, dist);
-                R go = masking_xy(wl, rn, ffn, ax, ay, dist);
-                R D = orc_eval_visible_normal(w_i, rn, ffn, ax, ay, dist);
-                R wr = gi * go * D / ((R)4.0f * fabs(vdot(w_i, ffn) * vdot(wl, ffn)));
+                R Rr;
+                R wr = rough_nee_reflect(w_i, wl, ffn, ax, ay, dist, recip, &Rr);
                 c = vscale(vscale(rad, Rr), wr);
             }
             ohit sh;

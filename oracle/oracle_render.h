@@ -122,6 +122,10 @@ void orc_dirpole_eval(void* s, int m, int n, const float* rows, float* out);
 /* The directional dipole's infinite-medium term S_infinite alone: rows of
  * (r^2, x.w12, no.w12, x.no); out: 3 per row. */
 void orc_dirpole_S_infinite_eval(void* s, int m, int n, const double* rows, double* out);
+/* The rough translucent boundary's NEE reflection factor R(l.h) G1(i) G1(l) D_wi(h) /
+ * (4 |i.n| |l.n|) for direction pairs about n = +z (3 floats each). */
+void orc_rough_nee_reflect_batch(int n, const float* wi, const float* wl, float ax, float ay, int dist, float recip,
+                                 double* out);
 void orc_sss_gather_eval(void* s, int m, const float* xo, const float* no, int reps, uint32_t seed, double* out);
 
 /* Differentiable form (new): textured albedo / roughness with frozen sampling,
