@@ -42,8 +42,21 @@ TRAFFIC_BSDF_2E22 = 218.4e6
 
 
 def _cpu_threads():
-    n = os.cpu_count() or 1
-    return max(1, min(16, n))
+    """Threads for the CPU baseline: the host cores this process may run on,
+    or the machine's CPU share when OMP_NUM_THREADS states it."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        n = os.cpu_count() or 1
+    share = os.environ.get("OMP_NUM_THREADS")
+    if share and share.isdigit() and int(share) > 0:
+        n = min(n, int(share))
+    return max(1, n)
+
+
+def _host_cores():
+    return {"nproc": os.cpu_count() or 0, "affinity": len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity")
+            else None, "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def _cpu_model():
@@ -113,7 +126,7 @@ def cpu_baseline_bsdf(host):
         el = time.perf_counter() - t0
         if el > 10.0:
             break
-    return {"value": done / el / 1e6, "unit": "Mqueries/s", "cores": threads, "kind": "port",
+    return {"value": done / el / 1e6, "unit": "Mqueries/s", "cores": threads, "kind": "port", "host": _host_cores(),
             "sample": "%d x 2^22 operator queries (oracle FP32 restatement, OpenMP %d threads, %s)"
                       % (done >> 22, threads, _cpu_model())}
 
@@ -198,7 +211,7 @@ def cpu_baseline_path(desc):
         el = time.perf_counter() - t0
         if el > 10.0:
             break
-    return {"value": paths / el / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port",
+    return {"value": paths / el / 1e6, "unit": "Mpaths/s", "cores": threads, "kind": "port", "host": _host_cores(),
             "sample": "%d frames of 512x512 at 1 spp of the same scene and seeds (oracle FP32 restatement, "
                       "OpenMP %d threads, %s)" % (frames, threads, _cpu_model())}
 

@@ -22,6 +22,10 @@ static DiffParams diff_params(orf_context_t* c, const orf_diff_desc* d, const fl
     ORF_REQUIRE(d->roughness_material < 0 || (c->mats[d->roughness_material].kind == MAT_CONDUCTOR &&
                                               c->mats[d->roughness_material].normal_distribution == GGX_DISTRIBUTION),
                 "diff: the roughness texture applies to a GGX conductor (metallic_shader) material");
+    // d log f / d alpha of the single-scattering BSDF is the adjoint's roughness
+    // term; the microsurface walk's weight is not that BSDF
+    ORF_REQUIRE(d->roughness_material < 0 || c->mats[d->roughness_material].microfacet_model != MULTISCATTERING_MODEL,
+                "diff: the roughness texture does not apply to the multiple-scattering (microsurface walk) model");
     ORF_REQUIRE(d->albedo_uv == 0 || d->albedo_uv == 1, "diff: albedo_uv must be 0 or 1");
     ORF_REQUIRE(d->roughness_uv == 0 || d->roughness_uv == 1, "diff: roughness_uv must be 0 or 1");
     DiffParams p;

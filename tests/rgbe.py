@@ -1,4 +1,4 @@
-"""Radiance RGBE (.hdr) writer for test inputs, from the format's published
+"""Radiance RGBE (.hdr) writer for test inputs and a reader of such files, from the format's published
 description (Ward, "Real pixels", Graphics Gems II): per pixel an 8-bit
 mantissa per channel and a shared exponent e with value (m + 1/2) 2^(e - 136);
 scanlines flat, or run-length encoded per channel (new-style: 2 2 hi lo header,
@@ -63,3 +63,44 @@ def write(path, rgbe, rle=True, exposure=None):
             body += row.tobytes()
     with open(path, "wb") as f:
         f.write(hdr + bytes(body))
+
+
+def read(path):
+    """(h, w, 4) uint8 RGBE of a '-Y h +X w' file, first row at the top, and its
+    header lines; flat or new-style run-length-encoded scanlines."""
+    data = open(path, "rb").read()
+    pos, header = 0, []
+    while True:
+        end = data.index(b"\n", pos)
+        line = data[pos:end]
+        pos = end + 1
+        if line.startswith(b"-Y") or line.startswith(b"+Y"):
+            break
+        header.append(line.decode("latin-1"))
+    tok = line.split()
+    assert tok[0] == b"-Y" and tok[2] == b"+X", line
+    h, w = int(tok[1]), int(tok[3])
+    out = np.zeros((h, w, 4), np.uint8)
+    buf = np.frombuffer(data, np.uint8)
+    for y in range(h):
+        if 8 <= w <= 0x7FFF and buf[pos] == 2 and buf[pos + 1] == 2 and not (buf[pos + 2] & 0x80):
+            assert (int(buf[pos + 2]) << 8) | int(buf[pos + 3]) == w
+            pos += 4
+            for c in range(4):
+                x = 0
+                while x < w:
+                    n = int(buf[pos])
+                    pos += 1
+                    if n > 128:
+                        out[y, x:x + n - 128, c] = buf[pos]
+                        x += n - 128
+                        pos += 1
+                    else:
+                        out[y, x:x + n, c] = buf[pos:pos + n]
+                        x += n
+                        pos += n
+                assert x == w
+        else:
+            out[y] = buf[pos:pos + 4 * w].reshape(w, 4)
+            pos += 4 * w
+    return out, header

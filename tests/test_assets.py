@@ -9,6 +9,10 @@ descriptions (tests/rgbe.py for Radiance RGBE; OBJ text below):
 - OBJ: v / vn / f in the v, v/t, v//n and v/t/n forms, polygons as fans,
   negative (relative) indices, comments; nindex is dropped when a face has no
   normals; out-of-range indices raise OrfError.
+- The reference's own assets, where its tree is present (this container):
+  data/CedarCity.hdr (1600 x 800, run-length encoded) decodes pixel for pixel as
+  tests/rgbe.py's reader + decoder read it, and data/Plane.obj (one quad, v/vt/vn
+  faces, normals +y) loads as two fan triangles of the unit square at y = 0.
 """
 import numpy as np
 import pytest
@@ -110,3 +114,34 @@ def test_obj_without_normals_and_errors(tmp_path):
     bad.write_text("v 0 0 0\nv 1 0 0\nf 1 2 3\n")
     with pytest.raises(rt.OrfError):
         rt.obj_read(bad)
+
+
+REF_DATA = "/root/reference/data"
+
+
+@pytest.mark.skipif(not __import__("os").path.exists(REF_DATA + "/CedarCity.hdr"), reason="reference tree absent")
+def test_reference_cedarcity_hdr():
+    enc, header = rgbe.read(REF_DATA + "/CedarCity.hdr")
+    assert enc.shape == (800, 1600, 4)
+    assert any(h.startswith("FORMAT=32-bit_rle_rgbe") for h in header)
+    exposure = [float(h.split("=", 1)[1]) for h in header if "EXPOSURE=" in h][-1]
+    got = rt.hdr_read(REF_DATA + "/CedarCity.hdr")
+    assert got.shape == (800, 1600, 4)
+    want = rgbe.decode(enc)[::-1] / np.float32(exposure)
+    np.testing.assert_array_equal(got[..., :3], want)
+    assert np.all(got[..., 3] == 1.0)
+    assert np.isfinite(got).all() and got[..., :3].max() > 1.0  # a real HDR sky
+
+
+@pytest.mark.skipif(not __import__("os").path.exists(REF_DATA + "/Plane.obj"), reason="reference tree absent")
+def test_reference_plane_obj():
+    o = rt.obj_read(REF_DATA + "/Plane.obj")
+    np.testing.assert_array_equal(o["verts"], [[-1, 0, 1], [1, 0, 1], [-1, 0, -1], [1, 0, -1]])
+    # f 1/1/1 2/2/2 4/4/3 3/3/4 as a fan from its first corner
+    np.testing.assert_array_equal(o["tris"], [[0, 1, 3], [0, 3, 2]])
+    np.testing.assert_array_equal(o["normals"], [[0, 1, 0]] * 4)
+    np.testing.assert_array_equal(o["nindex"], [[0, 1, 2], [0, 2, 3]])
+    v = o["verts"].astype(np.float64)
+    for t in o["tris"]:
+        n = np.cross(v[t[1]] - v[t[0]], v[t[2]] - v[t[0]])
+        assert n[1] > 0  # the file's corner order faces +y, the side of its vn normals

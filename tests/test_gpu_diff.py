@@ -168,3 +168,19 @@ def test_torch_function_matches_c_abi():
     torch.testing.assert_close(alb.grad, ga, rtol=1e-4, atol=1e-4 * float(ga.abs().max()))
     torch.testing.assert_close(rough.grad, gr, rtol=1e-4, atol=1e-4 * float(gr.abs().max()))
     R.close()
+
+
+def test_roughness_texture_refuses_the_microsurface_walk():
+    import torch
+
+    from open_rendering_framework_amd import rt
+
+    _say("diff: roughness texture on a multiple-scattering conductor is refused")
+    d = _scene()
+    d.materials[scenes.DIFF_ROUGHNESS_MATERIAL].uints["microfacet_model"] = (2,)
+    ctx = scenes.to_context(d)
+    views = scenes.ring_views(n=1, size=W, spp=1)
+    alb = torch.full((TEX, TEX, 3), 0.5, device="cuda")
+    rough = torch.full((TEX, TEX), 0.3, device="cuda")
+    with pytest.raises(rt.OrfError, match="multiple-scattering"):
+        ctx.diff_render(_desc(rt), views, alb, rough)
