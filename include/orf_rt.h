@@ -93,8 +93,49 @@ orf_status orf_context_create(int device, orf_context* out);
 /* OptixScene::destroyContext (ORF/OptixScene.cpp:68-75). */
 orf_status orf_context_destroy(orf_context ctx);
 
-/* Variable scopes of context["name"] / material["name"]. */
-enum { ORF_SCOPE_CONTEXT = 0, ORF_SCOPE_MATERIAL = 1 };
+/* Variable scopes of context["name"] / material["name"] / geometry["name"] /
+ * geometry_instance["name"].  A mesh is one Geometry inside one
+ * GeometryInstance here, so ORF_SCOPE_GEOMETRY and ORF_SCOPE_INSTANCE both
+ * take a mesh handle and address that instance: a material variable set there
+ * overrides the material's value for that mesh only (OptiX looks instance
+ * variables up before material ones), as the reference sets gi["radiance"] on
+ * light geometry (ORF/Light.cpp:242, 419-422) and gi["scattering_properties"]
+ * on translucent instances (ORF/OptixSceneLoader.cpp:497-501). */
+enum { ORF_SCOPE_CONTEXT = 0, ORF_SCOPE_MATERIAL = 1, ORF_SCOPE_GEOMETRY = 2, ORF_SCOPE_INSTANCE = 3 };
+
+/* ScatteringMaterialProperties (ORF/structs.h:116-144) byte for byte: 65 floats,
+ * 260 bytes.  Set as user data "scattering_properties" on a translucent
+ * material (RoughTranslucentMaterial::loadParameters, ORF/RoughTranslucentMaterial.cpp:
+ * 360-364) or instance; the shading then reads these fields as given, derived
+ * ones included, as the reference's programs read the struct.  ior_real and
+ * ior_imag are carried and unused, as in the reference's programs. */
+typedef struct orf_scattering_material_properties {
+    float relative_ior;
+    float ior_real[3], ior_imag[3];
+    float absorption[3], scattering[3], meancosine[3];
+    float extinction[3], reducedExtinction[3], deltaEddExtinction[3], D[3], transport[3];
+    float C_phi[3], C_phi_inv[3], C_E[3], A[3], albedo[3], reducedAlbedo[3], de[3], three_D[3], two_A_de[3];
+    float global_coeff[3], one_over_three_ext[3];
+    float mean_transport;
+} orf_scattering_material_properties;
+
+/* LightStruct (ORF/structs.h:42-48, 40 bytes) and TriangleLight (:95-102, 92
+ * bytes), the element types of the context buffers "light_buffer" and
+ * "triangle_light_buffer" (ORF/OptixSceneLoader.cpp:297-347). */
+typedef struct orf_light_struct {
+    uint32_t light_type;  /* 0 point, 1 directional, 2 triangles, 3 disk, 4 sphere (ORF/structs.h:146-153) */
+    float emitted_radiance[3];
+    float light_parameters[3];  /* point: position; directional: direction; triangles: count, first, last;
+                                   disk, sphere: position */
+    float padding[3];           /* disk: radius, phi, theta (degrees); sphere: radius */
+} orf_light_struct;
+typedef struct orf_triangle_light {
+    float v0[3], v1[3], v2[3];
+    float n0[3], n1[3], n2[3];
+    uint32_t has_normals;
+    float emission[3];
+    float area;
+} orf_triangle_light;
 
 /* Variable setters: context["name"]->setFloat/setInt/setUint(...) and
  * material["name"]->... with 1-4 components; setUserData(size, ptr) for
@@ -114,6 +155,25 @@ orf_status orf_variable_set_uint(orf_context ctx, int scope, int handle, const c
                                  const uint32_t* v, int n);
 orf_status orf_variable_set_user_data(orf_context ctx, int scope, int handle, const char* name,
                                       const void* data, size_t size);
+/* setMatrix4x4fv(transpose, m) (ORF/OptixSceneLoader.cpp:497-498, 557-558):
+ * row-major 4x4 unless transpose != 0.  Instance / geometry scope:
+ * "transform_matrix" is the mesh's object-to-world transform (as
+ * orf_mesh_set_transform), "inverse_transform_matrix" must be its inverse
+ * (checked to 1e-4, else ORF_ERROR_INVALID_VALUE).  Context scope:
+ * "transform_matrix" and "normal_matrix" of the probe program's current object
+ * are accepted and have no effect (the probe pass takes every translucent
+ * mesh's own transform). */
+orf_status orf_variable_set_matrix4x4(orf_context ctx, int scope, int handle, const char* name, int transpose,
+                                      const float m[16]);
+/* createBuffer(RT_BUFFER_INPUT, RT_FORMAT_USER) + setElementSize + map/copy/unmap
+ * + context[name]->set(buffer) for the light buffers
+ * (ORF/OptixSceneLoader.cpp:297-347): "light_buffer" (element_size 40,
+ * orf_light_struct) replaces the light list, its disk and sphere entries
+ * bringing their emitters as orf_light_add_disk / _sphere do;
+ * "triangle_light_buffer" (element_size 92, orf_triangle_light) replaces the
+ * triangles the triangle entries index.  Either may be set first; a triangle
+ * entry indexing past the triangle buffer is refused at the next launch. */
+orf_status orf_buffer_set(orf_context ctx, const char* name, const void* data, size_t element_size, size_t count);
 
 /* createMaterial + setClosestHitProgram(radiance, createProgramFromPTXFile(file,
  * "closest_hit")) (e.g. ORF/MetallicMaterial.cpp:108-118).  `program` is the
@@ -124,6 +184,18 @@ orf_status orf_variable_set_user_data(orf_context ctx, int scope, int handle, co
  * "transparent_shader.cu", "normal_shader.cu", "flat_shader.cu",
  * "rough_diffuse_shader.cu".  Returns the material handle. */
 orf_status orf_material_create(orf_context ctx, const char* program, int* material);
+/* setClosestHitProgram / setAnyHitProgram(ray_type, createProgramFromPTXFile(file,
+ * entry)) (ORF/MetallicMaterial.cpp:108-118).  ray_type 0 radiance: entry
+ * "closest_hit" of any program orf_material_create accepts selects the
+ * material's shading (its variables are kept); ray_type 1 shadow: entry
+ * "any_hit" -- every reference any-hit is opaque except normal_shader.cu's,
+ * which lets shadow rays through; ray_type 2 depth: "closest_hit" of
+ * "depth_shader.cu" or "rough_depth_shader.cu" (the depth integrator answers
+ * every material with the hit distance).  Other combinations are refused with
+ * ORF_ERROR_NOT_FOUND. */
+enum { ORF_RAY_RADIANCE = 0, ORF_RAY_SHADOW = 1, ORF_RAY_DEPTH = 2 };
+orf_status orf_material_set_program(orf_context ctx, int material, int ray_type, const char* program_file,
+                                    const char* entry);
 
 /* createGeometry + vertex/index/normal buffers + createGeometryInstance
  * (ORF/ObjLoader.cpp:198-410).  verts: 3 floats per vertex; vindex: 3 ints per
@@ -165,6 +237,18 @@ orf_status orf_lights_clear(orf_context ctx);
  * row 0 at the nadir.  importance_sample != 0 builds the FP64 2D CDF on the GPU
  * (spec: CU/env_cameras.cu) and adds the environment to the light list. */
 orf_status orf_envmap_set(orf_context ctx, const float* rgba, int width, int height, int importance_sample);
+/* createTextureSampler + setWrapMode/setFilteringModes + setBuffer(FLOAT4)
+ * (ORF/HDRLoader.cpp:226-289, ORF/Texture.h:18-84): an FP32 RGBA texture
+ * (host array, row 0 first).  Returns the texture handle. */
+enum { ORF_WRAP_REPEAT = 0, ORF_WRAP_CLAMP_TO_EDGE = 1 };
+enum { ORF_FILTER_LINEAR = 0, ORF_FILTER_NEAREST = 1 };
+orf_status orf_texture_create(orf_context ctx, const float* rgba, int width, int height, int wrap_mode,
+                              int filter_mode, int* texture);
+/* context[name]->setTextureSampler(texture): "envmap" (ORF/Background.cpp:57-105)
+ * makes the texture the environment (REPEAT wrap and LINEAR filtering, as
+ * loadHDRTexture creates it; others are refused), importance sampled unless
+ * the context variable "envmap_importance" is 0. */
+orf_status orf_variable_set_texture(orf_context ctx, int scope, int handle, const char* name, int texture);
 /* Read back the FP32 pdf/cdf tables (host arrays: h, h, h*w, h*w). */
 orf_status orf_envmap_tables(orf_context ctx, float* marg_pdf, float* marg_cdf, float* cond_pdf,
                              float* cond_cdf);

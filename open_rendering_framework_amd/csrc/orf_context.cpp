@@ -30,7 +30,7 @@ orf_context_t* ctx_check(orf_context ctx)
     return ctx;
 }
 
-static MaterialGPU default_material(int kind)
+MaterialGPU default_material(int kind)
 {
     MaterialGPU m;
     std::memset(&m, 0, sizeof(m));
@@ -46,7 +46,7 @@ static MaterialGPU default_material(int kind)
     return m;
 }
 
-static int program_kind(const std::string& prog)
+int program_kind(const std::string& prog)
 {
     std::string p = prog;
     if (p.size() > 3 && p.compare(p.size() - 3, 3, ".cu") == 0) p = p.substr(0, p.size() - 3);
@@ -99,6 +99,10 @@ static void set_context_int(orf_context_t* c, const std::string& name, const int
         dst[0] = (unsigned)v[0];
         dst[1] = (unsigned)v[1];
         c->patch_set = true;
+    } else if (name == "envmap_importance") {
+        // importance sampling of an environment bound by orf_variable_set_texture("envmap")
+        ORF_REQUIRE(n == 1, "envmap_importance takes one value");
+        c->envmap_importance = v[0] != 0;
     } else if (name == "probe_count" || name == "probe_samples" || name == "probe_seed") {
         // surface probes of the translucent object (count, gather samples per hit, seed)
         ORF_REQUIRE(n == 1 && v[0] >= 0 && v[0] <= (1 << 24), name + " takes one value in [0, 2^24]");
@@ -111,14 +115,14 @@ static void set_context_int(orf_context_t* c, const std::string& name, const int
     }
 }
 
-static MaterialGPU& material_ref(orf_context_t* c, int handle)
+MaterialGPU& material_ref(orf_context_t* c, int handle)
 {
     ORF_REQUIRE(handle >= 0 && handle < (int)c->mats.size(), "bad material handle");
     c->dirty_mat = true;
     return c->mats[handle];
 }
 
-static void set_material_float(orf_context_t* c, int handle, const std::string& name, const float* v, int n)
+void set_material_float(orf_context_t* c, int handle, const std::string& name, const float* v, int n)
 {
     MaterialGPU& m = material_ref(c, handle);
     if (name == "diffuse_color" || name == "radiance" || name == "glass_absorption" || name == "highlight_color") {
@@ -155,7 +159,7 @@ static void set_material_float(orf_context_t* c, int handle, const std::string& 
     }
 }
 
-static void set_material_int(orf_context_t* c, int handle, const std::string& name, const int64_t* v, int n)
+void set_material_int(orf_context_t* c, int handle, const std::string& name, const int64_t* v, int n)
 {
     MaterialGPU& m = material_ref(c, handle);
     ORF_REQUIRE(n == 1, "material variable '" + name + "' takes one value");
@@ -312,6 +316,14 @@ static void upload_env(orf_context_t* c, hipStream_t stream)
 static void upload_lights(orf_context_t* c)
 {
     std::vector<LightGPU> L = c->lights;
+    for (size_t i = 0; i < L.size(); ++i) {
+        if (L[i].type != TRIANGLES_AREA_LIGHT) continue;
+        // {count, first, last} into triangle_light_buffer (ORF/OptixSceneLoader.cpp:305-317)
+        const float cnt = L[i].params.x, first = L[i].params.y, last = L[i].params.z;
+        ORF_REQUIRE(cnt >= 1.0f && first >= 0.0f && last == first + cnt - 1.0f &&
+                        last < (float)c->tri_lights.size() && cnt == std::floor(cnt) && first == std::floor(first),
+                    "light " + std::to_string(i) + ": its triangles lie outside triangle_light_buffer");
+    }
     if (c->env_w > 0 && c->env_is) {
         LightGPU e;
         std::memset(&e, 0, sizeof(e));
@@ -477,6 +489,7 @@ extern "C" orf_status orf_variable_set_float(orf_context ctx, int scope, int han
     ORF_REQUIRE(name && v && n > 0 && n <= 4, "orf_variable_set_float: bad arguments");
     if (scope == ORF_SCOPE_CONTEXT) orf::set_context_float(c, name, v, n);
     else if (scope == ORF_SCOPE_MATERIAL) orf::set_material_float(c, handle, name, v, n);
+    else if (orf::instance_scope(scope)) orf::set_material_float(c, orf::instance_material(c, handle), name, v, n);
     else ORF_THROW(ORF_ERROR_INVALID_VALUE, "bad variable scope");
     ORF_API_END
 }
@@ -491,6 +504,7 @@ extern "C" orf_status orf_variable_set_int(orf_context ctx, int scope, int handl
     for (int k = 0; k < n; ++k) w[k] = v[k];
     if (scope == ORF_SCOPE_CONTEXT) orf::set_context_int(c, name, w, n);
     else if (scope == ORF_SCOPE_MATERIAL) orf::set_material_int(c, handle, name, w, n);
+    else if (orf::instance_scope(scope)) orf::set_material_int(c, orf::instance_material(c, handle), name, w, n);
     else ORF_THROW(ORF_ERROR_INVALID_VALUE, "bad variable scope");
     ORF_API_END
 }
@@ -505,6 +519,7 @@ extern "C" orf_status orf_variable_set_uint(orf_context ctx, int scope, int hand
     for (int k = 0; k < n; ++k) w[k] = v[k];
     if (scope == ORF_SCOPE_CONTEXT) orf::set_context_int(c, name, w, n);
     else if (scope == ORF_SCOPE_MATERIAL) orf::set_material_int(c, handle, name, w, n);
+    else if (orf::instance_scope(scope)) orf::set_material_int(c, orf::instance_material(c, handle), name, w, n);
     else ORF_THROW(ORF_ERROR_INVALID_VALUE, "bad variable scope");
     ORF_API_END
 }
@@ -515,10 +530,18 @@ extern "C" orf_status orf_variable_set_user_data(orf_context ctx, int scope, int
     ORF_API_BEGIN
     orf_context_t* c = ctx_check(ctx);
     ORF_REQUIRE(name && data, "orf_variable_set_user_data: bad arguments");
-    ORF_REQUIRE(scope == ORF_SCOPE_MATERIAL, "user data is only defined on materials");
+    ORF_REQUIRE(scope == ORF_SCOPE_MATERIAL || orf::instance_scope(scope),
+                "user data is defined on materials and geometry instances");
+    if (orf::instance_scope(scope)) handle = orf::instance_material(c, handle);
     std::string nm = name;
     orf::MaterialGPU& m = orf::material_ref(c, handle);
-    if (nm == "ior") {
+    if (nm == "scattering_properties") {
+        // ScatteringMaterialProperties (ORF/structs.h:116-144), read by the programs as given
+        ORF_REQUIRE(size == sizeof(orf_scattering_material_properties),
+                    "scattering_properties user data must be ScatteringMaterialProperties (260 bytes)");
+        ORF_REQUIRE(orf::is_translucent(m.kind), "scattering_properties: not a translucent material");
+        orf::set_user_scattering(c->mat_sss[handle], *(const orf_scattering_material_properties*)data);
+    } else if (nm == "ior") {
         // MyComplex3: three {double real, double im} (ORF/MyComplex.h:7-16)
         ORF_REQUIRE(size == 6 * sizeof(double), "ior user data must be MyComplex3 (48 bytes)");
         const double* d = (const double*)data;
@@ -670,9 +693,11 @@ extern "C" orf_status orf_light_add_triangles(orf_context ctx, const float* tv, 
     ORF_API_END
 }
 
+namespace orf {
+
 // Emitter material of a disk or sphere light (the arealight material each
 // DiskAreaLight / SphericalLight instance carries, ORF/Light.cpp:476-505, 583-611).
-static int emitter_material(orf_context_t* c, const float radiance[3])
+int emitter_material(orf_context_t* c, const float radiance[3])
 {
     c->mats.push_back(orf::default_material(orf::MAT_EMITTER));
     c->mats.back().radiance = orf::mk3(radiance[0], radiance[1], radiance[2]);
@@ -683,7 +708,7 @@ static int emitter_material(orf_context_t* c, const float radiance[3])
 
 // A disk or sphere light: the light record, its analytic emitter and the
 // emitter's material.
-static void add_analytic_light(orf_context_t* c, unsigned type, const float position[3], float radius, orf::f3 normal,
+void add_analytic_light(orf_context_t* c, unsigned type, const float position[3], float radius, orf::f3 normal,
                                const float radiance[3], int* material)
 {
     const int m = emitter_material(c, radiance);
@@ -703,6 +728,8 @@ static void add_analytic_light(orf_context_t* c, unsigned type, const float posi
     if (material) *material = m;
 }
 
+}  // namespace orf
+
 extern "C" orf_status orf_light_add_disk(orf_context ctx, const float position[3], float radius, float theta,
                                          float phi, const float radiance[3], int* emitter_material)
 {
@@ -713,7 +740,7 @@ extern "C" orf_status orf_light_add_disk(orf_context ctx, const float position[3
     // DiskAreaLight::computeNormal (ORF/Light.cpp:469-474): theta from +y, phi about y, in degrees
     const float t = theta * orf::kPi / 180.0f, p = phi * orf::kPi / 180.0f;
     const orf::f3 n = orf::mk3(orf_sinf(t) * orf_sinf(p), orf_cosf(t), orf_sinf(t) * orf_cosf(p));
-    add_analytic_light(c, orf::DISK_LIGHT, position, radius, n, radiance, emitter_material);
+    orf::add_analytic_light(c, orf::DISK_LIGHT, position, radius, n, radiance, emitter_material);
     ORF_API_END
 }
 
@@ -724,7 +751,7 @@ extern "C" orf_status orf_light_add_sphere(orf_context ctx, const float position
     orf_context_t* c = ctx_check(ctx);
     ORF_REQUIRE(position && radiance, "orf_light_add_sphere: null arguments");
     ORF_REQUIRE(radius > 0.0f && std::isfinite(radius), "orf_light_add_sphere: radius must be positive and finite");
-    add_analytic_light(c, orf::SPHERICAL_LIGHT, position, radius, orf::mk3(0.0f, 0.0f, 0.0f), radiance,
+    orf::add_analytic_light(c, orf::SPHERICAL_LIGHT, position, radius, orf::mk3(0.0f, 0.0f, 0.0f), radiance,
                        emitter_material);
     ORF_API_END
 }

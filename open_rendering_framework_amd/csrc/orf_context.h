@@ -18,6 +18,7 @@ struct MeshRec {
     int material = 0;
     float xf[16];
     bool has_xf = false;
+    bool own_material = false;  // `material` is this mesh's private copy (instance-scope variables)
 };
 
 }  // namespace orf
@@ -73,6 +74,14 @@ struct orf_context_t {
     // ray generation program: 0 path tracer, 1 depth tracer; the depth film
     int integrator = 0;
     orf::DevBuf d_depth;
+    // environment bound through orf_variable_set_texture: importance sampled?
+    bool envmap_importance = true;
+    // textures of orf_texture_create (FP32 RGBA, row 0 first)
+    struct Texture {
+        std::vector<float4> texels;
+        int w = 0, h = 0, wrap = 0, filter = 0;
+    };
+    std::vector<Texture> textures;
     // the scene document last read by orf_scene_load_json (saved back by orf_scene_save)
     std::string scene_doc;
 };
@@ -86,5 +95,19 @@ SceneGPU scene_view(orf_context_t* c);
 void run_probe_pass(orf_context_t* c, unsigned frame, hipStream_t stream);
 void film_ensure(orf_context_t* c, int w, int h);
 void film_clear(orf_context_t* c);
+MaterialGPU default_material(int kind);
+int program_kind(const std::string& prog);
+MaterialGPU& material_ref(orf_context_t* c, int handle);
+void set_material_float(orf_context_t* c, int handle, const std::string& name, const float* v, int n);
+void set_material_int(orf_context_t* c, int handle, const std::string& name, const int64_t* v, int n);
+int emitter_material(orf_context_t* c, const float radiance[3]);
+void add_analytic_light(orf_context_t* c, unsigned type, const float position[3], float radius, f3 normal,
+                        const float radiance[3], int* material);
+// Geometry and instance scopes address a mesh's single instance.
+inline bool instance_scope(int scope) { return scope == ORF_SCOPE_GEOMETRY || scope == ORF_SCOPE_INSTANCE; }
+// The mesh's private material (copied from its shared one on first use).
+int instance_material(orf_context_t* c, int mesh);
+// ScatteringMaterialProperties as given by the caller become the material's coefficients.
+void set_user_scattering(SssParamsHost& h, const orf_scattering_material_properties& p);
 }  // namespace orf
 

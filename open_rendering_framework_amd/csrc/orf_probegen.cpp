@@ -166,6 +166,11 @@ static float C_E(float ni) { return 0.5f * (1.0f - three_C2(ni)); }
 // with an outside ior of 1.
 ScatteringGPU scattering_coefficients(const SssParamsHost& h)
 {
+    if (h.has_user) {  // ScatteringMaterialProperties given whole (setUserData)
+        ScatteringGPU u = h.user;
+        u.dipole_model = h.dipole_model;
+        return u;
+    }
     ScatteringGPU p;
     std::memset(&p, 0, sizeof(p));
     p.dipole_model = h.dipole_model;

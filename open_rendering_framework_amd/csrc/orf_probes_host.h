@@ -19,6 +19,10 @@ struct SssParamsHost {
     float scale = 100.0f;  // ORF/RoughTranslucentMaterial.cpp:13
     float ior = 1.3f;
     int dipole_model = 0;
+    // coefficients given whole as ScatteringMaterialProperties user data: used
+    // as they are (dipole_model still from the material variable)
+    bool has_user = false;
+    ScatteringGPU user;
 };
 
 ScatteringGPU scattering_coefficients(const SssParamsHost& h);

@@ -63,6 +63,12 @@ _SIGS = {
     "orf_accel_build": [c_vp, c_vp],
     "orf_launch": [c_vp, ctypes.c_int, ctypes.c_int, c_u32, c_u32, c_vp],
     "orf_launch_noalloc": [c_vp, ctypes.c_int, ctypes.c_int, c_u32, c_u32, c_vp, c_vp, c_vp],
+    "orf_variable_set_matrix4x4": [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int, c_vp],
+    "orf_buffer_set": [c_vp, ctypes.c_char_p, c_vp, ctypes.c_size_t, ctypes.c_size_t],
+    "orf_material_set_program": [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p],
+    "orf_texture_create": [c_vp, c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                           ctypes.POINTER(ctypes.c_int)],
+    "orf_variable_set_texture": [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_char_p, ctypes.c_int],
     "orf_film_clear": [c_vp],
     "orf_film_read": [c_vp, c_vp, c_vp],
     "orf_film_read_sums": [c_vp, c_vp],
@@ -225,7 +231,29 @@ class SceneInfo(ctypes.Structure):
 
 LOAD_ENV_IMPORTANCE = 1
 INTEGRATOR_PATH_TRACER, INTEGRATOR_DEPTH = 0, 1
-SCOPE_CONTEXT, SCOPE_MATERIAL = 0, 1
+SCOPE_CONTEXT, SCOPE_MATERIAL, SCOPE_GEOMETRY, SCOPE_INSTANCE = 0, 1, 2, 3
+RAY_RADIANCE, RAY_SHADOW, RAY_DEPTH = 0, 1, 2
+WRAP_REPEAT, WRAP_CLAMP_TO_EDGE = 0, 1
+FILTER_LINEAR, FILTER_NEAREST = 0, 1
+
+
+def _structs():
+    import numpy as np
+
+    f3 = ("<f4", (3,))
+    light = np.dtype([("light_type", "<u4"), ("emitted_radiance", f3), ("light_parameters", f3), ("padding", f3)])
+    tri = np.dtype([("v0", f3), ("v1", f3), ("v2", f3), ("n0", f3), ("n1", f3), ("n2", f3), ("has_normals", "<u4"),
+                    ("emission", f3), ("area", "<f4")])
+    names = ["relative_ior", "ior_real", "ior_imag", "absorption", "scattering", "meancosine", "extinction",
+             "reducedExtinction", "deltaEddExtinction", "D", "transport", "C_phi", "C_phi_inv", "C_E", "A", "albedo",
+             "reducedAlbedo", "de", "three_D", "two_A_de", "global_coeff", "one_over_three_ext", "mean_transport"]
+    sss = np.dtype([(n, "<f4") if n in ("relative_ior", "mean_transport") else (n, f3) for n in names])
+    assert light.itemsize == 40 and tri.itemsize == 92 and sss.itemsize == 260
+    return light, tri, sss
+
+
+# numpy layouts of LightStruct, TriangleLight and ScatteringMaterialProperties (ORF/structs.h)
+LIGHT_STRUCT, TRIANGLE_LIGHT, SCATTERING_PROPERTIES = _structs()
 BATCH_WAVEFRONT, BATCH_MEGAKERNEL, BATCH_PERSISTENT = 0, 1, 2
 
 
@@ -293,23 +321,62 @@ class Context:
             pass
 
     # ---- variables
-    def _scope(self, material):
+    def _scope(self, material, instance=None):
+        if instance is not None:
+            return SCOPE_INSTANCE, instance
         return (SCOPE_CONTEXT, 0) if material is None else (SCOPE_MATERIAL, material)
 
-    def set_float(self, name, *v, material=None):
-        sc, h = self._scope(material)
+    def set_float(self, name, *v, material=None, instance=None):
+        sc, h = self._scope(material, instance)
         arr = (ctypes.c_float * len(v))(*v)
         call("orf_variable_set_float", self.handle, sc, h, name.encode(), arr, len(v))
 
-    def set_int(self, name, *v, material=None):
-        sc, h = self._scope(material)
+    def set_int(self, name, *v, material=None, instance=None):
+        sc, h = self._scope(material, instance)
         arr = (ctypes.c_int32 * len(v))(*v)
         call("orf_variable_set_int", self.handle, sc, h, name.encode(), arr, len(v))
 
-    def set_uint(self, name, *v, material=None):
-        sc, h = self._scope(material)
+    def set_uint(self, name, *v, material=None, instance=None):
+        sc, h = self._scope(material, instance)
         arr = (ctypes.c_uint32 * len(v))(*v)
         call("orf_variable_set_uint", self.handle, sc, h, name.encode(), arr, len(v))
+
+    def set_user_data(self, name, data, material=None, instance=None):
+        """setUserData(size, ptr): `data` is bytes (or a numpy array viewed as bytes)."""
+        sc, h = self._scope(material, instance)
+        b = bytes(data)
+        buf = ctypes.create_string_buffer(b, len(b))
+        call("orf_variable_set_user_data", self.handle, sc, h, name.encode(), buf, len(b))
+
+    def set_matrix4x4(self, name, m4, transpose=False, instance=None):
+        """setMatrix4x4fv(transpose, m) on the context or a mesh instance."""
+        import numpy as np
+
+        m = np.ascontiguousarray(m4, np.float32).reshape(16)
+        sc, h = self._scope(None, instance)
+        call("orf_variable_set_matrix4x4", self.handle, sc, h, name.encode(), int(bool(transpose)), _np_ptr(m))
+
+    def set_buffer(self, name, records):
+        """A user-format context buffer from a numpy structured array
+        (LIGHT_STRUCT for light_buffer, TRIANGLE_LIGHT for triangle_light_buffer)."""
+        import numpy as np
+
+        a = np.ascontiguousarray(records)
+        call("orf_buffer_set", self.handle, name.encode(), _np_ptr(a) if len(a) else None, a.dtype.itemsize, len(a))
+
+    def material_program(self, material, ray_type, program_file, entry):
+        call("orf_material_set_program", self.handle, material, ray_type, program_file.encode(), entry.encode())
+
+    def texture(self, rgba, wrap=0, filter=0):
+        import numpy as np
+
+        t = np.ascontiguousarray(rgba, np.float32)
+        h = ctypes.c_int()
+        call("orf_texture_create", self.handle, _np_ptr(t), t.shape[1], t.shape[0], wrap, filter, ctypes.byref(h))
+        return h.value
+
+    def set_texture(self, name, texture):
+        call("orf_variable_set_texture", self.handle, SCOPE_CONTEXT, 0, name.encode(), texture)
 
     def set_conductor_ior(self, material, ior):
         """ior: three (re, im) pairs, passed as MyComplex3 user data."""
