@@ -369,6 +369,7 @@ typedef struct orf_diff_desc {
     int32_t roughness_uv;
     float roughness_uv_params[4];
     int32_t tex_w, tex_h;
+    int32_t scattering_material;  /* translucent material handle whose coefficients are parameters, or -1 */
 } orf_diff_desc;
 
 orf_status orf_diff_render(orf_context ctx, const orf_diff_desc* desc, const orf_render_item* views, int32_t nviews,
@@ -376,6 +377,26 @@ orf_status orf_diff_render(orf_context ctx, const orf_diff_desc* desc, const orf
 orf_status orf_diff_adjoint(orf_context ctx, const orf_diff_desc* desc, const orf_render_item* views,
                             int32_t nviews, const float* albedo, const float* roughness, const float* d_images,
                             float* g_albedo, float* g_roughness, uint32_t frame_shift, void* stream);
+/* The same with the scattering coefficients of desc->scattering_material as a
+ * third parameter (a translucent material, subsurface_scattering_shader or
+ * rough_subsurface_scattering_shader, whose coefficients come from its
+ * variables, not from user data): `scattering` is a device array of 6 floats,
+ * absorption RGB then scattering RGB in 1/mm (the material variables; its
+ * scale, meancosine, ior and dipole model stay the material's).  The render
+ * shades the material with them (the probe pass included); the adjoint adds
+ * dLoss/d(absorption RGB, scattering RGB) into g_scattering (device, 6 floats,
+ * caller-zeroed): the extinction weights inside the medium are differentiated
+ * with their roulette probability held fixed, and the diffusion gather with its
+ * exit points, their pdf and the probes held fixed.  orf_diff_render and
+ * orf_diff_adjoint are these with scattering = NULL and refuse a desc whose
+ * scattering_material is set. */
+orf_status orf_diff_render_ex(orf_context ctx, const orf_diff_desc* desc, const orf_render_item* views, int32_t nviews,
+                              const float* albedo, const float* roughness, const float* scattering, float* images,
+                              void* stream);
+orf_status orf_diff_adjoint_ex(orf_context ctx, const orf_diff_desc* desc, const orf_render_item* views,
+                               int32_t nviews, const float* albedo, const float* roughness, const float* scattering,
+                               const float* d_images, float* g_albedo, float* g_roughness, float* g_scattering,
+                               uint32_t frame_shift, void* stream);
 
 /* ------------------------------------------------------------------------
  * Translucent materials ("subsurface_scattering_shader.cu", smooth boundary;

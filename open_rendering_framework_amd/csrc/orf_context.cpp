@@ -424,13 +424,14 @@ SceneGPU scene_view(orf_context_t* c)
 }
 
 // Probe pass of the translucent object for sample index `frame` (no-op without probes).
-void run_probe_pass(orf_context_t* c, unsigned frame, hipStream_t stream)
+void run_probe_pass(orf_context_t* c, unsigned frame, hipStream_t stream, ScatteringGPU* table)
 {
     if (c->probe_count <= 0) return;
     SceneGPU s = scene_view(c);
+    if (table) s.sss = table;
     int sss = s.probes.sss;
     s.probes.sss = -1;  // no gathers while the probes are being written
-    launch_probe_pass(s, c->d_sss.as<ScatteringGPU>(), sss, frame, stream);
+    launch_probe_pass(s, s.sss, sss, frame, stream);
 }
 
 void film_clear(orf_context_t* c)

@@ -69,7 +69,7 @@ struct orf_context_t {
     orf::WavefrontPool wf_pool;
     double last_batch_seconds = 0.0;
     // differentiable renders
-    orf::DevBuf d_diff_views, d_diff_stats;
+    orf::DevBuf d_diff_views, d_diff_stats, d_diff_sss;
     long long last_batch_iterations = 0;
     // ray generation program: 0 path tracer, 1 depth tracer; the depth film
     int integrator = 0;
@@ -92,7 +92,8 @@ void prepare(orf_context_t* c, hipStream_t stream);
 // True when every scene change is on the device (prepare() would do nothing).
 bool scene_ready(const orf_context_t* c);
 SceneGPU scene_view(orf_context_t* c);
-void run_probe_pass(orf_context_t* c, unsigned frame, hipStream_t stream);
+// table: the ScatteringGPU table to use (null: the context's)
+void run_probe_pass(orf_context_t* c, unsigned frame, hipStream_t stream, ScatteringGPU* table = nullptr);
 void film_ensure(orf_context_t* c, int w, int h);
 void film_clear(orf_context_t* c);
 MaterialGPU default_material(int kind);

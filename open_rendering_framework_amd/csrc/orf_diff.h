@@ -14,10 +14,18 @@
 //   Lambert  : d w_c / d rho_c = w_c / rho_c, and the NEE term is linear in rho_c;
 //   conductor: d log w / d alpha = d log D(m) + d log G1(wi) + d log G1(wo),
 //              D = 1 / (pi a^2 cos^4 (1 + tan^2/a^2)^2), G1 = 2 / (1 + sqrt(1 + a^2 tan^2)).
+// A translucent material's absorption and scattering coefficients (per RGB
+// channel, the material variables in 1/mm) are the third kind of parameter: its
+// extinction weights exp(-t sigma_t) / p inside the medium are differentiated
+// with p held fixed, and its diffusion gather with the exit points, their pdf
+// and the probes held fixed (forward-mode derivatives of the coefficients and
+// dipoles, orf_sss_grad.h).
 // Path replay (Vicini et al. 2021): pass 1 computes the sample's radiance L; pass 2
 // replays the same random numbers, keeping L_rem = radiance still to come, and at
 // each textured vertex adds  delta * L_rem / rho  (Lambert, per channel, before the
-// vertex's own NEE term is removed) or  <delta, L_rem> * d log w / d alpha.
+// vertex's own NEE term is removed) or  <delta, L_rem> * d log w / d alpha; at a
+// vertex of the translucent material  delta * L_rem * d log w / d sigma  (inside)
+// and  delta * d(gathered radiance) / d sigma.
 #pragma once
 
 #include "orf_path.h"
@@ -33,6 +41,8 @@ struct DiffParams {
     int albedo_mat, rough_mat;  // material handles carrying the textures (-1: none)
     int albedo_uv, rough_uv;
     float albedo_uvp[4], rough_uvp[4];  // planar: x0, z0, x1, z1; spherical: cx, cy, cz, -
+    int sss_mat;      // translucent material handle whose coefficients are differentiated (-1: none)
+    float sss_scale;  // its scale: the gradients are reported per unscaled coefficient
 };
 
 ORF_HD int diff_texel(int mode, const float* uvp, f3 p, int tw, int th)
@@ -71,16 +81,17 @@ ORF_HD float ggx_dlogw_dalpha(f3 wi, f3 wo, f3 n, float a)
 }
 
 struct DiffVertex {
-    int kind;   // 0: no textured parameter, 1: albedo texel, 2: roughness texel
+    int kind;   // 0: no textured parameter, 1: albedo texel, 2: roughness texel, 3: translucent coefficients
     int texel;
     f3 rho;
     float dlogw;
+    SssGrad sg;
 };
 
 // One vertex of the differentiable estimator: the plain shading step with the
 // texel's parameter substituted, plus what the adjoint needs.
 ORF_HD bool diff_vertex(const SceneGPU& sc, const DiffParams& dp, PathState& ps, const Hit& h, ShadowRay& sr,
-                        DiffVertex& dv)
+                        DiffVertex& dv, bool want_grad = false)
 {
     HitGeom g = hit_geometry(sc, ps, h);
     int mid = g.sh.w;
@@ -98,8 +109,13 @@ ORF_HD bool diff_vertex(const SceneGPU& sc, const DiffParams& dp, PathState& ps,
         dv.kind = 2;
         dv.texel = t;
     }
+    if (dv.kind == 0 && mid == dp.sss_mat && want_grad &&
+        (mat.kind == MAT_TRANSLUCENT || mat.kind == MAT_ROUGH_TRANSLUCENT)) {
+        dv.kind = 3;
+        sss_grad_clear(dv.sg);
+    }
     f3 wi = -ps.d;
-    bool more = shade_with(sc, mat, ps, h, g, sr);
+    bool more = shade_with(sc, mat, ps, h, g, sr, nullptr, 1, dv.kind == 3 ? &dv.sg : nullptr);
     if (dv.kind == 1 && mat.kind != MAT_LAMBERT) dv.kind = 0;
     if (dv.kind == 2) {
         if (mat.kind == MAT_CONDUCTOR && more && mat.normal_distribution == GGX_DISTRIBUTION)
@@ -155,12 +171,19 @@ ORF_HD void diff_replay(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, ui
         f3 rem_before = L_total - ps.L;  // radiance from this vertex on
         ShadowRay sr;
         DiffVertex dv;
-        bool more = diff_vertex(sc, dp, ps, h, sr, dv);
+        bool more = diff_vertex(sc, dp, ps, h, sr, dv, true);
         if (sr.valid) {
             Hit hs;
             if (!trace<true, false>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) shadow_unoccluded(ps, sr);
         }
-        if (dv.kind == 1) {
+        if (dv.kind == 3) {
+            f3 ga = delta * dv.sg.dL_a, gs = delta * dv.sg.dL_s;
+            if (dv.sg.beam) {
+                ga += delta * rem_before * dv.sg.dlog_a;
+                gs += delta * rem_before * dv.sg.dlog_s;
+            }
+            sink.sss(ga * dp.sss_scale, gs * dp.sss_scale);
+        } else if (dv.kind == 1) {
             if (dv.rho.x != 0.0f) sink.albedo(dv.texel, 0, delta.x * rem_before.x / dv.rho.x);
             if (dv.rho.y != 0.0f) sink.albedo(dv.texel, 1, delta.y * rem_before.y / dv.rho.y);
             if (dv.rho.z != 0.0f) sink.albedo(dv.texel, 2, delta.z * rem_before.z / dv.rho.z);

@@ -17,10 +17,13 @@ namespace orf {
 
 constexpr int kDiffBlock = 64;
 
-// Texel gradients land in global FP32 arrays through atomics.
+// Texel gradients land in global FP32 arrays through atomics; the translucent
+// coefficients' six gradients are summed per lane and added once per lane.
 struct AtomicSink {
     float* g_albedo;
     float* g_rough;
+    float* g_sss;  // d/d absorption RGB, d/d scattering RGB
+    f3 ga, gs;
     __device__ void albedo(int texel, int c, float v)
     {
         if (g_albedo) atomicAdd(g_albedo + 3 * (size_t)texel + c, v);
@@ -29,12 +32,24 @@ struct AtomicSink {
     {
         if (g_rough) atomicAdd(g_rough + texel, v);
     }
+    __device__ void sss(f3 a, f3 s)
+    {
+        ga += a;
+        gs += s;
+    }
+    __device__ void flush()
+    {
+        if (!g_sss) return;
+        const float v[6] = {ga.x, ga.y, ga.z, gs.x, gs.y, gs.z};
+        for (int k = 0; k < 6; ++k)
+            if (v[k] != 0.0f) atomicAdd(g_sss + k, v[k]);
+    }
 };
 
 __global__ __launch_bounds__(kDiffBlock) void k_diff(SceneGPU sc, DiffParams dp, const WfRender* __restrict__ views,
                                                      int tiles_x, int tiles_per_view, int adjoint,
                                                      unsigned frame_shift, float* images, const float* dimages,
-                                                     float* g_albedo, float* g_rough,
+                                                     float* g_albedo, float* g_rough, float* g_sss,
                                                      unsigned long long* counters)
 {
     __shared__ unsigned stack_lds[kLdsStack * kDiffBlock];
@@ -67,7 +82,7 @@ __global__ __launch_bounds__(kDiffBlock) void k_diff(SceneGPU sc, DiffParams dp,
         images[3 * px + 1] = acc.y * inv;
         images[3 * px + 2] = acc.z * inv;
     } else {
-        AtomicSink sink{g_albedo, g_rough};
+        AtomicSink sink{g_albedo, g_rough, g_sss, mk3(0.0f), mk3(0.0f)};
         float sc_inv = 1.0f / (float)max(1u, R.spp);
         f3 delta = mk3(dimages[3 * px], dimages[3 * px + 1], dimages[3 * px + 2]) * sc_inv;
         if (delta.x == 0.0f && delta.y == 0.0f && delta.z == 0.0f) return;
@@ -82,20 +97,42 @@ __global__ __launch_bounds__(kDiffBlock) void k_diff(SceneGPU sc, DiffParams dp,
             }
             diff_replay(sc, dp, R.cam.eye, dir, rng, seed64(s0), L, delta, sink, stk, kDiffBlock);
         }
+        sink.flush();
     }
     if (bad) atomicAdd(counters, (unsigned long long)bad);
 }
 
 void launch_diff(const SceneGPU& sc, const DiffParams& dp, const WfRender* d_views, int nviews, int width,
                  int height, bool adjoint, unsigned frame_shift, float* images, const float* dimages,
-                 float* g_albedo, float* g_rough, unsigned long long* counters, hipStream_t stream)
+                 float* g_albedo, float* g_rough, float* g_sss, unsigned long long* counters, hipStream_t stream)
 {
     int tiles_x = (width + 7) / 8, tiles_y = (height + 7) / 8;
     int64_t blocks = (int64_t)tiles_x * tiles_y * nviews;
     ORF_REQUIRE(blocks > 0 && blocks < 0x7fffffffLL, "diff launch: bad size");
     hipLaunchKernelGGL(k_diff, dim3((unsigned)blocks), dim3(kDiffBlock), 0, stream, sc, dp, d_views, tiles_x,
-                       tiles_x * tiles_y, adjoint ? 1 : 0, frame_shift, images, dimages, g_albedo, g_rough,
+                       tiles_x * tiles_y, adjoint ? 1 : 0, frame_shift, images, dimages, g_albedo, g_rough, g_sss,
                        counters);
+    ORF_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace orf
+
+namespace orf {
+
+// The coefficients of material `m` from device parameters (absorption RGB,
+// scattering RGB, unscaled), the rest of its record kept: one lane.
+__global__ void k_sss_coeffs(const float* __restrict__ params, float scale, ScatteringGPU* table, int m)
+{
+    if (threadIdx.x != 0) return;
+    const ScatteringGPU t = table[m];
+    const f3 sa = mk3(params[0], params[1], params[2]) * scale;
+    const f3 ss = mk3(params[3], params[4], params[5]) * scale;
+    table[m] = sss_coefficients(sa, ss, t.meancosine, t.relative_ior, t.dipole_model);
+}
+
+void launch_sss_coeffs(const float* params, float scale, ScatteringGPU* table, int m, hipStream_t stream)
+{
+    hipLaunchKernelGGL(k_sss_coeffs, dim3(1), dim3(64), 0, stream, params, scale, table, m);
     ORF_HIP_CHECK(hipGetLastError());
 }
 

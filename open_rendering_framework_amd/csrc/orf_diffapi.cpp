@@ -8,7 +8,8 @@
 
 namespace orf {
 
-static DiffParams diff_params(orf_context_t* c, const orf_diff_desc* d, const float* albedo, const float* rough)
+static DiffParams diff_params(orf_context_t* c, const orf_diff_desc* d, const float* albedo, const float* rough,
+                              const float* scattering)
 {
     ORF_REQUIRE(d, "diff: null description");
     ORF_REQUIRE(c->analytic.empty(), "diff: scenes with disk or sphere lights are not supported");
@@ -42,6 +43,18 @@ static DiffParams diff_params(orf_context_t* c, const orf_diff_desc* d, const fl
         p.albedo_uvp[k] = d->albedo_uv_params[k];
         p.rough_uvp[k] = d->roughness_uv_params[k];
     }
+    p.sss_mat = -1;
+    p.sss_scale = 1.0f;
+    const int sm = d->scattering_material;
+    ORF_REQUIRE(sm < nm, "diff: scattering material handle out of range");
+    ORF_REQUIRE(sm < 0 || scattering, "diff: scattering coefficients required");
+    if (sm >= 0) {
+        ORF_REQUIRE(is_translucent(c->mats[sm].kind), "diff: the scattering coefficients apply to a translucent material");
+        ORF_REQUIRE(!c->mat_sss[sm].has_user,
+                    "diff: the scattering material takes its coefficients from user data, not from absorption/scattering");
+        p.sss_mat = sm;
+        p.sss_scale = c->mat_sss[sm].scale;
+    }
     return p;
 }
 
@@ -72,6 +85,63 @@ static void stage_views(orf_context_t* c, const orf_render_item* v, int n, hipSt
 
 }  // namespace orf
 
+namespace orf {
+
+// The scene with the differentiated material's coefficients from `scattering`
+// (device) in a copy of the coefficient table, and the probe pass run with it.
+static SceneGPU diff_scene(orf_context_t* c, const DiffParams& dp, const float* scattering, unsigned frame,
+                           hipStream_t st)
+{
+    SceneGPU sc = scene_view(c);
+    ScatteringGPU* table = nullptr;
+    if (dp.sss_mat >= 0) {
+        c->d_diff_sss.ensure(c->d_sss.bytes);
+        ORF_HIP_CHECK(hipMemcpyAsync(c->d_diff_sss.p, c->d_sss.p, c->d_sss.bytes, hipMemcpyDeviceToDevice, st));
+        table = c->d_diff_sss.as<ScatteringGPU>();
+        launch_sss_coeffs(scattering, dp.sss_scale, table, dp.sss_mat, st);
+        sc.sss = table;
+    }
+    run_probe_pass(c, frame, st, table);
+    return sc;
+}
+
+}  // namespace orf
+
+namespace orf {
+
+static void diff_render(orf_context_t* c, const orf_diff_desc* desc, const orf_render_item* views, int32_t nviews,
+                        const float* albedo, const float* roughness, const float* scattering, float* images,
+                        hipStream_t st)
+{
+    ORF_REQUIRE(images, "orf_diff_render: null image buffer");
+    prepare(c, st);
+    DiffParams dp = diff_params(c, desc, albedo, roughness, scattering);
+    int w, h;
+    stage_views(c, views, nviews, st, w, h);
+    SceneGPU sc = diff_scene(c, dp, scattering, views[0].frame0, st);
+    launch_diff(sc, dp, c->d_diff_views.as<WfRender>(), nviews, w, h, false, 0, images, nullptr, nullptr, nullptr,
+                nullptr, c->d_diff_stats.as<unsigned long long>(), st);
+}
+
+static void diff_adjoint(orf_context_t* c, const orf_diff_desc* desc, const orf_render_item* views, int32_t nviews,
+                         const float* albedo, const float* roughness, const float* scattering, const float* d_images,
+                         float* g_albedo, float* g_roughness, float* g_scattering, uint32_t frame_shift,
+                         hipStream_t st)
+{
+    ORF_REQUIRE(d_images, "orf_diff_adjoint: null image gradient");
+    prepare(c, st);
+    DiffParams dp = diff_params(c, desc, albedo, roughness, scattering);
+    int w, h;
+    stage_views(c, views, nviews, st, w, h);
+    // the probes of the primal's sample index, so the replay sees the primal's probe light
+    SceneGPU sc = diff_scene(c, dp, scattering, views[0].frame0, st);
+    launch_diff(sc, dp, c->d_diff_views.as<WfRender>(), nviews, w, h, true, frame_shift, nullptr, d_images,
+                desc->albedo_material >= 0 ? g_albedo : nullptr, desc->roughness_material >= 0 ? g_roughness : nullptr,
+                dp.sss_mat >= 0 ? g_scattering : nullptr, c->d_diff_stats.as<unsigned long long>(), st);
+}
+
+}  // namespace orf
+
 using namespace orf;
 
 extern "C" orf_status orf_diff_render(orf_context ctx, const orf_diff_desc* desc, const orf_render_item* views,
@@ -80,15 +150,8 @@ extern "C" orf_status orf_diff_render(orf_context ctx, const orf_diff_desc* desc
 {
     ORF_API_BEGIN
     orf_context_t* c = ctx_check(ctx);
-    ORF_REQUIRE(images, "orf_diff_render: null image buffer");
-    hipStream_t st = (hipStream_t)stream;
-    prepare(c, st);
-    DiffParams dp = diff_params(c, desc, albedo, roughness);
-    int w, h;
-    stage_views(c, views, nviews, st, w, h);
-    run_probe_pass(c, views[0].frame0, st);
-    launch_diff(scene_view(c), dp, c->d_diff_views.as<WfRender>(), nviews, w, h, false, 0, images, nullptr,
-                nullptr, nullptr, c->d_diff_stats.as<unsigned long long>(), st);
+    ORF_REQUIRE(desc && desc->scattering_material < 0, "orf_diff_render: scattering coefficients need orf_diff_render_ex");
+    diff_render(c, desc, views, nviews, albedo, roughness, nullptr, images, (hipStream_t)stream);
     ORF_API_END
 }
 
@@ -99,14 +162,30 @@ extern "C" orf_status orf_diff_adjoint(orf_context ctx, const orf_diff_desc* des
 {
     ORF_API_BEGIN
     orf_context_t* c = ctx_check(ctx);
-    ORF_REQUIRE(d_images, "orf_diff_adjoint: null image gradient");
-    hipStream_t st = (hipStream_t)stream;
-    prepare(c, st);
-    DiffParams dp = diff_params(c, desc, albedo, roughness);
-    int w, h;
-    stage_views(c, views, nviews, st, w, h);
-    launch_diff(scene_view(c), dp, c->d_diff_views.as<WfRender>(), nviews, w, h, true, frame_shift, nullptr,
-                d_images, desc->albedo_material >= 0 ? g_albedo : nullptr,
-                desc->roughness_material >= 0 ? g_roughness : nullptr, c->d_diff_stats.as<unsigned long long>(), st);
+    ORF_REQUIRE(desc && desc->scattering_material < 0,
+                "orf_diff_adjoint: scattering coefficients need orf_diff_adjoint_ex");
+    diff_adjoint(c, desc, views, nviews, albedo, roughness, nullptr, d_images, g_albedo, g_roughness, nullptr,
+                 frame_shift, (hipStream_t)stream);
+    ORF_API_END
+}
+
+extern "C" orf_status orf_diff_render_ex(orf_context ctx, const orf_diff_desc* desc, const orf_render_item* views,
+                                         int32_t nviews, const float* albedo, const float* roughness,
+                                         const float* scattering, float* images, void* stream)
+{
+    ORF_API_BEGIN
+    diff_render(ctx_check(ctx), desc, views, nviews, albedo, roughness, scattering, images, (hipStream_t)stream);
+    ORF_API_END
+}
+
+extern "C" orf_status orf_diff_adjoint_ex(orf_context ctx, const orf_diff_desc* desc, const orf_render_item* views,
+                                          int32_t nviews, const float* albedo, const float* roughness,
+                                          const float* scattering, const float* d_images, float* g_albedo,
+                                          float* g_roughness, float* g_scattering, uint32_t frame_shift, void* stream)
+{
+    ORF_API_BEGIN
+    ORF_REQUIRE(!desc || desc->scattering_material < 0 || g_scattering, "orf_diff_adjoint_ex: null scattering gradient");
+    diff_adjoint(ctx_check(ctx), desc, views, nviews, albedo, roughness, scattering, d_images, g_albedo, g_roughness,
+                 g_scattering, frame_shift, (hipStream_t)stream);
     ORF_API_END
 }

@@ -7,6 +7,7 @@
 // diffusion term without the global coefficient (applied by the caller).
 #pragma once
 
+#include "orf_fresnel.h"
 #include "orf_scene.h"
 
 namespace orf {
@@ -14,6 +15,42 @@ namespace orf {
 ORF_HD f3 sqrt3(f3 v) { return mk3(sqrtf(v.x), sqrtf(v.y), sqrtf(v.z)); }
 ORF_HD f3 exp3(f3 v) { return mk3(orf_expf(v.x), orf_expf(v.y), orf_expf(v.z)); }
 ORF_HD f3 div3(f3 a, f3 b) { return mk3(a.x / b.x, a.y / b.y, a.z / b.z); }
+
+// RoughTranslucentMaterial::computeCoefficients (ORF/RoughTranslucentMaterial.cpp:366-395)
+// with an outside ior of 1, from the scaled coefficients (absorption and
+// scattering already multiplied by the material's scale).
+ORF_HD ScatteringGPU sss_coefficients(f3 absorption, f3 scattering, f3 meancosine, float ior, int dipole_model)
+{
+    ScatteringGPU p;
+    p.dipole_model = dipole_model;
+    p.absorption = absorption;
+    p.scattering = scattering;
+    p.meancosine = meancosine;
+    p.relative_ior = ior;
+    const f3 ior_rgb = mk3(ior);
+    const f3 reducedScattering = p.scattering * (1.0f - p.meancosine);
+    p.relative_ior /= (1.0f + 1.0f + 1.0f) / 3.0f;
+    p.extinction = p.scattering + p.absorption;
+    p.reducedExtinction = reducedScattering + p.absorption;
+    p.deltaEddExtinction = p.scattering * (1.0f - p.meancosine * p.meancosine) + p.absorption;
+    p.D = div3(mk3(1.0f), 3.0f * p.reducedExtinction);
+    p.transport = sqrt3(div3(p.absorption, p.D));
+    auto c_phi = [](float n) { return 0.25f * (1.0f - two_C1(n)); };
+    auto c_e = [](float n) { return 0.5f * (1.0f - three_C2(n)); };
+    p.C_phi = mk3(c_phi(ior_rgb.x), c_phi(ior_rgb.y), c_phi(ior_rgb.z));
+    p.C_phi_inv = mk3(c_phi(1.0f / ior_rgb.x), c_phi(1.0f / ior_rgb.y), c_phi(1.0f / ior_rgb.z));
+    p.C_E = mk3(c_e(ior_rgb.x), c_e(ior_rgb.y), c_e(ior_rgb.z));
+    p.albedo = div3(p.scattering, p.extinction);
+    p.reducedAlbedo = div3(reducedScattering, p.reducedExtinction);
+    p.de = div3(2.131f * p.D, sqrt3(p.reducedAlbedo));
+    p.A = div3(1.0f - p.C_E, 2.0f * p.C_phi);
+    p.three_D = 3.0f * p.D;
+    p.two_A_de = 2.0f * p.A * p.de;
+    p.global_coeff = div3(mk3(1.0f), 4.0f * p.C_phi_inv) * 1.0f / (4.0f * kPi * kPi);
+    p.one_over_three_ext = div3(mk3(1.0f), 3.0f * p.extinction);
+    p.mean_transport = (p.transport.x + p.transport.y + p.transport.z) / 3.0f;
+    return p;
+}
 
 // ORF/dipoles/standard_dipole.h:8-29.
 ORF_HD f3 dipole_bssrdf(float dist, const ScatteringGPU& p)

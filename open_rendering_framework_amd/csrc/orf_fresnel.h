@@ -92,4 +92,26 @@ ORF_HD float fresnel_dielectric(float cos_theta, float eta)
     return (rs * rs + rp * rp) * 0.5f;
 }
 
+// two_C1(n) (ORF/Fresnel.h:93-101): twice the first Fresnel moment, the
+// published polynomial fits of d'Eon & Irving 2011 (one per side of n = 1).
+ORF_HD float two_C1(float n)
+{
+    if (n >= 1.0f) return -9.23372f + n * (22.2272f + n * (-20.9292f + n * (10.2291f + n * (-2.54396f + n * 0.254913f))));
+    return 0.919317f + n * (-3.4793f + n * (6.75335f + n * (-7.80989f + n * (4.98554f - n * 1.36881f))));
+}
+
+// three_C2(n) (ORF/Fresnel.h:103-114): three times the second Fresnel moment,
+// the same authors' fits.
+ORF_HD float three_C2(float n)
+{
+    float r;
+    if (n >= 1.0f) {
+        r = -1641.1f + n * (1213.67f + n * (-568.556f + n * (164.798f + n * (-27.0181f + n * 1.91826f))));
+        r += (((135.926f / n) - 656.175f) / n + 1376.53f) / n;
+    } else {
+        r = 0.828421f + n * (-2.62051f + n * (3.36231f + n * (-1.95284f + n * (0.236494f + n * 0.145787f))));
+    }
+    return r;
+}
+
 }  // namespace orf

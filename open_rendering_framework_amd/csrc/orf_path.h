@@ -126,7 +126,7 @@ ORF_HD HitGeom hit_geometry(const SceneGPU& sc, const PathState& ps, const Hit& 
 
 template <unsigned kFeat = kFeatAll>
 ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps, const Hit& h, const HitGeom& g,
-                       ShadowRay& sr, unsigned* stk = nullptr, int stride = 1);
+                       ShadowRay& sr, unsigned* stk = nullptr, int stride = 1, SssGrad* sg = nullptr);
 
 // Shade one surface vertex.  Returns true when the path continues with ps.o/ps.d
 // as its next ray; `sr` receives the NEE shadow segment, if any.  kFeat: the
@@ -142,9 +142,11 @@ ORF_HD bool shade_vertex(const SceneGPU& sc, const int* remap, PathState& ps, co
 }
 
 // Shade a vertex with material `mat` (the material programs' closest hits).
+// `sg` (translucent materials): receives what the vertex contributes to the
+// derivatives with respect to the material's absorption and scattering.
 template <unsigned kFeat>
 ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps, const Hit& h, const HitGeom& g,
-                       ShadowRay& sr, unsigned* stk, int stride)
+                       ShadowRay& sr, unsigned* stk, int stride, SssGrad* sg)
 {
     constexpr bool kDiel = (kFeat & kFeatDielectric) != 0, kSss = (kFeat & kFeatTranslucent) != 0;
     sr.valid = false;
@@ -482,6 +484,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
             recip = P.relative_ior;
             no = -no;
             co = -co;
+            if (sg) sss_beam_grad(*sg, P, h.t);
         }
         float st2 = recip * recip * (1.0f - co * co);
         float ct = 1.0f, R = 1.0f;
@@ -492,8 +495,13 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         ps.thr = ps.thr * beam;
         f3 nd;
         if (reflect_xi >= R) {
-            if (!inside && mat.sss == sc.probes.sss)
-                ps.L += ps.thr * sss_gather(sc, P, mat.sss, p, no, ps.s64, stk, stride);
+            if (!inside && mat.sss == sc.probes.sss) {
+                ps.L += ps.thr * sss_gather(sc, P, mat.sss, p, no, ps.s64, stk, stride, sg);
+                if (sg) {
+                    sg->dL_a = ps.thr * sg->dL_a;
+                    sg->dL_s = ps.thr * sg->dL_s;
+                }
+            }
             nd = recip * (co * no - wo) - no * ct;
         } else {
             nd = 2.0f * co * no - wo;
@@ -587,6 +595,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
             if (rnd_accurate(ps.s64) >= prob) return false;
             beam = beam / prob;
             recip = P.relative_ior;
+            if (sg) sss_beam_grad(*sg, P, h.t);
         }
         const f3 no = inside ? -normal : normal;
         if (model != MULTISCATTERING_MODEL) {
@@ -610,8 +619,13 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         }
         ps.thr = ps.thr * beam;
         if (refract) {
-            if (!inside && mat.sss == sc.probes.sss)
-                ps.L += ps.thr * weight * sss_gather(sc, P, mat.sss, p, no, ps.s64, stk, stride);
+            if (!inside && mat.sss == sc.probes.sss) {
+                ps.L += ps.thr * weight * sss_gather(sc, P, mat.sss, p, no, ps.s64, stk, stride, sg);
+                if (sg) {
+                    sg->dL_a = ps.thr * weight * sg->dL_a;
+                    sg->dL_s = ps.thr * weight * sg->dL_s;
+                }
+            }
             ps.emit = 1;
         } else if (model == MULTISCATTERING_MODEL) {
             ps.emit = 1;

@@ -146,22 +146,6 @@ ProbeSetHost generate_probes(const float* V, const int32_t* I, const int4* shade
     return ps;
 }
 
-// two_C1 / three_C2 / C_phi / C_E (ORF/Fresnel.h:93-124): the published polynomial
-// fits of the first and second Fresnel moments.
-static float three_C2(float n)
-{
-    float r;
-    if (n >= 1.0f) {
-        r = -1641.1f + n * (1213.67f + n * (-568.556f + n * (164.798f + n * (-27.0181f + n * 1.91826f))));
-        r += (((135.926f / n) - 656.175f) / n + 1376.53f) / n;
-    } else {
-        r = 0.828421f + n * (-2.62051f + n * (3.36231f + n * (-1.95284f + n * (0.236494f + n * 0.145787f))));
-    }
-    return r;
-}
-static float C_phi(float ni) { return 0.25f * (1.0f - two_C1(ni)); }
-static float C_E(float ni) { return 0.5f * (1.0f - three_C2(ni)); }
-
 // RoughTranslucentMaterial::computeCoefficients (ORF/RoughTranslucentMaterial.cpp:366-395)
 // with an outside ior of 1.
 ScatteringGPU scattering_coefficients(const SssParamsHost& h)
@@ -171,34 +155,7 @@ ScatteringGPU scattering_coefficients(const SssParamsHost& h)
         u.dipole_model = h.dipole_model;
         return u;
     }
-    ScatteringGPU p;
-    std::memset(&p, 0, sizeof(p));
-    p.dipole_model = h.dipole_model;
-    p.absorption = h.absorption * h.scale;
-    p.scattering = h.scattering * h.scale;
-    p.meancosine = h.meancosine;
-    p.relative_ior = h.ior;
-    f3 ior_rgb = mk3(h.ior);
-    f3 reducedScattering = p.scattering * (1.0f - p.meancosine);
-    p.relative_ior /= (1.0f + 1.0f + 1.0f) / 3.0f;
-    p.extinction = p.scattering + p.absorption;
-    p.reducedExtinction = reducedScattering + p.absorption;
-    p.deltaEddExtinction = p.scattering * (1.0f - p.meancosine * p.meancosine) + p.absorption;
-    p.D = div3(mk3(1.0f), 3.0f * p.reducedExtinction);
-    p.transport = sqrt3(div3(p.absorption, p.D));
-    p.C_phi = mk3(C_phi(ior_rgb.x), C_phi(ior_rgb.y), C_phi(ior_rgb.z));
-    p.C_phi_inv = mk3(C_phi(1.0f / ior_rgb.x), C_phi(1.0f / ior_rgb.y), C_phi(1.0f / ior_rgb.z));
-    p.C_E = mk3(C_E(ior_rgb.x), C_E(ior_rgb.y), C_E(ior_rgb.z));
-    p.albedo = div3(p.scattering, p.extinction);
-    p.reducedAlbedo = div3(reducedScattering, p.reducedExtinction);
-    p.de = div3(2.131f * p.D, sqrt3(p.reducedAlbedo));
-    p.A = div3(1.0f - p.C_E, 2.0f * p.C_phi);
-    p.three_D = 3.0f * p.D;
-    p.two_A_de = 2.0f * p.A * p.de;
-    p.global_coeff = div3(mk3(1.0f), 4.0f * p.C_phi_inv) * 1.0f / (4.0f * kPi * kPi);
-    p.one_over_three_ext = div3(mk3(1.0f), 3.0f * p.extinction);
-    p.mean_transport = (p.transport.x + p.transport.y + p.transport.z) / 3.0f;
-    return p;
+    return sss_coefficients(h.absorption * h.scale, h.scattering * h.scale, h.meancosine, h.ior, h.dipole_model);
 }
 
 }  // namespace orf

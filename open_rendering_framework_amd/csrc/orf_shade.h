@@ -285,13 +285,6 @@ ORF_HD float fresnel_R_cos(float c, float eta)
     return fresnel_R3(c, sqrtf(1.0f - st2), eta);
 }
 
-// two_C1(n) (ORF/Fresnel.h:93-101): twice the first Fresnel moment, the
-// published polynomial fits of d'Eon & Irving 2011 (one per side of n = 1).
-ORF_HD float two_C1(float n)
-{
-    if (n >= 1.0f) return -9.23372f + n * (22.2272f + n * (-20.9292f + n * (10.2291f + n * (-2.54396f + n * 0.254913f))));
-    return 0.919317f + n * (-3.4793f + n * (6.75335f + n * (-7.80989f + n * (4.98554f - n * 1.36881f))));
-}
 
 // ---------------------------------------------------------------- multiple scattering
 // Microsurface random walk of Heitz et al. 2016 on a Gaussian-height GGX surface

@@ -20,6 +20,7 @@
 #pragma once
 
 #include "orf_dipole.h"
+#include "orf_sss_grad.h"
 #include "orf_shade.h"
 #include "orf_trace.h"
 
@@ -147,9 +148,11 @@ ORF_HD int probe_ray_hits(const SceneGPU& sc, int sss, f3 o, f3 dir, float tmax,
 // numbers come from the path's 64-bit stream, as the reference's gather loop
 // draws its roulette numbers (CU/subsurface_scattering_shader.cu:152-156).  The
 // probe rays use the caller's traversal stack (stk, stride) -- free between
-// traces -- or a private one when stk is null.
+// traces -- or a private one when stk is null.  With `grad`, the derivatives of
+// the returned term with respect to the scaled sigma_a and sigma_s of each
+// channel go to grad->dL_a, dL_s (the exit points and their pdf held fixed).
 ORF_HD f3 sss_gather(const SceneGPU& sc, const ScatteringGPU& P, int sss, f3 xo, f3 no, Seed64& rng,
-                     unsigned* stk = nullptr, int stride = 1)
+                     unsigned* stk = nullptr, int stride = 1, SssGrad* grad = nullptr)
 {
     const ProbesGPU& pr = sc.probes;
     f3 acc = mk3(0.0f);
@@ -203,6 +206,28 @@ ORF_HD f3 sss_gather(const SceneGPU& sc, const ScatteringGPU& P, int sss, f3 xo,
             S = dipole_bssrdf(sqrtf(dot(dx, dx)), P);
         }
         acc += E * S / pdf;
+        if (grad) {
+            const float4 nj = pr.sn[j];
+            const float e[3] = {E.x, E.y, E.z};
+            const float sa[3] = {P.absorption.x, P.absorption.y, P.absorption.z};
+            const float ss[3] = {P.scattering.x, P.scattering.y, P.scattering.z};
+            const float g[3] = {P.meancosine.x, P.meancosine.y, P.meancosine.z};
+            float da[3], ds[3];
+            for (int c = 0; c < 3; ++c) {
+                const SssCoefD cd = sss_coef_channel(sa[c], ss[c], g[c], P.relative_ior);
+                const Dual2 Sd = P.dipole_model == 1
+                                     ? dirpole_channel(xi, mk3(nj.x, nj.y, nj.z), mk3(wj.x, wj.y, wj.z), xo, no, cd)
+                                     : dipole_channel(sqrtf(dot(dx, dx)), cd);
+                da[c] = e[c] * Sd.a / pdf;
+                ds[c] = e[c] * Sd.s / pdf;
+            }
+            grad->dL_a += mk3(da[0], da[1], da[2]);
+            grad->dL_s += mk3(ds[0], ds[1], ds[2]);
+        }
+    }
+    if (grad) {
+        grad->dL_a = grad->dL_a * (1.0f / (float)pr.samples) * P.global_coeff;
+        grad->dL_s = grad->dL_s * (1.0f / (float)pr.samples) * P.global_coeff;
     }
     return acc * (1.0f / (float)pr.samples) * P.global_coeff;
 }

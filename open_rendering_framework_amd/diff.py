@@ -65,16 +65,19 @@ def load_extension():
 
 class DiffRenderer:
     """Renders views of a context's scene as a differentiable function of an
-    albedo texture (on a Lambert material) and a roughness texture (on a GGX
-    conductor): images = renderer(albedo[h,w,3], roughness[h,w])."""
+    albedo texture (on a Lambert material), a roughness texture (on a GGX
+    conductor) and, with `scattering_material`, the absorption and scattering
+    coefficients of a translucent material (6 values, 1/mm):
+    images = renderer(albedo[h,w,3], roughness[h,w] [, scattering[6]])."""
 
     def __init__(self, ctx, views, albedo_material, roughness_material, tex_size=(64, 64),
                  albedo_uv=(0, (-10.0, -10.0, 10.0, 10.0)), roughness_uv=(1, (0.0, 0.0, 0.0, 0.0)),
-                 frame_shift=0x40000000):
+                 frame_shift=0x40000000, scattering_material=-1):
         ext = load_extension()
         self.ctx = ctx
+        self.scattering_material = scattering_material
         desc = [albedo_material, albedo_uv[0], *albedo_uv[1], roughness_material, roughness_uv[0], *roughness_uv[1],
-                tex_size[0], tex_size[1]]
+                tex_size[0], tex_size[1], scattering_material]
         vv = [[v["width"], v["height"], v["spp"], v["frame0"], *v["eye"], *v["U"], *v["V"], *v["W"]] for v in views]
         self.views = views
         self.tex_size = tex_size
@@ -82,7 +85,10 @@ class DiffRenderer:
                                     int(frame_shift))
         self._ext = ext
 
-    def __call__(self, albedo, roughness):
+    def __call__(self, albedo, roughness, scattering=None):
+        if self.scattering_material >= 0:
+            assert scattering is not None, "the scattering coefficients are a parameter of this renderer"
+            return self._ext.render_sss(albedo, roughness, scattering, self.setup)
         return self._ext.render(albedo, roughness, self.setup)
 
     def close(self):

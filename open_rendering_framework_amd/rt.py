@@ -88,6 +88,8 @@ _SIGS = {
     "orf_probes_read": [c_vp, ctypes.POINTER(c_i64), c_dblp, c_f32p, c_f32p, c_vp, c_vp, c_vp, c_vp, c_vp],
     "orf_probe_pass": [c_vp, c_u32, c_vp],
     "orf_diff_adjoint": [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp],
+    "orf_diff_render_ex": [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp, c_vp, c_vp, c_vp, c_vp],
+    "orf_diff_adjoint_ex": [c_vp, c_vp, c_vp, ctypes.c_int32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_u32, c_vp],
     "orf_scene_load_json": [c_vp, ctypes.c_char_p, c_u32, c_vp],
     "orf_integrator_set": [c_vp, ctypes.c_int],
     "orf_render_frames": [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32, c_vp],
@@ -287,17 +289,18 @@ class DiffDesc(ctypes.Structure):
     _fields_ = [("albedo_material", ctypes.c_int32), ("albedo_uv", ctypes.c_int32),
                 ("albedo_uv_params", ctypes.c_float * 4), ("roughness_material", ctypes.c_int32),
                 ("roughness_uv", ctypes.c_int32), ("roughness_uv_params", ctypes.c_float * 4),
-                ("tex_w", ctypes.c_int32), ("tex_h", ctypes.c_int32)]
+                ("tex_w", ctypes.c_int32), ("tex_h", ctypes.c_int32), ("scattering_material", ctypes.c_int32)]
 
     @classmethod
     def make(cls, albedo_material, roughness_material, tex_size, albedo_uv=(0, (-10.0, -10.0, 10.0, 10.0)),
-             roughness_uv=(1, (0.0, 0.0, 0.0, 0.0))):
+             roughness_uv=(1, (0.0, 0.0, 0.0, 0.0)), scattering_material=-1):
         d = cls()
         d.albedo_material, d.roughness_material = albedo_material, roughness_material
         d.albedo_uv, d.roughness_uv = albedo_uv[0], roughness_uv[0]
         d.albedo_uv_params[:] = [float(x) for x in albedo_uv[1]]
         d.roughness_uv_params[:] = [float(x) for x in roughness_uv[1]]
         d.tex_w, d.tex_h = tex_size
+        d.scattering_material = scattering_material
         return d
 
 
@@ -535,14 +538,16 @@ class Context:
         call("orf_batch_counters", self.handle, _np_ptr(out))
         return dict(zip(("nonfinite", "ext_rays", "shadow_rays", "paths"), [int(x) for x in out]))
 
-    def diff_render(self, desc, views, albedo, roughness, images=None, stream=None):
-        """Per-view mean images (nviews, h, w, 3) of the textured scene (device tensors)."""
+    def diff_render(self, desc, views, albedo, roughness, images=None, stream=None, scattering=None):
+        """Per-view mean images (nviews, h, w, 3) of the textured scene (device tensors);
+        `scattering` (6 floats, device) when desc.scattering_material is set."""
         arr = self._items(views)
         if images is None:
+            dev = next(t for t in (albedo, roughness, scattering) if t is not None).device
             images = torch.empty((len(views), views[0]["height"], views[0]["width"], 3), dtype=torch.float32,
-                                 device=(albedo if albedo is not None else roughness).device)
-        call("orf_diff_render", self.handle, ctypes.byref(desc), arr, len(views), ptr(albedo), ptr(roughness),
-             ptr(images), stream_ptr(stream))
+                                 device=dev)
+        call("orf_diff_render_ex", self.handle, ctypes.byref(desc), arr, len(views), ptr(albedo), ptr(roughness),
+             ptr(scattering), ptr(images), stream_ptr(stream))
         return images
 
     def diff_adjoint(self, desc, views, albedo, roughness, d_images, g_albedo=None, g_roughness=None,
@@ -556,6 +561,18 @@ class Context:
         call("orf_diff_adjoint", self.handle, ctypes.byref(desc), arr, len(views), ptr(albedo), ptr(roughness),
              ptr(d_images), ptr(g_albedo), ptr(g_roughness), frame_shift, stream_ptr(stream))
         return g_albedo, g_roughness
+
+    def diff_adjoint_sss(self, desc, views, albedo, roughness, scattering, d_images, frame_shift=0x40000000,
+                         stream=None):
+        """dLoss/d(textures, scattering coefficients) by orf_diff_adjoint_ex (fresh zeroed buffers)."""
+        arr = self._items(views)
+        g_albedo = torch.zeros_like(albedo) if albedo is not None else None
+        g_roughness = torch.zeros_like(roughness) if roughness is not None else None
+        g_sss = torch.zeros(6, dtype=torch.float32, device=scattering.device)
+        call("orf_diff_adjoint_ex", self.handle, ctypes.byref(desc), arr, len(views), ptr(albedo), ptr(roughness),
+             ptr(scattering), ptr(d_images), ptr(g_albedo), ptr(g_roughness), ptr(g_sss), frame_shift,
+             stream_ptr(stream))
+        return g_albedo, g_roughness, g_sss
 
     SSS_FIELDS = ("absorption", "scattering", "meancosine", "extinction", "reducedExtinction",
                   "deltaEddExtinction", "D", "transport", "C_phi", "C_phi_inv", "C_E", "A", "albedo",

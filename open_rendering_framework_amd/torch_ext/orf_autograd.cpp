@@ -1,7 +1,7 @@
 // orf_autograd.cpp -- libtorch autograd Function over the differentiable C-ABI.
 //
-// forward : orf_diff_render   (textures -> per-view images)
-// backward: orf_diff_adjoint  (dLoss/dImages -> dLoss/dTextures, path replay)
+// forward : orf_diff_render_ex   (textures, scattering coefficients -> per-view images)
+// backward: orf_diff_adjoint_ex  (dLoss/dImages -> dLoss/dTextures, dLoss/dCoefficients, path replay)
 // Both run on the current HIP stream of the textures' device; the render
 // context, texture mapping and views live in a Setup owned by Python.
 #include <torch/extension.h>
@@ -38,18 +38,21 @@ void check_tex(const torch::Tensor& t, int64_t n, const char* what)
 class OrfRender : public torch::autograd::Function<OrfRender> {
 public:
     static torch::Tensor forward(torch::autograd::AutogradContext* actx, torch::Tensor albedo, torch::Tensor rough,
-                                 int64_t setup)
+                                 torch::Tensor scattering, int64_t setup)
     {
         auto* s = reinterpret_cast<Setup*>(setup);
         int64_t texels = (int64_t)s->desc.tex_w * s->desc.tex_h;
         check_tex(albedo, texels * 3, "albedo");
         check_tex(rough, texels, "roughness");
+        const bool sss = s->desc.scattering_material >= 0;
+        if (sss) check_tex(scattering, 6, "scattering coefficients");
         int64_t nv = (int64_t)s->views.size();
         auto img = torch::empty({nv, s->views[0].height, s->views[0].width, 3}, albedo.options());
         hipStream_t st = c10::hip::getCurrentHIPStream(albedo.device().index()).stream();
-        check(orf_diff_render(s->ctx, &s->desc, s->views.data(), (int32_t)nv, albedo.data_ptr<float>(),
-                              rough.data_ptr<float>(), img.data_ptr<float>(), st));
-        actx->save_for_backward({albedo, rough});
+        check(orf_diff_render_ex(s->ctx, &s->desc, s->views.data(), (int32_t)nv, albedo.data_ptr<float>(),
+                                 rough.data_ptr<float>(), sss ? scattering.data_ptr<float>() : nullptr,
+                                 img.data_ptr<float>(), st));
+        actx->save_for_backward({albedo, rough, sss ? scattering : torch::zeros({6}, albedo.options())});
         actx->saved_data["setup"] = setup;
         return img;
     }
@@ -58,23 +61,27 @@ public:
                                                  torch::autograd::tensor_list grads)
     {
         auto saved = actx->get_saved_variables();
-        auto albedo = saved[0], rough = saved[1];
+        auto albedo = saved[0], rough = saved[1], scattering = saved[2];
         auto* s = reinterpret_cast<Setup*>(actx->saved_data["setup"].toInt());
+        const bool sss = s->desc.scattering_material >= 0;
         auto gi = grads[0].contiguous().to(torch::kFloat32);
-        auto ga = torch::zeros_like(albedo), gr = torch::zeros_like(rough);
+        auto ga = torch::zeros_like(albedo), gr = torch::zeros_like(rough), gs = torch::zeros_like(scattering);
         hipStream_t st = c10::hip::getCurrentHIPStream(albedo.device().index()).stream();
-        check(orf_diff_adjoint(s->ctx, &s->desc, s->views.data(), (int32_t)s->views.size(), albedo.data_ptr<float>(),
-                               rough.data_ptr<float>(), gi.data_ptr<float>(), ga.data_ptr<float>(),
-                               gr.data_ptr<float>(), s->frame_shift, st));
-        return {ga, gr, torch::Tensor()};
+        check(orf_diff_adjoint_ex(s->ctx, &s->desc, s->views.data(), (int32_t)s->views.size(),
+                                  albedo.data_ptr<float>(), rough.data_ptr<float>(),
+                                  sss ? scattering.data_ptr<float>() : nullptr, gi.data_ptr<float>(),
+                                  ga.data_ptr<float>(), gr.data_ptr<float>(), sss ? gs.data_ptr<float>() : nullptr,
+                                  s->frame_shift, st));
+        return {ga, gr, sss ? gs : torch::Tensor(), torch::Tensor()};
     }
 };
 
-// desc: [albedo_material, albedo_uv, ax0, az0, ax1, az1, rough_material, rough_uv, rx0, ry0, rz0, r3, tex_w, tex_h]
+// desc: [albedo_material, albedo_uv, ax0, az0, ax1, az1, rough_material, rough_uv, rx0, ry0, rz0, r3, tex_w, tex_h,
+//        scattering_material (optional, -1: none)]
 // views: [n, 16] = width, height, spp, frame0, eye[3], U[3], V[3], W[3]
 int64_t make_setup(int64_t ctx, std::vector<double> desc, std::vector<std::vector<double>> views, int64_t frame_shift)
 {
-    TORCH_CHECK(desc.size() == 14, "desc must have 14 entries");
+    TORCH_CHECK(desc.size() == 14 || desc.size() == 15, "desc must have 14 or 15 entries");
     TORCH_CHECK(!views.empty(), "at least one view");
     auto s = std::make_unique<Setup>();
     s->ctx = reinterpret_cast<orf_context>(ctx);
@@ -86,6 +93,7 @@ int64_t make_setup(int64_t ctx, std::vector<double> desc, std::vector<std::vecto
     for (int k = 0; k < 4; ++k) s->desc.roughness_uv_params[k] = (float)desc[8 + k];
     s->desc.tex_w = (int32_t)desc[12];
     s->desc.tex_h = (int32_t)desc[13];
+    s->desc.scattering_material = desc.size() == 15 ? (int32_t)desc[14] : -1;
     for (auto& v : views) {
         TORCH_CHECK(v.size() == 16, "a view has 16 entries");
         orf_render_item it{};
@@ -109,7 +117,13 @@ void free_setup(int64_t setup) { delete reinterpret_cast<Setup*>(setup); }
 
 torch::Tensor render(torch::Tensor albedo, torch::Tensor rough, int64_t setup)
 {
-    return OrfRender::apply(albedo, rough, setup);
+    // no scattering parameter: an empty tensor on the textures' device stands in for it
+    return OrfRender::apply(albedo, rough, torch::zeros({0}, albedo.options()), setup);
+}
+
+torch::Tensor render_sss(torch::Tensor albedo, torch::Tensor rough, torch::Tensor scattering, int64_t setup)
+{
+    return OrfRender::apply(albedo, rough, scattering, setup);
 }
 
 }  // namespace
@@ -120,4 +134,5 @@ PYBIND11_MODULE(TORCH_EXTENSION_NAME, m)
     m.def("make_setup", &make_setup);
     m.def("free_setup", &free_setup);
     m.def("render", &render);
+    m.def("render_sss", &render_sss);
 }

@@ -84,6 +84,11 @@ typedef struct {
     R* rough_frozen;
     /* translucent materials: coefficients per material, probes */
     osss* sss;
+    /* differentiable form (new): material sss_diff_mat shades with sss[] (the
+     * evaluated coefficients) and samples -- gather exit points, extinction
+     * roulette -- with sss_frozen */
+    int32_t sss_diff_mat;
+    osss sss_frozen;
     int32_t nprobes, probe_samples, probe_mat;
     R probe_cell;
     v3* probe_pos;
@@ -1226,14 +1231,16 @@ static R three_C2(R n)
 static R C_phi(R ni) { return (R)0.25f * ((R)1.0f - two_C1(ni)); }
 static R C_E(R ni) { return (R)0.5f * ((R)1.0f - three_C2(ni)); }
 
-static osss sss_coefficients(const orc_material* m)
+/* Coefficients of material m with its absorption and scattering (1/mm) given
+ * as sa[3], ss[3] (the differentiable form's parameters). */
+static osss sss_coefficients_with(const orc_material* m, const double* sa, const double* ss)
 {
     osss p;
     memset(&p, 0, sizeof(p));
     R scale = (R)m->sss_scale;
     p.dipole_model = m->dipole_model;
-    p.absorption = vscale(V3(m->sss_absorption[0], m->sss_absorption[1], m->sss_absorption[2]), scale);
-    p.scattering = vscale(V3(m->sss_scattering[0], m->sss_scattering[1], m->sss_scattering[2]), scale);
+    p.absorption = vscale(V3((R)sa[0], (R)sa[1], (R)sa[2]), scale);
+    p.scattering = vscale(V3((R)ss[0], (R)ss[1], (R)ss[2]), scale);
     p.meancosine = V3(m->sss_meancosine[0], m->sss_meancosine[1], m->sss_meancosine[2]);
     p.relative_ior = (R)m->ior;
     R ior = (R)m->ior;
@@ -1259,6 +1266,16 @@ static osss sss_coefficients(const orc_material* m)
     p.one_over_three_ext = vdiv(V3(1, 1, 1), vscale(p.extinction, (R)3.0f));
     p.mean_transport = (p.transport.x + p.transport.y + p.transport.z) / (R)3.0f;
     return p;
+}
+
+static osss sss_coefficients(const orc_material* m)
+{
+    double sa[3], ss[3];
+    for (int k = 0; k < 3; k++) {
+        sa[k] = (double)m->sss_absorption[k];
+        ss[k] = (double)m->sss_scattering[k];
+    }
+    return sss_coefficients_with(m, sa, ss);
 }
 
 static v3 dipole_bssrdf(R dist, const osss* p)
@@ -1398,21 +1415,41 @@ static int probe_collect(const oscene* s, v3 o, v3 d, R tmax, int m, ohit* out)
  * member (a pointer to a struct's first member converts back to the struct). */
 static uint32_t* path_s64(uint32_t* t) { return ((orc_rng*)t)->s64; }
 
+/* The coefficients that drive sampling for material P (the frozen ones of the
+ * differentiated material, else P itself). */
+static const osss* sss_sampling(const oscene* s, const osss* P)
+{
+    return s->sss_diff_mat >= 0 && P == &s->sss[s->sss_diff_mat] ? &s->sss_frozen : P;
+}
+
+/* Roulette probability of the extinction weight `beam` over distance t: its
+ * channel mean, from the sampling coefficients. */
+static R beam_prob(const oscene* s, const osss* P, R t, v3 beam)
+{
+    const osss* Q = sss_sampling(s, P);
+    if (Q != P) {
+        v3 ext = Q->dipole_model == 1 ? Q->deltaEddExtinction : Q->extinction;
+        beam = V3(F_EXP((-t) * ext.x), F_EXP((-t) * ext.y), F_EXP((-t) * ext.z));
+    }
+    return (beam.x + beam.y + beam.z) / (R)3.0f;
+}
+
 /* The gather's draws come from the 64-bit stream, as the reference's gather
  * loop draws its roulette numbers (CU/subsurface_scattering_shader.cu:152-156). */
 static v3 sss_gather(const oscene* s, const osss* P, v3 xo, v3 no, uint32_t* s64)
 {
     v3 acc = V3(0, 0, 0);
     if (s->nprobes <= 0 || s->probe_samples <= 0) return acc;
+    const osss* Q = sss_sampling(s, P); /* exit-point sampling and its pdf */
     v3 t1, t2;
     orc_create_onb(no, &t1, &t2);
-    R smin = fmin(P->transport.x, fmin(P->transport.y, P->transport.z));
+    R smin = fmin(Q->transport.x, fmin(Q->transport.y, Q->transport.z));
     R r_max = SSS_RADIUS_SIGMAS / smin;
     for (int k = 0; k < s->probe_samples; k++) {
         R u = orc_rnd_accurate(s64), v = orc_rnd_accurate(s64), w = orc_rnd_accurate(s64), sx = orc_rnd_accurate(s64);
         int c = (int)((R)3.0f * u);
         if (c > 2) c = 2;
-        R sig = c == 0 ? P->transport.x : (c == 1 ? P->transport.y : P->transport.z);
+        R sig = c == 0 ? Q->transport.x : (c == 1 ? Q->transport.y : Q->transport.z);
         R r = -F_LOG((R)1.0f - v) / sig;
         if (!(r < r_max)) continue;
         R ang = (R)6.28318530717958647692f * w;
@@ -1433,8 +1470,8 @@ static v3 sss_gather(const oscene* s, const osss* P, v3 xo, v3 no, uint32_t* s64
         v3 dx = vsub(xi, xo);
         v3 inpl = vsub(dx, vscale(no, vdot(dx, no)));
         R rp = sqrt(vdot(inpl, inpl));
-        R pr_r = (P->transport.x * F_EXP(-P->transport.x * rp) + P->transport.y * F_EXP(-P->transport.y * rp) +
-                  P->transport.z * F_EXP(-P->transport.z * rp)) /
+        R pr_r = (Q->transport.x * F_EXP(-Q->transport.x * rp) + Q->transport.y * F_EXP(-Q->transport.y * rp) +
+                  Q->transport.z * F_EXP(-Q->transport.z * rp)) /
                  ((R)3.0f * (R)6.28318530717958647692f * rp);
         R pdf = pr_r * cosn / (R)nh;
         if (!(pdf > 0) || !isfinite(pdf)) continue;
@@ -1470,7 +1507,7 @@ static v3 shade_translucent(const oscene* s, int mi, v3 o, v3 d, const ohit* h, 
     if (inside) {
         v3 ext = P->dipole_model == 1 ? P->deltaEddExtinction : P->extinction;
         beam = V3(F_EXP((-h->t) * ext.x), F_EXP((-h->t) * ext.y), F_EXP((-h->t) * ext.z));
-        R prob = (beam.x + beam.y + beam.z) / (R)3.0f;
+        R prob = beam_prob(s, P, h->t, beam);
         if (orc_rnd_accurate(t64) >= prob) return zero;
         beam = V3(beam.x / prob, beam.y / prob, beam.z / prob);
         recip = P->relative_ior;
@@ -1608,7 +1645,7 @@ static v3 shade_rough_translucent(const oscene* s, int mi, v3 o, v3 d, const ohi
     if (inside) {
         v3 ext = P->dipole_model == 1 ? P->deltaEddExtinction : P->extinction;
         beam = V3(F_EXP((-h->t) * ext.x), F_EXP((-h->t) * ext.y), F_EXP((-h->t) * ext.z));
-        R prob = (beam.x + beam.y + beam.z) / (R)3.0f;
+        R prob = beam_prob(s, P, h->t, beam);
         if (orc_rnd_accurate(t64) >= prob) return result;
         beam = V3(beam.x / prob, beam.y / prob, beam.z / prob);
         recip = P->relative_ior;
@@ -1713,6 +1750,7 @@ static v3 pixel_sample(const oscene* s, int W, int H, int x, int y, uint32_t fra
 void* orc_scene_create(const orc_scene_desc* d)
 {
     oscene* s = (oscene*)calloc(1, sizeof(oscene));
+    s->sss_diff_mat = -1;
     s->ntris = d->ntris;
     s->p0 = (v3*)malloc(sizeof(v3) * (size_t)(d->ntris + 1));
     s->e1 = (v3*)malloc(sizeof(v3) * (size_t)(d->ntris + 1));
@@ -2054,6 +2092,22 @@ double orc_diff_eval(void* p, int W, int H, int nviews, const float* cams, const
     return loss;
 }
 
+/* Differentiable scattering coefficients of material m (new): eval6 and frozen6
+ * are (absorption RGB, scattering RGB) in 1/mm; the material shades with eval6
+ * and samples with frozen6 (NULL: eval6).  m < 0 restores every material's own
+ * coefficients.  The probes keep the values they hold. */
+void orc_diff_set_sss(void* p, int m, const double* eval6, const double* frozen6)
+{
+    oscene* s = (oscene*)p;
+    if (s->sss_diff_mat >= 0) s->sss[s->sss_diff_mat] = sss_coefficients(&s->mats[s->sss_diff_mat]);
+    s->sss_diff_mat = -1;
+    if (m < 0 || !eval6) return;
+    const double* f = frozen6 ? frozen6 : eval6;
+    s->sss[m] = sss_coefficients_with(&s->mats[m], eval6, eval6 + 3);
+    s->sss_frozen = sss_coefficients_with(&s->mats[m], f, f + 3);
+    s->sss_diff_mat = m;
+}
+
 /* ---------------------------------------------------------------- translucent API */
 
 void orc_probes_set(void* p, int n, const float* pos, const float* nrm, float cell, int samples, int material)
@@ -2082,13 +2136,14 @@ void orc_probes_set(void* p, int n, const float* pos, const float* nrm, float ce
     }
 }
 
-void orc_probes_values(void* p, const float* L, const float* w12)
+void orc_probes_values(void* p, const float* L, const float* w12, const float* sn)
 {
     oscene* s = (oscene*)p;
     for (int j = 0; j < s->nprobes; j++) {
         s->probe_L[j] = V3(L[4 * j], L[4 * j + 1], L[4 * j + 2]);
         s->probe_w12[j] = V3(w12[4 * j], w12[4 * j + 1], w12[4 * j + 2]);
         s->probe_T12[j] = (R)w12[4 * j + 3];
+        if (sn) s->probe_sn[j] = V3(sn[4 * j], sn[4 * j + 1], sn[4 * j + 2]);
     }
 }
 

@@ -109,10 +109,11 @@ void orc_closest_hits(void* s, int n, const float* o, const float* d, float tmin
 /* Translucent material (new estimator, see oracle_render.c): probes given by the
  * caller (4 floats per probe), nearest-probe search radius `cell`, gather samples
  * per entering hit; probe pass for sample index `frame` (outputs may be NULL);
- * probe values may also be loaded directly. */
+ * probe values (L, w12 with T12 in w, and the entry normals sn, which may be NULL)
+ * may also be loaded directly, 4 floats per probe each. */
 void orc_probes_set(void* s, int n, const float* pos, const float* nrm, float cell, int samples, int material);
 void orc_probe_pass(void* s, uint32_t frame, float* L, float* w12, float* sn, int threads);
-void orc_probes_values(void* s, const float* L, const float* w12);
+void orc_probes_values(void* s, const float* L, const float* w12, const float* sn);
 /* Derived coefficients of material m (64 floats, the product's field order). */
 void orc_sss_coefficients(void* s, int m, float* out);
 /* Standard / directional dipole of material m: standard at distances r[i];
@@ -133,6 +134,10 @@ void orc_sss_gather_eval(void* s, int m, const float* xo, const float* no, int r
 void orc_diff_set(void* s, int tex_w, int tex_h, int alb_mat, int alb_uv, const double* alb_uvp, int rough_mat,
                   int rough_uv, const double* rough_uvp, const double* albedo_eval, const double* albedo_frozen,
                   const double* rough_eval, const double* rough_frozen);
+/* Scattering coefficients of material m as parameters: shade with eval6, sample
+ * (gather exit points, extinction roulette) with frozen6 (absorption RGB,
+ * scattering RGB in 1/mm); m < 0 restores the materials' own. */
+void orc_diff_set_sss(void* s, int m, const double* eval6, const double* frozen6);
 double orc_diff_eval(void* s, int W, int H, int nviews, const float* cams, const uint32_t* frame0, uint32_t spp,
                      uint32_t shift, const double* dimg, double* images, int threads);
 

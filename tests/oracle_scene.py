@@ -73,7 +73,7 @@ class OracleScene:
         L.orc_closest_hits.argtypes = [c_vp, ctypes.c_int, c_vp, c_vp, ctypes.c_float, c_vp, c_vp, ctypes.c_int]
         L.orc_probes_set.argtypes = [c_vp, ctypes.c_int, c_vp, c_vp, ctypes.c_float, ctypes.c_int, ctypes.c_int]
         L.orc_probe_pass.argtypes = [c_vp, ctypes.c_uint32, c_vp, c_vp, c_vp, ctypes.c_int]
-        L.orc_probes_values.argtypes = [c_vp, c_vp, c_vp]
+        L.orc_probes_values.argtypes = [c_vp, c_vp, c_vp, c_vp]
         L.orc_sss_coefficients.argtypes = [c_vp, ctypes.c_int, c_vp]
         L.orc_dipole_eval.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, c_vp, c_vp]
         L.orc_dirpole_eval.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, c_vp, c_vp]
@@ -82,6 +82,7 @@ class OracleScene:
         L.orc_diff_set.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_vp, ctypes.c_int,
                                    ctypes.c_int, c_vp, c_vp, c_vp, c_vp, c_vp]
         L.orc_diff_eval.restype = ctypes.c_double
+        L.orc_diff_set_sss.argtypes = [c_vp, ctypes.c_int, c_vp, c_vp]
         L.orc_diff_eval.argtypes = [c_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, c_vp, c_vp, ctypes.c_uint32,
                                     ctypes.c_uint32, c_vp, c_vp, ctypes.c_int]
         self.desc = desc
@@ -262,6 +263,15 @@ class OracleScene:
         self.lib.orc_diff_set(self.handle, tw, th, albedo_material, albedo_uv[0], _p(au), roughness_material,
                               roughness_uv[0], _p(ru), _p(a), _p(af), _p(r), _p(rf))
 
+    def diff_set_sss(self, material, eval6, frozen6=None):
+        """Absorption RGB + scattering RGB (1/mm) of a translucent material as
+        parameters: shade with eval6, sample with frozen6 (default eval6);
+        material < 0 restores the materials' own coefficients."""
+        e = None if eval6 is None else np.ascontiguousarray(eval6, np.float64)
+        f = None if frozen6 is None else np.ascontiguousarray(frozen6, np.float64)
+        self._sss_keep = (e, f)
+        self.lib.orc_diff_set_sss(self.handle, material, _p(e), _p(f))
+
     def diff_eval(self, views, shift=0, dimg=None, images=True, threads=8):
         """(loss, images): per-view mean images over frames frame0 + shift + s and
         the linear loss sum(dimg * sum_s L_s / spp)."""
@@ -292,8 +302,9 @@ class OracleScene:
         self.lib.orc_probe_pass(self.handle, frame, _p(L), _p(w), _p(sn), threads)
         return (L, w, sn) if normals else (L, w)
 
-    def probes_values(self, L, w12):
-        self._probe_vals = [np.ascontiguousarray(L, np.float32), np.ascontiguousarray(w12, np.float32)]
+    def probes_values(self, L, w12, sn=None):
+        self._probe_vals = [np.ascontiguousarray(L, np.float32), np.ascontiguousarray(w12, np.float32),
+                            None if sn is None else np.ascontiguousarray(sn, np.float32)]
         self.lib.orc_probes_values(self.handle, *[_p(a) for a in self._probe_vals])
 
     def sss_coefficients(self, material):

@@ -102,3 +102,49 @@ def test_loss_is_linear_in_image_gradient():
     _, img = orc.diff_eval(views, shift=7)
     # loss == sum(dimg * image) when every sample is finite
     np.testing.assert_allclose(l1, (g1 * img).sum(), rtol=1e-9)
+
+
+SSS0 = np.array([0.0011, 0.0024, 0.014, 2.55, 3.21, 3.77])
+MILK = 1
+
+
+def _milk_scene(theta):
+    d = scenes.subsurface_scene(width=W, height=W, level=3, glass_level=1, env_size=(64, 32), probes=0)
+    d.materials[MILK].floats["absorption"] = tuple(theta[:3])
+    d.materials[MILK].floats["scattering"] = tuple(theta[3:])
+    return d
+
+
+def _milk_oracle(theta, probes):
+    o = OracleScene(_milk_scene(theta))
+    o.probes_set(probes, 4, MILK)
+    o.probes_values(probes["L"], probes["w12"], probes["sn"])
+    return o
+
+
+def test_scattering_reweighting_is_unbiased():
+    """The translucent material's coefficients as parameters: shaded at theta1,
+    with the gather's exit points and the extinction roulette drawn at theta0,
+    the estimator's mean is the plain render's at theta1 (same probe values on
+    both sides, the probes being frozen); and differs from the theta0 render."""
+    theta1 = SSS0 * np.array([8.0, 8.0, 8.0, 0.6, 0.6, 0.6])
+    m = _milk_scene(SSS0).meshes[MILK]
+    n = len(m.verts)
+    area = 0.5 * np.linalg.norm(np.cross(m.verts[m.tris[:, 1]] - m.verts[m.tris[:, 0]],
+                                         m.verts[m.tris[:, 2]] - m.verts[m.tris[:, 0]]), axis=1).sum()
+    pad = lambda a: np.concatenate([a, np.zeros((len(a), 1), np.float32)], 1).astype(np.float32)  # noqa: E731
+    probes = {"pos": pad(m.verts), "nrm": pad(m.normals), "cell": float(2.5 * np.sqrt(area / n))}
+    src = OracleScene(_milk_scene(SSS0))
+    src.probes_set(probes, 4, MILK)
+    L, w12, sn = src.probe_pass(0, normals=True)
+    probes.update(L=L, w12=w12, sn=sn)
+    views = scenes.ring_views(n=1, size=W, spp=16, radius=2.5, height=0.8)
+    rew = _milk_oracle(SSS0, probes)
+    rew.diff_set_sss(MILK, theta1, SSS0)
+    plain1 = _milk_oracle(theta1, probes)
+    plain0 = _milk_oracle(SSS0, probes)
+    m_rew, se_rew = _mean_and_se(rew, views, 16)
+    m_1, se_1 = _mean_and_se(plain1, views, 16)
+    m_0, se_0 = _mean_and_se(plain0, views, 16)
+    assert abs(m_rew - m_1) < 4.0 * np.hypot(se_rew, se_1) + 1e-7, (m_rew, m_1, se_rew, se_1)
+    assert abs(m_0 - m_1) > 4.0 * np.hypot(se_0, se_1), (m_0, m_1)
