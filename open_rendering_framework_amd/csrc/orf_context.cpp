@@ -348,12 +348,8 @@ void prepare(orf_context_t* c, hipStream_t stream)
         for (size_t i = 0; i < m.size(); ++i) {
             sss.push_back(scattering_coefficients(c->mat_sss[i]));
             m[i].sss = is_translucent(m[i].kind) ? (int)i : -1;
-            const bool ms = m[i].microfacet_model == MULTISCATTERING_MODEL;
-            ORF_REQUIRE(!(ms && m[i].kind == MAT_ROUGH_DIFFUSE),
-                        "material " + std::to_string(i) + ": the rough diffuse multiple-scattering walk is not built");
-            ORF_REQUIRE(!(ms && m[i].normal_distribution == BECKMANN_DISTRIBUTION && m[i].kind == MAT_ROUGH_TRANSLUCENT),
-                        "material " + std::to_string(i) +
-                            ": the Beckmann multiple-scattering walk of the rough translucent boundary is not built");
+            // the rough diffuse interface reflects as a conductor of index 1 / ior
+            if (m[i].kind == MAT_ROUGH_DIFFUSE) dielectric_eta_host(1.0f / m[i].ior, m[i].eta);
         }
         if (m.empty()) m.push_back(default_material(MAT_LAMBERT));
         if (sss.empty()) sss.push_back(scattering_coefficients(SssParamsHost()));

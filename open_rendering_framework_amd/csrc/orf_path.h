@@ -273,14 +273,19 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         return false;
     }
     if (kDiel && mat.kind == MAT_ROUGH_DIFFUSE) {
-        // CU/rough_diffuse_shader.cu:40-290 (visible-normal and Walter models
-        // alike; the multiple-scattering walk is refused at build): a Lambert
-        // base under a rough dielectric interface.  NEE adds the transmitted
-        // diffuse and the microfacet-reflected terms (shadow segment to the full
-        // light distance, emission off once a light is seen); the continuation
-        // is the diffuse bounce about the sampled microfacet normal with
-        // probability 1 - R_i, else the mirror about it, weighted as the
-        // reference weights them (not divided by the choice probability).
+        // CU/rough_diffuse_shader.cu:40-290: a Lambert base under a rough
+        // dielectric interface.  NEE adds the transmitted diffuse and the
+        // microfacet-reflected terms (shadow segment to the full light distance,
+        // emission off once a light is seen); the continuation is the diffuse
+        // bounce about the sampled microfacet normal with probability 1 - R_i,
+        // else the mirror about it, weighted as the reference weights them (not
+        // divided by the choice probability).  The multiple-scattering model
+        // replaces both NEE terms by walk estimates (diffuse and interface
+        // reflection, drawn before the shadow test as the rough translucent
+        // boundary's are) and both continuations by walks: the diffuse walk
+        // (weight rho^k n12^2 T / (1 - rho r10), T at its last visible normal)
+        // and the reflection walk with the interface as a conductor of index
+        // n12, weight 1, in the un-flipped normal's frame (:252-268).
         if (ps.depth > sc.max_depth) return false;
         const f3 w_i = -d;
         const float n12 = 1.0f / mat.ior;
@@ -289,6 +294,7 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         const int nd = mat.normal_distribution;
         const f3 rho = mat.diffuse_color;
         const f3 denom = mk3(1.0f - rho.x * r10, 1.0f - rho.y * r10, 1.0f - rho.z * r10);
+        const bool ms = mat.microfacet_model == MULTISCATTERING_MODEL;
         const float z1 = rnd_tea(ps.rng);
         const float z2 = rnd_tea(ps.rng);
         const f3 mn = sample_visible_normal(w_i, ffn, ax, ay, z1, z2, nd);
@@ -308,34 +314,64 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
             const float cl = dot(ffn, wl);
             // the environment reaches this material through escaping rays only
             if (cl > 0.0f && penv < 0.0f) {
-                const f3 rn = normalize(w_i + wl);
-                const float cr = dot(rn, wl);
-                float Rr = 1.0f;
-                st2 = n12 * n12 * (1.0f - cr * cr);
-                if (st2 < 1.0f) Rr = fresnel_R3(cr, sqrtf(1.0f - st2), n12);
-                const float cd = dot(mn, wl);
-                float Rd = 1.0f;
-                st2 = n12 * n12 * (1.0f - cd * cd);
-                if (st2 < 1.0f) Rd = fresnel_R3(cd, sqrtf(1.0f - st2), n12);
-                const float Td = 1.0f - Rd;
-                const float gi = masking_g1(w_i, rn, ffn, ax, ay, nd);
-                const float go = masking_g1(wl, rn, ffn, ax, ay, nd);
-                const float god = masking_g1(wl, mn, ffn, ax, ay, nd);
-                const float D = eval_visible_normal(w_i, rn, ffn, ax, ay, nd);
-                const float wr = gi * go * D / (4.0f * fabsf(dot(w_i, ffn) * dot(wl, ffn)));
-                const f3 diff = rad * kInvPi * cl * rho * n12 * n12 * Ti * Td / denom * god;
-                const f3 spec = rad * Rr * wr;
+                f3 c;
+                if (ms) {
+                    // :126-139
+                    const f3 lwi = to_local(w_i, ffn), lwl = to_local(wl, ffn);
+                    const f3 fd = ms_diffuse_eval(nd, lwi, lwl, ax, ay, rho, ps.rng);
+                    c = rad * (fd + ms_conductor_eval(nd, lwi, lwl, mat.eta, ax, ay, ps.rng));
+                } else {
+                    const f3 rn = normalize(w_i + wl);
+                    const float cr = dot(rn, wl);
+                    float Rr = 1.0f;
+                    st2 = n12 * n12 * (1.0f - cr * cr);
+                    if (st2 < 1.0f) Rr = fresnel_R3(cr, sqrtf(1.0f - st2), n12);
+                    const float cd = dot(mn, wl);
+                    float Rd = 1.0f;
+                    st2 = n12 * n12 * (1.0f - cd * cd);
+                    if (st2 < 1.0f) Rd = fresnel_R3(cd, sqrtf(1.0f - st2), n12);
+                    const float Td = 1.0f - Rd;
+                    const float gi = masking_g1(w_i, rn, ffn, ax, ay, nd);
+                    const float go = masking_g1(wl, rn, ffn, ax, ay, nd);
+                    const float god = masking_g1(wl, mn, ffn, ax, ay, nd);
+                    const float D = eval_visible_normal(w_i, rn, ffn, ax, ay, nd);
+                    const float wr = gi * go * D / (4.0f * fabsf(dot(w_i, ffn) * dot(wl, ffn)));
+                    const f3 diff = rad * kInvPi * cl * rho * n12 * n12 * Ti * Td / denom * god;
+                    c = diff + rad * Rr * wr;
+                }
                 sr.o = p;
                 sr.d = wl;
                 sr.tmax = dist;  // the reference's shadow ray runs to the light itself (:96)
-                sr.contrib = ps.thr * (diff + spec);
+                sr.contrib = ps.thr * c;
                 sr.emit = 0;
                 sr.valid = true;
             }
         }
         const float xi = rnd_tea(ps.rng);
         f3 nd_dir, w;
-        if (xi > Ri) {
+        if (ms) {
+            f3 lo;
+            if (xi > Ri) {
+                // :187-216
+                f3 lm, k;
+                ms_diffuse_sample(nd, to_local(w_i, ffn), ax, ay, rho, ps.rng, lo, lm, k);
+                nd_dir = to_world(lo, ffn);
+                const float cr = dot(nd_dir, to_world(lm, ffn));
+                float Rr = 1.0f;
+                st2 = n12 * n12 * (1.0f - cr * cr);
+                if (st2 < 1.0f) Rr = fresnel_R3(cr, sqrtf(1.0f - st2), n12);
+                w = k * (n12 * n12 * (1.0f - Rr)) / denom;
+            } else {
+                // :252-268
+                f3 k;
+                const bool ok = nd == GGX_DISTRIBUTION
+                                    ? ggx_ms_conductor_sample(to_local(w_i, normal), mat.eta, ax, ay, ps.rng, lo, k)
+                                    : beckmann_ms_conductor_sample(to_local(w_i, normal), mat.eta, ax, ay, ps.rng, lo, k);
+                if (!ok) return false;
+                nd_dir = to_world(lo, normal);
+                w = mk3(1.0f);
+            }
+        } else if (xi > Ri) {
             nd_dir = sample_cosine(mn, ps.rng);
             const float cr = dot(nd_dir, mn);
             const float gom = masking_g1(nd_dir, mn, ffn, ax, ay, nd);
@@ -554,7 +590,9 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
             if (!inside && cl > 0.0f) {
                 f3 c;
                 if (model == MULTISCATTERING_MODEL) {
-                    float f = ggx_ms_dielectric_eval(to_local(w_i, ffn), to_local(wl, ffn), recip, ax, ay, ps.rng);
+                    const f3 lwi = to_local(w_i, ffn), lwl = to_local(wl, ffn);
+                    float f = dist == GGX_DISTRIBUTION ? ggx_ms_dielectric_eval(lwi, lwl, recip, ax, ay, ps.rng)
+                                                       : beckmann_ms_dielectric_eval(lwi, lwl, recip, ax, ay, ps.rng);
                     c = rad * f;
                 } else {
                     f3 rn = normalize(w_i + wl);
@@ -582,7 +620,10 @@ ORF_HD bool shade_with(const SceneGPU& sc, const MaterialGPU& mat, PathState& ps
         if (model == MULTISCATTERING_MODEL) {
             // :360-455: the walk from the un-flipped normal's frame, weight 1
             f3 lo;
-            if (!ggx_ms_dielectric_sample(to_local(w_i, normal), 1.0f / P.relative_ior, ax, ay, ps.rng, lo))
+            const f3 lwi = to_local(w_i, normal);
+            const float eta = 1.0f / P.relative_ior;
+            if (!(dist == GGX_DISTRIBUTION ? ggx_ms_dielectric_sample(lwi, eta, ax, ay, ps.rng, lo)
+                                           : beckmann_ms_dielectric_sample(lwi, eta, ax, ay, ps.rng, lo)))
                 return false;
             nd = to_world(lo, normal);
             refract = dot(nd, ffn) < 0.0f;

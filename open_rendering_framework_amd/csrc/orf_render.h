@@ -88,5 +88,6 @@ void build_env_tables(const float4* tex, int w, int h, float* marg_pdf, float* m
 
 // Conductor constants as the reference forms eta = 1 / ior (ORF/MyComplex.h:155-160).
 void conductor_eta_host(const double ior_re[3], const double ior_im[3], ConductorEta& e);
+void dielectric_eta_host(float n12, ConductorEta& e);
 
 }  // namespace orf

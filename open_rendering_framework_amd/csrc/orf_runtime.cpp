@@ -28,6 +28,21 @@ void conductor_eta_host(const double ior_re[3], const double ior_im[3], Conducto
               ior_im[1] == ior_im[2]);
 }
 
+// A dielectric interface's real relative index n12 as a conductor's eta
+// (CU/rough_diffuse_shader.cu:132-133, :254-255 pass MyComplex{n1_over_n2, 0});
+// 1 / eta formed as operator/(float, MyComplex) forms it.
+void dielectric_eta_host(float n12, ConductorEta& e)
+{
+    const float ab = sqrtf(n12 * n12);
+    const double ie = (double)n12 * (1.0 / (double)powf(ab, 2.0f));
+    for (int c = 0; c < 3; ++c) {
+        e.eta[c] = cf{n12, 0.0f};
+        e.eta2[c] = cf{(float)((double)n12 * (double)n12), 0.0f};
+        e.ieta[c] = cf{(float)ie, 0.0f};
+        e.ieta2[c] = cf{(float)(ie * ie), 0.0f};
+    }
+    e.gray = 1;
+}
 
 static thread_local std::string g_last_error;
 

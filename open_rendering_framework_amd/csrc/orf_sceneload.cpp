@@ -8,7 +8,8 @@
 // (ORF/Geometry.cpp:73-131; transform T * Rz * Ry * Rx * R(axis, angle) * S,
 // :286-291), the material classes (mtl_name dispatch ORF/Geometry.cpp:186-216;
 // parameter names of DiffuseMaterial, MetallicMaterial, LambertianInterface-,
-// RoughTransparent-, Translucent- and RoughTranslucentMaterial::loadFromJSON)
+// RoughTransparent-, Transparent-, Normal-, Flat-, RoughDiffuse-, Translucent-
+// and RoughTranslucentMaterial::loadFromJSON)
 // and the lights (ORF/Light.cpp: PointLight :22-40, DirectionalLight :75-93,
 // TriangleAreaLight :149-290 with applyTransformationMatrix, DiskAreaLight
 // :367-505, SphericalLight :507-611).  Every scene
@@ -17,9 +18,9 @@
 // Differences: relative paths resolve against the JSON file's directory (the
 // reference resolves them against its build directory); keys a file omits
 // take this library's material defaults (the reference's JSON constructors
-// leave them unset); classes with no radiance program here (NormalMaterial,
-// TransparentMaterial, FlatMaterial, RoughDiffuseMaterial, Anisotropic-,
-// DepthTracer) are refused with an error naming them.
+// leave them unset); AnisotropicMaterial ("anisotropic_structure_shader"), the
+// one material class with no radiance program here, and integrators other than
+// PathTracer and DepthTracer are refused with an error naming them.
 #include <cctype>
 #include <cmath>
 #include <cstdio>

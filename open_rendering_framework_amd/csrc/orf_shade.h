@@ -553,6 +553,205 @@ ORF_HD float ggx_ms_dielectric_eval(f3 w_i, f3 w_o, float eta, float ax, float a
     return sum;
 }
 
+// ------------------------------------------------- walks of either distribution
+// The primitives above picked by the material's normal distribution, as the
+// wrappers of ORF/Microfacet.h:100-200 pick them.
+ORF_HD float ms_sample_height(int dist, f3 wr, float hr, float z, float ax, float ay)
+{
+    return dist == GGX_DISTRIBUTION ? ggx_sample_height(wr, hr, z, ax, ay) : beckmann_sample_height(wr, hr, z, ax, ay);
+}
+
+// ggx_G(wi, h) / beckmann_G(wi, h) (ORF/MicrofacetGGX.h:93-107,
+// ORF/MicrofacetBeckmann.h:108-127): C1(h)^Lambda(wi), 1 near the zenith, 0 below.
+ORF_HD float ms_masking_height(int dist, f3 wi, float h, float ax, float ay)
+{
+    if (wi.z > 0.9999f) return 1.0f;
+    if (wi.z <= 0.0f) return 0.0f;
+    const float lambda = dist == GGX_DISTRIBUTION ? ggx_smith_lambda(wi, ax, ay) : beckmann_smith_lambda(wi, ax, ay);
+    return orf_powf(height_cdf(h), lambda);
+}
+
+ORF_HD f3 ms_sample_visible(int dist, f3 wi, float ax, float ay, float z1, float z2)
+{
+    return dist == GGX_DISTRIBUTION ? ggx_sample_visible(wi, ax, ay, z1, z2) : beckmann_sample_visible(wi, ax, ay, z1, z2);
+}
+
+ORF_HD float ms_vndf(int dist, f3 wi, f3 wm, float ax, float ay)
+{
+    return dist == GGX_DISTRIBUTION ? ggx_vndf(wi, wm, ax, ay) : beckmann_vndf(wi, wm, ax, ay);
+}
+
+// ggx_sample_diffuse_phase / beckmann_sample_diffuse_phase (ORF/MicrofacetGGX.h:
+// 342-375, ORF/MicrofacetBeckmann.h:375-408): a visible normal w_m of w_i (draws
+// z1, z2), then a cosine-distributed direction about it (z3, z4) through the
+// concentric disk map of Shirley and Chiu 1997 in the make_frame basis of w_m,
+// not renormalised.
+ORF_HD f3 ms_sample_diffuse_phase(int dist, f3 w_i, float ax, float ay, uint32_t& rng, f3& w_m)
+{
+    const float z1 = rnd_tea(rng);
+    const float z2 = rnd_tea(rng);
+    const float z3 = rnd_tea(rng);
+    const float z4 = rnd_tea(rng);
+    w_m = ms_sample_visible(dist, w_i, ax, ay, z1, z2);
+    f3 u, v;
+    make_frame(w_m, u, v);
+    const float r1 = 2.0f * z3 - 1.0f, r2 = 2.0f * z4 - 1.0f;
+    float r, phi;
+    if (r1 == 0.0f && r2 == 0.0f) {
+        r = phi = 0.0f;
+    } else if (r1 * r1 > r2 * r2) {
+        r = r1;
+        phi = (kPi / 4.0f) * (r2 / r1);
+    } else {
+        r = r2;
+        phi = (kPi / 2.0f) - (r1 / r2) * (kPi / 4.0f);
+    }
+    float sp, cp;
+    orf_sincosf(phi, &sp, &cp);
+    const float x = r * cp, y = r * sp;
+    const float z = sqrtf(fmaxf(0.0f, 1.0f - x * x - y * y));
+    return u * x + v * y + w_m * z;
+}
+
+// Scattering events a rough diffuse walk may take: the reference's own bound
+// on its sampler (scatteringOrder < 100), also applied to the evaluation.
+constexpr int kMsDiffuseCap = 100;
+
+// ggx_multiscattering_diffuse_BSDF_sample / beckmann_... (ORF/MicrofacetGGX.h:
+// 519-544, ORF/MicrofacetBeckmann.h:613-640), local frame: the walk on a
+// Lambertian microsurface; every event multiplies the weight by rho.  w_m is
+// the last event's visible normal (the caller's Fresnel term uses it).  After
+// kMsDiffuseCap events the last sampled direction is returned as it is.
+ORF_HD void ms_diffuse_sample(int dist, f3 local_wi, float ax, float ay, f3 rho, uint32_t& rng, f3& w_out, f3& w_m,
+                              f3& weight)
+{
+    weight = mk3(1.0f);
+    w_m = mk3(0.0f, 0.0f, 1.0f);
+    float hr = 1.0f + height_inv_cdf(0.9999f);
+    f3 w_o = -local_wi;
+    for (int order = 0; order < kMsDiffuseCap;) {
+        const float u = rnd_tea(rng);
+        hr = ms_sample_height(dist, w_o, hr, u, ax, ay);
+        if (hr == kRayMax) break;
+        ++order;
+        weight = weight * rho;
+        w_o = ms_sample_diffuse_phase(dist, -w_o, ax, ay, rng, w_m);
+        if (hr != hr || w_o.z != w_o.z) {
+            w_o = mk3(0.0f, 0.0f, 1.0f);
+            break;
+        }
+    }
+    w_out = w_o;
+}
+
+// ggx_multiscattering_diffuse_BSDF_eval / beckmann_... (ORF/MicrofacetGGX.h:
+// 547-577, ORF/MicrofacetBeckmann.h:643-674): one walk estimate of BSDF x cos
+// for (w_i, w_o), the sum over the events of rho (once, as written there) x the
+// diffuse phase about a freshly drawn visible normal (max(0, w_o.w_m) / pi) x
+// the masking of w_o at the event's height; non-finite terms are skipped.
+// Truncated after kMsDiffuseCap events.
+ORF_HD f3 ms_diffuse_eval(int dist, f3 w_i, f3 w_o, float ax, float ay, f3 rho, uint32_t& rng)
+{
+    f3 w_r = -w_i;
+    float hr = 1.0f + height_inv_cdf(0.9999f);
+    f3 sum = mk3(0.0f);
+    for (int order = 0; order < kMsDiffuseCap; ++order) {
+        const float u = rnd_tea(rng);
+        hr = ms_sample_height(dist, w_r, hr, u, ax, ay);
+        if (hr == kRayMax) break;
+        const float z1 = rnd_tea(rng);
+        const float z2 = rnd_tea(rng);
+        const f3 wm = ms_sample_visible(dist, -w_r, ax, ay, z1, z2);
+        const float phase = kInvPi * fmaxf(0.0f, dot(w_o, wm));
+        const f3 I = rho * (phase * ms_masking_height(dist, w_o, hr, ax, ay));
+        if (__builtin_isfinite(I.x) && __builtin_isfinite(I.y) && __builtin_isfinite(I.z)) sum = sum + I;
+        f3 wm_s;
+        w_r = ms_sample_diffuse_phase(dist, -w_r, ax, ay, rng, wm_s);
+        if (hr != hr || w_r.z != w_r.z) return mk3(0.0f);
+    }
+    return sum;
+}
+
+// ggx_multiscattering_conductor_BSDF_eval / beckmann_... (ORF/MicrofacetGGX.h:
+// 580-611, ORF/MicrofacetBeckmann.h:677-708): the sum over the walk's events of
+// F(-w_r, h) x 0.25 D_{-w_r}(h) (h the half vector of -w_r and w_o, 0 when it
+// points below the mean plane) x the masking of w_o at the event's height, with
+// eta or 1 / eta as the arriving direction lies above or below; non-finite
+// terms are skipped.  Truncated after kMsWalkCap events, the orders the
+// conductor sampler keeps.
+ORF_HD f3 ms_conductor_eval(int dist, f3 w_i, f3 w_o, const ConductorEta& eta, float ax, float ay, uint32_t& rng)
+{
+    f3 w_r = -w_i;
+    float hr = 1.0f + height_inv_cdf(0.9999f);
+    f3 sum = mk3(0.0f);
+    for (int order = 0; order < kMsWalkCap; ++order) {
+        const float u = rnd_tea(rng);
+        hr = ms_sample_height(dist, w_r, hr, u, ax, ay);
+        if (hr == kRayMax) break;
+        const f3 w_t = -w_r;
+        const f3 h = normalize(w_t + w_o);
+        const float phase = h.z < 0.0f ? 0.0f : 0.25f * ms_vndf(dist, w_t, h, ax, ay);
+        const f3 F = w_t.z > 0.0f ? fresnel_conductor(w_t, h, eta) : fresnel_conductor_inv(w_t, h, eta);
+        const f3 I = F * (phase * ms_masking_height(dist, w_o, hr, ax, ay));
+        if (__builtin_isfinite(I.x) && __builtin_isfinite(I.y) && __builtin_isfinite(I.z)) sum = sum + I;
+        const float z1 = rnd_tea(rng);
+        const float z2 = rnd_tea(rng);
+        const f3 wm = ms_sample_visible(dist, w_t, ax, ay, z1, z2);
+        w_r = -w_t + (wm * 2.0f) * dot(w_t, wm);
+        if (hr != hr || w_r.z != w_r.z) return mk3(0.0f);
+    }
+    return sum;
+}
+
+// beckmann_eval_dielectric_phase (ORF/MicrofacetBeckmann.h:342-372): the GGX
+// form with the Beckmann visible-normal density.
+ORF_HD float beckmann_eval_dielectric_phase(f3 w_i, f3 w_o, bool wi_out, bool wo_out, float m_eta, float ax, float ay)
+{
+    float eta = wi_out ? m_eta : 1.0f / m_eta;
+    if (wi_out == wo_out) {
+        f3 w_h = normalize(w_i + w_o);
+        float D = wi_out ? beckmann_vndf(w_i, w_h, ax, ay) : beckmann_vndf(-w_i, -w_h, ax, ay);
+        float F = wi_out ? fresnel_R_dir(w_i, w_h, eta) : fresnel_R_dir(-w_i, -w_h, eta);
+        float a = wi_out ? 0.25f / dot(w_i, w_h) : 0.25f / dot(-w_i, -w_h);
+        return F * D * a;
+    }
+    f3 w_h = -normalize(w_i + w_o / eta);
+    w_h = w_h * (wi_out ? w_h.z / fabsf(w_h.z) : -w_h.z / fabsf(w_h.z));
+    if (dot(w_h, w_i) < 0.0f) return 0.0f;
+    float D = wi_out ? beckmann_vndf(w_i, w_h, ax, ay) : beckmann_vndf(-w_i, -w_h, ax, ay);
+    float F = wi_out ? fresnel_R_dir(w_i, w_h, eta) : fresnel_R_dir(-w_i, -w_h, eta);
+    float b = 1.0f / (eta * eta);
+    float c = wi_out ? fmaxf(0.0f, -dot(w_o, w_h)) : fmaxf(0.0f, -dot(-w_o, -w_h));
+    float e = wi_out ? dot(w_i, w_h) + dot(w_o, w_h) / eta : dot(-w_i, -w_h) + dot(-w_o, -w_h) / eta;
+    return b * c / (e * e) * D * (1.0f - F);
+}
+
+// beckmann_multiscattering_dielectric_BSDF_eval (ORF/MicrofacetBeckmann.h:479-527)
+// with every order summed (the render path passes order 0): the GGX estimator
+// of ggx_ms_dielectric_eval on the Beckmann walk, non-finite terms skipped,
+// truncated after kMsWalkCap events.
+ORF_HD float beckmann_ms_dielectric_eval(f3 w_i, f3 w_o, float eta, float ax, float ay, uint32_t& rng)
+{
+    f3 w_r = -w_i;
+    float hr = 1.0f + height_inv_cdf(0.9999f);
+    float sum = 0.0f;
+    bool wr_out = (-w_r.z) > 0.0f;
+    const bool wo_out = w_o.z > 0.0f;
+    for (int order = 0; order < kMsWalkCap; ++order) {
+        const float z = rnd_tea(rng);
+        hr = wr_out ? beckmann_sample_height(w_r, hr, z, ax, ay) : -beckmann_sample_height(-w_r, -hr, z, ax, ay);
+        if (hr == kRayMax || hr == -kRayMax || __builtin_isinf(hr)) break;
+        const float phase = beckmann_eval_dielectric_phase(-w_r, w_o, wr_out, wo_out, eta, ax, ay);
+        const float shadowing = wo_out ? ms_masking_height(BECKMANN_DISTRIBUTION, w_o, hr, ax, ay)
+                                       : ms_masking_height(BECKMANN_DISTRIBUTION, -w_o, -hr, ax, ay);
+        const float I = phase * shadowing;
+        if (__builtin_isfinite(I)) sum += I;
+        w_r = beckmann_sample_dielectric_phase(-w_r, eta, rng, wr_out, ax, ay);
+        if (hr != hr || w_r.z != w_r.z) return 0.0f;
+    }
+    return sum;
+}
+
 // One step of the rough dielectric (CU/rough_transparent_shader.cu:66-225,
 // visible-normal, Walter or multiple-scattering model): Beer absorption with Russian roulette when leaving the
 // medium, a visible microfacet normal, Fresnel-weighted choice between

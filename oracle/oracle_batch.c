@@ -304,6 +304,82 @@ void orc_ggx_ms_eval_batch(int n, const float* wi, const float* wo, float eta, f
     }
 }
 
+/* orc_ggx_ms_eval_batch on the Beckmann walk. */
+void orc_beckmann_ms_eval_batch(int n, const float* wi, const float* wo, float eta, float alpha, uint32_t seed, int cap,
+                                float* out)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        uint32_t t = orc_tea16((uint32_t)i, seed);
+        out[i] = (float)orc_beckmann_ms_dielectric_eval(V3(wi[0], wi[1], wi[2]),
+                                                        V3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), eta, alpha, alpha,
+                                                        &t, cap);
+    }
+}
+
+/* Rough diffuse walks (dist 1 GGX, 0 Beckmann) from the fixed local wi, albedo
+ * rho[3] (stream tea16(i, seed)): exit direction, last visible normal, weight
+ * (3 floats each) and order. */
+void orc_ms_diffuse_sample_batch(int n, const float* wi, float alpha, int dist, const float* rho, uint32_t seed, int cap,
+                                 float* wo, float* wm, float* weight, int32_t* order)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        uint32_t t = orc_tea16((uint32_t)i, seed);
+        v3 o, m, w;
+        int k = 0;
+        orc_ms_diffuse_sample(dist, V3(wi[0], wi[1], wi[2]), alpha, alpha, V3(rho[0], rho[1], rho[2]), &t, cap, &o, &m,
+                              &w, &k);
+        order[i] = k;
+        const v3 r[3] = {o, m, w};
+        float* dst[3] = {wo, wm, weight};
+        for (int j = 0; j < 3; ++j) {
+            dst[j][3 * i] = (float)r[j].x;
+            dst[j][3 * i + 1] = (float)r[j].y;
+            dst[j][3 * i + 2] = (float)r[j].z;
+        }
+    }
+}
+
+/* One rough diffuse walk estimate of BSDF x cos (3 channels) per direction wo[i]
+ * for the fixed local wi (stream tea16(i, seed)), at most `cap` events. */
+void orc_ms_diffuse_eval_batch(int n, const float* wi, const float* wo, float alpha, int dist, const float* rho,
+                               uint32_t seed, int cap, float* out)
+{
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        uint32_t t = orc_tea16((uint32_t)i, seed);
+        v3 r = orc_ms_diffuse_eval(dist, V3(wi[0], wi[1], wi[2]), V3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), alpha,
+                                   alpha, V3(rho[0], rho[1], rho[2]), &t, cap);
+        out[3 * i] = (float)r.x;
+        out[3 * i + 1] = (float)r.y;
+        out[3 * i + 2] = (float)r.z;
+    }
+}
+
+/* One conductor walk estimate of BSDF x cos per direction wo[i] for the fixed
+ * local wi, complex ior per channel as 6 doubles (eta = 1 / ior). */
+void orc_ms_conductor_eval_batch(int n, const float* wi, const float* wo, const double* ior, float alpha, int dist,
+                                 uint32_t seed, int cap, float* out)
+{
+    cplx3 eta, ieta;
+    ieta.x = (cplx){ior[0], ior[1]};
+    ieta.y = (cplx){ior[2], ior[3]};
+    ieta.z = (cplx){ior[4], ior[5]};
+    eta.x = orc_cdiv_fc((R)1.0f, ieta.x);
+    eta.y = orc_cdiv_fc((R)1.0f, ieta.y);
+    eta.z = orc_cdiv_fc((R)1.0f, ieta.z);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        uint32_t t = orc_tea16((uint32_t)i, seed);
+        v3 r = orc_ms_conductor_eval(dist, V3(wi[0], wi[1], wi[2]), V3(wo[3 * i], wo[3 * i + 1], wo[3 * i + 2]), &eta,
+                                     &ieta, alpha, alpha, &t, cap);
+        out[3 * i] = (float)r.x;
+        out[3 * i + 1] = (float)r.y;
+        out[3 * i + 2] = (float)r.z;
+    }
+}
+
 /* Walter-model microfacet normals about +z (microfacet_sample_normal). */
 void orc_walter_sample_batch(int n, const float* z1, const float* z2, float alpha, int dist, float* m)
 {

@@ -780,9 +780,6 @@ static v3 ggx_sample_dielectric_phase(v3 w_i, R m_eta, uint32_t* seed, int* outs
     return orc_normalize(vadd(vscale(w_i, -eta), vscale(w_m, eta * c - ct)));
 }
 
-/* ggx_multiscattering_dielectric_BSDF_sample (ORF/MicrofacetGGX.h:412-438) with
- * at most `cap` scattering events (the reference's loop is unbounded); returns
- * 0 when the walk exceeds the cap.  *order receives the event count. */
 /* beckmann_smith_lambda (ORF/MicrofacetBeckmann.h:24-38). */
 R orc_beckmann_smith_lambda(v3 wi, R ax, R ay)
 {
@@ -909,6 +906,9 @@ int orc_ggx_ms_conductor_sample(v3 local_wi, const cplx3* eta, const cplx3* ieta
     return 1;
 }
 
+/* ggx_multiscattering_dielectric_BSDF_sample (ORF/MicrofacetGGX.h:412-438) with
+ * at most `cap` scattering events (the reference's loop is unbounded); returns
+ * 0 when the walk exceeds the cap.  *order receives the event count. */
 int orc_ggx_ms_dielectric_sample(v3 local_wi, R eta, R ax, R ay, uint32_t* seed, int cap, v3* w_out, int* order)
 {
     R hr = (R)1.0f + orc_height_inv_cdf((R)0.9999f);
@@ -990,6 +990,198 @@ R orc_ggx_ms_dielectric_eval(v3 w_i, v3 w_o, R eta, R ax, R ay, uint32_t* seed, 
             break;
         }
         if (order >= cap) break;
+    }
+    return sum;
+}
+
+/* ---- the walks of ORF/Microfacet.h:100-200 for either distribution (dist 1:
+ * GGX, 0: Beckmann). */
+
+static R ms_sample_height(int dist, v3 wr, R hr, R z, R ax, R ay)
+{
+    return dist == 1 ? orc_ggx_sample_height(wr, hr, z, ax, ay) : orc_beckmann_sample_height(wr, hr, z, ax, ay);
+}
+
+/* ggx_G(wi, h) / beckmann_G(wi, h) (ORF/MicrofacetGGX.h:93-107,
+ * ORF/MicrofacetBeckmann.h:108-127). */
+R orc_ms_masking_height(int dist, v3 wi, R h, R ax, R ay)
+{
+    if (wi.z > (R)0.9999f) return 1;
+    if (wi.z <= 0) return 0;
+    R lambda = dist == 1 ? orc_ggx_smith_lambda(wi, ax, ay) : orc_beckmann_smith_lambda(wi, ax, ay);
+    return F_POW(orc_height_cdf(h), lambda);
+}
+
+static v3 ms_sample_vndf(int dist, v3 wi, R ax, R ay, R z1, R z2)
+{
+    return dist == 1 ? orc_ggx_sample_vndf(wi, ax, ay, z1, z2) : orc_beckmann_sample_vndf(wi, ax, ay, z1, z2);
+}
+
+/* ggx_sample_diffuse_phase / beckmann_sample_diffuse_phase (ORF/MicrofacetGGX.h:
+ * 342-375, ORF/MicrofacetBeckmann.h:375-408): visible normal (z1, z2), then the
+ * concentric disk map (Shirley and Chiu 1997) of (z3, z4) lifted to the
+ * hemisphere about it in its create_onb frame. */
+static v3 ms_sample_diffuse_phase(int dist, v3 w_i, R ax, R ay, uint32_t* seed, v3* w_m)
+{
+    R z1 = orc_rnd_tea(seed);
+    R z2 = orc_rnd_tea(seed);
+    R z3 = orc_rnd_tea(seed);
+    R z4 = orc_rnd_tea(seed);
+    *w_m = ms_sample_vndf(dist, w_i, ax, ay, z1, z2);
+    v3 w1, w2;
+    orc_create_onb(*w_m, &w1, &w2);
+    R r1 = (R)2.0f * z3 - (R)1.0f;
+    R r2 = (R)2.0f * z4 - (R)1.0f;
+    R r, phi;
+    if (r1 == 0 && r2 == 0) {
+        r = phi = 0;
+    } else if (r1 * r1 > r2 * r2) {
+        r = r1;
+        phi = (PI_R / (R)4.0f) * (r2 / r1);
+    } else {
+        r = r2;
+        phi = (PI_R / (R)2.0f) - (r1 / r2) * (PI_R / (R)4.0f);
+    }
+    R x = r * F_COS(phi);
+    R y = r * F_SIN(phi);
+    R zz = (R)1.0f - x * x - y * y;
+    R z = sqrt(zz > 0 ? zz : (R)0);
+    return vadd(vadd(vscale(w1, x), vscale(w2, y)), vscale(*w_m, z));
+}
+
+/* ggx_multiscattering_diffuse_BSDF_sample / beckmann_... (ORF/MicrofacetGGX.h:
+ * 519-544, ORF/MicrofacetBeckmann.h:613-640): the walk on a Lambertian
+ * microsurface, weight rho per event, at most `cap` events (the reference
+ * stops at 100 and returns the last sampled direction as it is); *w_m is the
+ * last event's visible normal. */
+void orc_ms_diffuse_sample(int dist, v3 local_wi, R ax, R ay, v3 rho, uint32_t* seed, int cap, v3* w_out, v3* w_m,
+                           v3* weight, int* order)
+{
+    *weight = V3(1, 1, 1);
+    *w_m = V3(0, 0, 1);
+    *order = 0;
+    R hr = (R)1.0f + orc_height_inv_cdf((R)0.9999f);
+    v3 w_o = vneg(local_wi);
+    while (*order < cap) {
+        R u = orc_rnd_tea(seed);
+        hr = ms_sample_height(dist, w_o, hr, u, ax, ay);
+        if (hr == RAY_MAX_R) break;
+        ++*order;
+        *weight = vmul(*weight, rho);
+        w_o = ms_sample_diffuse_phase(dist, vneg(w_o), ax, ay, seed, w_m);
+        if (hr != hr || w_o.z != w_o.z) {
+            w_o = V3(0, 0, 1);
+            break;
+        }
+    }
+    *w_out = w_o;
+}
+
+/* ggx_multiscattering_diffuse_BSDF_eval / beckmann_... (ORF/MicrofacetGGX.h:
+ * 547-577, ORF/MicrofacetBeckmann.h:643-674): sum over the walk's events of
+ * rho (once per event, as written there) x max(0, w_o.m) / pi for a visible
+ * normal m drawn afresh x the masking of w_o at the event's height, after at
+ * most `cap` events. */
+v3 orc_ms_diffuse_eval(int dist, v3 w_i, v3 w_o, R ax, R ay, v3 rho, uint32_t* seed, int cap)
+{
+    v3 w_r = vneg(w_i);
+    R hr = (R)1.0f + orc_height_inv_cdf((R)0.9999f);
+    v3 sum = V3(0, 0, 0);
+    for (int order = 0; order < cap; ++order) {
+        R u = orc_rnd_tea(seed);
+        hr = ms_sample_height(dist, w_r, hr, u, ax, ay);
+        if (hr == RAY_MAX_R) break;
+        R z1 = orc_rnd_tea(seed);
+        R z2 = orc_rnd_tea(seed);
+        v3 m = ms_sample_vndf(dist, vneg(w_r), ax, ay, z1, z2);
+        R c = vdot(w_o, m);
+        R phase = (R)0.318309886183790671538 * (c > 0 ? c : (R)0); /* M_1_PI */
+        R g = orc_ms_masking_height(dist, w_o, hr, ax, ay);
+        v3 term = V3(rho.x * (phase * g), rho.y * (phase * g), rho.z * (phase * g));
+        if (isfinite(term.x) && isfinite(term.y) && isfinite(term.z)) sum = vadd(sum, term);
+        v3 ms;
+        w_r = ms_sample_diffuse_phase(dist, vneg(w_r), ax, ay, seed, &ms);
+        if (hr != hr || w_r.z != w_r.z) return V3(0, 0, 0);
+    }
+    return sum;
+}
+
+/* ggx_multiscattering_conductor_BSDF_eval / beckmann_... (ORF/MicrofacetGGX.h:
+ * 580-611, ORF/MicrofacetBeckmann.h:677-708): sum over the walk's events of
+ * F(-w_r, h) x D_{-w_r}(h) / 4 (h the half vector of -w_r and w_o; 0 when h
+ * points below the mean plane) x the masking of w_o at the event's height,
+ * with eta or ieta as -w_r points up or down; at most `cap` events. */
+v3 orc_ms_conductor_eval(int dist, v3 w_i, v3 w_o, const cplx3* eta, const cplx3* ieta, R ax, R ay, uint32_t* seed,
+                         int cap)
+{
+    v3 w_r = vneg(w_i);
+    R hr = (R)1.0f + orc_height_inv_cdf((R)0.9999f);
+    v3 sum = V3(0, 0, 0);
+    for (int order = 0; order < cap; ++order) {
+        R u = orc_rnd_tea(seed);
+        hr = ms_sample_height(dist, w_r, hr, u, ax, ay);
+        if (hr == RAY_MAX_R) break;
+        v3 w_t = vneg(w_r);
+        v3 h = orc_normalize(vadd(w_t, w_o));
+        R phase = 0;
+        if (h.z >= 0) phase = (R)0.25f * (dist == 1 ? orc_ggx_eval_vndf(w_t, h, ax, ay) : orc_beckmann_eval_vndf(w_t, h, ax, ay));
+        v3 F = orc_fresnel_conductor(w_t, h, w_t.z > 0 ? eta : ieta);
+        R g = orc_ms_masking_height(dist, w_o, hr, ax, ay);
+        v3 term = vscale(F, phase * g);
+        if (isfinite(term.x) && isfinite(term.y) && isfinite(term.z)) sum = vadd(sum, term);
+        R z1 = orc_rnd_tea(seed);
+        R z2 = orc_rnd_tea(seed);
+        v3 m = ms_sample_vndf(dist, w_t, ax, ay, z1, z2);
+        w_r = vadd(vneg(w_t), vscale(vscale(m, (R)2.0f), vdot(w_t, m)));
+        if (hr != hr || w_r.z != w_r.z) return V3(0, 0, 0);
+    }
+    return sum;
+}
+
+/* beckmann_eval_dielectric_phase (ORF/MicrofacetBeckmann.h:342-372). */
+R orc_beckmann_eval_dielectric_phase(v3 w_i, v3 w_o, int wi_out, int wo_out, R m_eta, R ax, R ay)
+{
+    R eta = wi_out ? m_eta : (R)1.0f / m_eta;
+    if (wi_out == wo_out) {
+        v3 w_h = orc_normalize(vadd(w_i, w_o));
+        R D = wi_out ? orc_beckmann_eval_vndf(w_i, w_h, ax, ay) : orc_beckmann_eval_vndf(vneg(w_i), vneg(w_h), ax, ay);
+        R F = wi_out ? orc_fresnel_R_dir(w_i, w_h, eta) : orc_fresnel_R_dir(vneg(w_i), vneg(w_h), eta);
+        R a = wi_out ? (R)0.25f / vdot(w_i, w_h) : (R)0.25f / vdot(vneg(w_i), vneg(w_h));
+        return F * D * a;
+    }
+    v3 w_h = vneg(orc_normalize(vadd(w_i, V3(w_o.x / eta, w_o.y / eta, w_o.z / eta))));
+    w_h = vscale(w_h, wi_out ? w_h.z / fabs(w_h.z) : -w_h.z / fabs(w_h.z));
+    if (vdot(w_h, w_i) < 0) return 0;
+    R D = wi_out ? orc_beckmann_eval_vndf(w_i, w_h, ax, ay) : orc_beckmann_eval_vndf(vneg(w_i), vneg(w_h), ax, ay);
+    R F = wi_out ? orc_fresnel_R_dir(w_i, w_h, eta) : orc_fresnel_R_dir(vneg(w_i), vneg(w_h), eta);
+    R b = (R)1.0f / (eta * eta);
+    R cc = wi_out ? -vdot(w_o, w_h) : -vdot(vneg(w_o), vneg(w_h));
+    R c = cc > 0 ? cc : 0;
+    R e = wi_out ? vdot(w_i, w_h) + vdot(w_o, w_h) / eta : vdot(vneg(w_i), vneg(w_h)) + vdot(vneg(w_o), vneg(w_h)) / eta;
+    return b * c / (e * e) * D * ((R)1.0f - F);
+}
+
+/* beckmann_multiscattering_dielectric_BSDF_eval (ORF/MicrofacetBeckmann.h:
+ * 479-527) summing every order (order argument 0), non-finite terms skipped,
+ * after at most `cap` events. */
+R orc_beckmann_ms_dielectric_eval(v3 w_i, v3 w_o, R eta, R ax, R ay, uint32_t* seed, int cap)
+{
+    v3 w_r = vneg(w_i);
+    R hr = (R)1.0f + orc_height_inv_cdf((R)0.9999f);
+    R sum = 0;
+    int wr_out = (-w_r.z) > 0;
+    int wo_out = w_o.z > 0;
+    for (int order = 0; order < cap; ++order) {
+        R z = orc_rnd_tea(seed);
+        hr = wr_out ? orc_beckmann_sample_height(w_r, hr, z, ax, ay)
+                    : -orc_beckmann_sample_height(vneg(w_r), -hr, z, ax, ay);
+        if (hr == RAY_MAX_R || hr == -RAY_MAX_R || isinf(hr)) break;
+        R phase = orc_beckmann_eval_dielectric_phase(vneg(w_r), w_o, wr_out, wo_out, eta, ax, ay);
+        R shadowing = wo_out ? orc_ms_masking_height(0, w_o, hr, ax, ay) : orc_ms_masking_height(0, vneg(w_o), -hr, ax, ay);
+        R term = phase * shadowing;
+        if (isfinite(term)) sum += term;
+        w_r = beckmann_sample_dielectric_phase(vneg(w_r), eta, seed, &wr_out, ax, ay);
+        if (hr != hr || w_r.z != w_r.z) return 0;
     }
     return sum;
 }

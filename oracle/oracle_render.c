@@ -26,6 +26,7 @@
 #define INV_PI_R ((R)0.318309886183790671538)
 #define RAY_MAX ((R)1e27f)
 #define MS_WALK_CAP 10 /* scattering events per microsurface walk (new: the reference's loop is unbounded) */
+#define MS_DIFFUSE_CAP 100 /* scattering events per rough diffuse walk (the reference's own bound on its sampler) */
 
 typedef struct {
     double lo[3], hi[3];
@@ -931,14 +932,18 @@ static v3 shade_dielectric(const oscene* s, const orc_material* m, v3 o, v3 d, c
  * total internal reflection (CU/lambertian_interface_shader.cu:68-75). */
 static R two_C1(R n);
 
-/* CU/rough_diffuse_shader.cu:40-290 without the multiple-scattering walk: a
- * Lambert base under a rough dielectric.  NEE (a light other than the
- * environment, shadow ray to the full light distance) adds
- * Li/pi cos_l rho n12^2 T_i T_d / (1 - rho r10) G1(l, m) and
+/* CU/rough_diffuse_shader.cu:40-290: a Lambert base under a rough dielectric.
+ * NEE (a light other than the environment, shadow ray to the full light
+ * distance) adds Li/pi cos_l rho n12^2 T_i T_d / (1 - rho r10) G1(l, m) and
  * Li R_r G1(i, h) G1(l, h) D_wi(h) / (4 |i.n l.n|), and turns emission off once
  * a light is seen; the continuation is the cosine bounce about the sampled
  * microfacet normal m (weight rho n12^2 T_r / (1 - rho r10) G1(o, m)) when
- * xi > R_i, else the mirror about m (weight G1(o, m)). */
+ * xi > R_i, else the mirror about m (weight G1(o, m)).  The multiple-scattering
+ * model (:126-139, :187-216, :252-268) takes Li (diffuse walk estimate +
+ * interface walk estimate with the interface as a conductor of index n12),
+ * drawn before the shadow ray, and continues with the diffuse walk (weight
+ * rho^k n12^2 T / (1 - rho r10), T at its last visible normal) or the
+ * conductor walk in the un-flipped normal's frame (weight 1). */
 static v3 shade_rough_diffuse(const oscene* s, const orc_material* m, v3 o, v3 d, const ohit* h, int depth,
                               int emit_light, uint32_t* t, ostats* st)
 {
@@ -954,6 +959,10 @@ static v3 shade_rough_diffuse(const oscene* s, const orc_material* m, v3 o, v3 d
     int nd = m->normal_distribution;
     v3 rho = V3(m->diffuse[0], m->diffuse[1], m->diffuse[2]);
     v3 denom = V3((R)1.0f - rho.x * r10, (R)1.0f - rho.y * r10, (R)1.0f - rho.z * r10);
+    int ms = m->microfacet_model == 2;
+    cplx e12 = {n12, 0};
+    cplx ie12 = orc_cdiv_fc((R)1.0f, e12);
+    cplx3 eta3 = {e12, e12, e12}, ieta3 = {ie12, ie12, ie12};
     R z1 = orc_rnd_tea(t);
     R z2 = orc_rnd_tea(t);
     v3 l = orc_to_local(w_i, ffn);
@@ -977,9 +986,21 @@ static v3 shade_rough_diffuse(const oscene* s, const orc_material* m, v3 o, v3 d
         rad = V3(rad.x / light_pdf, rad.y / light_pdf, rad.z / light_pdf);
         R cl = vdot(ffn, w_l);
         if (cl > 0 && penv < 0) {
+            v3 msc = V3(0, 0, 0);
+            if (ms) {
+                v3 lwi = orc_to_local(w_i, ffn), lwl = orc_to_local(w_l, ffn);
+                v3 fd = orc_ms_diffuse_eval(nd, lwi, lwl, ax, ay, rho, t, MS_DIFFUSE_CAP);
+                v3 fr = orc_ms_conductor_eval(nd, lwi, lwl, &eta3, &ieta3, ax, ay, t, MS_WALK_CAP);
+                msc = vmul(rad, vadd(fd, fr));
+            }
             ohit sh;
             st->shd++;
-            if (!trace(s, hit_pos, w_l, s->eps, dist, 1, &sh)) {
+            if (ms) {
+                if (!trace(s, hit_pos, w_l, s->eps, dist, 1, &sh)) {
+                    result = vadd(result, msc);
+                    emit_light = 0;
+                }
+            } else if (!trace(s, hit_pos, w_l, s->eps, dist, 1, &sh)) {
                 v3 rn = orc_normalize(vadd(w_i, w_l));
                 R cr = vdot(rn, w_l);
                 R Rr = 1;
@@ -1013,6 +1034,31 @@ static v3 shade_rough_diffuse(const oscene* s, const orc_material* m, v3 o, v3 d
         }
     }
     R xi = orc_rnd_tea(t);
+    if (ms) {
+        v3 lo, k;
+        int order;
+        if (xi > Ri) {
+            v3 lm;
+            orc_ms_diffuse_sample(nd, orc_to_local(w_i, ffn), ax, ay, rho, t, MS_DIFFUSE_CAP, &lo, &lm, &k, &order);
+            v3 wo = orc_to_world(lo, ffn);
+            R cr = vdot(wo, orc_to_world(lm, ffn));
+            R Rr = 1;
+            st2 = n12 * n12 * ((R)1.0f - cr * cr);
+            if (st2 < (R)1.0f) Rr = orc_fresnel_R3(cr, sqrt((R)1.0f - st2), n12);
+            v3 child = radiance(s, hit_pos, wo, depth + 1, emit_light, t, (R)-1, st);
+            v3 c = vscale(vscale(vscale(child, n12), n12), (R)1.0f - Rr);
+            c = V3(c.x / denom.x, c.y / denom.y, c.z / denom.z);
+            result = vadd(result, vmul(c, k));
+        } else {
+            v3 lwi = orc_to_local(w_i, normal);
+            int ok = nd == 1 ? orc_ggx_ms_conductor_sample(lwi, &eta3, &ieta3, ax, ay, t, MS_WALK_CAP, &lo, &k, &order)
+                             : orc_beckmann_ms_conductor_sample(lwi, &eta3, &ieta3, ax, ay, t, MS_WALK_CAP, &lo, &k,
+                                                                &order);
+            if (!ok) return result;
+            result = vadd(result, radiance(s, hit_pos, orc_to_world(lo, normal), depth + 1, emit_light, t, (R)-1, st));
+        }
+        return result;
+    }
     if (xi > Ri) {
         v3 dd = orc_sample_cosine_weighted(mn, t);
         R cr = vdot(dd, mn);
@@ -1613,8 +1659,9 @@ static v3 shade_rough_translucent(const oscene* s, int mi, v3 o, v3 d, const ohi
         if (!inside && cl > 0) {
             v3 c;
             if (model == 2) {
-                R f = orc_ggx_ms_dielectric_eval(orc_to_local(w_i, ffn), orc_to_local(wl, ffn), recip, ax, ay, t,
-                                                 MS_WALK_CAP);
+                v3 lwi = orc_to_local(w_i, ffn), lwl = orc_to_local(wl, ffn);
+                R f = dist == 1 ? orc_ggx_ms_dielectric_eval(lwi, lwl, recip, ax, ay, t, MS_WALK_CAP)
+                                : orc_beckmann_ms_dielectric_eval(lwi, lwl, recip, ax, ay, t, MS_WALK_CAP);
                 c = vscale(rad, f);
             } else {
                 R Rr;
@@ -1636,8 +1683,10 @@ static v3 shade_rough_translucent(const oscene* s, int mi, v3 o, v3 d, const ohi
     if (model == 2) {
         v3 lo;
         int order;
-        if (!orc_ggx_ms_dielectric_sample(orc_to_local(w_i, normal), (R)1.0f / P->relative_ior, ax, ay, t, MS_WALK_CAP,
-                                          &lo, &order))
+        v3 lwi = orc_to_local(w_i, normal);
+        R eta = (R)1.0f / P->relative_ior;
+        if (!(dist == 1 ? orc_ggx_ms_dielectric_sample(lwi, eta, ax, ay, t, MS_WALK_CAP, &lo, &order)
+                        : orc_beckmann_ms_dielectric_sample(lwi, eta, ax, ay, t, MS_WALK_CAP, &lo, &order)))
             return result;
         nd = orc_to_world(lo, normal);
         refract = vdot(nd, ffn) < 0;

@@ -122,12 +122,28 @@ int orc_ggx_ms_conductor_sample(v3 local_wi, const cplx3* eta, const cplx3* ieta
 R orc_fresnel_R_dir(v3 w_i, v3 w_m, R eta);
 R orc_ggx_eval_dielectric_phase(v3 w_i, v3 w_o, int wi_out, int wo_out, R m_eta, R ax, R ay);
 R orc_ggx_ms_dielectric_eval(v3 w_i, v3 w_o, R eta, R ax, R ay, uint32_t* seed, int cap);
+R orc_ms_masking_height(int dist, v3 wi, R h, R ax, R ay);
+void orc_ms_diffuse_sample(int dist, v3 local_wi, R ax, R ay, v3 rho, uint32_t* seed, int cap, v3* w_out, v3* w_m,
+                           v3* weight, int* order);
+v3 orc_ms_diffuse_eval(int dist, v3 w_i, v3 w_o, R ax, R ay, v3 rho, uint32_t* seed, int cap);
+v3 orc_ms_conductor_eval(int dist, v3 w_i, v3 w_o, const cplx3* eta, const cplx3* ieta, R ax, R ay, uint32_t* seed,
+                         int cap);
+R orc_beckmann_eval_dielectric_phase(v3 w_i, v3 w_o, int wi_out, int wo_out, R m_eta, R ax, R ay);
+R orc_beckmann_ms_dielectric_eval(v3 w_i, v3 w_o, R eta, R ax, R ay, uint32_t* seed, int cap);
 /* Walter et al. 2007 model and the visible-normal density (ORF/Microfacet.h). */
 R orc_masking_iso(v3 v, v3 m, v3 n, R a, int dist);
 v3 orc_sample_normal_walter(v3 n, R z1, R z2, R a, int dist);
 R orc_eval_visible_normal(v3 wi, v3 wm, v3 n, R ax, R ay, int dist);
 void orc_ggx_ms_eval_batch(int n, const float* wi, const float* wo, float eta, float alpha, uint32_t seed, int cap,
                            float* out);
+void orc_beckmann_ms_eval_batch(int n, const float* wi, const float* wo, float eta, float alpha, uint32_t seed, int cap,
+                                float* out);
+void orc_ms_diffuse_sample_batch(int n, const float* wi, float alpha, int dist, const float* rho, uint32_t seed, int cap,
+                                 float* wo, float* wm, float* weight, int32_t* order);
+void orc_ms_diffuse_eval_batch(int n, const float* wi, const float* wo, float alpha, int dist, const float* rho,
+                               uint32_t seed, int cap, float* out);
+void orc_ms_conductor_eval_batch(int n, const float* wi, const float* wo, const double* ior, float alpha, int dist,
+                                 uint32_t seed, int cap, float* out);
 void orc_walter_sample_batch(int n, const float* z1, const float* z2, float alpha, int dist, float* m);
 void orc_reflect_weight_batch(int n, const float* wi, const float* z1, const float* z2, float alpha, int dist,
                               int model, float* out);

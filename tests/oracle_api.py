@@ -57,6 +57,13 @@ class Oracle:
                                               ctypes.c_int, c_vp, c_vp, c_vp]
         L.orc_ggx_ms_eval_batch.argtypes = [ctypes.c_int, c_vp, c_vp, ctypes.c_float, ctypes.c_float, ctypes.c_uint32,
                                             ctypes.c_int, c_vp]
+        L.orc_beckmann_ms_eval_batch.argtypes = L.orc_ggx_ms_eval_batch.argtypes
+        L.orc_ms_diffuse_sample_batch.argtypes = [ctypes.c_int, c_vp, ctypes.c_float, ctypes.c_int, c_vp,
+                                                  ctypes.c_uint32, ctypes.c_int, c_vp, c_vp, c_vp, c_vp]
+        L.orc_ms_diffuse_eval_batch.argtypes = [ctypes.c_int, c_vp, c_vp, ctypes.c_float, ctypes.c_int, c_vp,
+                                                ctypes.c_uint32, ctypes.c_int, c_vp]
+        L.orc_ms_conductor_eval_batch.argtypes = [ctypes.c_int, c_vp, c_vp, c_vp, ctypes.c_float, ctypes.c_int,
+                                                  ctypes.c_uint32, ctypes.c_int, c_vp]
         L.orc_walter_sample_batch.argtypes = [ctypes.c_int, c_vp, c_vp, ctypes.c_float, ctypes.c_int, c_vp]
         L.orc_reflect_weight_batch.argtypes = [ctypes.c_int, c_vp, c_vp, c_vp, ctypes.c_float, ctypes.c_int,
                                                ctypes.c_int, c_vp]
@@ -243,6 +250,42 @@ class Oracle:
         wo = np.ascontiguousarray(wo, np.float32)
         out = np.zeros(len(wo), np.float32)
         self.lib.orc_ggx_ms_eval_batch(len(wo), _p(wi), _p(wo), eta, alpha, seed, cap, _p(out))
+        return out
+
+    def beckmann_ms_eval(self, wi, wo, eta, alpha, seed=1, cap=10):
+        """ggx_ms_eval on the Beckmann walk."""
+        wi = np.ascontiguousarray(wi, np.float32)
+        wo = np.ascontiguousarray(wo, np.float32)
+        out = np.zeros(len(wo), np.float32)
+        self.lib.orc_beckmann_ms_eval_batch(len(wo), _p(wi), _p(wo), eta, alpha, seed, cap, _p(out))
+        return out
+
+    def ms_diffuse_sample(self, wi, alpha, rho, n, dist=1, seed=1, cap=100):
+        """Rough diffuse walks: (exit directions, last visible normals, weights) each (n, 3), and orders."""
+        wi = np.ascontiguousarray(wi, np.float32)
+        rho = np.ascontiguousarray(rho, np.float32)
+        wo, wm, w = (np.zeros((n, 3), np.float32) for _ in range(3))
+        order = np.zeros(n, np.int32)
+        self.lib.orc_ms_diffuse_sample_batch(n, _p(wi), alpha, dist, _p(rho), seed, cap, _p(wo), _p(wm), _p(w),
+                                             _p(order))
+        return wo, wm, w, order
+
+    def ms_diffuse_eval(self, wi, wo, alpha, rho, dist=1, seed=1, cap=100):
+        """One rough diffuse walk estimate of BSDF x cos (n, 3) per row of wo."""
+        wi = np.ascontiguousarray(wi, np.float32)
+        wo = np.ascontiguousarray(wo, np.float32)
+        rho = np.ascontiguousarray(rho, np.float32)
+        out = np.zeros((len(wo), 3), np.float32)
+        self.lib.orc_ms_diffuse_eval_batch(len(wo), _p(wi), _p(wo), alpha, dist, _p(rho), seed, cap, _p(out))
+        return out
+
+    def ms_conductor_eval(self, wi, wo, ior, alpha, dist=1, seed=1, cap=10):
+        """One conductor walk estimate of BSDF x cos (n, 3) per row of wo; ior ((re, im),) * 3."""
+        wi = np.ascontiguousarray(wi, np.float32)
+        wo = np.ascontiguousarray(wo, np.float32)
+        c = np.ascontiguousarray(np.asarray(ior, np.float64).ravel())
+        out = np.zeros((len(wo), 3), np.float32)
+        self.lib.orc_ms_conductor_eval_batch(len(wo), _p(wi), _p(wo), _p(c), alpha, dist, seed, cap, _p(out))
         return out
 
     def walter_sample(self, z1, z2, alpha, dist):

@@ -1,5 +1,6 @@
 """Pins the oracle's rough diffuse material (CU/rough_diffuse_shader.cu:40-290,
-visible-normal sampling; the multiple-scattering walk is not built), CPU only.
+visible-normal sampling; the multiple-scattering walks are pinned in
+test_oracle_ms_diffuse.py), CPU only.
 
 As the roughness goes to 0 the microfacet normal is the shading normal, the
 specular NEE term vanishes for a point light and both continuations keep weight
