@@ -89,7 +89,9 @@ struct DiffVertex {
 };
 
 // One vertex of the differentiable estimator: the plain shading step with the
-// texel's parameter substituted, plus what the adjoint needs.
+// texel's parameter substituted, plus what the adjoint needs.  kFeat selects the
+// material programs compiled in, as for the render kernels.
+template <unsigned kFeat>
 ORF_HD bool diff_vertex(const SceneGPU& sc, const DiffParams& dp, PathState& ps, const Hit& h, ShadowRay& sr,
                         DiffVertex& dv, bool want_grad = false)
 {
@@ -109,13 +111,13 @@ ORF_HD bool diff_vertex(const SceneGPU& sc, const DiffParams& dp, PathState& ps,
         dv.kind = 2;
         dv.texel = t;
     }
-    if (dv.kind == 0 && mid == dp.sss_mat && want_grad &&
+    if ((kFeat & kFeatTranslucent) != 0 && dv.kind == 0 && mid == dp.sss_mat && want_grad &&
         (mat.kind == MAT_TRANSLUCENT || mat.kind == MAT_ROUGH_TRANSLUCENT)) {
         dv.kind = 3;
         sss_grad_clear(dv.sg);
     }
     f3 wi = -ps.d;
-    bool more = shade_with(sc, mat, ps, h, g, sr, nullptr, 1, dv.kind == 3 ? &dv.sg : nullptr);
+    bool more = shade_with<kFeat>(sc, mat, ps, h, g, sr, nullptr, 1, dv.kind == 3 ? &dv.sg : nullptr);
     if (dv.kind == 1 && mat.kind != MAT_LAMBERT) dv.kind = 0;
     if (dv.kind == 2) {
         if (mat.kind == MAT_CONDUCTOR && more && mat.normal_distribution == GGX_DISTRIBUTION)
@@ -133,6 +135,7 @@ ORF_HD bool diff_vertex(const SceneGPU& sc, const DiffParams& dp, PathState& ps,
 // the same code was right.
 
 // Pass 1: the sample's radiance (the plain estimator with textured parameters).
+template <unsigned kFeat>
 ORF_HD f3 diff_path(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32_t rng, Seed64 s64, unsigned* stk,
                     int stride)
 {
@@ -146,7 +149,7 @@ ORF_HD f3 diff_path(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32
         }
         ShadowRay sr;
         DiffVertex dv;
-        bool more = diff_vertex(sc, dp, ps, h, sr, dv);
+        bool more = diff_vertex<kFeat>(sc, dp, ps, h, sr, dv);
         if (sr.valid) {
             Hit hs;
             if (!trace<true, false>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) shadow_unoccluded(ps, sr);
@@ -159,7 +162,7 @@ ORF_HD f3 diff_path(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32
 // Pass 2: replay the same random numbers and hand each textured vertex's
 // gradient terms to `sink` (albedo(texel, channel, value), rough(texel, value));
 // delta = dLoss/dL of this sample, L_total its pass-1 radiance.
-template <class Sink>
+template <unsigned kFeat, class Sink>
 ORF_HD void diff_replay(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32_t rng, Seed64 s64, f3 L_total,
                         f3 delta, Sink& sink, unsigned* stk, int stride)
 {
@@ -171,7 +174,7 @@ ORF_HD void diff_replay(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, ui
         f3 rem_before = L_total - ps.L;  // radiance from this vertex on
         ShadowRay sr;
         DiffVertex dv;
-        bool more = diff_vertex(sc, dp, ps, h, sr, dv, true);
+        bool more = diff_vertex<kFeat>(sc, dp, ps, h, sr, dv, true);
         if (sr.valid) {
             Hit hs;
             if (!trace<true, false>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) shadow_unoccluded(ps, sr);

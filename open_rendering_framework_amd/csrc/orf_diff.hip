@@ -17,6 +17,11 @@ namespace orf {
 
 constexpr int kDiffBlock = 64;
 
+// Waves per SIMD the register budget is sized for: the Lambert / conductor /
+// emitter form fits 128 VGPRs without spilling (4 waves); the larger material
+// sets keep the compiler's own allocation.
+constexpr int diff_waves(unsigned features) { return features == 0u ? 4 : 1; }
+
 // Texel gradients land in global FP32 arrays through atomics; the translucent
 // coefficients' six gradients are summed per lane and added once per lane.
 struct AtomicSink {
@@ -46,7 +51,8 @@ struct AtomicSink {
     }
 };
 
-__global__ __launch_bounds__(kDiffBlock) void k_diff(SceneGPU sc, DiffParams dp, const WfRender* __restrict__ views,
+template <unsigned kFeat>
+__global__ __launch_bounds__(kDiffBlock, diff_waves(kFeat)) void k_diff(SceneGPU sc, DiffParams dp, const WfRender* __restrict__ views,
                                                      int tiles_x, int tiles_per_view, int adjoint,
                                                      unsigned frame_shift, float* images, const float* dimages,
                                                      float* g_albedo, float* g_rough, float* g_sss,
@@ -69,7 +75,7 @@ __global__ __launch_bounds__(kDiffBlock) void k_diff(SceneGPU sc, DiffParams dp,
             f3 dir;
             uint32_t s0;
             uint32_t rng = camera_ray(R.cam, R.width, R.height, x, y, R.frame0 + s, dir, &s0);
-            f3 L = diff_path(sc, dp, R.cam.eye, dir, rng, seed64(s0), stk, kDiffBlock);
+            f3 L = diff_path<kFeat>(sc, dp, R.cam.eye, dir, rng, seed64(s0), stk, kDiffBlock);
             if (finite3(L)) {
                 acc += L;
                 cnt++;
@@ -90,12 +96,12 @@ __global__ __launch_bounds__(kDiffBlock) void k_diff(SceneGPU sc, DiffParams dp,
             f3 dir;
             uint32_t s0;
             uint32_t rng = camera_ray(R.cam, R.width, R.height, x, y, R.frame0 + frame_shift + s, dir, &s0);
-            f3 L = diff_path(sc, dp, R.cam.eye, dir, rng, seed64(s0), stk, kDiffBlock);
+            f3 L = diff_path<kFeat>(sc, dp, R.cam.eye, dir, rng, seed64(s0), stk, kDiffBlock);
             if (!finite3(L)) {
                 bad++;
                 continue;
             }
-            diff_replay(sc, dp, R.cam.eye, dir, rng, seed64(s0), L, delta, sink, stk, kDiffBlock);
+            diff_replay<kFeat>(sc, dp, R.cam.eye, dir, rng, seed64(s0), L, delta, sink, stk, kDiffBlock);
         }
         sink.flush();
     }
@@ -109,7 +115,10 @@ void launch_diff(const SceneGPU& sc, const DiffParams& dp, const WfRender* d_vie
     int tiles_x = (width + 7) / 8, tiles_y = (height + 7) / 8;
     int64_t blocks = (int64_t)tiles_x * tiles_y * nviews;
     ORF_REQUIRE(blocks > 0 && blocks < 0x7fffffffLL, "diff launch: bad size");
-    hipLaunchKernelGGL(k_diff, dim3((unsigned)blocks), dim3(kDiffBlock), 0, stream, sc, dp, d_views, tiles_x,
+    // the scene's material programs only, as the render kernels pick them
+    auto kernel = sc.features == 0u ? k_diff<0u> : sc.features == kFeatDielectric ? k_diff<kFeatDielectric>
+                                                                                   : k_diff<kFeatAll>;
+    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(kDiffBlock), 0, stream, sc, dp, d_views, tiles_x,
                        tiles_x * tiles_y, adjoint ? 1 : 0, frame_shift, images, dimages, g_albedo, g_rough, g_sss,
                        counters);
     ORF_HIP_CHECK(hipGetLastError());
