@@ -96,7 +96,7 @@ def test_batch_items_vs_fp64_oracle(batch):
         orc = orc[..., :3] / orc[..., 3:4]
         rel = np.sqrt(((img - orc) ** 2).mean()) / orc.mean()
         _say("item %d (%s, %dx%d, %d spp): rel rmse vs fp64 oracle %.3g" % (i, it["material"], w, h, it["spp"], rel))
-        assert rel < 2e-2
+        assert rel < 2.5e-3  # measured at most 1.0e-3
 
 
 def test_autotune_returns_valid_config(batch):
@@ -109,3 +109,25 @@ def test_autotune_returns_valid_config(batch):
     got = _films(ctx, items)
     for a, b in zip(got, ref):
         np.testing.assert_allclose(a[..., :3], b[..., :3], rtol=1e-9, atol=1e-12)
+
+
+def test_full_batch_workload_forms_agree():
+    """The benchmark's batch at full size (64 renders, 64x64 .. 1024x1024, 4 ..
+    256 spp, level-7 sphere, 1024x512 env): wavefront and persistent films equal
+    the per-item megakernel films on every item (relative 1e-9) with equal ray
+    counts."""
+    d, items, remap = scenes.batch_workload()
+    ctx = scenes.to_context(d)
+    ctx.render_batch(items, remap, mode=rt.BATCH_MEGAKERNEL)
+    ref = _films(ctx, items)
+    ref_counts = ctx.batch_counters()
+    assert ref_counts["paths"] == sum(it["width"] * it["height"] * it["spp"] for it in items)
+    for mode, name in ((rt.BATCH_WAVEFRONT, "wavefront"), (rt.BATCH_PERSISTENT, "persistent")):
+        secs = ctx.render_batch(items, remap, mode=mode)
+        counts = ctx.batch_counters()
+        _say("%s: %.3f s, counters %s (megakernel %s)" % (name, secs, counts, ref_counts))
+        for i, it in enumerate(items):
+            a = ctx.batch_film(i, it["width"], it["height"], sums=True)
+            assert np.array_equal(a[..., 3], ref[i][..., 3]), (name, i)
+            np.testing.assert_allclose(a[..., :3], ref[i][..., :3], rtol=1e-9, atol=1e-12, err_msg="%s %d" % (name, i))
+        assert counts == ref_counts, (name, counts, ref_counts)

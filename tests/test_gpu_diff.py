@@ -300,3 +300,57 @@ def test_torch_function_scattering_gradient_matches_c_abi():
     torch.testing.assert_close(theta.grad, gs, rtol=1e-4, atol=1e-4 * float(gs.abs().max()))
     assert float(theta.grad.abs().max()) > 0
     R.close()
+
+
+def test_inverse_rendering_workload_adjoint_matches_fp64_central_differences():
+    """The inverse-rendering workload's shapes: the benchmark scene at 256x256,
+    64x64 albedo and roughness textures, 1 spp x 2 ring views; the adjoint
+    against FP64 central differences (h = 1e-3) on 64 texture entries (48 among
+    those the views see, 16 anywhere), same bars as above."""
+    import torch
+
+    from open_rendering_framework_amd import rt
+    from tests.oracle_scene import OracleScene
+
+    size, tex = 256, 64
+    d = scenes.path_scene(width=size, height=size)
+    ctx = scenes.to_context(d)
+    views = scenes.ring_views(n=2, size=size, spp=1)
+    a, r = scenes.diff_textures(tex, seed=5, rough_range=(0.05, 0.8))
+    desc = rt.DiffDesc.make(scenes.DIFF_ALBEDO_MATERIAL, scenes.DIFF_ROUGHNESS_MATERIAL, (tex, tex),
+                            scenes.DIFF_ALBEDO_UV, scenes.DIFF_ROUGHNESS_UV)
+    dimg = np.random.default_rng(3).standard_normal((2, size, size, 3)).astype(np.float32)
+    ga, gr = ctx.diff_adjoint(desc, views, torch.from_numpy(a).cuda(), torch.from_numpy(r).cuda(),
+                              torch.from_numpy(dimg).cuda(), frame_shift=SHIFT)
+    g = np.concatenate([ga.cpu().numpy().ravel(), gr.cpu().numpy().ravel()]).astype(np.float64)
+    o64 = OracleScene(d, fp64=True)
+    a0, r0, dd = a.astype(np.float64), r.astype(np.float64), dimg.astype(np.float64)
+    h = 1e-3
+
+    def loss(aa, rr):
+        o64.diff_set((tex, tex), scenes.DIFF_ALBEDO_MATERIAL, scenes.DIFF_ROUGHNESS_MATERIAL, aa, rr, a0, r0)
+        return o64.diff_eval(views, shift=SHIFT, dimg=dd, images=False, threads=16)[0]
+
+    rng = np.random.default_rng(11)
+    seen = np.flatnonzero(g != 0.0)
+    assert len(seen) >= 1000
+    picks = np.concatenate([rng.choice(seen, size=48, replace=False), rng.choice(len(g), size=16, replace=False)])
+    na = a0.size
+    fd = []
+    for p in picks:
+        aa, rr = a0.copy(), r0.copy()
+        v = aa.reshape(-1) if p < na else rr.reshape(-1)
+        k = p if p < na else p - na
+        v[k] += h
+        lp = loss(aa, rr)
+        v[k] -= 2 * h
+        lm = loss(aa, rr)
+        fd.append((lp - lm) / (2 * h))
+    got, fd = g[picks], np.array(fd)
+    big = np.abs(fd) > 1e-2 * np.abs(fd).max()
+    rel = np.abs(got - fd) / np.maximum(np.abs(fd), 1e-30)
+    _say("  entries %d (large %d, seen %d): max rel %.3e, median rel %.3e" % (len(fd), big.sum(), len(seen),
+                                                                             rel[big].max(), np.median(rel[big])))
+    assert big.sum() >= 24
+    assert np.all(rel[big] <= 1e-2), list(zip(got[big][rel[big] > 1e-2], fd[big][rel[big] > 1e-2]))
+    assert np.all(np.abs(got - fd)[~big] <= 1e-4 * np.abs(fd).max())

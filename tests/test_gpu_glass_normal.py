@@ -3,8 +3,9 @@
 against the CPU oracle.
 
 Tolerances: per sample |gpu - oracle| <= 1e-5 |oracle| + 1e-6 on >= 99% of
-samples; converged image vs the FP64 oracle: relative RMSE of 8 x 8 block means < 3e-2
-(glass caustics make single pixels noisy once FP32 and FP64 paths part); wavefront
+samples; converged image: relative RMSE < 1e-6 against the FP32 oracle and
+< 0.1 against the FP64 oracle (glass caustics make single pixels noisy once FP32
+and FP64 paths part); wavefront
 and persistent batch forms vs the megakernel: relative 1e-9, equal ray counts.
 """
 import json
@@ -73,13 +74,16 @@ def test_converged_vs_fp64_oracle():
     ref, _ = OracleScene(d, fp64=True).render(W, W, 0, spp)
     ref = ref[..., :3] / ref[..., 3:4]
     rel = np.sqrt(((img - ref) ** 2).mean()) / ref.mean()
-    # glass caustics make single pixels noisy once FP32 and FP64 paths part ways:
-    # compare 8 x 8 pixel block means
-    blk = lambda a: a.reshape(W // 8, 8, W // 8, 8, 3).mean(axis=(1, 3))  # noqa: E731
-    rel_blk = np.sqrt(((blk(img) - blk(ref)) ** 2).mean()) / ref.mean()
-    _say("rel rmse vs fp64 %.3g (pixels), %.3g (8x8 blocks); counters %s" % (rel, rel_blk, ctx.counters()))
+    r32, _ = OracleScene(d).render(W, W, 0, spp)
+    r32 = r32[..., :3] / r32[..., 3:4]
+    rel32 = np.sqrt(((img - r32) ** 2).mean()) / r32.mean()
+    _say("rel rmse vs fp64 %.3g, vs fp32 %.3g; counters %s" % (rel, rel32, ctx.counters()))
     assert ctx.counters()["nonfinite"] == 0
-    assert rel_blk < 3e-2
+    # glass caustics: once FP32 and FP64 paths part at a refraction or roulette
+    # draw, a pixel's few samples through the glass differ wholesale (measured
+    # 5.3e-2 at 32 spp); the same samples in FP32 agree to 1e-5
+    assert rel < 0.1
+    assert rel32 < 1e-6  # measured 1.6e-7
 
 
 def test_batch_forms_match_megakernel():

@@ -148,7 +148,37 @@ def test_image_and_ray_counts_vs_fp64_oracle():
     assert c["paths"] == 64 * 64 * spp and c["nonfinite"] == 0
     assert abs(c["ext_rays"] - rc["ext_rays"]) <= 0.005 * rc["ext_rays"]
     assert abs(c["shadow_rays"] - rc["shadow_rays"]) <= 0.005 * rc["shadow_rays"]
-    assert rel_rmse < 2e-2
+    assert rel_rmse < 2.5e-3  # measured 1.1e-3
+
+
+def test_benchmark_scene_image_rmse_gate():
+    """The benchmark scene at its full size (512x512, 64 spp): the GPU image
+    against the FP64 oracle's image of the same samples, relative RMSE (RMSE /
+    image mean) <= 1e-3 (measured 9.2e-4: FP32 and FP64 arithmetic part on about
+    0.1% of the paths, mostly at Russian roulette and hit-distance ties), and
+    against the FP32 oracle <= 1e-5 (measured 2.7e-6)."""
+    from tests.oracle_scene import OracleScene
+
+    d = scenes.path_scene()
+    ctx = scenes.to_context(d)
+    ctx.accel_build()
+    spp = 64
+    ctx.film_clear()
+    ctx.launch(512, 512, 0, spp)
+    img = ctx.film(512, 512)[..., :3].astype(np.float64)
+    c = ctx.counters()
+    for fp64, bar in ((True, 1e-3), (False, 1e-5)):
+        ref, rc = OracleScene(d, fp64=fp64).render(512, 512, 0, spp, threads=16)
+        ref = ref[..., :3] / ref[..., 3:4]
+        rel = np.sqrt(((img - ref) ** 2).mean()) / ref.mean()
+        _say("fp%d oracle: rel rmse %.3g, rays gpu %d/%d oracle %d/%d" % (64 if fp64 else 32, rel, c["ext_rays"],
+                                                                          c["shadow_rays"], rc["ext_rays"],
+                                                                          rc["shadow_rays"]))
+        assert rel <= bar
+        if not fp64:
+            assert abs(c["ext_rays"] - rc["ext_rays"]) <= 1e-4 * rc["ext_rays"]
+            assert abs(c["shadow_rays"] - rc["shadow_rays"]) <= 1e-4 * rc["shadow_rays"]
+    assert c["nonfinite"] == 0
 
 
 def test_full_scene_samples_and_determinism():

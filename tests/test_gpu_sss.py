@@ -6,7 +6,10 @@ against the CPU oracle.
 Tolerances: coefficients rtol 1e-6 (same FP32 formulas); per-probe and
 per-sample |gpu - oracle| <= 1e-5 |oracle| + 1e-6 on >= 99% of probes / samples
 (both sides share the FP32 transcendentals; a path whose branch flips on a last
-bit diverges); converged image vs the FP64 oracle: relative RMSE < 3e-2.
+bit diverges); converged image: relative RMSE < 1e-7 against the FP32 oracle
+(same samples, same probe set) and < 3e-2 (smooth) / 3.6e-2 (rough boundary)
+against the FP64 oracle, about twice the measured 1.6e-2 / 1.8e-2: FP32 and
+FP64 part at the diffusion gather's radius and roulette draws.
 """
 import sys
 
@@ -34,6 +37,19 @@ def _scene(dipole_model=0, probes=8192, rough=None):
 
 def _ok(got, want):
     return (np.abs(got - want) <= RTOL * np.abs(want) + ATOL).all(axis=-1)
+
+
+def _rel_vs_fp32(d, ctx, img, spp):
+    """Relative RMSE of the GPU image (one launch from frame 0: one probe pass)
+    against the FP32 oracle's image of the same samples on the GPU's probe set."""
+    from tests.oracle_scene import OracleScene
+
+    o32 = OracleScene(d)
+    o32.probes_set(ctx.probes(), 4, MILK)
+    o32.probe_pass(0)
+    ref, _ = o32.render(W, W, 0, spp)
+    ref = ref[..., :3] / ref[..., 3:4]
+    return np.sqrt(((img - ref) ** 2).mean()) / ref.mean()
 
 
 def test_coefficients_match_oracle():
@@ -138,9 +154,11 @@ def test_converged_image_vs_fp64_oracle():
     ref, rc = orc.render(W, W, 0, spp)
     ref = ref[..., :3] / ref[..., 3:4]
     rel = np.sqrt(((img - ref) ** 2).mean()) / ref.mean()
-    _say("rel rmse vs fp64 %.3g counters %s / %s" % (rel, c, rc))
+    rel32 = _rel_vs_fp32(d, ctx, img, spp)
+    _say("rel rmse vs fp64 %.3g vs fp32 %.3g counters %s / %s" % (rel, rel32, c, rc))
     assert c["nonfinite"] == 0
-    assert rel < 3e-2
+    assert rel < 3e-2  # measured 1.6e-2
+    assert rel32 < 1e-7  # measured 1.4e-9
 
 
 def test_rough_boundary_converged_vs_fp64_oracle():
@@ -159,5 +177,41 @@ def test_rough_boundary_converged_vs_fp64_oracle():
     ref, rc = orc.render(W, W, 0, spp)
     ref = ref[..., :3] / ref[..., 3:4]
     rel = np.sqrt(((img - ref) ** 2).mean()) / ref.mean()
-    _say("rough: rel rmse vs fp64 %.3g" % rel)
-    assert np.isfinite(img).all() and rel < 3e-2
+    rel32 = _rel_vs_fp32(d, ctx, img, spp)
+    _say("rough: rel rmse vs fp64 %.3g vs fp32 %.3g" % (rel, rel32))
+    assert np.isfinite(img).all() and rel < 3.6e-2  # measured 1.8e-2
+    assert rel32 < 1e-7  # measured 1.4e-9
+
+
+def test_benchmark_scene_full_size_parity():
+    """The subsurface benchmark scene at full size (512x512, level-8 milk sphere
+    with 2^20 probes, multiple-scattering glass, 1024x512 env): every probe of
+    the probe pass and 3,000 random pixels' samples against the FP32 oracle,
+    same per-probe / per-sample bar as the small scenes."""
+    from tests.oracle_scene import OracleScene
+
+    d = scenes.subsurface_scene()
+    ctx = scenes.to_context(d)
+    ctx.accel_build()
+    orc = OracleScene(d)
+    rng = np.random.default_rng(17)
+    px, py = rng.integers(0, 512, 3000), rng.integers(0, 512, 3000)
+    first = True
+    for frame in (0, 5):
+        ctx.film_clear()
+        ctx.launch(512, 512, frame, 1)
+        gp = ctx.probes()
+        if first:
+            assert gp["count"] > (1 << 19)
+            orc.probes_set(gp, 4, MILK)
+            first = False
+        oL, ow, osn = orc.probe_pass(frame, threads=16, normals=True)
+        okL, okw, oks = _ok(gp["L"], oL), _ok(gp["w12"], ow), _ok(gp["sn"], osn)
+        _say("frame %d: %d probes, L ok %.5f w12 ok %.5f sn ok %.5f" % (frame, gp["count"], okL.mean(), okw.mean(),
+                                                                       oks.mean()))
+        assert okL.mean() >= 0.99 and okw.mean() >= 0.99 and oks.mean() >= 0.99
+        got = ctx.film_sums(512, 512)[py, px, :3]
+        want = orc.samples(512, 512, px, py, frame, threads=16)
+        ok = _ok(got, want)
+        _say("  samples ok %.5f mean %.4g / %.4g" % (ok.mean(), got.mean(), want.mean()))
+        assert ok.mean() >= 0.99
