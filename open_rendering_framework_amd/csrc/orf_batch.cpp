@@ -119,7 +119,7 @@ static double run_batch(orf_context_t* c, long long paths, int mode, const Wavef
         c->d_batch_items.upload(pr);
         launch_batch_persistent(sc, c->d_batch_renders.as<WfRender>(), c->d_batch_items.as<PersistRender>(),
                                 (int)pr.size(), c->d_batch_remap.as<int>(), film, stats, stats + 8, items,
-                                persistent_grid(items, mega_waves(sc.features)), st);
+                                persistent_waves(items, mega_waves(sc.features)), st);
         ORF_HIP_CHECK(hipStreamSynchronize(st));
     } else if (mode == ORF_BATCH_MEGAKERNEL) {
         const int* remap = c->d_batch_remap.as<int>();

@@ -356,18 +356,51 @@ BvhResult build_bvh(const float* verts, int64_t nverts, const int32_t* idx, int6
             }
             for (int k = n; k < 4; ++k) kids[k] = -1;
         };
-        std::vector<int> flat4(tn.size(), -1), order4;
-        std::vector<std::pair<int, int>> st{{0, 0}};
-        while (!st.empty()) {
-            const int i = st.back().first, d = st.back().second;
-            st.pop_back();
+        // Node order: the first kBvhTopOrdered internal nodes breadth-first, level by
+        // level with larger boxes first within a level (a kernel may keep a prefix
+        // of them in LDS: the most visited nodes), then each remaining subtree
+        // depth-first from its root, children in gather order.
+        std::vector<int> flat4(tn.size(), -1), order4, dep(tn.size(), 0);
+        auto place = [&](int i) {
             flat4[i] = (int)order4.size();
             order4.push_back(i);
-            res.depth4 = std::max(res.depth4, d);
+            res.depth4 = std::max(res.depth4, dep[i]);
+        };
+        auto internal_kids = [&](int i, std::vector<int>& out) {
             int kids[4];
             gather(i, kids);
-            for (int k = 3; k >= 0; --k)
-                if (kids[k] >= 0 && tn[kids[k]].left >= 0 && kids[k] != i) st.push_back({kids[k], d + 1});
+            for (int k = 0; k < 4; ++k)
+                if (kids[k] >= 0 && tn[kids[k]].left >= 0 && kids[k] != i) {
+                    dep[kids[k]] = dep[i] + 1;
+                    out.push_back(kids[k]);
+                }
+        };
+        std::vector<int> lvl{0}, rest;
+        while (!lvl.empty()) {
+            std::stable_sort(lvl.begin(), lvl.end(),
+                             [&](int a, int b) { return tn[a].box.area() > tn[b].box.area(); });
+            std::vector<int> next;
+            for (int i : lvl) {
+                if ((int)order4.size() < kBvhTopOrdered) {
+                    place(i);
+                    internal_kids(i, next);
+                } else {
+                    rest.push_back(i);
+                }
+            }
+            lvl.swap(next);
+        }
+        res.top4 = (int)order4.size();
+        for (int r : rest) {
+            std::vector<int> st{r}, kids;
+            while (!st.empty()) {
+                const int i = st.back();
+                st.pop_back();
+                place(i);
+                kids.clear();
+                internal_kids(i, kids);
+                for (int k = (int)kids.size() - 1; k >= 0; --k) st.push_back(kids[k]);
+            }
         }
         // stack bound: a visit pushes (live children - 1) entries at most
         std::vector<int> need(tn.size(), 0);

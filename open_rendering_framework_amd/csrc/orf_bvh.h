@@ -28,8 +28,11 @@ struct BvhResult {
     // The same tree collapsed to 4 children per node (each BVH2 node absorbs
     // the children of its largest internal children): 8 float4 per node,
     // lo.x, hi.x, lo.y, hi.y, lo.z, hi.z of the 4 children, their links, pad.
+    // The first top4 nodes are the top levels breadth-first; the rest are
+    // depth-first subtrees.
     std::vector<float4> nodes4;
     int depth4 = 0;             // deepest BVH4 node level
+    int top4 = 0;               // leading BVH4 nodes in breadth-first order (kBvhTopOrdered at most)
     int stack4 = 0;             // most traversal-stack entries a BVH4 ray can hold
     std::vector<float4> tris;   // 3 float4 per leaf-ordered triangle (p0|id, e1, e2)
     unsigned root_link = 0;     // leaf link when the whole scene is one leaf

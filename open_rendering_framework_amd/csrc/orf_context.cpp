@@ -252,6 +252,7 @@ static void build_geometry(orf_context_t* c, hipStream_t)
     ORF_REQUIRE(bvh.stack4 <= kStackDepth, "scene: the BVH needs a deeper traversal stack than this build has");
     c->d_bvh.upload(bvh.nodes);
     c->d_bvh4.upload(bvh.nodes4);
+    c->nodes4 = (int)(bvh.nodes4.size() / 8);
     c->d_tri.upload(bvh.tris);
     c->d_shade.upload(shade);
     c->d_normals.upload(normals);
@@ -372,6 +373,7 @@ SceneGPU scene_view(orf_context_t* c)
     std::memset(&s, 0, sizeof(s));
     s.bvh = c->d_bvh.as<float4>();
     s.bvh4 = c->d_bvh4.as<float4>();
+    s.nodes4 = c->nodes4;
     s.tri = c->d_tri.as<float4>();
     s.shade = c->d_shade.as<int4>();
     s.normals = c->d_normals.as<float4>();

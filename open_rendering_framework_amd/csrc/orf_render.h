@@ -48,10 +48,22 @@ struct FilmGPU {
     unsigned long long* qctr;      // 8 work-queue counters (device, zeroed by each launch)
 };
 
-// Workgroups that fill the current device at the megakernels' occupancy (at most `items`).
-// Workgroups of a persistent megakernel launch: CUs x 4 SIMDs x waves, capped at
-// the item count.
-int persistent_grid(long long items, int waves);
+// Waves that fill the current device at the megakernels' occupancy: CUs x 4
+// SIMDs x waves, capped at the item count (each wave fetches its own items).
+int persistent_waves(long long items, int waves);
+// Waves per workgroup of the persistent megakernels and the number of top BVH
+// nodes each workgroup keeps in LDS.
+#ifndef ORF_MEGA_BLOCK_WAVES
+#define ORF_MEGA_BLOCK_WAVES 8
+#endif
+#ifndef ORF_MEGA_TOP
+#define ORF_MEGA_TOP 0
+#endif
+#ifndef ORF_PERSIST_TOP
+#define ORF_PERSIST_TOP 200
+#endif
+// Workgroups of ORF_MEGA_BLOCK_WAVES waves holding `waves` waves.
+inline int mega_blocks(int waves) { return (waves + ORF_MEGA_BLOCK_WAVES - 1) / ORF_MEGA_BLOCK_WAVES; }
 // Zero 8 queue counters with a kernel (safe inside captured streams).
 void zero_queue_counters(unsigned long long* qctr, hipStream_t stream);
 
@@ -76,10 +88,11 @@ struct PersistRender {
     int pad;
 };
 struct WfRender;
-// qctr: 8 zeroed u64 queue counters (device).  grid: workgroups of 64 lanes.
+// qctr: 8 zeroed u64 queue counters (device).  waves: persistent waves (grid of
+// workgroups of 8 waves).
 void launch_batch_persistent(const SceneGPU& sc, const WfRender* renders, const PersistRender* pr, int nrenders,
                              const int* remap, double* film, unsigned long long* stats, unsigned long long* qctr,
-                             long long total_items, int grid, hipStream_t stream);
+                             long long total_items, int waves, hipStream_t stream);
 
 // FP64 construction of the environment's FP32 pdf/cdf tables (spec CU/env_cameras.cu).
 // scratch: (h + 1) doubles.

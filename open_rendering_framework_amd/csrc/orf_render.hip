@@ -19,7 +19,10 @@ __global__ __launch_bounds__(kMegaBlock, mega_waves(kFeat)) void k_render_mega(S
                                                                              int chunks, long long items)
 {
     __shared__ unsigned stack_lds[kLdsStack * kMegaBlock];
+    __shared__ float4 top_lds[(kMegaTop > 0 ? kMegaTop : 1u) * kTopFloat4];
     unsigned* stk = stack_lds + threadIdx.x;
+    const unsigned ntop = kMegaTop > 0 ? load_top_nodes(sc, top_lds, kMegaTop) : 0u;
+    const unsigned lane = threadIdx.x & 63u;
     const int home = (int)(blockIdx.x % kXcdParts);
     unsigned long long bad = 0, npaths = 0;
     PathStats st;
@@ -30,11 +33,12 @@ __global__ __launch_bounds__(kMegaBlock, mega_waves(kFeat)) void k_render_mega(S
             const long long item = mega_next(film.qctr, part, lo, hi);
             if (item < 0) break;  // uniform across the wave
             const int tile = (int)(item / chunks), chunk = (int)(item - (long long)tile * chunks);
-            const int x = (tile % tiles_x) * 8 + (int)(threadIdx.x & 7), y = (tile / tiles_x) * 8 + (int)(threadIdx.x >> 3);
+            const int x = (tile % tiles_x) * 8 + (int)(lane & 7), y = (tile / tiles_x) * 8 + (int)(lane >> 3);
             if (x < film.width && y < film.height) {
                 const unsigned k0 = (unsigned)chunk * per, k1 = min(k0 + per, spp);
-                mega_pixel<kFeat>(sc, cam, film.width, film.height, x, y, remap, frame0, k0, k1, chunks == 1,
-                                  film.sums + 4 * ((size_t)y * film.width + x), stk, st, bad);
+                mega_pixel<kFeat, kMegaTop>(sc, cam, film.width, film.height, x, y, remap, frame0, k0, k1, chunks == 1,
+                                            film.sums + 4 * ((size_t)y * film.width + x), stk, (lds_cf4*)top_lds, ntop,
+                                            st, bad);
                 npaths += k1 - k0;
             }
         }
@@ -57,7 +61,7 @@ void zero_queue_counters(unsigned long long* qctr, hipStream_t stream)
     ORF_HIP_CHECK(hipGetLastError());
 }
 
-int persistent_grid(long long items, int waves)
+int persistent_waves(long long items, int waves)
 {
     static int cus[64] = {0};
     int dev = 0;
@@ -76,9 +80,11 @@ void launch_megakernel(const SceneGPU& sc, const CameraGPU& cam, const FilmGPU& 
     const int tiles_x = (film.width + 7) / 8, tiles_y = (film.height + 7) / 8;
     const int waves = mega_waves(sc.features);
     // Samples per item: the largest of kMegaChunk, /2, ... that still gives every
-    // wave of a full persistent grid kMegaItemsPerWave items (the last items
-    // of a launch run alone, so a few big ones leave the chip idle).
-    const long long full = (long long)persistent_grid(1LL << 40, waves);
+    // wave of a full persistent grid kMegaItemsPerWave items.  Larger chunks
+    // shorten the waves' idle lanes per path but leave the chip idle behind the
+    // last expensive items (tiles differ several-fold in cost); 2 samples per
+    // item measured best on the 512x512 scene.
+    const long long full = (long long)persistent_waves(1LL << 40, waves);
     unsigned per = kMegaChunk;
     while (per > 1 && (long long)tiles_x * tiles_y * ((spp + per - 1) / per) < kMegaItemsPerWave * full) per >>= 1;
     per = std::min(per, spp);
@@ -86,7 +92,7 @@ void launch_megakernel(const SceneGPU& sc, const CameraGPU& cam, const FilmGPU& 
     const long long items = (long long)tiles_x * tiles_y * chunks;
     ORF_REQUIRE(items / kXcdParts < 0xF0000000LL, "launch: too many work items");
     zero_queue_counters(film.qctr, stream);
-    const dim3 grid((unsigned)persistent_grid(items, waves)), block(kMegaBlock);
+    const dim3 grid((unsigned)mega_blocks(persistent_waves(items, waves))), block(kMegaBlock);
     if (sc.features == 0u)
         hipLaunchKernelGGL(k_render_mega<0u>, grid, block, 0, stream, sc, cam, film, remap, frame0, spp, per, tiles_x,
                            chunks, items);

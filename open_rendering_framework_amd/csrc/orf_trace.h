@@ -17,11 +17,25 @@
 // workgroup's LDS array (stride = block size), deeper ones (up to kStackDepth)
 // in a private array (the context refuses a tree whose 4-wide traversal could
 // hold more than kStackDepth entries).
+// Top nodes: kernels that copy the first `ntop` 4-wide nodes (the breadth-first
+// top levels, kTopFloat4 float4 each) into LDS pass that copy; a lane at such a
+// node reads it from LDS, so the most visited nodes cost no vector-memory
+// instructions (the texture address/data path is what bounds the traversal).
+// Only where the node's bytes come from changes, never the visit order.
 #pragma once
 
 #include "orf_scene.h"
 
 namespace orf {
+
+// Pointer to float4 data in LDS.  A generic pointer to the LDS copy of the top
+// nodes compiles to flat loads, which still pass through the texture address
+// unit; the LDS address space gives ds_read_b128.
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) const float4 lds_cf4;
+#else
+typedef const float4 lds_cf4;
+#endif
 
 struct Hit {
     float t;
@@ -175,9 +189,9 @@ ORF_HD unsigned stack_pop(unsigned* stk, int stride, const unsigned* ovf, int& s
 // extension and shadow rays of different lanes at once).  kEmit: the disk and
 // sphere emitters are tested; kPass: shadow rays pass through triangles marked
 // kTriNoShadow (kernels for scenes that cannot hold either leave it out).
-template <int kMode, bool kEmit = true, bool kPass = true>
+template <int kMode, bool kEmit = true, bool kPass = true, bool kTop = false>
 ORF_HD bool trace_impl(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, bool any, Hit& h, unsigned* stk,
-                       int stride)
+                       int stride, lds_cf4* top = nullptr, unsigned ntop = 0u)
 {
     const bool ANY = kMode == 1 || (kMode == 2 && any);
     RayPre r = ray_pre(o, d);
@@ -194,8 +208,16 @@ ORF_HD bool trace_impl(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, b
         if (!link_is_leaf(node)) {
             // 4-wide node: the entered children in near-to-far order; the
             // nearest is taken next, the others pushed farthest first
-            const float4* nd = sc.bvh4 + 8 * (size_t)node;
-            const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], ln = nd[6];
+            float4 lx, hx, ly, hy, lz, hz, ln;
+            if (kTop && node < ntop) {
+                lds_cf4* nd = top + kTopFloat4 * node;
+                lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], ln = nd[6];
+                // keeps the two arms apart: merged, they become flat loads through phi'd pointers
+                __asm__ volatile("" ::: "memory");
+            } else {
+                const float4* nd = sc.bvh4 + 8 * (size_t)node;
+                lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], ln = nd[6];
+            }
             float e[4] = {box_enter(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, tmin, tbest),
                           box_enter(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, tmin, tbest),
                           box_enter(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, tmin, tbest),
@@ -276,6 +298,27 @@ template <bool ANY, bool kEmit = true, bool kPass = true>
 ORF_HD bool trace(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, Hit& h, unsigned* stk, int stride)
 {
     return trace_impl<ANY ? 1 : 0, kEmit, kPass>(sc, o, d, tmin, tmax, ANY, h, stk, stride);
+}
+
+// As trace, reading the first ntop nodes from the LDS copy `top`.
+template <bool ANY, bool kEmit = true, bool kPass = true>
+ORF_HD bool trace_top(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, Hit& h, unsigned* stk, int stride,
+                      lds_cf4* top, unsigned ntop)
+{
+    return trace_impl<ANY ? 1 : 0, kEmit, kPass, true>(sc, o, d, tmin, tmax, ANY, h, stk, stride, top, ntop);
+}
+
+// Cooperative copy of the first min(nodes4, cap) nodes into the LDS array `top`
+// by the whole workgroup (every thread calls it; ends with a barrier).
+__device__ __forceinline__ unsigned load_top_nodes(const SceneGPU& sc, float4* top, unsigned cap)
+{
+    const unsigned n = min((unsigned)sc.nodes4, min(cap, (unsigned)kBvhTopOrdered));
+    for (unsigned i = threadIdx.x; i < n * kTopFloat4; i += blockDim.x) {
+        const unsigned node = i / kTopFloat4, row = i - node * kTopFloat4;
+        top[i] = sc.bvh4[8 * (size_t)node + row];
+    }
+    __syncthreads();
+    return n;
 }
 
 ORF_HD bool trace_dyn(const SceneGPU& sc, f3 o, f3 d, float tmin, float tmax, bool any, Hit& h, unsigned* stk,
