@@ -226,7 +226,7 @@ orf_status orf_light_add_triangles(orf_context ctx, const float* tri_verts, cons
  * remaps must cover it like any mesh material. */
 orf_status orf_light_add_disk(orf_context ctx, const float position[3], float radius, float theta, float phi,
                               const float radiance[3], int* emitter_material);
-/* SphericalLight (ORF/Light.cpp:507-611): a sphere emitter (CU/sphere_mesh.cu:42-91)
+/* SphericalLight (ORF/Light.cpp:507-607): a sphere emitter (CU/sphere_mesh.cu:42-91)
  * sampled by evaluate_spherical_area_light (ORF/LightSampler.h:103-141). */
 orf_status orf_light_add_sphere(orf_context ctx, const float position[3], float radius, const float radiance[3],
                                 int* emitter_material);

@@ -695,7 +695,7 @@ extern "C" orf_status orf_light_add_triangles(orf_context ctx, const float* tv, 
 namespace orf {
 
 // Emitter material of a disk or sphere light (the arealight material each
-// DiskAreaLight / SphericalLight instance carries, ORF/Light.cpp:476-505, 583-611).
+// DiskAreaLight / SphericalLight instance carries, ORF/Light.cpp:476-502, 583-607).
 int emitter_material(orf_context_t* c, const float radiance[3])
 {
     c->mats.push_back(orf::default_material(orf::MAT_EMITTER));

@@ -1,11 +1,11 @@
 // orf_diff.h -- differentiable form: textured albedo/roughness and path-replay adjoint.
 //
 // No counterpart in the reference (it has neither textures on these materials --
-// CU/diffuse_shader.cu:130 reads a texture but shades with diffuse_color -- nor
+// CU/diffuse_shader.cu:50 reads a texture but shades with diffuse_color -- nor
 // derivatives).  Two materials take their parameter from a texture, looked up at
 // the nearest texel of a planar (x,z) or spherical mapping of the hit position:
 //   Lambert  : albedo rho(uv) (RGB), Russian roulette on mean(rho) as in
-//              CU/diffuse_shader.cu:165-168;
+//              CU/diffuse_shader.cu:86-88;
 //   conductor: GGX roughness alpha(uv) (isotropic), visible-normal sampling.
 // The estimator is the plain integrator's (shade_with() with the texel's values),
 // so a constant texture reproduces the plain render.  Sampling is "detached": the

@@ -6,10 +6,10 @@
 // becomes a loop carrying a throughput; the TEA state is threaded through as the
 // reference threads prd.seed, so every draw lands at the same stream position.
 // Depth rules are the reference's: Lambert shades while depth <= max_depth
-// (CU/diffuse_shader.cu:125), conductor and dielectric while depth < max_depth
+// (CU/diffuse_shader.cu:45), conductor and dielectric while depth < max_depth
 // (CU/metallic_shader.cu:47, CU/rough_transparent_shader.cu:68); emitters and the
 // environment answer at any depth.  emit_light follows CU/path_tracer.cu:46,
-// CU/diffuse_shader.cu:160,176 and CU/metallic_shader.cu:127.
+// CU/diffuse_shader.cu:80,96 and CU/metallic_shader.cu:127.
 //
 // shade_vertex() is the one place the shading arithmetic lives: the megakernel
 // calls it inside its loop, the wavefront kernels call it on queued hits, so

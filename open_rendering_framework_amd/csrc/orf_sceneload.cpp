@@ -12,7 +12,7 @@
 // and RoughTranslucentMaterial::loadFromJSON)
 // and the lights (ORF/Light.cpp: PointLight :22-40, DirectionalLight :75-93,
 // TriangleAreaLight :149-290 with applyTransformationMatrix, DiskAreaLight
-// :367-505, SphericalLight :507-611).  Every scene
+// :367-502, SphericalLight :507-607).  Every scene
 // element becomes the same C-ABI calls a hand-written host would make.
 //
 // Differences: relative paths resolve against the JSON file's directory (the

@@ -8,7 +8,7 @@
 //   search is a lower bound (first index with cdf >= xi).  env_pdf is the matching
 //   solid-angle density p_uv / (2 pi^2 sin(theta)).
 // Lights: ORF/LightSampler.h:20-37 (point, directional), :40-71 (triangle area light).
-// Lambert step: CU/diffuse_shader.cu:122-187; conductor step: CU/metallic_shader.cu:84-158.
+// Lambert step: CU/diffuse_shader.cu:42-107; conductor step: CU/metallic_shader.cu:84-158.
 // The environment as a light (NEE + power-heuristic MIS against cosine sampling) is
 // new: the reference's env importance sampling (CU/env_cameras.cu, CU/envmap_shader.cu)
 // is never compiled and its miss program ignores emit_light.
@@ -555,7 +555,7 @@ ORF_HD float ggx_ms_dielectric_eval(f3 w_i, f3 w_o, float eta, float ax, float a
 
 // ------------------------------------------------- walks of either distribution
 // The primitives above picked by the material's normal distribution, as the
-// wrappers of ORF/Microfacet.h:100-200 pick them.
+// wrappers of ORF/Microfacet.h:101-184 pick them.
 ORF_HD float ms_sample_height(int dist, f3 wr, float hr, float z, float ax, float ay)
 {
     return dist == GGX_DISTRIBUTION ? ggx_sample_height(wr, hr, z, ax, ay) : beckmann_sample_height(wr, hr, z, ax, ay);
@@ -810,7 +810,7 @@ ORF_HD float mis_power(float pa, float pb)
     return a2 + b2 > 0.0f ? a2 / (a2 + b2) : 0.0f;
 }
 
-// Shading frame of a hit: interpolated vertex normal (CU/triangle_mesh.cu:334),
+// Shading frame of a hit: interpolated vertex normal (CU/triangle_mesh.cu:69),
 // normalised again as rtTransformNormal + normalize do in the closest-hit programs.
 ORF_HD f3 shading_normal(const SceneGPU& sc, int prim, float b, float g, f3 e1, f3 e2, int4& sh)
 {

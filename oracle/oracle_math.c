@@ -994,7 +994,7 @@ R orc_ggx_ms_dielectric_eval(v3 w_i, v3 w_o, R eta, R ax, R ay, uint32_t* seed, 
     return sum;
 }
 
-/* ---- the walks of ORF/Microfacet.h:100-200 for either distribution (dist 1:
+/* ---- the walks of ORF/Microfacet.h:101-184 for either distribution (dist 1:
  * GGX, 0: Beckmann). */
 
 static R ms_sample_height(int dist, v3 wr, R hr, R z, R ax, R ay)

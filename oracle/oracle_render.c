@@ -692,7 +692,7 @@ static v3 hit_normal(const oscene* s, const ohit* h)
         n = orc_normalize(vcross(s->e1[t], s->e2[t]));
     } else {
         v3 n0 = s->normals[s->nidx[3 * t]], n1 = s->normals[s->nidx[3 * t + 1]], n2 = s->normals[s->nidx[3 * t + 2]];
-        /* normalize(n1*beta + n2*gamma + n0*(1-beta-gamma)) (CU/triangle_mesh.cu:334) */
+        /* normalize(n1*beta + n2*gamma + n0*(1-beta-gamma)) (CU/triangle_mesh.cu:69) */
         n = orc_normalize(vadd(vadd(vscale(n1, h->b), vscale(n2, h->g)), vscale(n0, (R)1 - h->b - h->g)));
     }
     /* normalize(rtTransformNormal(RT_OBJECT_TO_WORLD, shading_normal)): world-space geometry */
@@ -719,7 +719,7 @@ static int diff_texel(int mode, const R* uvp, v3 p, int tw, int th)
     return iv * tw + iu;
 }
 
-/* CU/diffuse_shader.cu:122-187 (+ environment MIS, new). */
+/* CU/diffuse_shader.cu:42-107 (+ environment MIS, new). */
 static v3 shade_lambert(const oscene* s, const orc_material* m, v3 o, v3 d, const ohit* h, int depth,
                         int* emit_light, uint32_t* t, ostats* st)
 {
