@@ -70,7 +70,7 @@ struct orf_context_t {
     orf::WavefrontPool wf_pool;
     double last_batch_seconds = 0.0;
     // differentiable renders
-    orf::DevBuf d_diff_views, d_diff_stats, d_diff_sss;
+    orf::DevBuf d_diff_views, d_diff_stats, d_diff_sss, d_diff_partial;
     long long last_batch_iterations = 0;
     // ray generation program: 0 path tracer, 1 depth tracer; the depth film
     int integrator = 0;

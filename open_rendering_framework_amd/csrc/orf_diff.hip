@@ -1,10 +1,15 @@
 // orf_diff.hip -- differentiable form: primal and path-replay adjoint kernels (gfx950).
 //
-// Both kernels map one wave64 workgroup to an 8x8 pixel tile of one view and run
-// every sample of a pixel in one lane (views of a differentiable render share a
-// size, so tiles are uniform).  The primal writes the per-view FP32 mean image;
-// the adjoint replays each sample twice (pass 1: radiance L; pass 2: gradient
-// terms against the radiance still to come) and adds texel gradients with FP32
+// Both kernels are persistent: work items are (view, 8x8 pixel tile, chunk of
+// kDiffChunk samples), split into 8 contiguous partitions, one per XCD; a wave
+// takes items of its XCD's partition with one atomic each, then steals from
+// the others (orf_mega.h), so an expensive tile no longer holds a whole wave
+// while the rest of the chip idles.  Views of a differentiable render share a
+// size, so tiles are uniform.  The primal writes each chunk's FP32 sums and
+// finite count per pixel; a second kernel adds a pixel's chunks in chunk order
+// and writes the per-view FP32 mean image (deterministic: no atomics).  The
+// adjoint replays each sample twice (pass 1: radiance L; pass 2: gradient terms
+// against the radiance still to come) and adds texel gradients with FP32
 // atomics (16 B per shading event).  Adjoint samples use frames shifted by
 // `frame_shift` so they are decorrelated from the primal's.
 #include <algorithm>
@@ -12,6 +17,7 @@
 #include "orf_common.h"
 #include "orf_diff.h"
 #include "orf_diff_host.h"
+#include "orf_mega.h"
 
 namespace orf {
 
@@ -52,76 +58,113 @@ struct AtomicSink {
 };
 
 template <unsigned kFeat>
-__global__ __launch_bounds__(kDiffBlock, diff_waves(kFeat)) void k_diff(SceneGPU sc, DiffParams dp, const WfRender* __restrict__ views,
-                                                     int tiles_x, int tiles_per_view, int adjoint,
-                                                     unsigned frame_shift, float* images, const float* dimages,
-                                                     float* g_albedo, float* g_rough, float* g_sss,
-                                                     unsigned long long* counters)
+__global__ __launch_bounds__(kDiffBlock, diff_waves(kFeat)) void k_diff(
+    SceneGPU sc, DiffParams dp, const WfRender* __restrict__ views, int tiles_x, int tiles_per_view, int chunks,
+    long long items, int adjoint, unsigned frame_shift, float4* partial, long long npix, const float* dimages,
+    float* g_albedo, float* g_rough, float* g_sss, unsigned long long* counters, unsigned long long* qctr)
 {
     __shared__ unsigned stack_lds[kLdsStack * kDiffBlock];
     unsigned* stk = stack_lds + threadIdx.x;
-    int view = blockIdx.x / tiles_per_view;
-    int tile = blockIdx.x % tiles_per_view;
-    const WfRender& R = views[view];
-    int x = (tile % tiles_x) * 8 + (threadIdx.x & 7);
-    int y = (tile / tiles_x) * 8 + (threadIdx.x >> 3);
-    if (x >= R.width || y >= R.height) return;
-    size_t px = ((size_t)view * R.height + y) * R.width + x;
+    const unsigned lane = threadIdx.x & 63u;
+    const int home = (int)(blockIdx.x % kXcdParts);
     unsigned bad = 0;
-    if (!adjoint) {
-        f3 acc = mk3(0.0f);
-        unsigned cnt = 0;
-        for (unsigned s = 0; s < R.spp; ++s) {
-            f3 dir;
-            uint32_t s0;
-            uint32_t rng = camera_ray(R.cam, R.width, R.height, x, y, R.frame0 + s, dir, &s0);
-            f3 L = diff_path<kFeat>(sc, dp, R.cam.eye, dir, rng, seed64(s0), stk, kDiffBlock);
-            if (finite3(L)) {
-                acc += L;
-                cnt++;
+    AtomicSink sink{g_albedo, g_rough, g_sss, mk3(0.0f), mk3(0.0f)};
+    for (int q = 0; q < kXcdParts; ++q) {
+        const int part = (home + q) % kXcdParts;
+        const long long lo = items * part / kXcdParts, hi = items * (part + 1) / kXcdParts;
+        for (;;) {
+            const long long item = mega_next(qctr, part, lo, hi);
+            if (item < 0) break;  // uniform across the wave
+            const long long vt = item / chunks;
+            const int chunk = (int)(item - vt * chunks);
+            const int view = (int)(vt / tiles_per_view), tile = (int)(vt - (long long)view * tiles_per_view);
+            const WfRender& R = views[view];
+            const int x = (tile % tiles_x) * 8 + (int)(lane & 7), y = (tile / tiles_x) * 8 + (int)(lane >> 3);
+            if (x >= R.width || y >= R.height) continue;  // every lane reaches the next item fetch
+            const size_t px = ((size_t)view * R.height + y) * R.width + x;
+            const unsigned k0 = (unsigned)chunk * kDiffChunk, k1 = min(k0 + kDiffChunk, R.spp);
+            if (!adjoint) {
+                f3 acc = mk3(0.0f);
+                unsigned cnt = 0;
+                for (unsigned s = k0; s < k1; ++s) {
+                    f3 dir;
+                    uint32_t s0;
+                    uint32_t rng = camera_ray(R.cam, R.width, R.height, x, y, R.frame0 + s, dir, &s0);
+                    f3 L = diff_path<kFeat>(sc, dp, R.cam.eye, dir, rng, seed64(s0), stk, kDiffBlock);
+                    if (finite3(L)) {
+                        acc += L;
+                        cnt++;
+                    } else {
+                        bad++;
+                    }
+                }
+                partial[(size_t)chunk * (size_t)npix + px] = make_float4(acc.x, acc.y, acc.z, (float)cnt);
             } else {
-                bad++;
+                const float sc_inv = 1.0f / (float)max(1u, R.spp);
+                const f3 delta = mk3(dimages[3 * px], dimages[3 * px + 1], dimages[3 * px + 2]) * sc_inv;
+                if (delta.x == 0.0f && delta.y == 0.0f && delta.z == 0.0f) continue;
+                for (unsigned s = k0; s < k1; ++s) {
+                    f3 dir;
+                    uint32_t s0;
+                    uint32_t rng = camera_ray(R.cam, R.width, R.height, x, y, R.frame0 + frame_shift + s, dir, &s0);
+                    f3 L = diff_path<kFeat>(sc, dp, R.cam.eye, dir, rng, seed64(s0), stk, kDiffBlock);
+                    if (!finite3(L)) {
+                        bad++;
+                        continue;
+                    }
+                    diff_replay<kFeat>(sc, dp, R.cam.eye, dir, rng, seed64(s0), L, delta, sink, stk, kDiffBlock);
+                }
             }
         }
-        float inv = cnt ? 1.0f / (float)cnt : 0.0f;
-        images[3 * px + 0] = acc.x * inv;
-        images[3 * px + 1] = acc.y * inv;
-        images[3 * px + 2] = acc.z * inv;
-    } else {
-        AtomicSink sink{g_albedo, g_rough, g_sss, mk3(0.0f), mk3(0.0f)};
-        float sc_inv = 1.0f / (float)max(1u, R.spp);
-        f3 delta = mk3(dimages[3 * px], dimages[3 * px + 1], dimages[3 * px + 2]) * sc_inv;
-        if (delta.x == 0.0f && delta.y == 0.0f && delta.z == 0.0f) return;
-        for (unsigned s = 0; s < R.spp; ++s) {
-            f3 dir;
-            uint32_t s0;
-            uint32_t rng = camera_ray(R.cam, R.width, R.height, x, y, R.frame0 + frame_shift + s, dir, &s0);
-            f3 L = diff_path<kFeat>(sc, dp, R.cam.eye, dir, rng, seed64(s0), stk, kDiffBlock);
-            if (!finite3(L)) {
-                bad++;
-                continue;
-            }
-            diff_replay<kFeat>(sc, dp, R.cam.eye, dir, rng, seed64(s0), L, delta, sink, stk, kDiffBlock);
-        }
-        sink.flush();
     }
+    if (adjoint) sink.flush();
     if (bad) atomicAdd(counters, (unsigned long long)bad);
 }
 
-void launch_diff(const SceneGPU& sc, const DiffParams& dp, const WfRender* d_views, int nviews, int width,
-                 int height, bool adjoint, unsigned frame_shift, float* images, const float* dimages,
-                 float* g_albedo, float* g_rough, float* g_sss, unsigned long long* counters, hipStream_t stream)
+// Mean image of every pixel from its chunks' sums, added in chunk order.
+__global__ __launch_bounds__(256) void k_diff_image(const float4* __restrict__ partial, int chunks, long long npix,
+                                                    float* __restrict__ images)
 {
-    int tiles_x = (width + 7) / 8, tiles_y = (height + 7) / 8;
-    int64_t blocks = (int64_t)tiles_x * tiles_y * nviews;
-    ORF_REQUIRE(blocks > 0 && blocks < 0x7fffffffLL, "diff launch: bad size");
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= npix) return;
+    f3 acc = mk3(0.0f);
+    float cnt = 0.0f;
+    for (int c = 0; c < chunks; ++c) {
+        const float4 p = partial[(size_t)c * (size_t)npix + i];
+        acc += mk3(p.x, p.y, p.z);
+        cnt += p.w;
+    }
+    const float inv = cnt > 0.0f ? 1.0f / cnt : 0.0f;
+    images[3 * i + 0] = acc.x * inv;
+    images[3 * i + 1] = acc.y * inv;
+    images[3 * i + 2] = acc.z * inv;
+}
+
+void launch_diff(const SceneGPU& sc, const DiffParams& dp, const WfRender* d_views, int nviews, int width, int height,
+                 unsigned spp, bool adjoint, unsigned frame_shift, float* images, float4* partial, const float* dimages,
+                 float* g_albedo, float* g_rough, float* g_sss, unsigned long long* counters,
+                 unsigned long long* qctr, hipStream_t stream)
+{
+    const int tiles_x = (width + 7) / 8, tiles_y = (height + 7) / 8;
+    const int chunks = (int)((spp + kDiffChunk - 1) / kDiffChunk);
+    const long long items = (long long)tiles_x * tiles_y * nviews * chunks;
+    const long long npix = (long long)width * height * nviews;
+    ORF_REQUIRE(items > 0 && items / kXcdParts < 0xF0000000LL && npix < 0x7fffffffLL * 256LL, "diff launch: bad size");
+    ORF_REQUIRE(adjoint || partial, "diff launch: no partial-sum buffer");
+    zero_queue_counters(qctr, stream);
+    const int waves = diff_waves(sc.features);
+    const dim3 grid((unsigned)persistent_waves(items, waves)), block(kDiffBlock);
     // the scene's material programs only, as the render kernels pick them
     auto kernel = sc.features == 0u ? k_diff<0u> : sc.features == kFeatDielectric ? k_diff<kFeatDielectric>
                                                                                    : k_diff<kFeatAll>;
-    hipLaunchKernelGGL(kernel, dim3((unsigned)blocks), dim3(kDiffBlock), 0, stream, sc, dp, d_views, tiles_x,
-                       tiles_x * tiles_y, adjoint ? 1 : 0, frame_shift, images, dimages, g_albedo, g_rough, g_sss,
-                       counters);
+    hipLaunchKernelGGL(kernel, grid, block, 0, stream, sc, dp, d_views, tiles_x, tiles_x * tiles_y, chunks, items,
+                       adjoint ? 1 : 0, frame_shift, partial, npix, dimages, g_albedo, g_rough, g_sss, counters, qctr);
     ORF_HIP_CHECK(hipGetLastError());
+    if (!adjoint) {
+        hipLaunchKernelGGL(k_diff_image, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, stream, partial, chunks,
+                           npix, images);
+        ORF_HIP_CHECK(hipGetLastError());
+    }
 }
 
 }  // namespace orf

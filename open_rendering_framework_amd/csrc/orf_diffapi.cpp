@@ -58,14 +58,17 @@ static DiffParams diff_params(orf_context_t* c, const orf_diff_desc* d, const fl
     return p;
 }
 
-static void stage_views(orf_context_t* c, const orf_render_item* v, int n, hipStream_t st, int& w, int& h)
+static void stage_views(orf_context_t* c, const orf_render_item* v, int n, hipStream_t st, int& w, int& h,
+                        unsigned& spp)
 {
     ORF_REQUIRE(v && n > 0 && n <= 4096, "diff: 1..4096 views required");
     w = v[0].width;
     h = v[0].height;
+    spp = v[0].spp;
     std::vector<WfRender> R(n);
     for (int i = 0; i < n; ++i) {
         ORF_REQUIRE(v[i].width == w && v[i].height == h, "diff: all views must share one size");
+        ORF_REQUIRE(v[i].spp == spp, "diff: all views must share one sample count");
         ORF_REQUIRE(w > 0 && h > 0 && w <= 16384 && h <= 16384 && v[i].spp > 0, "diff: bad view");
         std::memset(&R[i], 0, sizeof(WfRender));
         R[i].cam.eye = mk3(v[i].eye[0], v[i].eye[1], v[i].eye[2]);
@@ -80,7 +83,7 @@ static void stage_views(orf_context_t* c, const orf_render_item* v, int n, hipSt
     // the previous launch may still read the view table
     ORF_HIP_CHECK(hipStreamSynchronize(st));
     c->d_diff_views.upload(R);
-    c->d_diff_stats.ensure(8 * sizeof(unsigned long long));
+    c->d_diff_stats.ensure(16 * sizeof(unsigned long long));  // [0] non-finite, [8..15] work queues
 }
 
 }  // namespace orf
@@ -117,10 +120,13 @@ static void diff_render(orf_context_t* c, const orf_diff_desc* desc, const orf_r
     prepare(c, st);
     DiffParams dp = diff_params(c, desc, albedo, roughness, scattering);
     int w, h;
-    stage_views(c, views, nviews, st, w, h);
+    unsigned spp;
+    stage_views(c, views, nviews, st, w, h, spp);
     SceneGPU sc = diff_scene(c, dp, scattering, views[0].frame0, st);
-    launch_diff(sc, dp, c->d_diff_views.as<WfRender>(), nviews, w, h, false, 0, images, nullptr, nullptr, nullptr,
-                nullptr, c->d_diff_stats.as<unsigned long long>(), st);
+    c->d_diff_partial.ensure((size_t)diff_partial_entries(nviews, w, h, spp) * sizeof(float4));
+    unsigned long long* stats = c->d_diff_stats.as<unsigned long long>();
+    launch_diff(sc, dp, c->d_diff_views.as<WfRender>(), nviews, w, h, spp, false, 0, images,
+                c->d_diff_partial.as<float4>(), nullptr, nullptr, nullptr, nullptr, stats, stats + 8, st);
 }
 
 static void diff_adjoint(orf_context_t* c, const orf_diff_desc* desc, const orf_render_item* views, int32_t nviews,
@@ -132,12 +138,15 @@ static void diff_adjoint(orf_context_t* c, const orf_diff_desc* desc, const orf_
     prepare(c, st);
     DiffParams dp = diff_params(c, desc, albedo, roughness, scattering);
     int w, h;
-    stage_views(c, views, nviews, st, w, h);
+    unsigned spp;
+    stage_views(c, views, nviews, st, w, h, spp);
     // the probes of the primal's sample index, so the replay sees the primal's probe light
     SceneGPU sc = diff_scene(c, dp, scattering, views[0].frame0, st);
-    launch_diff(sc, dp, c->d_diff_views.as<WfRender>(), nviews, w, h, true, frame_shift, nullptr, d_images,
-                desc->albedo_material >= 0 ? g_albedo : nullptr, desc->roughness_material >= 0 ? g_roughness : nullptr,
-                dp.sss_mat >= 0 ? g_scattering : nullptr, c->d_diff_stats.as<unsigned long long>(), st);
+    unsigned long long* stats = c->d_diff_stats.as<unsigned long long>();
+    launch_diff(sc, dp, c->d_diff_views.as<WfRender>(), nviews, w, h, spp, true, frame_shift, nullptr, nullptr,
+                d_images, desc->albedo_material >= 0 ? g_albedo : nullptr,
+                desc->roughness_material >= 0 ? g_roughness : nullptr, dp.sss_mat >= 0 ? g_scattering : nullptr, stats,
+                stats + 8, st);
 }
 
 }  // namespace orf
