@@ -103,7 +103,7 @@ static double run_batch(orf_context_t* c, long long paths, int mode, const Wavef
     auto t0 = std::chrono::steady_clock::now();
     run_probe_pass(c, c->batch_renders.empty() ? 0u : c->batch_renders[0].frame0, st);
     if (mode == ORF_BATCH_PERSISTENT) {
-        // work items: (render, 8x8 tile, chunk of up to 8 samples), render by render
+        // work items: (render, 8x8 tile, chunk of up to kMegaChunk samples), render by render
         std::vector<PersistRender> pr(c->batch_renders.size());
         long long items = 0;
         for (size_t i = 0; i < pr.size(); ++i) {

@@ -31,10 +31,16 @@ constexpr unsigned kPersistTop = ORF_PERSIST_TOP;
 static_assert(kMegaTop <= (unsigned)kBvhTopOrdered && kPersistTop <= (unsigned)kBvhTopOrdered,
               "LDS top nodes must lie in the breadth-first prefix");
 constexpr int kXcdParts = 8;
+// Most samples per work item.  A lane runs its item's samples back to back, so
+// longer runs leave fewer lanes idle while a wave's longest run finishes; the
+// per-render launch shrinks the chunk until every wave has kMegaItemsPerWave
+// items (512x512 at 64 spp gets 2), the batched persistent launch, with
+// hundreds of items per wave, takes up to 16 (1,519 -> 1,628 Mpaths/s on the
+// 64-render batch against 4; 32 measured the same, 64 slower).
 #ifndef ORF_MEGA_CHUNK
-#define ORF_MEGA_CHUNK 4
+#define ORF_MEGA_CHUNK 16
 #endif
-constexpr unsigned kMegaChunk = ORF_MEGA_CHUNK;  // most samples per work item
+constexpr unsigned kMegaChunk = ORF_MEGA_CHUNK;
 #ifndef ORF_MEGA_ITEMS_PER_WAVE
 #define ORF_MEGA_ITEMS_PER_WAVE 16
 #endif
