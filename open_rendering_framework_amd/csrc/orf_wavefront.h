@@ -177,24 +177,30 @@ ORF_HD void start_path(const WfView& v, int slot, long long id)
     v.flags[slot] = SLOT_ACTIVE;
 }
 
-// Material kind of the queued hit of `slot` (or Q_MISS).
+// Shade queue of the queued hit of `slot`.  Q_MISS, Q_EMIT, Q_LAMBERT and Q_COND
+// hold only what the smallest material program set shades (misses, triangle
+// emitters, Lambert, single-scattering conductors), so their shade kernels are
+// compiled without the other materials (unless the scene needs the all-feature
+// set for its disk or sphere lights); every other hit -- the remaining
+// materials, multiple-scattering conductors, disk and sphere emitters -- goes
+// to Q_DIEL, shaded by the scene's full program set.
 ORF_HD int hit_kind(const WfView& v, int slot, const Hit& h)
 {
     if (h.prim < 0) return Q_MISS;
+    if (h.leaf < 0) return Q_DIEL;
     int4 sh = v.sc.shade[h.prim];
     const int* remap = v.remap + v.renders[v.st[slot].z].remap_offset;
-    int k = hit_material(v.sc, remap, sh.w).kind;
-    if (k == MAT_EMITTER) return Q_EMIT;
-    if (k == MAT_LAMBERT || k == MAT_INTERFACE) return Q_LAMBERT;
-    if (k == MAT_CONDUCTOR) return Q_COND;
-    if (k == MAT_DIELECTRIC || k == MAT_TRANSLUCENT || k == MAT_ROUGH_TRANSLUCENT || k == MAT_TRANSPARENT ||
-        k == MAT_NORMAL || k == MAT_FLAT || k == MAT_ROUGH_DIFFUSE)
-        return Q_DIEL;
-    return Q_EMIT;  // kinds without a program end their path like an emitter that does not emit
+    const MaterialGPU m = hit_material(v.sc, remap, sh.w);
+    if (m.kind == MAT_EMITTER) return Q_EMIT;
+    if (m.kind == MAT_LAMBERT) return Q_LAMBERT;
+    if (m.kind == MAT_CONDUCTOR && m.microfacet_model != MULTISCATTERING_MODEL) return Q_COND;
+    return Q_DIEL;
 }
 
-// Shade the hit (or miss) of `slot`; returns true when the path continues, and
-// fills `sr` with an NEE segment when one is due.
+// Shade the hit (or miss) of `slot` with the material programs of kFeat; returns
+// true when the path continues, and fills `sr` with an NEE segment when one is
+// due.
+template <unsigned kFeat>
 ORF_HD bool shade_slot(const WfView& v, int slot, ShadowRay& sr)
 {
     PathState ps;
@@ -213,7 +219,7 @@ ORF_HD bool shade_slot(const WfView& v, int slot, ShadowRay& sr)
         more = false;
     } else {
         const int* remap = v.remap + v.renders[v.st[slot].z].remap_offset;
-        more = shade_vertex(v.sc, remap, ps, h, sr);
+        more = shade_vertex<kFeat>(v.sc, remap, ps, h, sr);
     }
     store_path(v, slot, ps);
     if (!more) v.flags[slot] = SLOT_DONE;

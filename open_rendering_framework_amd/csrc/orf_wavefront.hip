@@ -6,7 +6,10 @@
 // (p = 0, 1) are captured into one hipGraph and replayed until every work item
 // has been traced and flushed; all launch shapes are fixed (grid-stride loops
 // over device-side queue counts), so replays need no host decisions.  Counters
-// are reset by a one-lane kernel, never by memset nodes.
+// are reset by a one-lane kernel, never by memset nodes.  The traversal and
+// shade kernels are compiled per material program set (kFeat, as the
+// megakernel's variants): with per-material queues, the miss, emitter, Lambert
+// and conductor queues run the smallest set whatever the scene holds.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -68,7 +71,7 @@ __global__ __launch_bounds__(256) void k_wf_regen(WfView v, int parity)
     }
 }
 
-template <int BLOCK>
+template <int BLOCK, unsigned kFeat>
 __global__ __launch_bounds__(BLOCK) void k_wf_extend(WfView v, int parity)
 {
     __shared__ unsigned stack_lds[kLdsStack * BLOCK];
@@ -83,7 +86,8 @@ __global__ __launch_bounds__(BLOCK) void k_wf_extend(WfView v, int parity)
         if (active) {
             float4 o = v.o[slot], d = v.d[slot];
             Hit h;
-            trace<false>(v.sc, mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z), v.sc.scene_epsilon, kRayMax, h, stk, BLOCK);
+            trace<false, kFeat == kFeatAll, (kFeat != 0u)>(v.sc, mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z),
+                                                           v.sc.scene_epsilon, kRayMax, h, stk, BLOCK);
             v.hit[slot] = make_float4(h.t, h.b, h.g, __uint_as_float((unsigned)h.prim));
             v.hleaf[slot] = h.leaf;
             kind = v.sort_policy ? hit_kind(v, slot, h) : Q_ALL;
@@ -102,6 +106,7 @@ __global__ __launch_bounds__(BLOCK) void k_wf_extend(WfView v, int parity)
     if (rays) atomicAdd(v.stats + 1, (unsigned long long)rays);
 }
 
+template <unsigned kFeat>
 __global__ __launch_bounds__(256) void k_wf_shade(WfView v, int parity, int kind)
 {
     unsigned n = v.cnt[C_KIND0 + kind];
@@ -114,7 +119,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfView v, int parity, int kind
         ShadowRay sr;
         sr.valid = false;
         bool more = false;
-        if (active) more = shade_slot(v, slot, sr);
+        if (active) more = shade_slot<kFeat>(v, slot, sr);
         unsigned e = wave_push(next_cnt, more);
         if (more) next_q[e] = slot;
         unsigned s = wave_push(v.cnt + C_SHADOW, sr.valid);
@@ -126,7 +131,7 @@ __global__ __launch_bounds__(256) void k_wf_shade(WfView v, int parity, int kind
     }
 }
 
-template <int BLOCK>
+template <int BLOCK, unsigned kFeat>
 __global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfView v)
 {
     __shared__ unsigned stack_lds[kLdsStack * BLOCK];
@@ -139,8 +144,8 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfView v)
             float4 o = v.sh_o[i], d = v.sh_d[i];
             int slot = (int)__float_as_uint(d.w);
             Hit h;
-            bool occ = trace<true>(v.sc, mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z), v.sc.scene_epsilon, o.w, h, stk,
-                                   BLOCK);
+            bool occ = trace<true, kFeat == kFeatAll, (kFeat != 0u)>(v.sc, mk3(o.x, o.y, o.z), mk3(d.x, d.y, d.z),
+                                                                     v.sc.scene_epsilon, o.w, h, stk, BLOCK);
             if (!occ) {
                 float4 c = v.sh_c[i];
                 float4 l = v.L[slot];
@@ -160,30 +165,43 @@ __global__ __launch_bounds__(BLOCK) void k_wf_shadow(WfView v)
 
 // ------------------------------------------------------------------ driver
 
-static const int kShadeKinds[] = {Q_MISS, Q_EMIT, Q_LAMBERT, Q_COND, Q_DIEL};
+static const int kLeanKinds[] = {Q_MISS, Q_EMIT, Q_LAMBERT, Q_COND};
 
-template <int BLOCK>
+template <int BLOCK, unsigned kFeat>
 static void enqueue_iteration(const WfView& v, int parity, int grid, hipStream_t st)
 {
     hipLaunchKernelGGL(k_wf_reset, dim3(1), dim3(64), 0, st, v, parity);
     int g256 = std::max(1, std::min(grid * BLOCK / 256, (v.capacity + 255) / 256));
     hipLaunchKernelGGL(k_wf_regen, dim3(g256), dim3(256), 0, st, v, parity);
-    hipLaunchKernelGGL(k_wf_extend<BLOCK>, dim3(grid), dim3(BLOCK), 0, st, v, parity);
+    hipLaunchKernelGGL((k_wf_extend<BLOCK, kFeat>), dim3(grid), dim3(BLOCK), 0, st, v, parity);
     if (v.sort_policy) {
-        for (int k : kShadeKinds) hipLaunchKernelGGL(k_wf_shade, dim3(g256), dim3(256), 0, st, v, parity, k);
+        // the smallest program set leaves out disk/sphere light sampling, which
+        // only scenes of the all-feature set hold
+        constexpr unsigned kLean = kFeat == kFeatAll ? kFeatAll : 0u;
+        for (int k : kLeanKinds) hipLaunchKernelGGL(k_wf_shade<kLean>, dim3(g256), dim3(256), 0, st, v, parity, k);
+        hipLaunchKernelGGL(k_wf_shade<kFeat>, dim3(g256), dim3(256), 0, st, v, parity, (int)Q_DIEL);
     } else {
-        hipLaunchKernelGGL(k_wf_shade, dim3(g256), dim3(256), 0, st, v, parity, (int)Q_ALL);
+        hipLaunchKernelGGL(k_wf_shade<kFeat>, dim3(g256), dim3(256), 0, st, v, parity, (int)Q_ALL);
     }
-    hipLaunchKernelGGL(k_wf_shadow<BLOCK>, dim3(grid), dim3(BLOCK), 0, st, v);
+    hipLaunchKernelGGL((k_wf_shadow<BLOCK, kFeat>), dim3(grid), dim3(BLOCK), 0, st, v);
 }
 
+template <unsigned kFeat>
+static void enqueue_iteration_feat(const WfView& v, int block, int parity, int grid, hipStream_t st)
+{
+    if (block == 64) enqueue_iteration<64, kFeat>(v, parity, grid, st);
+    else if (block == 128) enqueue_iteration<128, kFeat>(v, parity, grid, st);
+    else enqueue_iteration<256, kFeat>(v, parity, grid, st);
+}
+
+// The scene's material program set picks the kernel variants, as for the megakernel.
 static void enqueue_pairs(const WfView& v, int block, int grid, int pairs, hipStream_t st)
 {
     for (int k = 0; k < pairs; ++k)
         for (int p = 0; p < 2; ++p) {
-            if (block == 64) enqueue_iteration<64>(v, p, grid, st);
-            else if (block == 128) enqueue_iteration<128>(v, p, grid, st);
-            else enqueue_iteration<256>(v, p, grid, st);
+            if (v.sc.features == 0u) enqueue_iteration_feat<0u>(v, block, p, grid, st);
+            else if (v.sc.features == kFeatDielectric) enqueue_iteration_feat<kFeatDielectric>(v, block, p, grid, st);
+            else enqueue_iteration_feat<kFeatAll>(v, block, p, grid, st);
         }
     ORF_HIP_CHECK(hipGetLastError());
 }
