@@ -397,6 +397,13 @@ orf_status orf_diff_adjoint_ex(orf_context ctx, const orf_diff_desc* desc, const
                                int32_t nviews, const float* albedo, const float* roughness, const float* scattering,
                                const float* d_images, float* g_albedo, float* g_roughness, float* g_scattering,
                                uint32_t frame_shift, void* stream);
+/* Statistics of the last differentiable launch (render or adjoint), after a
+ * device synchronisation: out[0] = samples dropped as non-finite, out[1] =
+ * adjoint samples whose replay ended on a radiance other than their first
+ * pass's.  The replay re-runs the first pass's path on the same random numbers,
+ * so out[1] is zero unless the two passes took different paths -- the gradients
+ * of such a launch are not to be trusted. */
+orf_status orf_diff_counters(orf_context ctx, uint64_t out[2]);
 
 /* ------------------------------------------------------------------------
  * Translucent materials ("subsurface_scattering_shader.cu", smooth boundary;

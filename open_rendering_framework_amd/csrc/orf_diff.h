@@ -130,9 +130,13 @@ ORF_HD bool diff_vertex(const SceneGPU& sc, const DiffParams& dp, PathState& ps,
 
 // The differentiable form takes scenes without disk and sphere emitters (the
 // C-ABI refuses them), so its traversals leave the emitter test out.  With the
-// test compiled in (and never taken), hipcc of ROCm 7.2 produced an adjoint
-// kernel whose roughness gradients were wrong on gfx950 while a host build of
-// the same code was right.
+// test compiled in (and never taken: the scenes hold no such emitter), the
+// adjoint kernel built by hipcc of ROCm 7.2 for gfx950 computes a different
+// pass-1 radiance than its replay for about a fifth of the samples, although
+// both are this same source on the same random numbers (the primal kernel's
+// images stay bit-identical to the build without the test), and the texture
+// gradients come out wrong.  The adjoint kernel therefore checks every replay
+// against its pass 1 and counts the samples that differ (orf_diff_counters).
 
 // Pass 1: the sample's radiance (the plain estimator with textured parameters).
 template <unsigned kFeat>
@@ -161,16 +165,21 @@ ORF_HD f3 diff_path(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32
 
 // Pass 2: replay the same random numbers and hand each textured vertex's
 // gradient terms to `sink` (albedo(texel, channel, value), rough(texel, value));
-// delta = dLoss/dL of this sample, L_total its pass-1 radiance.
+// delta = dLoss/dL of this sample, L_total its pass-1 radiance.  Returns the
+// replay's own radiance, which equals L_total bit for bit when both passes ran
+// the same path.
 template <unsigned kFeat, class Sink>
-ORF_HD void diff_replay(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32_t rng, Seed64 s64, f3 L_total,
-                        f3 delta, Sink& sink, unsigned* stk, int stride)
+ORF_HD f3 diff_replay(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32_t rng, Seed64 s64, f3 L_total,
+                      f3 delta, Sink& sink, unsigned* stk, int stride)
 {
     PathState ps;
     path_begin(ps, o, d, rng, s64);
     for (;;) {
         Hit h;
-        if (!trace<false, false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk, stride)) break;
+        if (!trace<false, false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk, stride)) {
+            shade_miss(sc, ps);
+            break;
+        }
         f3 rem_before = L_total - ps.L;  // radiance from this vertex on
         ShadowRay sr;
         DiffVertex dv;
@@ -196,6 +205,7 @@ ORF_HD void diff_replay(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, ui
         }
         if (!more) break;
     }
+    return ps.L;
 }
 
 }  // namespace orf

@@ -11,7 +11,9 @@
 // adjoint replays each sample twice (pass 1: radiance L; pass 2: gradient terms
 // against the radiance still to come) and adds texel gradients with FP32
 // atomics (16 B per shading event).  Adjoint samples use frames shifted by
-// `frame_shift` so they are decorrelated from the primal's.
+// `frame_shift` so they are decorrelated from the primal's.  The adjoint counts
+// the samples whose replay ends on a radiance other than their pass 1's
+// (counters[1]; zero unless the two passes took different paths).
 #include <algorithm>
 
 #include "orf_common.h"
@@ -67,7 +69,7 @@ __global__ __launch_bounds__(kDiffBlock, diff_waves(kFeat)) void k_diff(
     unsigned* stk = stack_lds + threadIdx.x;
     const unsigned lane = threadIdx.x & 63u;
     const int home = (int)(blockIdx.x % kXcdParts);
-    unsigned bad = 0;
+    unsigned bad = 0, mismatch = 0;
     AtomicSink sink{g_albedo, g_rough, g_sss, mk3(0.0f), mk3(0.0f)};
     for (int q = 0; q < kXcdParts; ++q) {
         const int part = (home + q) % kXcdParts;
@@ -112,13 +114,16 @@ __global__ __launch_bounds__(kDiffBlock, diff_waves(kFeat)) void k_diff(
                         bad++;
                         continue;
                     }
-                    diff_replay<kFeat>(sc, dp, R.cam.eye, dir, rng, seed64(s0), L, delta, sink, stk, kDiffBlock);
+                    const f3 Lr =
+                        diff_replay<kFeat>(sc, dp, R.cam.eye, dir, rng, seed64(s0), L, delta, sink, stk, kDiffBlock);
+                    if (!(Lr.x == L.x && Lr.y == L.y && Lr.z == L.z)) mismatch++;
                 }
             }
         }
     }
     if (adjoint) sink.flush();
     if (bad) atomicAdd(counters, (unsigned long long)bad);
+    if (mismatch) atomicAdd(counters + 1, (unsigned long long)mismatch);
 }
 
 // Mean image of every pixel from its chunks' sums, added in chunk order.
@@ -152,6 +157,7 @@ void launch_diff(const SceneGPU& sc, const DiffParams& dp, const WfRender* d_vie
     ORF_REQUIRE(items > 0 && items / kXcdParts < 0xF0000000LL && npix < 0x7fffffffLL * 256LL, "diff launch: bad size");
     ORF_REQUIRE(adjoint || partial, "diff launch: no partial-sum buffer");
     zero_queue_counters(qctr, stream);
+    zero_queue_counters(counters, stream);  // this launch's statistics (8 entries)
     const int waves = diff_waves(sc.features);
     const dim3 grid((unsigned)persistent_waves(items, waves)), block(kDiffBlock);
     // the scene's material programs only, as the render kernels pick them

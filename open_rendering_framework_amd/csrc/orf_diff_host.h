@@ -17,8 +17,10 @@ inline long long diff_partial_entries(int nviews, int width, int height, unsigne
 
 // One persistent launch over `nviews` views of width x height with `spp`
 // samples each: primal (images, through `partial`: diff_partial_entries float4)
-// or adjoint (dimages -> g_albedo, g_rough, g_sss).  counters[0] receives
-// dropped non-finite samples; qctr: 8 device work-queue counters.
+// or adjoint (dimages -> g_albedo, g_rough, g_sss).  counters: 8 device
+// statistics zeroed at each launch, [0] dropped non-finite samples, [1] (adjoint)
+// samples whose replay ended on a radiance other than their pass 1's; qctr: 8
+// device work-queue counters.
 void launch_diff(const SceneGPU& sc, const DiffParams& dp, const WfRender* d_views, int nviews, int width, int height,
                  unsigned spp, bool adjoint, unsigned frame_shift, float* images, float4* partial, const float* dimages,
                  float* g_albedo, float* g_rough, float* g_sss, unsigned long long* counters,

@@ -83,7 +83,8 @@ static void stage_views(orf_context_t* c, const orf_render_item* v, int n, hipSt
     // the previous launch may still read the view table
     ORF_HIP_CHECK(hipStreamSynchronize(st));
     c->d_diff_views.upload(R);
-    c->d_diff_stats.ensure(16 * sizeof(unsigned long long));  // [0] non-finite, [8..15] work queues
+    // [0] non-finite samples, [1] replay mismatches (adjoint), [8..15] work queues
+    c->d_diff_stats.ensure(16 * sizeof(unsigned long long));
 }
 
 }  // namespace orf
@@ -196,5 +197,17 @@ extern "C" orf_status orf_diff_adjoint_ex(orf_context ctx, const orf_diff_desc* 
     ORF_REQUIRE(!desc || desc->scattering_material < 0 || g_scattering, "orf_diff_adjoint_ex: null scattering gradient");
     diff_adjoint(ctx_check(ctx), desc, views, nviews, albedo, roughness, scattering, d_images, g_albedo, g_roughness,
                  g_scattering, frame_shift, (hipStream_t)stream);
+    ORF_API_END
+}
+
+extern "C" orf_status orf_diff_counters(orf_context ctx, uint64_t out[2])
+{
+    ORF_API_BEGIN
+    orf_context_t* c = ctx_check(ctx);
+    ORF_REQUIRE(out, "orf_diff_counters: null output");
+    out[0] = out[1] = 0;
+    if (!c->d_diff_stats.p) return ORF_SUCCESS;
+    ORF_HIP_CHECK(hipDeviceSynchronize());
+    ORF_HIP_CHECK(hipMemcpy(out, c->d_diff_stats.p, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost));
     ORF_API_END
 }

@@ -73,6 +73,7 @@ _SIGS = {
     "orf_film_read": [c_vp, c_vp, c_vp],
     "orf_film_read_sums": [c_vp, c_vp],
     "orf_counters": [c_vp, c_vp],
+    "orf_diff_counters": [c_vp, c_vp],
     "orf_gen_icosphere": [ctypes.c_int, c_u32, ctypes.c_float, ctypes.c_float, c_vp, c_vp,
                           ctypes.POINTER(c_i64), c_vp, ctypes.POINTER(c_i64)],
     "orf_gen_sky_envmap": [ctypes.c_int, ctypes.c_int, c_vp, ctypes.c_float, ctypes.c_float, c_vp],
@@ -537,6 +538,15 @@ class Context:
         out = np.zeros(4, np.uint64)
         call("orf_batch_counters", self.handle, _np_ptr(out))
         return dict(zip(("nonfinite", "ext_rays", "shadow_rays", "paths"), [int(x) for x in out]))
+
+    def diff_counters(self):
+        """Statistics of the last differentiable launch: non-finite samples and
+        adjoint samples whose replay did not reproduce their first pass."""
+        import numpy as np
+
+        out = np.zeros(2, np.uint64)
+        call("orf_diff_counters", self.handle, _np_ptr(out))
+        return dict(zip(("nonfinite", "replay_mismatch"), [int(x) for x in out]))
 
     def diff_render(self, desc, views, albedo, roughness, images=None, stream=None, scattering=None):
         """Per-view mean images (nviews, h, w, 3) of the textured scene (device tensors);

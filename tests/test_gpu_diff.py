@@ -10,7 +10,9 @@ the C-ABI and the torch autograd extension.
     values, on 64 random texture entries: relative error <= 1e-2 where the
     gradient is above 1% of the largest, absolute <= 1e-4 * max elsewhere;
   * the torch Function's forward equals the C-ABI call bit for bit, its backward
-    equals orf_diff_adjoint up to FP32 atomic summation order.
+    equals orf_diff_adjoint up to FP32 atomic summation order;
+  * every adjoint launch reports zero samples whose replay did not end on the
+    radiance of its first pass (orf_diff_counters).
 """
 import sys
 
@@ -95,7 +97,14 @@ def _fd_setup():
     dimg = rng.standard_normal((2, W, W, 3)).astype(np.float32)
     ga, gr = ctx.diff_adjoint(_desc(rt), views, torch.from_numpy(a).cuda(), torch.from_numpy(r).cuda(),
                               torch.from_numpy(dimg).cuda(), frame_shift=SHIFT)
+    _assert_replays_match(ctx)
     return d, views, a, r, dimg, ga.cpu().numpy(), gr.cpu().numpy()
+
+
+def _assert_replays_match(ctx):
+    st = ctx.diff_counters()
+    _say("  adjoint counters %s" % st)
+    assert st["replay_mismatch"] == 0, st
 
 
 def test_adjoint_matches_fp64_central_differences():
@@ -223,6 +232,7 @@ def test_scattering_adjoint_matches_fp64_central_differences(dipole_model):
     rng = np.random.default_rng(17)
     dimg = rng.standard_normal(img.shape).astype(np.float32)
     _, _, g = ctx.diff_adjoint_sss(desc, views, None, None, theta, torch.from_numpy(dimg).cuda(), frame_shift=SHIFT)
+    _assert_replays_match(ctx)
     g = g.cpu().numpy().astype(np.float64)
     # the probes are frozen sampling inputs: the oracle gathers from the GPU's probe
     # values (its own probe pass differs from them in the rare probes whose paths
@@ -322,6 +332,7 @@ def test_inverse_rendering_workload_adjoint_matches_fp64_central_differences():
     dimg = np.random.default_rng(3).standard_normal((2, size, size, 3)).astype(np.float32)
     ga, gr = ctx.diff_adjoint(desc, views, torch.from_numpy(a).cuda(), torch.from_numpy(r).cuda(),
                               torch.from_numpy(dimg).cuda(), frame_shift=SHIFT)
+    _assert_replays_match(ctx)
     g = np.concatenate([ga.cpu().numpy().ravel(), gr.cpu().numpy().ravel()]).astype(np.float64)
     o64 = OracleScene(d, fp64=True)
     a0, r0, dd = a.astype(np.float64), r.astype(np.float64), dimg.astype(np.float64)
