@@ -208,4 +208,85 @@ ORF_HD f3 diff_replay(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint
     return ps.L;
 }
 
+// Gradient-carrying vertices an adjoint sample keeps in registers and scratch
+// before it falls back to a replay (a path shades at most max_depth + 1 vertices
+// of the texture kinds; 8 + 1 at the configured depth).
+constexpr int kDiffRec = 12;
+
+// Adjoint of one sample in a single pass: the path of diff_path, whose
+// gradient-carrying vertices keep what diff_replay's terms need -- the path's
+// radiance where the term takes the rest (before the vertex for albedo and the
+// translucent beam, after its scattering for roughness) and the vertex's
+// coefficients -- until the sample's radiance L_total is known; then every term
+// goes to `sink` as diff_replay computes it.  A replay measures those radiances
+// at the same points of the same computation, so the terms are the replay's,
+// without the second traversal of the path.  Returns the radiance; `spill` is
+// set, and nothing reaches the sink, when the path holds more than kDiffRec such
+// vertices (the caller then replays it).
+template <unsigned kFeat, class Sink>
+ORF_HD f3 diff_adjoint_sample(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32_t rng, Seed64 s64,
+                              f3 delta, Sink& sink, unsigned* stk, int stride, bool& spill)
+{
+    constexpr bool kSss = (kFeat & kFeatTranslucent) != 0;
+    int kt[kDiffRec];  // kind << 28 | texel
+    f3 lr[kDiffRec], cr[kDiffRec];
+    SssGrad sgr[kSss ? kDiffRec : 1];
+    int n = 0;
+    spill = false;
+    PathState ps;
+    path_begin(ps, o, d, rng, s64);
+    for (;;) {
+        Hit h;
+        if (!trace<false, false>(sc, ps.o, ps.d, sc.scene_epsilon, kRayMax, h, stk, stride)) {
+            shade_miss(sc, ps);
+            break;
+        }
+        const f3 L_before = ps.L;
+        ShadowRay sr;
+        DiffVertex dv;
+        bool more = diff_vertex<kFeat>(sc, dp, ps, h, sr, dv, true);
+        if (sr.valid) {
+            Hit hs;
+            if (!trace<true, false>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) shadow_unoccluded(ps, sr);
+        }
+        if (dv.kind != 0) {
+            if (n == kDiffRec) {
+                spill = true;
+            } else {
+                kt[n] = (dv.kind << 28) | (dv.kind == 3 ? 0 : dv.texel);
+                lr[n] = dv.kind == 2 ? ps.L : L_before;
+                cr[n] = dv.kind == 1 ? dv.rho : mk3(dv.kind == 2 ? dv.dlogw : 0.0f, 0.0f, 0.0f);
+                if constexpr (kSss) {
+                    if (dv.kind == 3) sgr[n] = dv.sg;
+                }
+                ++n;
+            }
+        }
+        if (!more) break;
+    }
+    const f3 L_total = ps.L;
+    if (spill || !finite3(L_total)) return L_total;
+    for (int k = 0; k < n; ++k) {
+        const int kind = kt[k] >> 28, texel = kt[k] & 0x0FFFFFFF;
+        const f3 rem = L_total - lr[k];
+        if (kSss && kind == 3) {
+            const SssGrad& g = sgr[kSss ? k : 0];
+            f3 ga = delta * g.dL_a, gs = delta * g.dL_s;
+            if (g.beam) {
+                ga += delta * rem * g.dlog_a;
+                gs += delta * rem * g.dlog_s;
+            }
+            sink.sss(ga * dp.sss_scale, gs * dp.sss_scale);
+        } else if (kind == 1) {
+            const f3 rho = cr[k];
+            if (rho.x != 0.0f) sink.albedo(texel, 0, delta.x * rem.x / rho.x);
+            if (rho.y != 0.0f) sink.albedo(texel, 1, delta.y * rem.y / rho.y);
+            if (rho.z != 0.0f) sink.albedo(texel, 2, delta.z * rem.z / rho.z);
+        } else if (kind == 2) {
+            sink.rough(texel, dot(delta, rem) * cr[k].x);
+        }
+    }
+    return L_total;
+}
+
 }  // namespace orf

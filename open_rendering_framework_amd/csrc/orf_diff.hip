@@ -8,12 +8,15 @@
 // size, so tiles are uniform.  The primal writes each chunk's FP32 sums and
 // finite count per pixel; a second kernel adds a pixel's chunks in chunk order
 // and writes the per-view FP32 mean image (deterministic: no atomics).  The
-// adjoint replays each sample twice (pass 1: radiance L; pass 2: gradient terms
-// against the radiance still to come) and adds texel gradients with FP32
-// atomics (16 B per shading event).  Adjoint samples use frames shifted by
-// `frame_shift` so they are decorrelated from the primal's.  The adjoint counts
-// the samples whose replay ends on a radiance other than their pass 1's
-// (counters[1]; zero unless the two passes took different paths).
+// adjoint computes path-replay gradient terms (each gradient vertex's share of
+// the radiance still to come) and adds texel gradients with FP32 atomics (16 B
+// per shading event).  Adjoint samples use frames shifted by
+// `frame_shift` so they are decorrelated from the primal's.  An adjoint sample
+// runs its path once, keeping each gradient vertex's terms until the radiance is
+// known (diff_adjoint_sample); a path with more such vertices than that record
+// holds is replayed (diff_replay), and the adjoint counts the replays that end
+// on a radiance other than their first pass's (counters[1]; zero unless the two
+// passes took different paths).
 #include <algorithm>
 
 #include "orf_common.h"
@@ -109,14 +112,18 @@ __global__ __launch_bounds__(kDiffBlock, diff_waves(kFeat)) void k_diff(
                     f3 dir;
                     uint32_t s0;
                     uint32_t rng = camera_ray(R.cam, R.width, R.height, x, y, R.frame0 + frame_shift + s, dir, &s0);
-                    f3 L = diff_path<kFeat>(sc, dp, R.cam.eye, dir, rng, seed64(s0), stk, kDiffBlock);
+                    bool spill;
+                    f3 L = diff_adjoint_sample<kFeat>(sc, dp, R.cam.eye, dir, rng, seed64(s0), delta, sink, stk,
+                                                      kDiffBlock, spill);
                     if (!finite3(L)) {
                         bad++;
                         continue;
                     }
-                    const f3 Lr =
-                        diff_replay<kFeat>(sc, dp, R.cam.eye, dir, rng, seed64(s0), L, delta, sink, stk, kDiffBlock);
-                    if (!(Lr.x == L.x && Lr.y == L.y && Lr.z == L.z)) mismatch++;
+                    if (spill) {  // more gradient vertices than the record holds: replay the path
+                        const f3 Lr = diff_replay<kFeat>(sc, dp, R.cam.eye, dir, rng, seed64(s0), L, delta, sink,
+                                                         stk, kDiffBlock);
+                        if (!(Lr.x == L.x && Lr.y == L.y && Lr.z == L.z)) mismatch++;
+                    }
                 }
             }
         }
