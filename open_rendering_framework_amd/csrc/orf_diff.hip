@@ -18,6 +18,7 @@
 // on a radiance other than their first pass's (counters[1]; zero unless the two
 // passes took different paths).
 #include <algorithm>
+#include <cstdlib>
 
 #include "orf_common.h"
 #include "orf_diff.h"
@@ -26,27 +27,38 @@
 
 namespace orf {
 
-constexpr int kDiffBlock = 64;
-
 // Waves per SIMD the register budget is sized for: the Lambert / conductor /
 // emitter form fits 128 VGPRs without spilling (4 waves); the larger material
 // sets keep the compiler's own allocation.
 constexpr int diff_waves(unsigned features) { return features == 0u ? 4 : 1; }
+// One workgroup fills a CU at that occupancy (4 SIMDs), so the CU's waves share
+// one LDS copy of the texel gradients.
+constexpr int diff_block(unsigned features) { return 256 * diff_waves(features); }
+// LDS a workgroup may give the texel gradients (the 64x64 albedo + roughness
+// textures of the inverse-rendering workload take 64 KB); larger textures add
+// straight to global memory.
+constexpr size_t kDiffLdsGradBytes = 96 * 1024;
 
-// Texel gradients land in global FP32 arrays through atomics; the translucent
-// coefficients' six gradients are summed per lane and added once per lane.
+// Texel gradients go to FP32 arrays through atomics: the workgroup's LDS copy
+// when the launch gives one (flushed to global memory once at the end), else
+// the global arrays; the translucent coefficients' six gradients are summed per
+// lane and added once per lane.
 struct AtomicSink {
     float* g_albedo;
     float* g_rough;
     float* g_sss;  // d/d absorption RGB, d/d scattering RGB
     f3 ga, gs;
+    float* l_albedo;  // LDS copies (null: global atomics)
+    float* l_rough;
     __device__ void albedo(int texel, int c, float v)
     {
-        if (g_albedo) atomicAdd(g_albedo + 3 * (size_t)texel + c, v);
+        if (l_albedo) atomicAdd(l_albedo + 3 * texel + c, v);
+        else if (g_albedo) atomicAdd(g_albedo + 3 * (size_t)texel + c, v);
     }
     __device__ void rough(int texel, float v)
     {
-        if (g_rough) atomicAdd(g_rough + texel, v);
+        if (l_rough) atomicAdd(l_rough + texel, v);
+        else if (g_rough) atomicAdd(g_rough + texel, v);
     }
     __device__ void sss(f3 a, f3 s)
     {
@@ -63,17 +75,29 @@ struct AtomicSink {
 };
 
 template <unsigned kFeat>
-__global__ __launch_bounds__(kDiffBlock, diff_waves(kFeat)) void k_diff(
+__global__ __launch_bounds__(diff_block(kFeat), diff_waves(kFeat)) void k_diff(
     SceneGPU sc, DiffParams dp, const WfRender* __restrict__ views, int tiles_x, int tiles_per_view, int chunks,
     long long items, int adjoint, unsigned frame_shift, float4* partial, long long npix, const float* dimages,
-    float* g_albedo, float* g_rough, float* g_sss, unsigned long long* counters, unsigned long long* qctr)
+    float* g_albedo, float* g_rough, float* g_sss, unsigned long long* counters, unsigned long long* qctr,
+    int lds_grad)
 {
+    constexpr int kDiffBlock = diff_block(kFeat);
     __shared__ unsigned stack_lds[kLdsStack * kDiffBlock];
     unsigned* stk = stack_lds + threadIdx.x;
     const unsigned lane = threadIdx.x & 63u;
     const int home = (int)(blockIdx.x % kXcdParts);
     unsigned bad = 0, mismatch = 0;
-    AtomicSink sink{g_albedo, g_rough, g_sss, mk3(0.0f), mk3(0.0f)};
+    AtomicSink sink{g_albedo, g_rough, g_sss, mk3(0.0f), mk3(0.0f), nullptr, nullptr};
+    // the workgroup's LDS copy of the texel gradients (dynamic LDS, zeroed here)
+    extern __shared__ float grad_lds[];
+    const int texels = dp.tw * dp.th;
+    const int n_albedo = g_albedo ? 3 * texels : 0, n_rough = g_rough ? texels : 0;
+    if (lds_grad) {
+        sink.l_albedo = g_albedo ? grad_lds : nullptr;
+        sink.l_rough = g_rough ? grad_lds + n_albedo : nullptr;
+        for (int i = threadIdx.x; i < n_albedo + n_rough; i += kDiffBlock) grad_lds[i] = 0.0f;
+        __syncthreads();
+    }
     for (int q = 0; q < kXcdParts; ++q) {
         const int part = (home + q) % kXcdParts;
         const long long lo = items * part / kXcdParts, hi = items * (part + 1) / kXcdParts;
@@ -129,6 +153,17 @@ __global__ __launch_bounds__(kDiffBlock, diff_waves(kFeat)) void k_diff(
         }
     }
     if (adjoint) sink.flush();
+    if (lds_grad) {  // every wave of the workgroup has left its item loop
+        __syncthreads();
+        for (int i = threadIdx.x; i < n_albedo; i += kDiffBlock) {
+            const float v = grad_lds[i];
+            if (v != 0.0f) atomicAdd(g_albedo + i, v);
+        }
+        for (int i = threadIdx.x; i < n_rough; i += kDiffBlock) {
+            const float v = grad_lds[n_albedo + i];
+            if (v != 0.0f) atomicAdd(g_rough + i, v);
+        }
+    }
     if (bad) atomicAdd(counters, (unsigned long long)bad);
     if (mismatch) atomicAdd(counters + 1, (unsigned long long)mismatch);
 }
@@ -165,13 +200,24 @@ void launch_diff(const SceneGPU& sc, const DiffParams& dp, const WfRender* d_vie
     ORF_REQUIRE(adjoint || partial, "diff launch: no partial-sum buffer");
     zero_queue_counters(qctr, stream);
     zero_queue_counters(counters, stream);  // this launch's statistics (8 entries)
-    const int waves = diff_waves(sc.features);
-    const dim3 grid((unsigned)persistent_waves(items, waves)), block(kDiffBlock);
+    const int waves = diff_waves(sc.features), bsize = diff_block(sc.features), wpb = bsize / 64;
+    const dim3 grid((unsigned)((persistent_waves(items, waves) + wpb - 1) / wpb)), block((unsigned)bsize);
+    // texel gradients in LDS when they fit beside the traversal stacks
+    const size_t grad_bytes =
+        adjoint ? ((g_albedo ? 3u : 0u) + (g_rough ? 1u : 0u)) * (size_t)dp.tw * (size_t)dp.th * sizeof(float) : 0;
+    int dev = 0, lds_max = 0;
+    ORF_HIP_CHECK(hipGetDevice(&dev));
+    ORF_HIP_CHECK(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const size_t stack_bytes = (size_t)kLdsStack * bsize * sizeof(unsigned);
+    const char* force_global = std::getenv("ORF_DIFF_GLOBAL_GRAD_ATOMICS");
+    const bool lds = grad_bytes > 0 && grad_bytes <= kDiffLdsGradBytes &&
+                     stack_bytes + grad_bytes <= (size_t)lds_max && !(force_global && force_global[0] == '1');
     // the scene's material programs only, as the render kernels pick them
     auto kernel = sc.features == 0u ? k_diff<0u> : sc.features == kFeatDielectric ? k_diff<kFeatDielectric>
                                                                                    : k_diff<kFeatAll>;
-    hipLaunchKernelGGL(kernel, grid, block, 0, stream, sc, dp, d_views, tiles_x, tiles_x * tiles_y, chunks, items,
-                       adjoint ? 1 : 0, frame_shift, partial, npix, dimages, g_albedo, g_rough, g_sss, counters, qctr);
+    hipLaunchKernelGGL(kernel, grid, block, lds ? grad_bytes : 0, stream, sc, dp, d_views, tiles_x, tiles_x * tiles_y,
+                       chunks, items, adjoint ? 1 : 0, frame_shift, partial, npix, dimages, g_albedo, g_rough, g_sss,
+                       counters, qctr, lds ? 1 : 0);
     ORF_HIP_CHECK(hipGetLastError());
     if (!adjoint) {
         hipLaunchKernelGGL(k_diff_image, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, stream, partial, chunks,

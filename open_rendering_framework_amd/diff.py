@@ -36,7 +36,10 @@ def build_extension(verbose=False):
     cmd += ["-I" + d for d in ce.include_paths("cuda")]
     cmd += [SOURCE, "-o", out + ".tmp"]
     cmd += ["-L" + d for d in ce.library_paths("cuda")]
-    cmd += ["-L" + PACKAGE_DIR, "-lorf_rt", "-Wl,-rpath," + PACKAGE_DIR,
+    # liborf_rt.so is found next to the package, wherever the package sits (the
+    # extension lives in BUILD_DIR, one level below it)
+    rel = os.path.relpath(PACKAGE_DIR, BUILD_DIR)
+    cmd += ["-L" + PACKAGE_DIR, "-lorf_rt", "-Wl,-rpath,$ORIGIN/" + rel,
             "-lc10", "-lc10_hip", "-ltorch", "-ltorch_cpu", "-ltorch_hip", "-ltorch_python", "-lamdhip64"]
     if verbose:
         print(" ".join(cmd))

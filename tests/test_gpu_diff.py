@@ -151,6 +151,34 @@ def test_adjoint_matches_fp64_central_differences():
     assert np.all(np.abs(got - fd)[~big] <= 1e-4 * np.abs(fd).max())
 
 
+def test_lds_and_global_gradient_accumulation_agree():
+    """The adjoint adds texel gradients into a per-workgroup LDS copy when the
+    textures fit (flushed once per workgroup) and straight into global memory
+    otherwise; both give the same gradients up to FP32 summation order."""
+    import os
+
+    import torch
+
+    from open_rendering_framework_amd import rt
+
+    ctx = scenes.to_context(_scene())
+    views = scenes.ring_views(n=2, size=W, spp=8)
+    a, r = scenes.diff_textures(TEX, seed=5)
+    dimg = np.random.default_rng(3).standard_normal((2, W, W, 3)).astype(np.float32)
+    args = (_desc(rt), views, torch.from_numpy(a).cuda(), torch.from_numpy(r).cuda(), torch.from_numpy(dimg).cuda())
+    ga0, gr0 = ctx.diff_adjoint(*args, frame_shift=SHIFT)
+    os.environ["ORF_DIFF_GLOBAL_GRAD_ATOMICS"] = "1"
+    try:
+        ga1, gr1 = ctx.diff_adjoint(*args, frame_shift=SHIFT)
+    finally:
+        del os.environ["ORF_DIFF_GLOBAL_GRAD_ATOMICS"]
+    _assert_replays_match(ctx)
+    for x, y in ((ga0, ga1), (gr0, gr1)):
+        x, y = x.cpu().numpy().astype(np.float64), y.cpu().numpy().astype(np.float64)
+        assert np.abs(x).max() > 0
+        np.testing.assert_allclose(x, y, rtol=1e-4, atol=1e-6 * np.abs(y).max())
+
+
 def test_torch_function_matches_c_abi():
     import torch
 
