@@ -19,6 +19,7 @@
 // passes took different paths).
 #include <algorithm>
 #include <cstdlib>
+#include <type_traits>
 
 #include "orf_common.h"
 #include "orf_diff.h"
@@ -27,13 +28,17 @@
 
 namespace orf {
 
-// Waves per SIMD the register budget is sized for: the Lambert / conductor /
-// emitter form fits 128 VGPRs without spilling (4 waves); the larger material
-// sets keep the compiler's own allocation.
-constexpr int diff_waves(unsigned features) { return features == 0u ? 4 : 1; }
-// One workgroup fills a CU at that occupancy (4 SIMDs), so the CU's waves share
-// one LDS copy of the texel gradients.
-constexpr int diff_block(unsigned features) { return 256 * diff_waves(features); }
+// Waves per SIMD the register budget is sized for.  The Lambert / conductor /
+// emitter primal runs at the plain megakernel's 8 waves (64 VGPRs); its adjoint
+// fits 128 VGPRs without spilling (4 waves); the larger material sets keep the
+// compiler's own allocation.
+constexpr int diff_waves(unsigned features, bool adjoint) { return features == 0u ? (adjoint ? 4 : 8) : 1; }
+// Adjoint workgroups fill a CU at that occupancy (4 SIMDs), so the CU's waves
+// share one LDS copy of the texel gradients; primal workgroups hold 8 waves.
+constexpr int diff_block(unsigned features, bool adjoint)
+{
+    return adjoint ? 256 * diff_waves(features, true) : (features == 0u ? 512 : 256);
+}
 // LDS a workgroup may give the texel gradients (the 64x64 albedo + roughness
 // textures of the inverse-rendering workload take 64 KB); larger textures add
 // straight to global memory.
@@ -74,14 +79,14 @@ struct AtomicSink {
     }
 };
 
-template <unsigned kFeat>
-__global__ __launch_bounds__(diff_block(kFeat), diff_waves(kFeat)) void k_diff(
+template <unsigned kFeat, bool kAdjoint>
+__global__ __launch_bounds__(diff_block(kFeat, kAdjoint), diff_waves(kFeat, kAdjoint)) void k_diff(
     SceneGPU sc, DiffParams dp, const WfRender* __restrict__ views, int tiles_x, int tiles_per_view, int chunks,
-    long long items, int adjoint, unsigned frame_shift, float4* partial, long long npix, const float* dimages,
-    float* g_albedo, float* g_rough, float* g_sss, unsigned long long* counters, unsigned long long* qctr,
-    int lds_grad)
+    long long items, unsigned frame_shift, float4* partial, long long npix, const float* dimages, float* g_albedo,
+    float* g_rough, float* g_sss, unsigned long long* counters, unsigned long long* qctr, int lds_grad)
 {
-    constexpr int kDiffBlock = diff_block(kFeat);
+    constexpr int kDiffBlock = diff_block(kFeat, kAdjoint);
+    constexpr bool adjoint = kAdjoint;
     __shared__ unsigned stack_lds[kLdsStack * kDiffBlock];
     unsigned* stk = stack_lds + threadIdx.x;
     const unsigned lane = threadIdx.x & 63u;
@@ -112,7 +117,7 @@ __global__ __launch_bounds__(diff_block(kFeat), diff_waves(kFeat)) void k_diff(
             if (x >= R.width || y >= R.height) continue;  // every lane reaches the next item fetch
             const size_t px = ((size_t)view * R.height + y) * R.width + x;
             const unsigned k0 = (unsigned)chunk * kDiffChunk, k1 = min(k0 + kDiffChunk, R.spp);
-            if (!adjoint) {
+            if constexpr (!kAdjoint) {
                 f3 acc = mk3(0.0f);
                 unsigned cnt = 0;
                 for (unsigned s = k0; s < k1; ++s) {
@@ -200,7 +205,7 @@ void launch_diff(const SceneGPU& sc, const DiffParams& dp, const WfRender* d_vie
     ORF_REQUIRE(adjoint || partial, "diff launch: no partial-sum buffer");
     zero_queue_counters(qctr, stream);
     zero_queue_counters(counters, stream);  // this launch's statistics (8 entries)
-    const int waves = diff_waves(sc.features), bsize = diff_block(sc.features), wpb = bsize / 64;
+    const int waves = diff_waves(sc.features, adjoint), bsize = diff_block(sc.features, adjoint), wpb = bsize / 64;
     const dim3 grid((unsigned)((persistent_waves(items, waves) + wpb - 1) / wpb)), block((unsigned)bsize);
     // texel gradients in LDS when they fit beside the traversal stacks
     const size_t grad_bytes =
@@ -213,11 +218,16 @@ void launch_diff(const SceneGPU& sc, const DiffParams& dp, const WfRender* d_vie
     const bool lds = grad_bytes > 0 && grad_bytes <= kDiffLdsGradBytes &&
                      stack_bytes + grad_bytes <= (size_t)lds_max && !(force_global && force_global[0] == '1');
     // the scene's material programs only, as the render kernels pick them
-    auto kernel = sc.features == 0u ? k_diff<0u> : sc.features == kFeatDielectric ? k_diff<kFeatDielectric>
-                                                                                   : k_diff<kFeatAll>;
+    auto pick = [&](auto adj) {
+        constexpr bool kA = decltype(adj)::value;
+        return sc.features == 0u                 ? k_diff<0u, kA>
+               : sc.features == kFeatDielectric ? k_diff<kFeatDielectric, kA>
+                                                 : k_diff<kFeatAll, kA>;
+    };
+    auto kernel = adjoint ? pick(std::true_type{}) : pick(std::false_type{});
     hipLaunchKernelGGL(kernel, grid, block, lds ? grad_bytes : 0, stream, sc, dp, d_views, tiles_x, tiles_x * tiles_y,
-                       chunks, items, adjoint ? 1 : 0, frame_shift, partial, npix, dimages, g_albedo, g_rough, g_sss,
-                       counters, qctr, lds ? 1 : 0);
+                       chunks, items, frame_shift, partial, npix, dimages, g_albedo, g_rough, g_sss, counters, qctr,
+                       lds ? 1 : 0);
     ORF_HIP_CHECK(hipGetLastError());
     if (!adjoint) {
         hipLaunchKernelGGL(k_diff_image, dim3((unsigned)((npix + 255) / 256)), dim3(256), 0, stream, partial, chunks,
