@@ -34,6 +34,11 @@ namespace orf {
 
 enum DiffUV : int { UV_PLANAR_XZ = 0, UV_SPHERICAL = 1 };
 
+// Gradient-carrying vertices an adjoint sample keeps in registers and scratch
+// before it falls back to a replay (a path shades at most max_depth + 1 vertices
+// of the texture kinds; 8 + 1 at the configured depth).
+constexpr int kDiffRec = 12;
+
 struct DiffParams {
     const float* albedo;  // tex_h * tex_w * 3
     const float* rough;   // tex_h * tex_w
@@ -43,6 +48,7 @@ struct DiffParams {
     float albedo_uvp[4], rough_uvp[4];  // planar: x0, z0, x1, z1; spherical: cx, cy, cz, -
     int sss_mat;      // translucent material handle whose coefficients are differentiated (-1: none)
     float sss_scale;  // its scale: the gradients are reported per unscaled coefficient
+    int max_rec;      // gradient vertices an adjoint sample keeps (<= kDiffRec) before replaying
 };
 
 ORF_HD int diff_texel(int mode, const float* uvp, f3 p, int tw, int th)
@@ -208,11 +214,6 @@ ORF_HD f3 diff_replay(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint
     return ps.L;
 }
 
-// Gradient-carrying vertices an adjoint sample keeps in registers and scratch
-// before it falls back to a replay (a path shades at most max_depth + 1 vertices
-// of the texture kinds; 8 + 1 at the configured depth).
-constexpr int kDiffRec = 12;
-
 // Adjoint of one sample in a single pass: the path of diff_path, whose
 // gradient-carrying vertices keep what diff_replay's terms need -- the path's
 // radiance where the term takes the rest (before the vertex for albedo and the
@@ -221,8 +222,8 @@ constexpr int kDiffRec = 12;
 // goes to `sink` as diff_replay computes it.  A replay measures those radiances
 // at the same points of the same computation, so the terms are the replay's,
 // without the second traversal of the path.  Returns the radiance; `spill` is
-// set, and nothing reaches the sink, when the path holds more than kDiffRec such
-// vertices (the caller then replays it).
+// set, and nothing reaches the sink, when the path holds more than dp.max_rec
+// (at most kDiffRec) such vertices (the caller then replays it).
 template <unsigned kFeat, class Sink>
 ORF_HD f3 diff_adjoint_sample(const SceneGPU& sc, const DiffParams& dp, f3 o, f3 d, uint32_t rng, Seed64 s64,
                               f3 delta, Sink& sink, unsigned* stk, int stride, bool& spill)
@@ -250,7 +251,7 @@ ORF_HD f3 diff_adjoint_sample(const SceneGPU& sc, const DiffParams& dp, f3 o, f3
             if (!trace<true, false>(sc, sr.o, sr.d, sc.scene_epsilon, sr.tmax, hs, stk, stride)) shadow_unoccluded(ps, sr);
         }
         if (dv.kind != 0) {
-            if (n == kDiffRec) {
+            if (n >= dp.max_rec) {
                 spill = true;
             } else {
                 kt[n] = (dv.kind << 28) | (dv.kind == 3 ? 0 : dv.texel);

@@ -1,4 +1,5 @@
 // orf_diffapi.cpp -- C-ABI of the differentiable form (primal and adjoint launches).
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -54,6 +55,14 @@ static DiffParams diff_params(orf_context_t* c, const orf_diff_desc* d, const fl
                     "diff: the scattering material takes its coefficients from user data, not from absorption/scattering");
         p.sss_mat = sm;
         p.sss_scale = c->mat_sss[sm].scale;
+    }
+    // gradient vertices an adjoint sample keeps before it replays its path
+    // (kDiffRec; ORF_DIFF_RECORDS lowers it so the tests reach the replay)
+    p.max_rec = kDiffRec;
+    if (const char* e = std::getenv("ORF_DIFF_RECORDS")) {
+        const int n = std::atoi(e);
+        ORF_REQUIRE(n >= 0 && n <= kDiffRec, "diff: ORF_DIFF_RECORDS out of range");
+        p.max_rec = n;
     }
     return p;
 }

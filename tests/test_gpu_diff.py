@@ -179,6 +179,36 @@ def test_lds_and_global_gradient_accumulation_agree():
         np.testing.assert_allclose(x, y, rtol=1e-4, atol=1e-6 * np.abs(y).max())
 
 
+def test_replayed_paths_give_the_same_gradients():
+    """An adjoint sample keeps its gradient vertices' terms until its radiance is
+    known and replays the path when it holds more of them than the record takes;
+    with the record cut to 1 vertex (ORF_DIFF_RECORDS) most paths are replayed,
+    every replay reproduces its first pass, and the gradients are the same up to
+    FP32 summation order."""
+    import os
+
+    import torch
+
+    from open_rendering_framework_amd import rt
+
+    ctx = scenes.to_context(_scene())
+    views = scenes.ring_views(n=2, size=W, spp=8)
+    a, r = scenes.diff_textures(TEX, seed=5)
+    dimg = np.random.default_rng(3).standard_normal((2, W, W, 3)).astype(np.float32)
+    args = (_desc(rt), views, torch.from_numpy(a).cuda(), torch.from_numpy(r).cuda(), torch.from_numpy(dimg).cuda())
+    ga0, gr0 = ctx.diff_adjoint(*args, frame_shift=SHIFT)
+    os.environ["ORF_DIFF_RECORDS"] = "1"
+    try:
+        ga1, gr1 = ctx.diff_adjoint(*args, frame_shift=SHIFT)
+    finally:
+        del os.environ["ORF_DIFF_RECORDS"]
+    _assert_replays_match(ctx)
+    for x, y in ((ga0, ga1), (gr0, gr1)):
+        x, y = x.cpu().numpy().astype(np.float64), y.cpu().numpy().astype(np.float64)
+        assert np.abs(x).max() > 0
+        np.testing.assert_allclose(x, y, rtol=1e-4, atol=1e-6 * np.abs(y).max())
+
+
 def test_torch_function_matches_c_abi():
     import torch
 
