@@ -44,6 +44,15 @@ constexpr int diff_block(unsigned features, bool adjoint)
 // straight to global memory.
 constexpr size_t kDiffLdsGradBytes = 96 * 1024;
 
+// LDS float pointer: the LDS address space gives ds_add_f32 for the gradient
+// atomics (a generic pointer compiles to flat atomics, which also pass through
+// the texture address unit).
+#if defined(__HIP_DEVICE_COMPILE__)
+typedef __attribute__((address_space(3))) float lds_float;
+#else
+typedef float lds_float;
+#endif
+
 // Texel gradients go to FP32 arrays through atomics: the workgroup's LDS copy
 // when the launch gives one (flushed to global memory once at the end), else
 // the global arrays; the translucent coefficients' six gradients are summed per
@@ -53,17 +62,26 @@ struct AtomicSink {
     float* g_rough;
     float* g_sss;  // d/d absorption RGB, d/d scattering RGB
     f3 ga, gs;
-    float* l_albedo;  // LDS copies (null: global atomics)
-    float* l_rough;
+    lds_float* l_albedo;  // LDS copies (null: global atomics)
+    lds_float* l_rough;
     __device__ void albedo(int texel, int c, float v)
     {
-        if (l_albedo) atomicAdd(l_albedo + 3 * texel + c, v);
-        else if (g_albedo) atomicAdd(g_albedo + 3 * (size_t)texel + c, v);
+        if (l_albedo) {
+            atomicAdd(l_albedo + 3 * texel + c, v);
+            // keeps the two arms apart: merged, they become one flat atomic
+            __asm__ volatile("" ::: "memory");
+        } else if (g_albedo) {
+            atomicAdd(g_albedo + 3 * (size_t)texel + c, v);
+        }
     }
     __device__ void rough(int texel, float v)
     {
-        if (l_rough) atomicAdd(l_rough + texel, v);
-        else if (g_rough) atomicAdd(g_rough + texel, v);
+        if (l_rough) {
+            atomicAdd(l_rough + texel, v);
+            __asm__ volatile("" ::: "memory");
+        } else if (g_rough) {
+            atomicAdd(g_rough + texel, v);
+        }
     }
     __device__ void sss(f3 a, f3 s)
     {
@@ -98,8 +116,8 @@ __global__ __launch_bounds__(diff_block(kFeat, kAdjoint), diff_waves(kFeat, kAdj
     const int texels = dp.tw * dp.th;
     const int n_albedo = g_albedo ? 3 * texels : 0, n_rough = g_rough ? texels : 0;
     if (lds_grad) {
-        sink.l_albedo = g_albedo ? grad_lds : nullptr;
-        sink.l_rough = g_rough ? grad_lds + n_albedo : nullptr;
+        sink.l_albedo = g_albedo ? (lds_float*)grad_lds : nullptr;
+        sink.l_rough = g_rough ? (lds_float*)grad_lds + n_albedo : nullptr;
         for (int i = threadIdx.x; i < n_albedo + n_rough; i += kDiffBlock) grad_lds[i] = 0.0f;
         __syncthreads();
     }
