@@ -250,7 +250,6 @@ static void build_geometry(orf_context_t* c, hipStream_t)
     std::vector<float4> normals(N.size() / 3 + 1);
     for (size_t k = 0; k + 2 < N.size(); k += 3) normals[k / 3] = make_float4(N[k], N[k + 1], N[k + 2], 0.0f);
     ORF_REQUIRE(bvh.stack4 <= kStackDepth, "scene: the BVH needs a deeper traversal stack than this build has");
-    c->d_bvh.upload(bvh.nodes);
     c->d_bvh4.upload(bvh.nodes4);
     c->nodes4 = (int)(bvh.nodes4.size() / 8);
     c->d_tri.upload(bvh.tris);
@@ -371,7 +370,6 @@ SceneGPU scene_view(orf_context_t* c)
 {
     SceneGPU s;
     std::memset(&s, 0, sizeof(s));
-    s.bvh = c->d_bvh.as<float4>();
     s.bvh4 = c->d_bvh4.as<float4>();
     s.nodes4 = c->nodes4;
     s.tri = c->d_tri.as<float4>();

@@ -48,7 +48,7 @@ struct orf_context_t {
     orf::BvhStats bvh_stats;
     int nodes4 = 0;  // 4-wide BVH nodes on the device
     // device copies
-    orf::DevBuf d_bvh, d_bvh4, d_tri, d_shade, d_normals, d_mats, d_lights, d_trilights, d_analytic;
+    orf::DevBuf d_bvh4, d_tri, d_shade, d_normals, d_mats, d_lights, d_trilights, d_analytic;
     orf::DevBuf d_env, d_mpdf, d_mcdf, d_cpdf, d_ccdf, d_envscratch;
     orf::DevBuf d_film, d_counters;
     // translucent material: coefficients and surface probes

@@ -2,9 +2,8 @@
 //
 // HBM layout (all arrays 16 B aligned, structure-of-arrays where a kernel
 // streams one field):
-//   bvh      : BVH2 nodes, 64 B each, both child boxes stored in the parent so one
-//              node fetch tests two boxes (4 x float4: c0.lo,c0.hi,c1.lo,c1.hi
-//              packed, then the two child links)
+//   bvh4     : 4-wide BVH nodes, 128 B each: the four child boxes as six SoA
+//              rows (lo.x, hi.x, lo.y, hi.y, lo.z, hi.z) and the four child links
 //   tri      : triangles in leaf order, 48 B each: p0, e1 = p1-p0, e2 = p2-p0 and the
 //              original triangle id (3 x float4), read by the intersection loop only
 //   shade    : per original triangle: vertex-normal indices and material id,
@@ -125,8 +124,7 @@ struct ProbesGPU {
 };
 
 struct SceneGPU {
-    const float4* bvh;        // 4 float4 per node (BVH2; the translucent crossing search)
-    const float4* bvh4;       // 8 float4 per node (the same tree, 4-wide; every other traversal)
+    const float4* bvh4;       // 8 float4 per node (the binned-SAH tree collapsed to 4 children; every traversal)
     int nodes4;               // 4-wide nodes (the first min(nodes4, kBvhTopOrdered) breadth-first)
     const float4* tri;        // 3 float4 per leaf-ordered triangle
     const int4* shade;        // per original triangle: nidx.xyz, material id

@@ -100,27 +100,43 @@ ORF_HD int probe_ray_hits(const SceneGPU& sc, int sss, f3 o, f3 dir, float tmax,
     unsigned ovf[kStackDepth - kLdsStack + 1];
     auto push = [&](unsigned v) { stack_push(stk, stride, ovf, sp, v); };
     auto pop = [&]() -> unsigned { return stack_pop(stk, stride, ovf, sp); };
+    const float inf = __builtin_huge_valf();
     for (;;) {
         if (!link_is_leaf(node)) {
-            const float4* nd = sc.bvh + 4 * (size_t)node;
-            float4 a = nd[0], b = nd[1], c = nd[2], l = nd[3];
-            float e0 = box_enter(r, a.x, a.y, a.z, a.w, b.x, b.y, 0.0f, tcull);
-            float e1 = box_enter(r, b.z, b.w, c.x, c.y, c.z, c.w, 0.0f, tcull);
-            unsigned l0 = f2u(l.x), l1 = f2u(l.y);
-            bool h0 = e0 != __builtin_huge_valf(), h1 = e1 != __builtin_huge_valf();
-            if (h0 && h1) {
-                unsigned nearc = e0 <= e1 ? l0 : l1;
-                unsigned farc = e0 <= e1 ? l1 : l0;
-                if (sp < kStackDepth) push(farc);
-                node = nearc;
-            } else if (h0) {
-                node = l0;
-            } else if (h1) {
-                node = l1;
-            } else {
+            // 4-wide node: entered children near to far, the nearest next, the
+            // others pushed farthest first (near-first culls sooner once the
+            // list is full; the list itself does not depend on the order)
+            const float4* nd = sc.bvh4 + 8 * (size_t)node;
+            const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], ln = nd[6];
+            float e[4] = {box_enter(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, 0.0f, tcull),
+                          box_enter(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, 0.0f, tcull),
+                          box_enter(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, 0.0f, tcull),
+                          box_enter(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, 0.0f, tcull)};
+            unsigned c[4] = {f2u(ln.x), f2u(ln.y), f2u(ln.z), f2u(ln.w)};
+            auto order = [&](int a, int b) {
+                if (e[b] < e[a]) {
+                    const float te = e[a];
+                    e[a] = e[b];
+                    e[b] = te;
+                    const unsigned tc = c[a];
+                    c[a] = c[b];
+                    c[b] = tc;
+                }
+            };
+            order(0, 1);
+            order(2, 3);
+            order(0, 2);
+            order(1, 3);
+            order(1, 2);
+            if (e[0] == inf) {
                 if (sp == 0) break;
                 node = pop();
+                continue;
             }
+            if (e[3] != inf && sp < kStackDepth) push(c[3]);
+            if (e[2] != inf && sp < kStackDepth) push(c[2]);
+            if (e[1] != inf && sp < kStackDepth) push(c[1]);
+            node = c[0];
             continue;
         }
         unsigned first = leaf_first(node), cnt = leaf_count(node);

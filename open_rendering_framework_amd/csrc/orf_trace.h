@@ -1,4 +1,4 @@
-// orf_trace.h -- ray/triangle intersection and BVH2 traversal (device).
+// orf_trace.h -- ray/triangle intersection and 4-wide BVH traversal (device).
 //
 // Replaces OptiX traversal + intersect_triangle_earlyexit as used by
 // CU/triangle_mesh.cu:46-86 (closest hit for radiance rays, any hit for shadow
