@@ -701,7 +701,8 @@ static v3 hit_normal(const oscene* s, const ohit* h)
 
 /* Differentiable form (new): nearest texel of a planar (x,z) map over
  * [uvp0,uvp2] x [uvp1,uvp3] (mode 0) or a spherical map about uvp[0..2] (mode 1).
- * Parity unpinned beyond the GPU comparison: no reference counterpart. */
+ * No reference counterpart; pinned by texel-centre points of the rectangle and
+ * of the sphere's (theta, phi) grid (tests/test_oracle_diff.py). */
 static int diff_texel(int mode, const R* uvp, v3 p, int tw, int th)
 {
     R u, v;
@@ -717,6 +718,13 @@ static int diff_texel(int mode, const R* uvp, v3 p, int tw, int th)
     iu = iu < 0 ? 0 : (iu > tw - 1 ? tw - 1 : iu);
     iv = iv < 0 ? 0 : (iv > th - 1 ? th - 1 : iv);
     return iv * tw + iu;
+}
+
+void orc_diff_texel_batch(int n, int mode, const double* uvp, const double* pts, int tw, int th, int32_t* out)
+{
+    R u4[4] = {(R)uvp[0], (R)uvp[1], (R)uvp[2], (R)uvp[3]};
+    for (int i = 0; i < n; i++)
+        out[i] = diff_texel(mode, u4, V3((R)pts[3 * i], (R)pts[3 * i + 1], (R)pts[3 * i + 2]), tw, th);
 }
 
 /* CU/diffuse_shader.cu:42-107 (+ environment MIS, new). */

@@ -138,6 +138,9 @@ void orc_diff_set(void* s, int tex_w, int tex_h, int alb_mat, int alb_uv, const 
  * (gather exit points, extinction roulette) with frozen6 (absorption RGB,
  * scattering RGB in 1/mm); m < 0 restores the materials' own. */
 void orc_diff_set_sss(void* s, int m, const double* eval6, const double* frozen6);
+/* The differentiable form's texel lookup for n points (xyz each): mode 0 planar
+ * x-z over [uvp0, uvp2] x [uvp1, uvp3], mode 1 spherical about uvp[0..2]. */
+void orc_diff_texel_batch(int n, int mode, const double* uvp, const double* pts, int tw, int th, int32_t* out);
 double orc_diff_eval(void* s, int W, int H, int nviews, const float* cams, const uint32_t* frame0, uint32_t spp,
                      uint32_t shift, const double* dimg, double* images, int threads);
 

@@ -8,7 +8,11 @@ Pins:
     theta (importance-sampling identity) -- checked for the albedo (RR prob from
     theta0) and the GGX roughness (VNDF drawn at alpha0, weight
     F G1(wi) G1(wo) D / (G1_0(wi) D_0)); a dropped G1 or D term fails it;
-  * the loss is linear in the image gradient (sum of two losses == loss of sum).
+  * the loss is linear in the image gradient (sum of two losses == loss of sum);
+  * the texel lookup (no reference counterpart): points at the texel centres of
+    the planar rectangle and of the sphere's (theta, phi) grid land on their
+    texels, points outside clamp to the edge texels (a swapped u/v, a wrong
+    atan2 argument order or an off-by-one fails it).
 """
 import numpy as np
 import pytest
@@ -148,3 +152,53 @@ def test_scattering_reweighting_is_unbiased():
     m_0, se_0 = _mean_and_se(plain0, views, 16)
     assert abs(m_rew - m_1) < 4.0 * np.hypot(se_rew, se_1) + 1e-7, (m_rew, m_1, se_rew, se_1)
     assert abs(m_0 - m_1) > 4.0 * np.hypot(se_0, se_1), (m_0, m_1)
+
+
+def _texel_lookup(mode, uvp, pts, tw, th, fp64):
+    import ctypes
+
+    import oracle
+
+    L = oracle.lib(fp64=fp64)
+    L.orc_diff_texel_batch.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                       ctypes.c_int, ctypes.c_void_p]
+    uvp = np.ascontiguousarray(uvp, np.float64)
+    pts = np.ascontiguousarray(pts, np.float64)
+    out = np.zeros(len(pts), np.int32)
+    L.orc_diff_texel_batch(len(pts), mode, uvp.ctypes.data, pts.ctypes.data, tw, th, out.ctypes.data)
+    return out
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_planar_texel_lookup_hits_texel_centres(fp64):
+    tw, th = 7, 5
+    x0, z0, x1, z1 = -1.5, 0.25, 2.0, 3.0
+    iu, iv = np.meshgrid(np.arange(tw), np.arange(th))
+    x = x0 + (iu.ravel() + 0.5) / tw * (x1 - x0)
+    z = z0 + (iv.ravel() + 0.5) / th * (z1 - z0)
+    pts = np.stack([x, np.full_like(x, 0.3), z], 1)  # y plays no part
+    got = _texel_lookup(0, [x0, z0, x1, z1], pts, tw, th, fp64)
+    np.testing.assert_array_equal(got, iv.ravel() * tw + iu.ravel())
+    # outside the rectangle: clamped to the edge texels
+    outside = np.array([[x0 - 1.0, 0.0, z0 - 1.0], [x1 + 1.0, 0.0, z1 + 1.0], [x1 + 1.0, 0.0, z0 - 1.0]])
+    np.testing.assert_array_equal(_texel_lookup(0, [x0, z0, x1, z1], outside, tw, th, fp64),
+                                  [0, th * tw - 1, tw - 1])
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_spherical_texel_lookup_hits_texel_centres(fp64):
+    # column iu <-> azimuth phi = atan2(q.x, q.z) in [-pi, pi), row iv <-> polar
+    # angle theta = acos(q.y) in [0, pi]: a point at radius 2.5 about the centre
+    # in direction (sin t sin p, cos t, sin t cos p) at a texel centre's angles
+    tw, th = 8, 6
+    c = np.array([0.3, -0.7, 1.1])
+    iu, iv = np.meshgrid(np.arange(tw), np.arange(th))
+    phi = -np.pi + (iu.ravel() + 0.5) / tw * 2.0 * np.pi
+    theta = (iv.ravel() + 0.5) / th * np.pi
+    q = np.stack([np.sin(theta) * np.sin(phi), np.cos(theta), np.sin(theta) * np.cos(phi)], 1)
+    got = _texel_lookup(1, [c[0], c[1], c[2], 0.0], c + 2.5 * q, tw, th, fp64)
+    np.testing.assert_array_equal(got, iv.ravel() * tw + iu.ravel())
+    # the poles: top row and bottom row
+    poles = np.array([c + [0.0, 1.0, 0.0], c + [0.0, -1.0, 0.0]])
+    got = _texel_lookup(1, [c[0], c[1], c[2], 0.0], poles, tw, th, fp64)
+    assert got[0] // tw == 0 and got[1] // tw == th - 1
