@@ -215,3 +215,35 @@ def test_benchmark_scene_full_size_parity():
         ok = _ok(got, want)
         _say("  samples ok %.5f mean %.4g / %.4g" % (ok.mean(), got.mean(), want.mean()))
         assert ok.mean() >= 0.99
+
+
+def test_batch_forms_match_megakernel_on_the_translucent_scene():
+    """The batched wavefront (per-material shade queues: misses, emitters, Lambert
+    and conductor hits on the smallest program set, the translucent and glass
+    hits on the scene's) and the persistent batch give every item the per-item
+    megakernel's film (FP64 sums, relative 1e-9) and the same ray counts; the
+    batch shares one probe pass."""
+    from open_rendering_framework_amd import rt
+
+    d = _scene()
+    ctx = scenes.to_context(d)
+    items = [{"width": W, "height": W, "spp": 2, "frame0": f, "eye": d.eye, "U": d.U, "V": d.V, "W": d.W}
+             for f in (0, 5)]
+    ctx.render_batch(items, None, mode=rt.BATCH_MEGAKERNEL)
+    ref = [ctx.batch_film(i, W, W, sums=True) for i in range(len(items))]
+    ref_counts = ctx.batch_counters()
+    for mode, policy in ((rt.BATCH_WAVEFRONT, 1), (rt.BATCH_WAVEFRONT, 0), (rt.BATCH_PERSISTENT, None)):
+        cfg = None
+        if policy is not None:
+            cfg = rt.WavefrontConfig()
+            cfg.block_size, cfg.pool_size, cfg.sort_policy, cfg.regen_threshold = 128, 1 << 14, policy, 1.0
+            cfg.use_graph, cfg.graph_pairs = 1, 2
+        ctx.render_batch(items, None, mode=mode, config=cfg)
+        counts = ctx.batch_counters()
+        _say("mode %d policy %s counters %s (megakernel %s)" % (mode, policy, counts, ref_counts))
+        for i in range(len(items)):
+            a = ctx.batch_film(i, W, W, sums=True)
+            assert np.array_equal(a[..., 3], ref[i][..., 3]), i
+            np.testing.assert_allclose(a[..., :3], ref[i][..., :3], rtol=1e-9, atol=1e-12)
+        assert counts["ext_rays"] == ref_counts["ext_rays"]
+        assert counts["shadow_rays"] == ref_counts["shadow_rays"]
