@@ -277,6 +277,14 @@ orf_status orf_film_clear(orf_context ctx);
 /* Film mean as RGBA (A = 1) in FP64 (rgba64) and/or FP32 (rgba32); host arrays
  * of 4*width*height, row 0 = bottom row (launch index y = 0). */
 orf_status orf_film_read(orf_context ctx, double* rgba64, float* rgba32);
+/* Display buffer: the film mean as orf_film_read gives it (RGBA, FP32, same
+ * layout) written into a DEVICE array of 4*width*height floats on `stream`,
+ * without synchronising.  Replaces the reference's output buffer created on a GL
+ * pixel-buffer object (createBufferFromGLBO, ORF/OptixScene.cpp:252-257) that
+ * OptixWindow::paintBuffer uploads to an RGBA32F texture
+ * (ORF/OptixWindow.cpp:146-196): a GL host registers its PBO with HIP-GL
+ * interop, maps it and passes the mapped pointer here once per frame. */
+orf_status orf_film_display(orf_context ctx, float* rgba32_device, void* stream);
 /* Raw film (sum R, G, B, count) as FP64 host array of 4*width*height. */
 orf_status orf_film_read_sums(orf_context ctx, double* sums);
 /* Counters since the last clear: [0] non-finite samples dropped,

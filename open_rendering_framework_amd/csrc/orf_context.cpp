@@ -873,6 +873,19 @@ extern "C" orf_status orf_film_read_sums(orf_context ctx, double* sums)
     ORF_API_END
 }
 
+extern "C" orf_status orf_film_display(orf_context ctx, float* rgba32_device, void* stream)
+{
+    ORF_API_BEGIN
+    orf_context_t* c = ctx_check(ctx);
+    ORF_REQUIRE(c->d_film.p, "orf_film_display: no film");
+    ORF_REQUIRE(rgba32_device, "orf_film_display: null buffer");
+    const long long n = (long long)c->film_w * c->film_h;
+    const double* depth = c->integrator == 1 ? c->d_depth.as<double>() : nullptr;
+    ORF_REQUIRE(c->integrator != 1 || depth, "orf_film_display: no depth film");
+    orf::launch_film_display(c->d_film.as<double>(), depth, n, (float4*)rgba32_device, (hipStream_t)stream);
+    ORF_API_END
+}
+
 extern "C" orf_status orf_film_read(orf_context ctx, double* rgba64, float* rgba32)
 {
     ORF_API_BEGIN

@@ -66,6 +66,9 @@ int persistent_waves(long long items, int waves);
 inline int mega_blocks(int waves) { return (waves + ORF_MEGA_BLOCK_WAVES - 1) / ORF_MEGA_BLOCK_WAVES; }
 // Zero 8 queue counters with a kernel (safe inside captured streams).
 void zero_queue_counters(unsigned long long* qctr, hipStream_t stream);
+// Film mean RGBA32F (n pixels) into a device buffer: RGB = sums / count, A = 1
+// or depth sum / count when `depth` is given.
+void launch_film_display(const double* sums, const double* depth, long long n, float4* out, hipStream_t stream);
 
 // Megakernel: every (pixel, sample) path of frames frame0 .. frame0+spp-1.
 // remap: per-render material table (device, nullptr = identity).

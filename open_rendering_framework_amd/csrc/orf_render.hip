@@ -49,6 +49,29 @@ __global__ __launch_bounds__(kMegaBlock, mega_waves(kFeat)) void k_render_mega(S
     atomicAdd(film.counters + 3, npaths);
 }
 
+// Film mean as RGBA32F into a caller's device buffer (a display buffer): RGB =
+// FP64 sums / count, A = 1, or the mean hit distance with the depth sums.
+__global__ __launch_bounds__(256) void k_film_display(const double* __restrict__ sums,
+                                                      const double* __restrict__ depth, long long n,
+                                                      float4* __restrict__ out)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double cnt = sums[4 * i + 3];
+    const double r = cnt > 0 ? sums[4 * i + 0] / cnt : 0.0;
+    const double g = cnt > 0 ? sums[4 * i + 1] / cnt : 0.0;
+    const double b = cnt > 0 ? sums[4 * i + 2] / cnt : 0.0;
+    const double a = depth ? (cnt > 0 ? depth[i] / cnt : 0.0) : 1.0;
+    out[i] = make_float4((float)r, (float)g, (float)b, (float)a);
+}
+
+void launch_film_display(const double* sums, const double* depth, long long n, float4* out, hipStream_t stream)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_film_display, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, sums, depth, n, out);
+    ORF_HIP_CHECK(hipGetLastError());
+}
+
 __global__ void k_zero_u64(unsigned long long* p, int n)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;

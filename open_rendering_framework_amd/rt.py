@@ -73,6 +73,7 @@ _SIGS = {
     "orf_film_read": [c_vp, c_vp, c_vp],
     "orf_film_read_sums": [c_vp, c_vp],
     "orf_counters": [c_vp, c_vp],
+    "orf_film_display": [c_vp, c_vp, c_vp],
     "orf_diff_counters": [c_vp, c_vp],
     "orf_gen_icosphere": [ctypes.c_int, c_u32, ctypes.c_float, ctypes.c_float, c_vp, c_vp,
                           ctypes.POINTER(c_i64), c_vp, ctypes.POINTER(c_i64)],
@@ -493,6 +494,12 @@ class Context:
 
         out = np.zeros((height, width, 4), np.float64)
         call("orf_film_read_sums", self.handle, _np_ptr(out))
+        return out
+
+    def film_display(self, out, stream=None):
+        """Film mean RGBA (as film()) into the device float32 tensor `out`
+        (height, width, 4) on `stream`, without synchronising."""
+        call("orf_film_display", self.handle, ptr(out), stream_ptr(stream))
         return out
 
     def counters(self):

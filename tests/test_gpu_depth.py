@@ -3,6 +3,9 @@ CU/depth_shader.cu:13-19) through the C-ABI against the CPU oracle: per pixel,
 the film's RGB sums after frames 0..3 equal the sum of the oracle's normal
 colours and the film's alpha equals the mean hit distance (|d| <= 1e-5 |ref| +
 1e-6 on >= 99% of pixels), and a scene file naming DepthTracer selects it.
+The display buffer (orf_film_display: the film mean written into a device
+array, the renderer's half of the reference's GL pixel-buffer display) equals
+orf_film_read's FP32 output bit for bit, for both integrators.
 """
 import json
 import sys
@@ -58,3 +61,24 @@ def test_scene_file_selects_depth_tracer(tmp_path):
     img = ctx.film(SW, SH)
     assert img[..., 3].max() > 1.0  # hit distances, not the path tracer's alpha 1
     assert np.all((img[..., :3] >= 0) & (img[..., :3] <= 1))
+
+
+@pytest.mark.parametrize("integrator", [rt.INTEGRATOR_PATH_TRACER, rt.INTEGRATOR_DEPTH])
+def test_display_buffer_equals_film_read(integrator):
+    import torch
+
+    d = scenes.path_scene(width=W, height=H, level=3, env_size=(64, 32))
+    ctx = scenes.to_context(d)
+    ctx.integrator(integrator)
+    ctx.film_clear()
+    ctx.render_frames(W, H, 0, 3)
+    want = np.zeros((H, W, 4), np.float32)
+    rt.call("orf_film_read", ctx.handle, None, want.ctypes.data_as(rt.c_vp))
+    out = torch.full((H, W, 4), -1.0, dtype=torch.float32, device="cuda")
+    stream = torch.cuda.current_stream()
+    ctx.film_display(out, stream=stream)
+    torch.cuda.synchronize()
+    got = out.cpu().numpy()
+    _say("display buffer integrator %d: mean rgb %.4f, alpha %.4f" % (integrator, got[..., :3].mean(),
+                                                                       got[..., 3].mean()))
+    assert np.array_equal(got, want)
