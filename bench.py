@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 # HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE passes of these exact
 # workloads, scaled by a 1 GiB copy kernel (profiles/hbm_traffic_pmc.md)
 TRAFFIC_PATH_512_64SPP = 16.17e9
-TRAFFIC_SSS_512_128SPP = 233.8e9
+TRAFFIC_SSS_512_128SPP = 142.7e9
 TRAFFIC_BSDF_2E22 = 218.4e6
 
 
@@ -327,11 +327,13 @@ def diff_measure(steps, warmup):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     losses.append(float(loss))
+    checks = ctx.diff_counters()
     R.close()
     paths = DIFF_SIZE * DIFF_SIZE * DIFF_SPP * DIFF_VIEWS
     return {"grad_steps_per_s": steps / wall, "ms_per_step": wall * 1e3 / steps, "views": DIFF_VIEWS,
             "size": DIFF_SIZE, "spp_primal": DIFF_SPP, "spp_adjoint": DIFF_SPP, "texture": DIFF_TEX,
-            "primal_paths_per_step": paths, "replayed_paths_per_step": paths,
+            "primal_paths_per_step": paths, "adjoint_paths_per_step": paths,
+            "adjoint_replay_mismatches": checks["replay_mismatch"], "adjoint_nonfinite": checks["nonfinite"],
             "loss_first": losses[0], "loss_last": losses[-1], "steps": steps}
 
 

@@ -36,27 +36,49 @@ ORF_HD unsigned probe_hash(int ix, int iy, int iz, unsigned mask)
 
 ORF_HD int probe_cell(float v, float inv_cell) { return (int)floorf(v * inv_cell); }
 
-// Nearest probe strictly within distance `cell` of y (ties: lower index), or -1.
+// Nearest probe strictly within distance `cell` of y (ties: lower index), or -1:
+// the smallest (d^2, index) over the probes of the 27 cells around y.  The home
+// cell is searched first; a neighbour cell is skipped when its box lies farther
+// from y than the best probe so far, since none of its probes can then be the
+// answer or tie with it.  (Visiting y's nearer neighbours next, by a per-lane
+// mirrored order, measured 3% slower.)  The box distance is shortened by a
+// margin that covers the rounding of the cell coordinates (v * inv, 2^-18
+// (|v * inv| + 1) cell units) and of the probes' d^2 (a 2^-16 relative margin),
+// so the skip is conservative and the result is the full search's.
 ORF_HD int nearest_probe(const ProbesGPU& pr, f3 y)
 {
-    float inv = 1.0f / pr.cell;
-    int cx = probe_cell(y.x, inv), cy = probe_cell(y.y, inv), cz = probe_cell(y.z, inv);
-    float best_d2 = pr.cell * pr.cell;
+    const float inv = 1.0f / pr.cell;
+    const float ax = y.x * inv, ay = y.y * inv, az = y.z * inv;
+    const int cx = probe_cell(y.x, inv), cy = probe_cell(y.y, inv), cz = probe_cell(y.z, inv);
+    // gap from y to the lower / upper neighbour along each axis, in cell units
+    auto gap_lo = [](float a, int c) { return fmaxf(a - (float)c - 3.814697265625e-6f * (fabsf(a) + 1.0f), 0.0f); };
+    auto gap_hi = [](float a, int c) {
+        return fmaxf((float)c + 1.0f - a - 3.814697265625e-6f * (fabsf(a) + 1.0f), 0.0f);
+    };
+    const float lx = gap_lo(ax, cx), hx = gap_hi(ax, cx), ly = gap_lo(ay, cy), hy = gap_hi(ay, cy),
+                lz = gap_lo(az, cz), hz = gap_hi(az, cz);
+    const float cell2 = pr.cell * pr.cell;
+    const float scale = cell2 * 0.9999847412109375f;  // cell^2 (1 - 2^-16)
+    float best_d2 = cell2;
     int best = -1;
-    for (int dz = -1; dz <= 1; ++dz)
-        for (int dy = -1; dy <= 1; ++dy)
-            for (int dx = -1; dx <= 1; ++dx) {
-                uint2 c = pr.cells[probe_hash(cx + dx, cy + dy, cz + dz, pr.table_mask)];
-                for (unsigned j = c.x; j < c.x + c.y; ++j) {
-                    float4 q = pr.pos[j];
-                    f3 e = mk3(q.x, q.y, q.z) - y;
-                    float d2 = dot(e, e);
-                    if (d2 < best_d2 || (d2 == best_d2 && best >= 0 && (int)j < best)) {
-                        best_d2 = d2;
-                        best = (int)j;
-                    }
-                }
+    for (int i = 0; i < 27; ++i) {
+        const int k = i < 14 ? i + 13 : i - 14;  // 13 (the home cell) first
+        const int dx = k % 3 - 1, dy = (k / 3) % 3 - 1, dz = k / 9 - 1;
+        const float gx = dx < 0 ? lx : (dx > 0 ? hx : 0.0f);
+        const float gy = dy < 0 ? ly : (dy > 0 ? hy : 0.0f);
+        const float gz = dz < 0 ? lz : (dz > 0 ? hz : 0.0f);
+        if ((gx * gx + gy * gy + gz * gz) * scale > best_d2) continue;
+        uint2 c = pr.cells[probe_hash(cx + dx, cy + dy, cz + dz, pr.table_mask)];
+        for (unsigned j = c.x; j < c.x + c.y; ++j) {
+            float4 q = pr.pos[j];
+            f3 e = mk3(q.x, q.y, q.z) - y;
+            float d2 = dot(e, e);
+            if (d2 < best_d2 || (d2 == best_d2 && best >= 0 && (int)j < best)) {
+                best_d2 = d2;
+                best = (int)j;
             }
+        }
+    }
     return best;
 }
 
