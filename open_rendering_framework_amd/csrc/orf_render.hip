@@ -103,13 +103,17 @@ void launch_megakernel(const SceneGPU& sc, const CameraGPU& cam, const FilmGPU& 
     const int tiles_x = (film.width + 7) / 8, tiles_y = (film.height + 7) / 8;
     const int waves = mega_waves(sc.features);
     // Samples per item: the largest of kMegaChunk, /2, ... that still gives every
-    // wave of a full persistent grid kMegaItemsPerWave items.  Larger chunks
-    // shorten the waves' idle lanes per path but leave the chip idle behind the
-    // last expensive items (tiles differ several-fold in cost); 2 samples per
-    // item measured best on the 512x512 scene.
+    // wave of a full persistent grid kMegaItemsPerWave items (twice as many with
+    // translucent materials, whose gathers make tiles differ most in cost).
+    // Larger chunks shorten the waves' idle lanes per path but leave the chip
+    // idle behind the last expensive items.  Measured: the 512x512, 64 spp path
+    // scene is best at 16 items per wave (2 samples per item; 32 items, 1 sample:
+    // -2.3%); the 512x512, 128 spp subsurface scene at 32 (2 samples per item;
+    // +2.9% over 16 items of 4 samples, the same as 64 items of 1 sample).
     const long long full = (long long)persistent_waves(1LL << 40, waves);
+    const long long per_wave = (sc.features & kFeatTranslucent) ? 2 * kMegaItemsPerWave : kMegaItemsPerWave;
     unsigned per = kMegaChunk;
-    while (per > 1 && (long long)tiles_x * tiles_y * ((spp + per - 1) / per) < kMegaItemsPerWave * full) per >>= 1;
+    while (per > 1 && (long long)tiles_x * tiles_y * ((spp + per - 1) / per) < per_wave * full) per >>= 1;
     per = std::min(per, spp);
     const int chunks = (int)((spp + per - 1) / per);
     const long long items = (long long)tiles_x * tiles_y * chunks;
