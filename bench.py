@@ -37,7 +37,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 # HBM bytes per launch from rocprofv3 FETCH_SIZE/WRITE_SIZE passes of these exact
 # workloads, scaled by a 1 GiB copy kernel (profiles/hbm_traffic_pmc.md)
 TRAFFIC_PATH_512_64SPP = 16.17e9
-TRAFFIC_SSS_512_128SPP = 142.7e9
+TRAFFIC_SSS_512_128SPP = 133.3e9
 TRAFFIC_BSDF_2E22 = 218.4e6
 
 
